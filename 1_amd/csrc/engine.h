@@ -1,0 +1,179 @@
+// Device-resident LLG engine: shared declarations between the convolution
+// kernels (conv.hip), the Runge-Kutta kernels (rk.hip) and the host-side
+// simulation (sim.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "gk_common.h"
+
+namespace gk {
+
+// ---------------------------------------------------------------- device state
+// One per simulation, in device memory.  Control kernels (one thread) own
+// the scalar fields; every stage kernel reads dt/time from here, so a whole
+// run can be enqueued (or graph-replayed) without host round trips.
+struct EngineState {
+    double time;        // internal units
+    double time0;       // start of the current step
+    double stop;        // Run(): done when time >= stop
+    long long steps;
+    long long steps_target;  // Steps(n): done when steps >= steps_target
+    long long trials_total, rejects_total;
+    float dt;           // dt for the next trial (rk.dt)
+    float dt_used;      // dt of the current trial
+    float max_torque;   // maxabs(k0) of the current step (rk.go:353)
+    float err;          // error estimate of the last trial (rk.go:403)
+    unsigned int red_k;    // atomicMax(float bits) of |k| of the stage being evaluated
+    unsigned int red_k6;   // same for the last stage of an FSAL tableau (k0 of the next step)
+    unsigned int red_err;  // atomicMax(float bits) of |m_low - m_high|
+    int trials;         // trials of the current step (MAX_STEP_TRIALS = 10)
+    int accept;         // last trial accepted
+    int step_end;       // last trial ended the step (accepted or out of trials)
+    int fsal_init;      // an FSAL k is available (rk.go:311)
+    int done;           // no further trials: kernels become no-ops
+    int trial_active;   // latched at the start of each trial
+    int new_step;       // next trial starts a new step
+    int log_n;          // trials logged
+    int nan_dt;         // dt became NaN (rk.go:448-452)
+    double energy;
+};
+
+constexpr int kMaxStepTrials = 10;  // rk.go:277
+
+// Applied field: 0 none, 1 static, >= 2 points interpolated linearly in time
+// (simfield.go:101-153).  Times in seconds, fields in tesla, internal order.
+struct FieldTable {
+    int n;
+    const double* t;  // device
+    const float* b;   // device, 3n
+    float b0[3];      // static value (n == 1)
+};
+
+__device__ __forceinline__ void applied_field(const FieldTable& f, double t_si, float out[3]) {
+    out[0] = out[1] = out[2] = 0.f;
+    if (f.n == 0) return;
+    if (f.n == 1) {
+        out[0] = f.b0[0];
+        out[1] = f.b0[1];
+        out[2] = f.b0[2];
+        return;
+    }
+    int i = 0;
+    while (i < f.n && f.t[i] < t_si) i++;  // first point at or past t
+    if (i - 1 < 0 || i >= f.n) return;    // out of range: zero field
+    double dt = f.t[i] - f.t[i - 1];
+    float s = (float)((t_si - f.t[i - 1]) / dt);
+    for (int c = 0; c < 3; c++) out[c] = f.b[3 * (i - 1) + c] + s * (f.b[3 * i + c] - f.b[3 * (i - 1) + c]);
+}
+
+// ---------------------------------------------------------------- stage build
+// m_i = m + sum_j (dt a_ij) k_j, evaluated while loading the forward Z pass
+// (rk.go:316-322).  Coefficients a_ij are float32 like the reference; dt is
+// read from the device state.
+struct StageIn {
+    const float* m;
+    const float* k[6];
+    float a[6];
+    int nk;
+    float* mout;             // where m_i is stored for the torque pass (may be null)
+    const EngineState* st;   // null: plain input (no stage build)
+};
+
+// ---------------------------------------------------------------- convolution plan
+struct ConvPlan {
+    int N[3];   // data size (internal X,Y,Z)
+    int P[3];   // padded FFT size
+    int M, Q;   // Pz/2, Pz/2+1
+    int nK;     // 6 (3-D) or 4 (2-D)
+    float* Kc;  // compressed kernel spectrum (see conv.hip)
+    float2* B1; // [3][Nx][Q][Ny]
+    float2* B2; // 3-D: [3][Q][Py][Nx]
+    size_t bytes;
+};
+
+// Choose padded sizes, allocate buffers.  Kc is filled by conv_build_kernel.
+void conv_plan_init(ConvPlan& cp, const int N[3]);
+void conv_plan_free(ConvPlan& cp);
+// FaceKernel6 (accuracy x accuracy charges per face) in FP64 on the device,
+// optionally plus the 6-neighbour exchange kernel, FFT in FP64, stored
+// compressed in cp.Kc.  If full != null, also writes nK x [Px][Py][Q] floats
+// in the reference's layout (conv.go:110-167).  Returns max|Im|/max|Re| of the
+// spectrum (the reference warns above 1e-4, conv.go:153).
+double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool exch, float* full_host,
+                         float* full_dev);
+// real-space demag (+exchange) kernel over the padded grid P, 6 x P floats to the host
+void conv_realspace_kernel(const int P[3], const double cell[3], int accuracy, bool exch, float* out_host);
+// Optional per-kernel-class event timing (bench/profiling).  Classes:
+// 0 Z fwd, 1 Y fwd, 2 X.K.X (3-D) or Y.K.Y (2-D), 3 Y inv, 4 Z inv, 5 torque, 6 RK final, 7 control/copy.
+struct Prof {
+    static constexpr int kClasses = 8;
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, int>> marks;  // (class, index of start event; end = start+1)
+    int used = 0;
+    void begin(int cls, hipStream_t s);
+    void end(hipStream_t s);
+    void collect(double ms[kClasses], int counts[kClasses]);
+    void reset() { marks.clear(); used = 0; }
+    ~Prof();
+};
+// h = K * m (planar 3N each); in = stage build description, out = 3N floats.
+void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof = nullptr);
+
+// ---------------------------------------------------------------- RK kernels
+struct TorqueArgs {
+    float alpha, dtg;       // damping, 1/(1+alpha^2)
+    int anisType;
+    float kAnis;            // 2K1/(mu0 Ms^2) (init.go:103-108)
+    float u[3];
+    FieldTable field;
+    float unitField;        // tesla per internal field unit
+    double unitTime;        // seconds per internal time unit
+    float c;                // stage node c_i
+    int red6;               // accumulate |k| into red_k6 instead of red_k
+    int exch;               // add the 6-neighbour exchange stencil here
+    int n[3];
+    float cexch[4];         // cx, cy, cz, c0
+};
+// k = torque(m_i, H + local fields (+ exchange)); in place over hk.
+void launch_torque(const float* mi, float* hk, EngineState* st, const TorqueArgs& a, int N, hipStream_t s);
+
+struct FinalArgs {
+    int S;
+    float b[7], b2[7];
+    int adaptive;
+};
+// high/low order solutions; adaptive: mtmp = m_high, red_err = max|m_low-m_high|;
+// fixed step: m = normalize(m_high) in place.
+void launch_rk_final(float* m, const float* const k[7], float* mtmp, EngineState* st, const FinalArgs& a, int N,
+                     hipStream_t s);
+void launch_copy_if_active(float* dst, const float* src, long n, const EngineState* st, hipStream_t s);
+
+struct CtrlParams {
+    int adaptive;
+    float maxError, errorOrder;
+    float inputDt;          // internal
+    float minDm, maxDm, minDt, maxDt;  // minDt/maxDt in internal units
+    double* log_dt;
+    int* log_accept;
+    int log_cap;
+};
+void launch_ctrl_begin(EngineState* st, hipStream_t s);
+void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s);
+void launch_ctrl_end(EngineState* st, const CtrlParams& p, hipStream_t s);
+void launch_commit(float* m, const float* mtmp, EngineState* st, int N, hipStream_t s);
+void launch_normalize_active(float* m, const EngineState* st, int N, hipStream_t s);
+
+void launch_energy(const float* m, const float* h, const float hext[3], int anisType, float kAnis, const float u[3],
+                   int N, double* out, hipStream_t s);
+// steps_target = steps + nsteps (nsteps < 0: unlimited), stop time; recomputes `done`
+void launch_set_target(EngineState* st, long long nsteps, double stop, hipStream_t s);
+// FP64 reductions for outputs: sums[0..2] = sum of each component; energy
+void launch_sum3(const float* v, int N, double* out_dev, hipStream_t s);
+void launch_sum1(const float* v, int N, double* out_dev, hipStream_t s);
+
+}  // namespace gk

@@ -1,0 +1,59 @@
+// Common host/device helpers for libgpukern (MI355X / gfx950).
+//
+// Error behaviour follows the reference library: every failure prints a
+// message to stderr and aborts the process (gpu_safe.h:36-57); there are no
+// return codes on the reference entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define GK_CHECK(call)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            fprintf(stderr, "libgpukern: HIP error %s (%d) at %s:%d: %s\n", hipGetErrorString(e_), \
+                    (int)e_, __FILE__, __LINE__, #call);                                           \
+            abort();                                                                               \
+        }                                                                                          \
+    } while (0)
+
+#define GK_ASSERT(cond, ...)                                                        \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            fprintf(stderr, "libgpukern: assertion failed at %s:%d: %s: ", __FILE__, \
+                    __LINE__, #cond);                                               \
+            fprintf(stderr, __VA_ARGS__);                                           \
+            fprintf(stderr, "\n");                                                  \
+            abort();                                                                \
+        }                                                                           \
+    } while (0)
+
+#define GK_LAUNCH_CHECK() GK_CHECK(hipGetLastError())
+
+namespace gk {
+
+// Stream every libgpukern launch goes to.  The reference entry points are
+// synchronous (gpu_safe.h:41); the engine entry points enqueue on this stream
+// and synchronise only where their contract says so.
+hipStream_t stream();
+void sync();  // hipStreamSynchronize(stream()) + error check
+
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// 1-D launch geometry for N elements, 256 threads (4 wave64s) per block.
+constexpr int kBlock = 256;
+inline dim3 grid1d(long n, int block = kBlock) {
+    long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 2147483647L) g = 2147483647L;
+    return dim3((unsigned)g);
+}
+
+// Twiddle tables exp(-2 pi i k / L), k < L, in float2 and double2, computed
+// on the host in long double and uploaded once per length (cached).
+const float2* twiddle_f32(int L);
+const double2* twiddle_f64(int L);
+
+}  // namespace gk

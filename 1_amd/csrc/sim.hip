@@ -1,0 +1,825 @@
+// Host side of the LLG engine: the reference's Sim/RK driver (sim.go,
+// init.go:80-287, rk.go:41-445, simrun.go:19-92, field.go, torque.go)
+// restated in C++ on top of libgpukern.
+//
+// MM_MODE_FUSED enqueues each Runge-Kutta trial as a fixed sequence of fused
+// kernels whose scalars (dt, time, accept) live on the device, so Run()
+// only synchronises once per batch of trials.  MM_MODE_ABI replays the
+// reference's own call sequence through the drop-in gpukern.h functions, one
+// synchronous call at a time, as the Go driver would.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpukern.h"
+#include "../../include/mumax_engine.h"
+#include "engine.h"
+
+using namespace gk;
+
+namespace {
+
+struct Tableau {
+    int S = 0, fsal = 0, adaptive = 0;
+    float order = 0;
+    float a[7][7] = {}, b[7] = {}, b2[7] = {}, c[7] = {};
+};
+
+// rk.go:62-212, coefficients rounded to float32 as in the reference
+Tableau make_tableau(int solver) {
+    Tableau t;
+    switch (solver) {
+    case MM_RK2:
+    case MM_RK12:
+        t.S = 2;
+        t.c[1] = 1.f;
+        t.a[1][0] = 1.f;
+        t.b[0] = .5f;
+        t.b[1] = .5f;
+        if (solver == MM_RK12) {
+            t.adaptive = 1;
+            t.order = 1.f;
+            t.b2[0] = 1.f;
+        }
+        break;
+    case MM_RK3:
+        t.S = 3;
+        t.c[1] = 1.f / 2.f;
+        t.c[2] = 1.f;
+        t.a[1][0] = 1.f / 2.f;
+        t.a[2][0] = -1.f;
+        t.a[2][1] = 2.f;
+        t.b[0] = 1.f / 6.f;
+        t.b[1] = 2.f / 3.f;
+        t.b[2] = 1.f / 6.f;
+        break;
+    case MM_RK23:
+        t.S = 4;
+        t.fsal = 1;
+        t.c[1] = 1.f / 2.f;
+        t.c[2] = 3.f / 4.f;
+        t.c[3] = 1.f;
+        t.a[1][0] = 1.f / 2.f;
+        t.a[2][1] = 3.f / 4.f;
+        t.a[3][0] = 2.f / 9.f;
+        t.a[3][1] = 1.f / 3.f;
+        t.a[3][2] = 4.f / 9.f;
+        t.b[0] = 2.f / 9.f;
+        t.b[1] = 1.f / 3.f;
+        t.b[2] = 4.f / 9.f;
+        t.adaptive = 1;
+        t.order = 2.f;
+        t.b2[0] = 7.f / 24.f;
+        t.b2[1] = 1.f / 4.f;
+        t.b2[2] = 1.f / 3.f;
+        t.b2[3] = 1.f / 8.f;
+        break;
+    case MM_RK4:
+        t.S = 4;
+        t.c[1] = 1.f / 2.f;
+        t.c[2] = 1.f / 2.f;
+        t.c[3] = 1.f;
+        t.a[1][0] = 1.f / 2.f;
+        t.a[2][1] = 1.f / 2.f;
+        t.a[3][2] = 1.f;
+        t.b[0] = 1.f / 6.f;
+        t.b[1] = 1.f / 3.f;
+        t.b[2] = 1.f / 3.f;
+        t.b[3] = 1.f / 6.f;
+        break;
+    case MM_RKCK:
+        t.S = 6;
+        t.c[1] = 1.f / 5.f;
+        t.c[2] = 3.f / 10.f;
+        t.c[3] = 3.f / 5.f;
+        t.c[4] = 1.f;
+        t.c[5] = 7.f / 8.f;
+        t.a[1][0] = 1.f / 5.f;
+        t.a[2][0] = 3.f / 40.f;
+        t.a[2][1] = 9.f / 40.f;
+        t.a[3][0] = 3.f / 10.f;
+        t.a[3][1] = -9.f / 10.f;
+        t.a[3][2] = 6.f / 5.f;
+        t.a[4][0] = -11.f / 54.f;
+        t.a[4][1] = 5.f / 2.f;
+        t.a[4][2] = -70.f / 27.f;
+        t.a[4][3] = 35.f / 27.f;
+        t.a[5][0] = 1631.f / 55296.f;
+        t.a[5][1] = 175.f / 512.f;
+        t.a[5][2] = 575.f / 13824.f;
+        t.a[5][3] = 44275.f / 110592.f;
+        t.a[5][4] = 253.f / 4096.f;
+        t.b[0] = 37.f / 378.f;
+        t.b[2] = 250.f / 621.f;
+        t.b[3] = 125.f / 594.f;
+        t.b[5] = 512.f / 1771.f;
+        t.adaptive = 1;
+        t.order = 4.f;
+        t.b2[0] = 2825.f / 27648.f;
+        t.b2[2] = 18575.f / 48384.f;
+        t.b2[3] = 13525.f / 55296.f;
+        t.b2[4] = 277.f / 14336.f;
+        t.b2[5] = 1.f / 4.f;
+        break;
+    case MM_RKDP:
+        t.S = 7;
+        t.fsal = 1;
+        t.c[1] = 1.f / 5.f;
+        t.c[2] = 3.f / 10.f;
+        t.c[3] = 4.f / 5.f;
+        t.c[4] = 8.f / 9.f;
+        t.c[5] = 1.f;
+        t.c[6] = 1.f;
+        t.a[1][0] = 1.f / 5.f;
+        t.a[2][0] = 3.f / 40.f;
+        t.a[2][1] = 9.f / 40.f;
+        t.a[3][0] = 44.f / 45.f;
+        t.a[3][1] = -56.f / 15.f;
+        t.a[3][2] = 32.f / 9.f;
+        t.a[4][0] = 19372.f / 6561.f;
+        t.a[4][1] = -25360.f / 2187.f;
+        t.a[4][2] = 64448.f / 6561.f;
+        t.a[4][3] = -212.f / 729.f;
+        t.a[5][0] = 9017.f / 3168.f;
+        t.a[5][1] = -355.f / 33.f;
+        t.a[5][2] = 46732.f / 5247.f;
+        t.a[5][3] = 49.f / 176.f;
+        t.a[5][4] = -5103.f / 18656.f;
+        t.a[6][0] = 35.f / 384.f;
+        t.a[6][2] = 500.f / 1113.f;
+        t.a[6][3] = 125.f / 192.f;
+        t.a[6][4] = -2187.f / 6784.f;
+        t.a[6][5] = 11.f / 84.f;
+        t.b[0] = 35.f / 384.f;
+        t.b[2] = 500.f / 1113.f;
+        t.b[3] = 125.f / 192.f;
+        t.b[4] = -2187.f / 6784.f;
+        t.b[5] = 11.f / 84.f;
+        t.adaptive = 1;
+        t.order = 4.f;
+        t.b2[0] = 5179.f / 57600.f;
+        t.b2[2] = 7571.f / 16695.f;
+        t.b2[3] = 393.f / 640.f;
+        t.b2[4] = -92097.f / 339200.f;
+        t.b2[5] = 187.f / 2100.f;
+        t.b2[6] = 1.f / 40.f;
+        break;
+    default: fprintf(stderr, "libgpukern: unknown solver type %d\n", solver); abort();
+    }
+    return t;
+}
+
+const double kMu0 = 4.0e-7 * 3.14159265358979323846;
+const double kGamma0 = 2.211e5;
+
+}  // namespace
+
+struct mmSim {
+    mmParams p;
+    int mode = MM_MODE_FUSED;
+    int N = 0;
+    long N3 = 0;
+    double unitLength = 0, unitTime = 0, unitField = 0, unitEnergy = 0, unitEdens = 0;
+    double cell[3] = {};  // internal units
+    float kAnis = 0;
+    float u[3] = {};
+    Tableau tb;
+    std::vector<double> fieldT;
+    std::vector<float> fieldB;
+    hipStream_t s = nullptr;
+    // device
+    float* m = nullptr;
+    float* k[7] = {};
+    float* mi = nullptr;
+    float* mtmp = nullptr;
+    EngineState* st = nullptr;
+    double* log_dt = nullptr;
+    int* log_acc = nullptr;
+    int log_cap = 1 << 16;
+    double* dsum = nullptr;
+    double* fT = nullptr;
+    float* fB = nullptr;
+    FieldTable field{};
+    ConvPlan cp{};
+    bool conv = false;
+    double kernelImagRatio = 0;
+    bool fsalHost = false;
+    unsigned long long bytes = 0;
+    Prof prof;
+    // ABI mode
+    gpuFFT3dPlan* plan = nullptr;
+    int Pabi[3] = {};
+    float* abuf[3] = {};
+    float* akern[6] = {};
+    float* am1 = nullptr;
+    float* abackup = nullptr;
+    float* redDev = nullptr;
+    std::vector<float> redHost;
+    int redBlocks = 0, redThreads = 0;
+    // host mirror of the reference's RK state for ABI mode
+    double htime = 0;
+    float hdt = 0;
+    long long hsteps = 0, htrials = 0, hrejects = 0;
+    float herr = 0, htorque = 0;
+    bool hfsal = false;
+    std::vector<double> hlog_dt;
+    std::vector<int> hlog_acc;
+};
+
+static void* dmalloc(mmSim* s, size_t bytes) {
+    void* p = nullptr;
+    GK_CHECK(hipMalloc(&p, bytes));
+    s->bytes += bytes;
+    return p;
+}
+
+static void applied_field_host(const mmSim* s, double t_si, float out[3]) {
+    out[0] = out[1] = out[2] = 0.f;
+    int n = s->p.nField;
+    if (n == 0) return;
+    if (n == 1) {
+        for (int c = 0; c < 3; c++) out[c] = s->fieldB[c];
+        return;
+    }
+    int i = 0;
+    while (i < n && s->fieldT[i] < t_si) i++;
+    if (i - 1 < 0 || i >= n) return;
+    double dt = s->fieldT[i] - s->fieldT[i - 1];
+    float f = (float)((t_si - s->fieldT[i - 1]) / dt);
+    for (int c = 0; c < 3; c++) out[c] = s->fieldB[3 * (i - 1) + c] + f * (s->fieldB[3 * i + c] - s->fieldB[3 * (i - 1) + c]);
+}
+
+static EngineState read_state(mmSim* s) {
+    EngineState h;
+    GK_CHECK(hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, s->s));
+    GK_CHECK(hipStreamSynchronize(s->s));
+    return h;
+}
+
+static CtrlParams ctrl_params(const mmSim* s) {
+    CtrlParams c{};
+    c.adaptive = s->tb.adaptive;
+    c.maxError = (float)s->p.maxError;
+    c.errorOrder = s->tb.order;
+    c.inputDt = (float)(s->p.dt / s->unitTime);
+    c.minDm = (float)s->p.minDm;
+    c.maxDm = (float)s->p.maxDm;
+    c.minDt = (float)(s->p.minDt / s->unitTime);
+    c.maxDt = (float)(s->p.maxDt / s->unitTime);
+    c.log_dt = s->log_dt;
+    c.log_accept = s->log_acc;
+    c.log_cap = s->log_cap;
+    return c;
+}
+
+static TorqueArgs torque_args(const mmSim* s, float c, int red6) {
+    TorqueArgs a{};
+    a.alpha = (float)s->p.alpha;
+    a.dtg = (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha));
+    a.anisType = s->p.anisType;
+    a.kAnis = s->kAnis;
+    for (int i = 0; i < 3; i++) a.u[i] = s->u[i];
+    a.field = s->field;
+    a.unitField = (float)s->unitField;
+    a.unitTime = s->unitTime;
+    a.c = c;
+    a.red6 = red6;
+    return a;
+}
+
+// ---------------------------------------------------------------- fused mode
+static void enqueue_trial(mmSim* s) {
+    const Tableau& t = s->tb;
+    const CtrlParams cp = ctrl_params(s);
+    Prof& pf = s->prof;
+    pf.begin(7, s->s);
+    launch_ctrl_begin(s->st, s->s);
+    pf.end(s->s);
+    for (int i = 0; i < t.S; i++) {
+        if (i == 0 && t.fsal && s->fsalHost) {
+            pf.begin(7, s->s);
+            launch_copy_if_active(s->k[0], s->k[t.S - 1], s->N3, s->st, s->s);  // rk.go:311-315
+            launch_ctrl_after_k0(s->st, 1, cp, s->s);
+            pf.end(s->s);
+            continue;
+        }
+        StageIn si{};
+        si.m = s->m;
+        si.st = s->st;
+        for (int j = 0; j < t.S; j++)
+            if (t.a[i][j] != 0.f) {
+                si.k[si.nk] = s->k[j];
+                si.a[si.nk] = t.a[i][j];
+                si.nk++;
+            }
+        si.mout = si.nk ? s->mi : nullptr;
+        conv_apply(s->cp, si, s->k[i], s->s, &pf);
+        pf.begin(5, s->s);
+        launch_torque(si.nk ? s->mi : s->m, s->k[i], s->st, torque_args(s, t.c[i], t.fsal && i == t.S - 1), s->N,
+                      s->s);
+        pf.end(s->s);
+        if (i == 0) {
+            pf.begin(7, s->s);
+            launch_ctrl_after_k0(s->st, 0, cp, s->s);
+            pf.end(s->s);
+        }
+    }
+    FinalArgs fa{};
+    fa.S = t.S;
+    for (int i = 0; i < 7; i++) {
+        fa.b[i] = t.b[i];
+        fa.b2[i] = t.b2[i];
+    }
+    fa.adaptive = t.adaptive;
+    const float* kk[7];
+    for (int i = 0; i < 7; i++) kk[i] = s->k[i] ? s->k[i] : s->k[0];
+    pf.begin(6, s->s);
+    launch_rk_final(s->m, kk, s->mtmp, s->st, fa, s->N, s->s);
+    pf.end(s->s);
+    pf.begin(7, s->s);
+    launch_ctrl_end(s->st, cp, s->s);
+    if (t.adaptive) launch_commit(s->m, s->mtmp, s->st, s->N, s->s);
+    pf.end(s->s);
+    if (t.fsal) s->fsalHost = true;
+}
+
+static long fused_run(mmSim* s, long long nsteps, double stop, bool wait) {
+    EngineState h = read_state(s);
+    s->fsalHost = h.fsal_init != 0;
+    const long long steps0 = h.steps;
+    launch_set_target(s->st, nsteps, stop, s->s);
+    if (!wait) {
+        // enqueue exactly nsteps trials (fixed-step tableaux never reject)
+        for (long long i = 0; i < nsteps; i++) enqueue_trial(s);
+        return nsteps;
+    }
+    int batch = 4;
+    while (true) {
+        for (int i = 0; i < batch; i++) enqueue_trial(s);
+        h = read_state(s);
+        if (h.nan_dt) {
+            fprintf(stderr, "libgpukern: dt = NaN (rk.go:448-452)\n");
+            abort();
+        }
+        if (h.done) break;
+        if (batch < 64) batch *= 2;
+    }
+    return (long)(h.steps - steps0);
+}
+
+// ---------------------------------------------------------------- ABI mode (reference call sequence)
+static void abi_calc_heff(mmSim* s, float* m, float* h) {
+    // conv.go:57-100
+    const int N = s->N;
+    for (int c = 0; c < 3; c++) gpuFFT3dPlan_forward(s->plan, m + c * N, s->abuf[c]);
+    const int nReal = s->Pabi[0] * s->Pabi[1] * (s->Pabi[2] + 2);
+    if (s->Pabi[0] > 1)
+        gpu_kernelmul6(s->abuf[0], s->abuf[1], s->abuf[2], s->akern[0], s->akern[1], s->akern[2], s->akern[3],
+                       s->akern[4], s->akern[5], nReal);
+    else
+        gpu_kernelmul4(s->abuf[0], s->abuf[1], s->abuf[2], s->akern[0], s->akern[1], s->akern[2], s->akern[3],
+                       nReal);
+    for (int c = 0; c < 3; c++) gpuFFT3dPlan_inverse(s->plan, s->abuf[c], h + c * N);
+    // addLocalFields (field.go:36-61)
+    float b[3];
+    applied_field_host(s, s->htime * s->unitTime, b);
+    float hext[3];
+    for (int c = 0; c < 3; c++) hext[c] = b[c] / (float)s->unitField;
+    float kk = s->kAnis;
+    float ax[3] = {s->u[0], s->u[1], s->u[2]};
+    gpu_add_local_fields(m, h, N, hext, nullptr, s->p.anisType, &kk, ax);
+}
+
+static void abi_torque(mmSim* s, float* m, float* h) {
+    // torque.go:33-53 with dt = 1
+    const float alpha = (float)s->p.alpha;
+    gpu_deltaM(m, h, alpha, nullptr, 1.f / (1.f + alpha * alpha), s->N);
+}
+
+static void abi_step(mmSim* s) {
+    // rk.go:279-445
+    const Tableau& t = s->tb;
+    const int order = t.S;
+    const int n3 = (int)s->N3;
+    float* m = s->m;
+    float* m1 = s->am1;
+    const double time0 = s->htime;
+    memcpy_on_gpu(m, s->abackup, n3);
+    bool good = false;
+    int trials = 0;
+    while (!good && trials < kMaxStepTrials) {
+        trials++;
+        s->htrials++;
+        for (int i = 0; i < order; i++) {
+            if (t.fsal && i == 0 && s->hfsal) {
+                memcpy_on_gpu(s->k[order - 1], s->k[0], n3);
+            } else {
+                s->htime = time0 + (double)(t.c[i] * s->hdt);
+                memcpy_on_gpu(m, m1, n3);
+                for (int j = 0; j < order; j++)
+                    if (t.a[i][j] != 0.f) gpu_madd(m1, s->hdt * t.a[i][j], s->k[j], n3);
+                abi_calc_heff(s, m1, s->k[i]);
+                abi_torque(s, m1, s->k[i]);
+                s->hfsal = true;
+            }
+            if (i == 0) {
+                if (!t.adaptive) s->hdt = (float)(s->p.dt / s->unitTime);
+                float maxTorque = gpu_reduce(REDUCE_MAXABS, s->k[0], s->redDev, s->redHost.data(), s->redBlocks,
+                                             s->redThreads, n3);
+                s->htorque = maxTorque;
+                float dm = s->hdt * maxTorque;
+                if (dm < (float)s->p.minDm) s->hdt = (float)s->p.minDm / maxTorque;
+                if (s->hdt * s->unitTime < s->p.minDt) s->hdt = (float)(s->p.minDt / s->unitTime);
+                if (s->p.maxDm != 0 && dm > (float)s->p.maxDm) s->hdt = (float)s->p.maxDm / maxTorque;
+                if (s->p.maxDt != 0 && s->hdt * s->unitTime > s->p.maxDt) s->hdt = (float)(s->p.maxDt / s->unitTime);
+                GK_ASSERT(!std::isnan(s->hdt), "dt = NaN");
+            }
+        }
+        const float dtu = s->hdt;
+        s->htime = time0 + (double)s->hdt;
+        if (t.adaptive) {
+            memcpy_on_gpu(m, m1, n3);
+            for (int i = 0; i < order; i++)
+                if (t.b2[i] != 0.f) gpu_madd(m1, t.b2[i] * s->hdt, s->k[i], n3);
+        }
+        for (int i = 0; i < order; i++)
+            if (t.b[i] != 0.f) gpu_madd(m, t.b[i] * s->hdt, s->k[i], n3);
+        bool bad = false;
+        if (t.adaptive) {
+            gpu_madd(m1, -1.f, m, n3);
+            float err = gpu_reduce(REDUCE_MAXABS, m1, s->redDev, s->redHost.data(), s->redBlocks, s->redThreads, n3);
+            s->herr = err;
+            float factor = (float)pow((double)((float)s->p.maxError / err), 1.0 / (double)t.order);
+            if (factor > 2.f) factor = 2.f;
+            if (factor < 0.01f) factor = 0.01f;
+            s->hdt = s->hdt * factor;
+            GK_ASSERT(!std::isnan(s->hdt), "dt = NaN");
+            if (err > 2.f * (float)s->p.maxError) {
+                memcpy_on_gpu(s->abackup, m, n3);
+                bad = true;
+                s->hrejects++;
+            } else {
+                if (s->hdt * s->unitTime < s->p.minDt) s->hdt = (float)(s->p.minDt / s->unitTime);
+                if (s->p.maxDt != 0 && s->hdt * s->unitTime > s->p.maxDt) s->hdt = (float)(s->p.maxDt / s->unitTime);
+            }
+        }
+        good = !bad;
+        s->hlog_dt.push_back(dtu);
+        s->hlog_acc.push_back(!bad);
+    }
+    gpu_normalize_uniform(m, s->N);
+    s->hsteps++;
+}
+
+static void abi_init(mmSim* s, const double cellInt[3]) {
+    // initConv (init.go:234-274) + NewConv/loadKernel6 (conv.go:35-167) through the drop-in API
+    int size[3] = {s->p.size[0], s->p.size[1], s->p.size[2]};
+    for (int i = 0; i < 3; i++) s->Pabi[i] = size[i] > 1 && !s->p.periodic[i] ? 2 * size[i] : size[i];
+    s->plan = new_gpuFFT3dPlan_padded(size, s->Pabi);
+    const long phys = (long)s->Pabi[0] * s->Pabi[1] * (s->Pabi[2] + 2);
+    for (int c = 0; c < 3; c++) s->abuf[c] = new_gpu_array((int)phys);
+    // real-space kernel in FP64 on the device (FaceKernel6 + Exch6), rounded to float32;
+    // the spectrum of the real-space kernel is taken with the drop-in plan, as loadKernel6 does
+    const int nK = s->Pabi[0] == 1 ? 4 : 6;
+    const long NP = (long)s->Pabi[0] * s->Pabi[1] * s->Pabi[2];
+    std::vector<float> Kr((size_t)6 * NP);
+    conv_realspace_kernel(s->Pabi, cellInt, s->p.demagAccuracy > 0 ? s->p.demagAccuracy : 8,
+                          s->p.exchInConv && s->p.exchType == 6, Kr.data());
+    gpuFFT3dPlan* kplan = new_gpuFFT3dPlan_padded(s->Pabi, s->Pabi);
+    float* devIn = new_gpu_array((int)NP);
+    float* devOut = new_gpu_array((int)phys);
+    std::vector<float> hostOut(phys);
+    const float norm = 1.0f / (float)gpuFFT3dPlan_normalization(kplan);
+    const int Q = s->Pabi[2] / 2 + 1;
+    const long ksize = (long)s->Pabi[0] * s->Pabi[1] * Q;
+    for (int k = 0; k < nK; k++) {
+        memcpy_to_gpu(Kr.data() + k * NP, devIn, (int)NP);
+        gpuFFT3dPlan_forward(kplan, devIn, devOut);
+        memcpy_from_gpu(devOut, hostOut.data(), (int)phys);
+        for (long j = 0; j < ksize; j++) hostOut[j] = hostOut[2 * j] * norm;
+        s->akern[k] = new_gpu_array((int)ksize);
+        memcpy_to_gpu(hostOut.data(), s->akern[k], (int)ksize);
+    }
+    free_gpu_array(devIn);
+    free_gpu_array(devOut);
+    delete_gpuFFT3dPlan(kplan);
+    s->am1 = new_gpu_array((int)s->N3);
+    s->abackup = new_gpu_array((int)s->N3);
+    // Reductor init (redux.go:103-121)
+    s->redThreads = gpu_maxthreads() / 2;
+    if (s->redThreads == 0) s->redThreads = 1;
+    while (s->N3 <= s->redThreads) s->redThreads /= 2;
+    s->redBlocks = (int)((s->N3 + s->redThreads * 2 - 1) / (s->redThreads * 2));
+    s->redDev = new_gpu_array(s->redBlocks);
+    s->redHost.resize(s->redBlocks);
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
+    mmSim* s = new mmSim;
+    s->p = *p;
+    s->mode = mode;
+    s->s = stream();
+    GK_ASSERT(p->size[0] > 0 && p->size[1] > 0 && p->size[2] > 1, "bad grid size %d %d %d", p->size[0], p->size[1],
+              p->size[2]);
+    GK_ASSERT(!p->periodic[0] && !p->periodic[1] && !p->periodic[2], "periodic boundaries are not supported");
+    GK_ASSERT(p->msat > 0 && p->aexch > 0, "msat and aexch must be set (init.go:87-97)");
+    GK_ASSERT(p->wantDemag, "the engine needs demag on");
+    GK_ASSERT(p->exchType == 0 || (p->exchType == 6 && p->exchInConv),
+              "exchange must be 0 or 6-neighbour folded into the convolution");
+    s->N = p->size[0] * p->size[1] * p->size[2];
+    s->N3 = 3L * s->N;
+    // units (material.go:36-110)
+    s->unitLength = sqrt(2.0 * p->aexch / (kMu0 * p->msat * p->msat));
+    s->unitTime = 1.0 / (kGamma0 * p->msat);
+    s->unitField = kMu0 * p->msat;
+    s->unitEnergy = p->aexch * s->unitLength;
+    s->unitEdens = s->unitEnergy / (s->unitLength * s->unitLength * s->unitLength);
+    for (int i = 0; i < 3; i++) s->cell[i] = p->cellsize[i] / s->unitLength;
+    s->kAnis = (float)(p->k1 / s->unitEdens);
+    for (int i = 0; i < 3; i++) s->u[i] = (float)p->anisAxis[i];
+    s->tb = make_tableau(p->solver);
+    GK_ASSERT(!s->tb.adaptive || p->maxError > 0, "maxError must be > 0 for an adaptive solver");
+    GK_ASSERT(s->tb.adaptive || p->dt > 0, "a fixed-step solver needs dt > 0");
+    if (p->nField > 0) {
+        s->fieldT.assign(p->fieldT, p->fieldT + p->nField);
+        s->fieldB.resize(3 * p->nField);
+        for (int i = 0; i < 3 * p->nField; i++) s->fieldB[i] = (float)p->fieldB[i];
+    }
+    s->p.fieldT = nullptr;
+    s->p.fieldB = nullptr;
+    // device buffers
+    s->m = (float*)dmalloc(s, s->N3 * sizeof(float));
+    GK_CHECK(hipMemcpy(s->m, m0, s->N3 * sizeof(float), hipMemcpyHostToDevice));
+    for (int i = 0; i < s->tb.S; i++) {
+        s->k[i] = (float*)dmalloc(s, s->N3 * sizeof(float));
+        GK_CHECK(hipMemset(s->k[i], 0, s->N3 * sizeof(float)));
+    }
+    s->mi = (float*)dmalloc(s, s->N3 * sizeof(float));
+    s->mtmp = (float*)dmalloc(s, s->N3 * sizeof(float));
+    s->st = (EngineState*)dmalloc(s, sizeof(EngineState));
+    s->log_dt = (double*)dmalloc(s, s->log_cap * sizeof(double));
+    s->log_acc = (int*)dmalloc(s, s->log_cap * sizeof(int));
+    s->dsum = (double*)dmalloc(s, 4 * sizeof(double));
+    EngineState h{};
+    h.dt = (float)(p->dt / s->unitTime);  // init.go:280-284
+    if (h.dt == 0.f) h.dt = 1e-4f;
+    h.new_step = 1;
+    h.stop = 0;
+    h.done = 1;
+    GK_CHECK(hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
+    s->hdt = h.dt;
+    s->field.n = p->nField;
+    if (p->nField == 1)
+        for (int c = 0; c < 3; c++) s->field.b0[c] = s->fieldB[c];
+    if (p->nField >= 2) {
+        s->fT = (double*)dmalloc(s, p->nField * sizeof(double));
+        s->fB = (float*)dmalloc(s, 3 * p->nField * sizeof(float));
+        GK_CHECK(hipMemcpy(s->fT, s->fieldT.data(), p->nField * sizeof(double), hipMemcpyHostToDevice));
+        GK_CHECK(hipMemcpy(s->fB, s->fieldB.data(), 3 * p->nField * sizeof(float), hipMemcpyHostToDevice));
+        s->field.t = s->fT;
+        s->field.b = s->fB;
+    }
+    // convolution (init.go:234-274)
+    conv_plan_init(s->cp, p->size);
+    s->bytes += s->cp.bytes;
+    s->kernelImagRatio = conv_build_kernel(s->cp, s->cell, p->demagAccuracy > 0 ? p->demagAccuracy : 8,
+                                           p->exchType == 6 && p->exchInConv, nullptr, nullptr);
+    if (s->kernelImagRatio > 1e-4)
+        fprintf(stderr, "Warning: FFT Kernel max imaginary part ratio = %g\n", s->kernelImagRatio);
+    s->conv = true;
+    if (mode == MM_MODE_ABI) abi_init(s, s->cell);
+    GK_CHECK(hipDeviceSynchronize());
+    return s;
+}
+
+void mm_sim_free(mmSim* s) {
+    if (!s) return;
+    GK_CHECK(hipStreamSynchronize(s->s));
+    for (int i = 0; i < 7; i++)
+        if (s->k[i]) GK_CHECK(hipFree(s->k[i]));
+    for (float* p : {s->m, s->mi, s->mtmp, s->fB})
+        if (p) GK_CHECK(hipFree(p));
+    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT})
+        if (p) GK_CHECK(hipFree(p));
+    conv_plan_free(s->cp);
+    if (s->plan) delete_gpuFFT3dPlan(s->plan);
+    for (int c = 0; c < 3; c++)
+        if (s->abuf[c]) free_gpu_array(s->abuf[c]);
+    for (int k = 0; k < 6; k++)
+        if (s->akern[k]) free_gpu_array(s->akern[k]);
+    for (float* p : {s->am1, s->abackup, s->redDev})
+        if (p) free_gpu_array(p);
+    delete s;
+}
+
+long mm_sim_run(mmSim* s, double seconds) {
+    if (s->mode == MM_MODE_ABI) {
+        const double stop = s->htime + seconds / s->unitTime;
+        gpu_normalize_uniform(s->m, s->N);
+        long n = 0;
+        while (s->htime < stop) {
+            abi_step(s);
+            n++;
+        }
+        return n;
+    }
+    EngineState h = read_state(s);
+    const double stop = h.time + seconds / s->unitTime;
+    launch_normalize_active(s->m, nullptr, s->N, s->s);  // simrun.go:24
+    return fused_run(s, -1, stop, true);
+}
+
+void mm_sim_steps(mmSim* s, int n) {
+    if (s->mode == MM_MODE_ABI) {
+        for (int i = 0; i < n; i++) abi_step(s);
+        return;
+    }
+    fused_run(s, n, INFINITY, true);
+}
+
+void mm_sim_steps_async(mmSim* s, int n) {
+    if (s->mode == MM_MODE_ABI) {
+        mm_sim_steps(s, n);
+        return;
+    }
+    GK_ASSERT(!s->tb.adaptive, "mm_sim_steps_async needs a fixed-step solver (no rejected trials)");
+    fused_run(s, n, INFINITY, false);
+}
+
+void mm_sim_sync(mmSim* s) { GK_CHECK(hipStreamSynchronize(s->s)); }
+
+void mm_sim_get_m(mmSim* s, float* out) {
+    GK_CHECK(hipMemcpyAsync(out, s->m, s->N3 * sizeof(float), hipMemcpyDeviceToHost, s->s));
+    GK_CHECK(hipStreamSynchronize(s->s));
+}
+
+void mm_sim_set_m(mmSim* s, const float* in) {
+    GK_CHECK(hipMemcpyAsync(s->m, in, s->N3 * sizeof(float), hipMemcpyHostToDevice, s->s));
+    GK_CHECK(hipStreamSynchronize(s->s));
+}
+
+float* mm_sim_device_m(mmSim* s) { return s->m; }
+
+void mm_sim_state(mmSim* s, double* st) {
+    if (s->mode == MM_MODE_ABI) {
+        st[0] = s->htime;
+        st[1] = s->hdt;
+        st[2] = (double)s->hsteps;
+        st[3] = s->herr;
+        st[4] = s->htorque;
+        st[5] = (double)s->htrials;
+        st[6] = (double)s->hrejects;
+        st[7] = 0;
+    } else {
+        EngineState h = read_state(s);
+        st[0] = h.time;
+        st[1] = h.dt;
+        st[2] = (double)h.steps;
+        st[3] = h.err;
+        st[4] = h.max_torque;
+        st[5] = (double)h.trials_total;
+        st[6] = (double)h.rejects_total;
+        st[7] = h.energy;
+    }
+    st[8] = s->unitTime;
+    st[9] = s->unitField;
+    for (int i = 0; i < 3; i++) st[10 + i] = s->mode == MM_MODE_ABI ? s->Pabi[i] : s->cp.P[i];
+}
+
+int mm_sim_trial_log(mmSim* s, double* dt, int* accept, int max) {
+    if (s->mode == MM_MODE_ABI) {
+        int n = (int)std::min<size_t>(max, s->hlog_dt.size());
+        size_t off = s->hlog_dt.size() - n;
+        for (int i = 0; i < n; i++) {
+            dt[i] = s->hlog_dt[off + i];
+            accept[i] = s->hlog_acc[off + i];
+        }
+        return n;
+    }
+    EngineState h = read_state(s);
+    int n = std::min(h.log_n, s->log_cap);
+    n = std::min(n, max);
+    if (n <= 0) return 0;
+    GK_CHECK(hipMemcpy(dt, s->log_dt, n * sizeof(double), hipMemcpyDeviceToHost));
+    GK_CHECK(hipMemcpy(accept, s->log_acc, n * sizeof(int), hipMemcpyDeviceToHost));
+    return n;
+}
+
+static void eval_heff(mmSim* s, const float* m_host, float* h_dev) {
+    GK_CHECK(hipMemcpyAsync(s->mi, m_host, s->N3 * sizeof(float), hipMemcpyHostToDevice, s->s));
+    if (s->mode == MM_MODE_ABI) {
+        GK_CHECK(hipStreamSynchronize(s->s));
+        abi_calc_heff(s, s->mi, h_dev);
+        return;
+    }
+    StageIn si{};
+    si.m = s->mi;
+    conv_apply(s->cp, si, h_dev, s->s);
+    EngineState h = read_state(s);
+    float b[3], hext[3];
+    applied_field_host(s, h.time * s->unitTime, b);
+    for (int c = 0; c < 3; c++) hext[c] = b[c] / (float)s->unitField;
+    float kk = s->kAnis;
+    float ax[3] = {s->u[0], s->u[1], s->u[2]};
+    gpu_add_local_fields(s->mi, h_dev, s->N, hext, nullptr, s->p.anisType, &kk, ax);
+}
+
+void mm_sim_heff(mmSim* s, const float* m, float* h) {
+    eval_heff(s, m, s->mtmp);
+    GK_CHECK(hipMemcpy(h, s->mtmp, s->N3 * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+void mm_sim_torque(mmSim* s, const float* m, float* k) {
+    eval_heff(s, m, s->mtmp);
+    const float alpha = (float)s->p.alpha;
+    gpu_deltaM(s->mi, s->mtmp, alpha, nullptr, (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha)), s->N);
+    GK_CHECK(hipMemcpy(k, s->mtmp, s->N3 * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+double mm_sim_energy(mmSim* s) {
+    StageIn si{};
+    si.m = s->m;
+    conv_apply(s->cp, si, s->mtmp, s->s);
+    double t = s->mode == MM_MODE_ABI ? s->htime : read_state(s).time;
+    float b[3], hext[3];
+    applied_field_host(s, t * s->unitTime, b);
+    for (int c = 0; c < 3; c++) hext[c] = b[c] / (float)s->unitField;
+    GK_CHECK(hipMemsetAsync(s->dsum, 0, sizeof(double), s->s));
+    launch_energy(s->m, s->mtmp, hext, s->p.anisType, s->kAnis, s->u, s->N, s->dsum, s->s);
+    double e = 0;
+    GK_CHECK(hipMemcpyAsync(&e, s->dsum, sizeof(double), hipMemcpyDeviceToHost, s->s));
+    GK_CHECK(hipStreamSynchronize(s->s));
+    return e * s->cell[0] * s->cell[1] * s->cell[2];
+}
+
+void mm_sim_average_m(mmSim* s, double* avg) {
+    GK_CHECK(hipMemsetAsync(s->dsum, 0, 3 * sizeof(double), s->s));
+    launch_sum3(s->m, s->N, s->dsum, s->s);
+    GK_CHECK(hipMemcpyAsync(avg, s->dsum, 3 * sizeof(double), hipMemcpyDeviceToHost, s->s));
+    GK_CHECK(hipStreamSynchronize(s->s));
+    for (int c = 0; c < 3; c++) avg[c] /= (double)s->N;
+}
+
+long mm_sim_kernel_spectrum(mmSim* s, float* out) {
+    const int Q = s->cp.Q;
+    const long fulls = (long)s->cp.P[0] * s->cp.P[1] * Q;
+    const long total = s->cp.nK * fulls;
+    if (!out) return total;
+    float* dev = nullptr;
+    GK_CHECK(hipMalloc(&dev, total * sizeof(float)));
+    ConvPlan tmp = s->cp;
+    conv_build_kernel(tmp, s->cell, s->p.demagAccuracy > 0 ? s->p.demagAccuracy : 8,
+                      s->p.exchType == 6 && s->p.exchInConv, out, dev);
+    GK_CHECK(hipFree(dev));
+    return total;
+}
+
+unsigned long long mm_sim_device_bytes(mmSim* s) { return s->bytes; }
+
+double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts) {
+    GK_ASSERT(s->mode == MM_MODE_FUSED, "profiling needs the fused mode");
+    GK_ASSERT(!s->tb.adaptive, "profiling needs a fixed-step solver");
+    hipEvent_t e0, e1;
+    GK_CHECK(hipEventCreate(&e0));
+    GK_CHECK(hipEventCreate(&e1));
+    s->prof.reset();
+    s->prof.on = true;
+    GK_CHECK(hipEventRecord(e0, s->s));
+    fused_run(s, n, INFINITY, false);
+    GK_CHECK(hipEventRecord(e1, s->s));
+    GK_CHECK(hipEventSynchronize(e1));
+    float total = 0.f;
+    GK_CHECK(hipEventElapsedTime(&total, e0, e1));
+    s->prof.collect(ms, counts);
+    s->prof.on = false;
+    s->prof.reset();
+    GK_CHECK(hipEventDestroy(e0));
+    GK_CHECK(hipEventDestroy(e1));
+    return total;
+}
+
+double mm_sim_time_steps(mmSim* s, int n) {
+    GK_ASSERT(s->mode == MM_MODE_FUSED && !s->tb.adaptive, "timing needs the fused mode and a fixed-step solver");
+    hipEvent_t e0, e1;
+    GK_CHECK(hipEventCreate(&e0));
+    GK_CHECK(hipEventCreate(&e1));
+    GK_CHECK(hipEventRecord(e0, s->s));
+    fused_run(s, n, INFINITY, false);
+    GK_CHECK(hipEventRecord(e1, s->s));
+    GK_CHECK(hipEventSynchronize(e1));
+    float total = 0.f;
+    GK_CHECK(hipEventElapsedTime(&total, e0, e1));
+    GK_CHECK(hipEventDestroy(e0));
+    GK_CHECK(hipEventDestroy(e1));
+    return total;
+}
+
+}  // extern "C"

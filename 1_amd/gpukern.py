@@ -1,0 +1,144 @@
+"""ctypes binding of libgpukern.so (include/gpukern.h + include/mumax_engine.h).
+
+The library is the product; this module only loads it and declares argument
+types.  It raises immediately if the library has not been built: there is no
+fallback implementation of any entry point.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpukern.so")
+
+P = C.c_void_p
+F = C.c_float
+I = C.c_int
+FP = C.POINTER(C.c_float)
+IP = C.POINTER(C.c_int)
+
+# name: (restype, argtypes) for every entry point of include/gpukern.h
+GPUKERN = {
+    "gpu_init": (None, [I, I]),
+    "gpu_set_device": (None, [I]),
+    "gpu_maxthreads": (I, []),
+    "gpu_print_properties_stdout": (None, []),
+    "new_gpu_array": (P, [I]),
+    "free_gpu_array": (None, [P]),
+    "gpu_usedmem": (C.c_ulonglong, []),
+    "gpu_stride_float": (I, []),
+    "gpu_override_stride": (None, [I]),
+    "memcpy_gpu_dir": (None, [P, P, I, I]),
+    "memcpy_to_gpu": (None, [P, P, I]),
+    "memcpy_from_gpu": (None, [P, P, I]),
+    "memcpy_on_gpu": (None, [P, P, I]),
+    "gpu_array_get": (F, [P, I]),
+    "gpu_array_set": (None, [P, I, F]),
+    "gpu_zero": (None, [P, I]),
+    "gpu_add": (None, [P, P, I]),
+    "gpu_madd": (None, [P, F, P, I]),
+    "gpu_madd2": (None, [P, P, P, I]),
+    "gpu_add_constant": (None, [P, F, I]),
+    "gpu_linear_combination": (None, [P, P, F, F, I]),
+    "gpu_linear_combination_many": (None, [P, P, P, I, I]),
+    "gpu_scale_dot_product": (None, [P, P, P, F, I]),
+    "gpu_reduce": (F, [I, P, P, P, I, I, I]),
+    "gpu_normalize_uniform": (None, [P, I]),
+    "gpu_normalize_map": (None, [P, P, I]),
+    "gpu_deltaM": (None, [P, P, F, P, F, I]),
+    "gpu_spintorque_deltaM": (None, [P, P, F, F, F, P, P, F, I, I, I]),
+    "gpu_add_local_fields": (None, [P, P, I, P, P, I, P, P]),
+    "gpu_add_local_fields_H_and_phi": (None, [P, P, P, I, P, P, I, P, P]),
+    "gpu_add_lin_anisotropy": (None, [P] * 12 + [I]),
+    "gpu_add_exch": (None, [P, P, P, P, P, P, I]),
+    "gpu_anal_fw_step": (None, [F, F, I, P, P, P]),
+    "gpu_kernelmul6": (None, [P] * 9 + [I]),
+    "gpu_kernelmul4": (None, [P] * 7 + [I]),
+    "gpu_copy_pad": (None, [P, P, I, I, I, I, I, I]),
+    "gpu_copy_unpad": (None, [P, P, I, I, I, I, I, I]),
+    "new_gpuFFT3dPlan_padded": (P, [P, P]),
+    "delete_gpuFFT3dPlan": (None, [P]),
+    "gpuFFT3dPlan_forward": (None, [P, P, P]),
+    "gpuFFT3dPlan_inverse": (None, [P, P, P]),
+    "gpuFFT3dPlan_normalization": (I, [P]),
+    "gpu_gaussian_noise": (None, [P, F, F, I]),
+}
+
+ENGINE = {
+    "mm_sim_new": (P, [P, P, I]),
+    "mm_sim_free": (None, [P]),
+    "mm_sim_run": (C.c_long, [P, C.c_double]),
+    "mm_sim_steps": (None, [P, I]),
+    "mm_sim_steps_async": (None, [P, I]),
+    "mm_sim_sync": (None, [P]),
+    "mm_sim_get_m": (None, [P, P]),
+    "mm_sim_set_m": (None, [P, P]),
+    "mm_sim_device_m": (P, [P]),
+    "mm_sim_state": (None, [P, P]),
+    "mm_sim_trial_log": (I, [P, P, P, I]),
+    "mm_sim_heff": (None, [P, P, P]),
+    "mm_sim_torque": (None, [P, P, P]),
+    "mm_sim_energy": (C.c_double, [P]),
+    "mm_sim_average_m": (None, [P, P]),
+    "mm_sim_kernel_spectrum": (C.c_long, [P, P]),
+    "mm_sim_device_bytes": (C.c_ulonglong, [P]),
+    "mm_sim_time_steps": (C.c_double, [P, I]),
+    "mm_sim_profile_steps": (C.c_double, [P, I, P, P]),
+}
+
+
+class Params(C.Structure):
+    """mmParams (include/mumax_engine.h)."""
+
+    _fields_ = [
+        ("size", C.c_int * 3),
+        ("periodic", C.c_int * 3),
+        ("cellsize", C.c_double * 3),
+        ("msat", C.c_double),
+        ("aexch", C.c_double),
+        ("alpha", C.c_double),
+        ("anisType", C.c_int),
+        ("k1", C.c_double),
+        ("anisAxis", C.c_double * 3),
+        ("exchType", C.c_int),
+        ("exchInConv", C.c_int),
+        ("wantDemag", C.c_int),
+        ("demagAccuracy", C.c_int),
+        ("solver", C.c_int),
+        ("maxError", C.c_double),
+        ("dt", C.c_double),
+        ("minDm", C.c_double),
+        ("maxDm", C.c_double),
+        ("minDt", C.c_double),
+        ("maxDt", C.c_double),
+        ("nField", C.c_int),
+        ("fieldT", C.POINTER(C.c_double)),
+        ("fieldB", C.POINTER(C.c_double)),
+        ("wantEnergy", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libgpukern.so (once) and declare every entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libgpukern.so is not built (%s); run `python 1_amd/build.py` or __graft_entry__.build()" % path)
+    lib = C.CDLL(path)
+    for table in (GPUKERN, ENGINE):
+        for name, (res, args) in table.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols(path=LIB_PATH):
+    """Names from both tables that the shared object exports (no GPU needed)."""
+    lib = C.CDLL(path)
+    return {n for n in list(GPUKERN) + list(ENGINE) if hasattr(lib, n)}

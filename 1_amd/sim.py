@@ -1,0 +1,318 @@
+"""Python mirror of the reference's simulation commands over libgpukern.
+
+``Sim`` keeps the reference's command names and argument order (sim*.go,
+lib/mumax.py): vector arguments are given in the user's (z, y, x) order and
+stored in internal (X, Y, Z) order, exactly as ``GridSize(z, y, x)``,
+``CellSize``, ``ApplyStatic(what, hz, hy, hx)`` and ``AnisUniaxial(uz, uy, ux)``
+do (simgeom.go:25-55, simfield.go:87-153, simanis.go:32-45).  Magnetisation
+arrays are in the internal planar layout ``[3][X][Y][Z]``.
+
+The commands build an ``mmParams`` and the first ``run``/``steps`` creates the
+device-resident engine (``mm_sim_new``), like ``Sim.init()`` (init.go:36-77).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import gpukern
+
+MODES = {"fused": 0, "abi": 1}
+SOLVERS = {"rk2": 2, "rk12": 12, "rk3": 3, "rk23": 23, "rk4": 4, "rkck": 45, "rkdp": 54, "rk45": 54}
+
+
+class InputErr(ValueError):
+    """Invalid command argument (the reference panics with InputErr)."""
+
+
+class Sim:
+    def __init__(self, mode="fused"):
+        if mode not in MODES:
+            raise InputErr("mode must be one of %s" % list(MODES))
+        self.mode = mode
+        self._size = None
+        self._cell = None
+        self._msat = 0.0
+        self._aexch = 0.0
+        self._alpha = 0.0
+        self._k1 = 0.0
+        self._axis = None
+        self._field = []  # list of (t, [bX, bY, bZ]); static: single entry with t=None
+        self._static = None
+        self._solver = "rk12"       # DEFAULT_SOLVERTYPE (sim.go:24)
+        self._maxerror = 1e-4       # DEFAULT_MAXERROR (sim.go:25)
+        self._dt = 0.0
+        self._mindm = self._maxdm = self._mindt = self._maxdt = 0.0
+        self._exchinconv = True
+        self._demag_accuracy = 8
+        self._m0 = None
+        self._ptr = None
+        self._lib = gpukern.load()
+        self._keep = []
+
+    # ------------------------------------------------------------ material (simmaterial)
+    def msat(self, v):
+        self._invalidate()
+        self._msat = float(v)
+
+    def aexch(self, v):
+        self._invalidate()
+        self._aexch = float(v)
+
+    def alpha(self, v):
+        self._invalidate()
+        self._alpha = float(v)
+
+    def k1(self, v):
+        self._invalidate()
+        self._k1 = float(v)
+
+    def anisuniaxial(self, uz, uy, ux):
+        self._invalidate()
+        n = np.sqrt(ux * ux + uy * uy + uz * uz)
+        if n == 0:
+            raise InputErr("Anisotropy axis should not be 0")
+        self._axis = (ux / n, uy / n, uz / n)
+
+    # ------------------------------------------------------------ geometry (simgeom.go)
+    def gridsize(self, z, y, x):
+        self._invalidate()
+        if x <= 0 or y <= 0 or z <= 0:
+            raise InputErr("Size should be > 0")
+        if z == 1:
+            raise InputErr("For a 2D geometry, use (X, Y, 1) cells, not (1, X, Y)")
+        self._size = (int(x), int(y), int(z))
+
+    def cellsize(self, z, y, x):
+        self._invalidate()
+        if x <= 0 or y <= 0 or z <= 0:
+            raise InputErr("Cell size should be > 0")
+        self._cell = (float(x), float(y), float(z))
+
+    def partsize(self, z, y, x):
+        if self._size is None:
+            raise InputErr("gridsize must be set before partsize")
+        X, Y, Z = self._size
+        self.cellsize(z / Z, y / Y, x / X)
+
+    # ------------------------------------------------------------ fields (simfield.go)
+    def applystatic(self, what, hz, hy, hx):
+        self._invalidate()
+        if what != "field":
+            raise InputErr("only 'field' can be applied")
+        self._static = (hx, hy, hz)
+        self._field = []
+
+    def applypointwise(self, what, t, hz, hy, hx):
+        self._invalidate()
+        if what != "field":
+            raise InputErr("only 'field' can be applied")
+        self._static = None
+        self._field.append((float(t), (hx, hy, hz)))
+
+    # ------------------------------------------------------------ solver (simsolver.go)
+    def solvertype(self, s):
+        self._invalidate()
+        s = s.lower()
+        if s not in SOLVERS:
+            raise InputErr("Unknown solver type: %s" % s)
+        self._solver = s
+
+    def maxerror(self, e):
+        self._maxerror = float(e)
+
+    def dt(self, t):
+        self._invalidate()
+        self._dt = float(t)
+
+    def mindm(self, v):
+        self._mindm = float(v)
+
+    def maxdm(self, v):
+        self._maxdm = float(v)
+
+    def mindt(self, v):
+        self._mindt = float(v)
+
+    def maxdt(self, v):
+        self._maxdt = float(v)
+
+    def exchinconv(self, flag):
+        self._invalidate()
+        self._exchinconv = bool(flag)
+
+    # ------------------------------------------------------------ magnetisation (simconfig.go)
+    def uniform(self, mz, my, mx):
+        self._invalidate()
+        X, Y, Z = self._need_size()
+        m = np.empty((3, X, Y, Z), np.float32)
+        m[0], m[1], m[2] = mx, my, mz
+        self._m0 = m
+
+    def setm_array(self, m):
+        """Initial magnetisation in the internal layout (3, X, Y, Z)."""
+        self._invalidate()
+        m = np.ascontiguousarray(m, np.float32).reshape((3,) + self._need_size())
+        self._m0 = m.copy()
+
+    # ------------------------------------------------------------ run (simrun.go)
+    def run(self, seconds):
+        self._init()
+        return int(self._lib.mm_sim_run(self._ptr, float(seconds)))
+
+    def steps(self, n):
+        self._init()
+        self._lib.mm_sim_steps(self._ptr, int(n))
+
+    def step(self):
+        self.steps(1)
+
+    def steps_async(self, n):
+        self._init()
+        self._lib.mm_sim_steps_async(self._ptr, int(n))
+
+    def sync(self):
+        self._lib.mm_sim_sync(self._ptr)
+
+    # ------------------------------------------------------------ outputs
+    @property
+    def m(self):
+        self._init()
+        out = np.empty((3,) + self._size, np.float32)
+        self._lib.mm_sim_get_m(self._ptr, out.ctypes.data)
+        return out
+
+    @m.setter
+    def m(self, v):
+        self._init()
+        v = np.ascontiguousarray(v, np.float32).reshape((3,) + self._size)
+        self._lib.mm_sim_set_m(self._ptr, v.ctypes.data)
+
+    def state(self):
+        self._init()
+        st = np.zeros(13)
+        self._lib.mm_sim_state(self._ptr, st.ctypes.data)
+        keys = ("time", "dt", "steps", "error", "max_torque", "trials", "rejects", "energy", "unit_time",
+                "unit_field", "px", "py", "pz")
+        return dict(zip(keys, st))
+
+    def trial_log(self, max_entries=1 << 16):
+        self._init()
+        dt = np.zeros(max_entries)
+        acc = np.zeros(max_entries, np.int32)
+        n = self._lib.mm_sim_trial_log(self._ptr, dt.ctypes.data, acc.ctypes.data, max_entries)
+        return dt[:n], acc[:n]
+
+    def heff(self, m):
+        self._init()
+        m = np.ascontiguousarray(m, np.float32)
+        h = np.empty_like(m)
+        self._lib.mm_sim_heff(self._ptr, m.ctypes.data, h.ctypes.data)
+        return h
+
+    def torque(self, m):
+        self._init()
+        m = np.ascontiguousarray(m, np.float32)
+        k = np.empty_like(m)
+        self._lib.mm_sim_torque(self._ptr, m.ctypes.data, k.ctypes.data)
+        return k
+
+    def energy(self):
+        """Total energy in internal units x cell volume (energy.go:38-52)."""
+        self._init()
+        return float(self._lib.mm_sim_energy(self._ptr))
+
+    def energy_si(self):
+        st = self.state()
+        L = np.sqrt(2 * self._aexch / (4e-7 * np.pi * self._msat ** 2))
+        return self.energy() * self._aexch * L
+
+    def average_m(self):
+        self._init()
+        out = np.zeros(3)
+        self._lib.mm_sim_average_m(self._ptr, out.ctypes.data)
+        return out
+
+    def kernel_spectrum(self):
+        self._init()
+        n = self._lib.mm_sim_kernel_spectrum(self._ptr, None)
+        out = np.empty(n, np.float32)
+        self._lib.mm_sim_kernel_spectrum(self._ptr, out.ctypes.data)
+        st = self.state()
+        P = (int(st["px"]), int(st["py"]), int(st["pz"]))
+        return out.reshape(-1, P[0], P[1], P[2] // 2 + 1)
+
+    def device_bytes(self):
+        self._init()
+        return int(self._lib.mm_sim_device_bytes(self._ptr))
+
+    def free(self):
+        if self._ptr:
+            self._lib.mm_sim_free(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ internals
+    def _need_size(self):
+        if self._size is None:
+            raise InputErr("The gridsize should be set first. E.g.: gridsize 32 32 4")
+        return self._size
+
+    def _invalidate(self):
+        if self._ptr:
+            # init.go:20-26: re-initialisation is not allowed
+            raise InputErr("This parameter could only be set once.")
+
+    def params(self):
+        p = gpukern.Params()
+        if self._msat == 0:
+            raise InputErr("Saturation magnetization should first be set. E.g. msat 800E3")
+        if self._aexch == 0:
+            raise InputErr("Exchange constant should first be set. E.g. aexch 12E-13")
+        if self._cell is None:
+            raise InputErr("The cell size should be set first. E.g. cellsize 1E-9 1E-9 1E-9")
+        p.size[:] = list(self._need_size())
+        p.cellsize[:] = list(self._cell)
+        p.msat, p.aexch, p.alpha = self._msat, self._aexch, self._alpha
+        if self._axis is not None:
+            p.anisType = 1
+            p.anisAxis[:] = list(self._axis)
+        p.k1 = self._k1
+        p.exchType = 6
+        p.exchInConv = int(self._exchinconv)
+        p.wantDemag = 1
+        p.demagAccuracy = self._demag_accuracy
+        p.solver = SOLVERS[self._solver]
+        p.maxError, p.dt = self._maxerror, self._dt
+        p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt
+        if self._static is not None:
+            t = np.zeros(1)
+            b = np.asarray(self._static, np.float64)
+        elif self._field:
+            t = np.asarray([f[0] for f in self._field], np.float64)
+            b = np.asarray([f[1] for f in self._field], np.float64).reshape(-1)
+        else:
+            t = b = None
+        if t is not None:
+            t = np.ascontiguousarray(t)
+            b = np.ascontiguousarray(b)
+            self._keep = [t, b]
+            p.nField = len(t)
+            p.fieldT = t.ctypes.data_as(C.POINTER(C.c_double))
+            p.fieldB = b.ctypes.data_as(C.POINTER(C.c_double))
+        return p
+
+    def _init(self):
+        if self._ptr:
+            return
+        if self._m0 is None:
+            raise InputErr("Initial magnetization was not set")
+        p = self.params()
+        m0 = np.ascontiguousarray(self._m0, np.float32)
+        self._ptr = self._lib.mm_sim_new(C.byref(p), m0.ctypes.data, MODES[self.mode])
+        if not self._ptr:
+            raise RuntimeError("mm_sim_new failed")

@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Single-GPU benchmark of the LLG step on MI355X.
+
+Workload: cuboid of 256 x 256 x 16 cells (4 nm), Ms 1e6 A/m, A 1.5e-11 J/m,
+Ku1 5e5 J/m^3 along the thickness, alpha 0.1, seeded random m0, zero field,
+fixed-dt (1e-14 s) RK4, demag+exchange convolution padded to 512 x 512 x 32.
+One "step" is one RK4 step (4 field evaluations) of the whole grid, with the
+state resident in HBM before the timed region starts.
+
+Prints ONE JSON line.  `value` is Mcell-steps/s (cells x steps / s) of the
+timed region, measured with HIP events on the engine's own stream; the
+`roofline` object reports the dominant kernel's algorithmic bytes per launch
+over its average launch time (events around each launch in a separate
+profiled pass); `cpu_baseline` is the FP64 CPU oracle timed on this host on a
+bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+CLASSES = ["z_fwd_r2c", "y_fwd", "x_fwd_kmul_x_inv", "y_inv", "z_inv_c2r", "torque", "rk_final", "control"]
+
+
+def model_bytes(N, P, Nx, Ny, nK):
+    """Algorithmic bytes of one field evaluation, per kernel class, and of one
+    RK4 step (B_step = 4 B_conv + 12 N 27)."""
+    Q = P[2] // 2 + 1
+    A1 = Nx * Ny * Q
+    A2 = Nx * P[1] * Q
+    A3 = P[0] * P[1] * Q
+    per = {
+        "z_fwd_r2c": 3 * (4 * N + 8 * A1),
+        "y_fwd": 3 * (8 * A1 + 8 * A2),
+        "x_fwd_kmul_x_inv": 3 * 16 * A2 + 4 * nK * A3,
+        "y_inv": 3 * (8 * A2 + 8 * A1),
+        "z_inv_c2r": 3 * (8 * A1 + 4 * N),
+        "torque": 12 * N * 3,
+    }
+    b_conv = 3 * (8 * N + 32 * A1 + 32 * A2) + 4 * nK * A3
+    b_step = 4 * b_conv + 12 * N * 27
+    return per, b_conv, b_step
+
+
+def cpu_baseline(cfg, seconds):
+    """FP64 oracle (C, OpenMP) on the same workload: time whole RK4 steps until
+    about `seconds` of CPU work have run (at least one step)."""
+    from oracle import Oracle
+
+    orc = Oracle("f64")
+    kw = importlib.import_module("1_amd").workloads.oracle_kwargs(cfg)
+    t0 = time.perf_counter()
+    s = orc.sim(cfg["m0"], **kw)
+    setup = time.perf_counter() - t0
+    n = 0
+    t1 = time.perf_counter()
+    while True:
+        s.step()
+        n += 1
+        el = time.perf_counter() - t1
+        if el >= seconds or (el / n) * (n + 1) > 2.0 * seconds:
+            break
+    N = int(np.prod(cfg["size"]))
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": N * n / el / 1e6,
+        "unit": "Mcell-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "%d RK4 step(s) of the same 256x256x16 workload in %.1f s (FP64 oracle, OpenMP, kernel setup "
+                  "%.1f s excluded); cpu: %s" % (n, el, setup, model),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs each simulation on one MI355X; runs on %d GPUs are not part of this build "
+              "(use --gpus 1)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    mm = importlib.import_module("1_amd")
+    lib = mm.load()
+    cfg = mm.workloads.cuboid_rk4()
+    N = int(np.prod(cfg["size"]))
+    sim = mm.workloads.make_sim(cfg)
+    sim.steps(args.warmup)  # also builds the kernel and allocates everything
+    st = sim.state()
+    P = (int(st["px"]), int(st["py"]), int(st["pz"]))
+
+    ms = lib.mm_sim_time_steps(sim._ptr, args.steps)
+    ms_per_step = ms / args.steps
+    value = N * args.steps / (ms / 1e3) / 1e6
+
+    prof_ms = np.zeros(8)
+    prof_n = np.zeros(8, np.int32)
+    lib.mm_sim_profile_steps(sim._ptr, args.profile_steps, prof_ms.ctypes.data, prof_n.ctypes.data)
+    avg = {CLASSES[i]: (prof_ms[i] / prof_n[i] if prof_n[i] else 0.0) for i in range(8)}
+    per, b_conv, b_step = model_bytes(N, P, cfg["size"][0], cfg["size"][1], 6 if P[0] > 1 else 4)
+    dominant = max(per, key=lambda k: prof_ms[CLASSES.index(k)])
+    ach = per[dominant] / (avg[dominant] / 1e3) / 1e9
+    step_gbs = b_step / (ms_per_step / 1e3) / 1e9
+
+    m = sim.m
+    norm_err = float(np.abs(np.linalg.norm(m, axis=0) - 1).max())
+
+    out = {
+        "metric": "Mcell-steps/s",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded (SplitMix64 seed 2) random m0 uniform on the sphere",
+        "config": {
+            "workload": cfg["name"],
+            "cells": N,
+            "grid_internal_xyz": list(cfg["size"]),
+            "padded": list(P),
+            "solver": "rk4",
+            "dt_s": cfg["dt"],
+            "kernel": "FaceKernel6 (8x8 charges/face) + 6-neighbour exchange, FP64 setup",
+        },
+        "hbm_pct_peak_model": round(100 * step_gbs / HBM_PEAK_GBS, 2),
+        "step_model_bytes": b_step,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": per[dominant],
+            "avg_launch_ms": round(avg[dominant], 5),
+        },
+        "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+        "norm_max_dev": norm_err,
+    }
+    sim.free()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,115 @@
+/*
+ * mumax_engine.h -- additions to libgpukern.so: the device-resident LLG engine.
+ *
+ * None of these entry points exists in the reference.  They are the host
+ * side of the reference's solver (the Go packages sim/rk.go, sim/conv.go,
+ * sim/field.go, sim/torque.go, sim/simrun.go) restated in C++ on top of the
+ * kernels, so a caller gets a whole time step per call instead of ~150
+ * synchronous kernel calls.  Two step implementations are selectable:
+ *
+ *   MM_MODE_FUSED  MI355X-native: zero-skipping transposed FFT convolution
+ *                  with the k-space multiply fused into the outermost axis,
+ *                  Runge-Kutta stage build fused into the forward Z pass,
+ *                  local fields + torque + step-control reductions fused into
+ *                  one pass, dt control on the device (no host round trip
+ *                  per step).
+ *   MM_MODE_ABI    the reference's call sequence (rk.go:279-445 ->
+ *                  conv.go:57-100 -> field.go:74-80 -> torque.go:33-53) issued
+ *                  through the drop-in gpukern.h functions, one synchronous
+ *                  call at a time, as gpu.go would.
+ *
+ * Arrays crossing this boundary are host float/double arrays in the
+ * reference's internal layout: planar [3][X][Y][Z], Z fastest, X = thickness.
+ * Errors print to stderr and abort(), like the rest of libgpukern.
+ */
+#ifndef MUMAX_ENGINE_H
+#define MUMAX_ENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MM_MODE_FUSED = 0, MM_MODE_ABI = 1 };
+enum { MM_ANIS_NONE = 0, MM_ANIS_UNIAXIAL = 1 };
+/* solver codes: rk.go tableaux (solver.go:16-53) */
+enum { MM_RK2 = 2, MM_RK12 = 12, MM_RK3 = 3, MM_RK23 = 23, MM_RK4 = 4, MM_RKCK = 45, MM_RKDP = 54 };
+
+/* Simulation parameters: the reference's input commands (sim.go:50-170),
+ * already in internal axis order. */
+typedef struct {
+    int size[3];            /* cells, internal X,Y,Z (gridsize z y x) */
+    int periodic[3];        /* must be 0 in this build */
+    double cellsize[3];     /* metres, internal order (cellsize z y x) */
+    double msat, aexch, alpha;
+    int anisType;           /* MM_ANIS_* */
+    double k1;              /* J/m^3 */
+    double anisAxis[3];     /* internal order, unit vector */
+    int exchType;           /* 0 or 6 */
+    int exchInConv;         /* 1: exchange folded into the demag kernel (sim.go:203) */
+    int wantDemag;
+    int demagAccuracy;      /* face-charge points per face edge (8 in the reference) */
+    int solver;             /* MM_RK* */
+    double maxError, dt;    /* dt in seconds, 0 = default 1e-4 internal units */
+    double minDm, maxDm, minDt, maxDt;
+    int nField;             /* 0 none, 1 static, >=2 pointwise linear (simfield.go:87-153) */
+    const double* fieldT;   /* nField times (s) */
+    const double* fieldB;   /* nField*3 tesla, internal order */
+    int wantEnergy;         /* energy at the FSAL stage (rk.go:323-337) */
+} mmParams;
+
+typedef struct mmSim mmSim;
+
+/* m0: host 3*N floats, any norm (normalised by the first run/steps call) */
+mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode);
+void mm_sim_free(mmSim* s);
+
+/* Run(duration): steps until time >= stop (simrun.go:19-57); returns accepted steps */
+long mm_sim_run(mmSim* s, double seconds);
+/* Steps(n) (simrun.go:88-92) */
+void mm_sim_steps(mmSim* s, int n);
+/* Enqueue n steps without waiting (fused mode; for timing); mm_sim_sync waits. */
+void mm_sim_steps_async(mmSim* s, int n);
+void mm_sim_sync(mmSim* s);
+
+void mm_sim_get_m(mmSim* s, float* m);          /* 3N host floats */
+void mm_sim_set_m(mmSim* s, const float* m);
+float* mm_sim_device_m(mmSim* s);               /* device pointer to planar m (3N) */
+
+/* st[0]=time (internal) [1]=dt (internal) [2]=steps [3]=last error [4]=max torque
+ * [5]=trials [6]=rejects [7]=energy (internal) [8]=unit time (s) [9]=unit field (T)
+ * [10..12] padded size used by the convolution */
+void mm_sim_state(mmSim* s, double* st);
+
+/* per-trial schedule (dt internal, accepted flag) of the last `max` trials since
+ * creation, oldest first; returns the number written */
+int mm_sim_trial_log(mmSim* s, double* dt, int* accept, int max);
+
+/* one evaluation of the effective field / torque of m (host 3N floats) */
+void mm_sim_heff(mmSim* s, const float* m, float* h);
+void mm_sim_torque(mmSim* s, const float* m, float* k);
+
+/* energy of the current m, internal units x cell volume (energy.go:38-52), FP64 reductions */
+double mm_sim_energy(mmSim* s);
+/* <m> per component, FP64 reduction (simoutput.go:234-238), internal order */
+void mm_sim_average_m(mmSim* s, double* avg);
+
+/* Demag(+exchange) kernel spectrum used by the engine, as the reference stores it:
+ * nK x [Px][Py][Pz/2+1] floats, Re(FFT(K))/(Px Py Pz), nK = 6 (3-D) or 4 (2-D,
+ * xx yy zz yz).  out may be NULL to query; returns the number of floats. */
+long mm_sim_kernel_spectrum(mmSim* s, float* out);
+
+/* bytes of device memory held by the simulation */
+unsigned long long mm_sim_device_bytes(mmSim* s);
+
+/* Fixed-step fused mode only.  Enqueue n steps bracketed by HIP events on the
+ * engine's stream; returns the elapsed ms.  The _profile variant also brackets
+ * every kernel class (0 Z fwd, 1 Y fwd, 2 X.K.X or Y.K.Y, 3 Y inv, 4 Z inv,
+ * 5 torque, 6 RK final, 7 control) and returns per-class total ms and launch
+ * counts in ms[8], counts[8]. */
+double mm_sim_time_steps(mmSim* s, int n);
+double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,977 @@
+/*
+ * oracle.c -- CPU restatement of the MuMax 1.1 (barnex/1) LLG hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libgpukern.so.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library.
+ *
+ * Each function cites the reference file:line it restates (paths relative to
+ * the reference root).  How each part is pinned (tests/test_oracle_*.py):
+ *   or_units             closed form: lex(Py)=5.686 nm, t_u=5.654 ps (SURVEY a14 values)
+ *   or_facekernel6       self term: trace -> -1, cube -> -1/3 each; thin film Nxx -> -1;
+ *                        far field -> point dipole; symmetry K(-r)=+-K(r); float32 twin
+ *                        bit-compared with or_facekernel6_f32 (the literal transcription)
+ *   or_add_exch6_kernel  spin-wave eigenvalue -(2/h^2)sum(1-cos kh); conv exchange == stencil (3-D)
+ *   or_fft_*             brute-force DFT on small n; round trip; Parseval; impulse -> constant;
+ *                        numpy.fft (pocketfft) on random data
+ *   or_conv_*            brute-force direct convolution; unit kernel -> identity; zero kernel -> 0;
+ *                        padded transform == manually padded transform (fft_test.go:79-165)
+ *   or_local_fields      closed forms (Zeeman constant, uniaxial k(m.u)u)
+ *   or_deltaM            m.k = 0; |k| = |m x h| sqrt(1+a^2)/(1+a^2); Larmor precession
+ *   or_reduce            redux_test.go:13-102 known answers
+ *   or_sim (RK)          tableau order on dm/dt=linear; analytic macrospin precession+damping;
+ *                        forced reject path with a tiny maxError; SP4 <m>(t) vs solution3.txt (loose)
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define X 0
+#define Y 1
+#define Z 2
+
+int or_real_size(void) { return (int)sizeof(or_real); }
+
+static void* xcalloc(size_t n, size_t sz) {
+    void* p = calloc(n ? n : 1, sz);
+    if (!p) { fprintf(stderr, "oracle: out of memory (%zu x %zu)\n", n, sz); abort(); }
+    return p;
+}
+
+/* ====================================================================== */
+/* units: material.go:36-110, init.go:80-113                               */
+/* ====================================================================== */
+static const double MU0 = 4.0e-7 * 3.14159265358979323846;
+static const double GAMMA0 = 2.211e5;
+
+void or_units(double msat, double aexch, double out[5]) {
+    double L = sqrt(2.0 * aexch / (MU0 * msat * msat)); /* UnitLength, material.go:253-256 */
+    double T = 1.0 / (GAMMA0 * msat);                   /* UnitTime */
+    double B = MU0 * msat;                              /* UnitField */
+    double E = aexch * L;                               /* UnitEnergy */
+    double Ed = E / (L * L * L);                        /* UnitEnergyDensity */
+    out[0] = L; out[1] = T; out[2] = B; out[3] = E; out[4] = Ed;
+}
+
+/* ====================================================================== */
+/* kernels: kernel.go:62-98, demag.go:26-84,142-183, exchange.go:44-69     */
+/* ====================================================================== */
+static int wrapi(int number, int max) { /* kernel.go:62-72 */
+    while (number < 0) number += max;
+    while (number >= max) number -= max;
+    return number;
+}
+
+void or_pad_size(const int size[3], const int periodic[3], int padded[3]) { /* kernel.go:79-90 */
+    for (int i = 0; i < 3; i++)
+        padded[i] = (size[i] > 1 && periodic[i] == 0) ? 2 * size[i] : size[i];
+}
+
+static const int KERN_IDX[3][3] = {{OR_XX, OR_XY, OR_XZ}, {OR_XY, OR_YY, OR_YZ}, {OR_XZ, OR_YZ, OR_ZZ}};
+
+/* faceIntegral (demag.go:142-183) in or_real: field at R of a cell uniformly
+ * magnetised along s, from n x n point charges on each of the two faces normal to s. */
+static void face_integral(or_real B[3], const or_real R[3], const or_real cell[3], int s, int n) {
+    int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
+    or_real charge = cell[v] * cell[w];
+    or_real pu1 = cell[u] / 2, pu2 = -pu1;
+    const or_real fourpi = (or_real)(4.0 * 3.14159265358979323846);
+    B[0] = B[1] = B[2] = 0;
+    for (int i = 0; i < n; i++) {
+        or_real pv = -(cell[v] / 2) + cell[v] / (or_real)(2 * n) + (or_real)i * (cell[v] / (or_real)n);
+        for (int j = 0; j < n; j++) {
+            or_real pw = -(cell[w] / 2) + cell[w] / (or_real)(2 * n) + (or_real)j * (cell[w] / (or_real)n);
+            for (int pole = 0; pole < 2; pole++) {
+                or_real pos[3];
+                pos[u] = pole == 0 ? pu1 : pu2;
+                pos[v] = pv;
+                pos[w] = pw;
+                or_real d[3] = {R[0] - pos[0], R[1] - pos[1], R[2] - pos[2]};
+                or_real r = (or_real)sqrt((double)(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+                or_real q = (pole == 0 ? charge : -charge) / (fourpi * r * r);
+                /* normalised direction times q */
+                B[0] += d[0] / r * q;
+                B[1] += d[1] / r * q;
+                B[2] += d[2] / r * q;
+            }
+        }
+    }
+    or_real inv = (or_real)1 / (or_real)(n * n);
+    B[0] *= inv; B[1] *= inv; B[2] *= inv;
+}
+
+void or_facekernel6(const int P[3], const or_real cell[3], int accuracy, const int periodic[3], or_real* K) {
+    size_t NP = (size_t)P[0] * P[1] * P[2];
+    memset(K, 0, 6 * NP * sizeof(or_real));
+    int x1 = -(P[X] - 1) / 2, x2 = P[X] / 2 - 1;
+    if (P[X] == 1 && periodic[X] == 0) x2 = 0; /* demag.go:37-40 */
+    int y1 = -(P[Y] - 1) / 2, y2 = P[Y] / 2 - 1;
+    int z1 = -(P[Z] - 1) / 2, z2 = P[Z] / 2 - 1;
+    x1 *= periodic[X] + 1; x2 *= periodic[X] + 1;
+    y1 *= periodic[Y] + 1; y2 *= periodic[Y] + 1;
+    z1 *= periodic[Z] + 1; z2 *= periodic[Z] + 1;
+    for (int s = 0; s < 3; s++) {
+        /* periodic images may add to the same wrapped cell: keep the loop serial over x */
+        for (int x = x1; x <= x2; x++) {
+            int xw = wrapi(x, P[X]);
+#pragma omp parallel for schedule(static)
+            for (int y = y1; y <= y2; y++) {
+                int yw = wrapi(y, P[Y]);
+                for (int z = z1; z <= z2; z++) {
+                    int zw = wrapi(z, P[Z]);
+                    or_real R[3] = {(or_real)x * cell[X], (or_real)y * cell[Y], (or_real)z * cell[Z]};
+                    or_real B[3];
+                    face_integral(B, R, cell, s, accuracy);
+                    size_t idx = ((size_t)xw * P[Y] + yw) * P[Z] + zw;
+                    for (int d = s; d < 3; d++) K[KERN_IDX[s][d] * NP + idx] += B[d];
+                }
+            }
+        }
+    }
+}
+
+/* literal float32 transcription of faceIntegral for bit-level checks:
+ * tensor.Vector is [3]float32, Norm() is float32(math.Sqrt(float64(...))),
+ * Normalize() multiplies by 1/norm, Scale multiplies (demag.go:142-183, vector.go:62-71). */
+static void face_integral_f32(float B[3], const float R[3], const float cell[3], int s, int n) {
+    int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
+    float surface = cell[v] * cell[w];
+    float charge = surface;
+    float pu1 = cell[u] / 2.f, pu2 = -pu1;
+    const float fourpi = (float)(4.0 * 3.14159265358979323846);
+    B[0] = B[1] = B[2] = 0.f;
+    for (int i = 0; i < n; i++) {
+        volatile float pv = -(cell[v] / 2.f) + cell[v] / (float)(2 * n) + (float)i * (cell[v] / (float)n);
+        for (int j = 0; j < n; j++) {
+            volatile float pw = -(cell[w] / 2.f) + cell[w] / (float)(2 * n) + (float)j * (cell[w] / (float)n);
+            for (int pole = 0; pole < 2; pole++) {
+                float pos[3];
+                pos[u] = pole == 0 ? pu1 : pu2;
+                pos[v] = pv;
+                pos[w] = pw;
+                volatile float R2[3] = {R[0] - pos[0], R[1] - pos[1], R[2] - pos[2]};
+                volatile float ss = R2[0] * R2[0];
+                ss = ss + R2[1] * R2[1];
+                ss = ss + R2[2] * R2[2];
+                float r = (float)sqrt((double)ss);
+                volatile float inv = 1.f / r;
+                for (int c = 0; c < 3; c++) R2[c] = R2[c] * inv;
+                volatile float den = fourpi * r;
+                den = den * r;
+                volatile float sc = (pole == 0 ? charge : -charge) / den;
+                for (int c = 0; c < 3; c++) R2[c] = R2[c] * sc;
+                for (int c = 0; c < 3; c++) B[c] = B[c] + R2[c];
+            }
+        }
+    }
+    float inv = 1.f / (float)(n * n);
+    for (int c = 0; c < 3; c++) B[c] = B[c] * inv;
+}
+
+void or_facekernel6_f32(const int P[3], const float cell[3], int accuracy, const int periodic[3], float* K) {
+    size_t NP = (size_t)P[0] * P[1] * P[2];
+    memset(K, 0, 6 * NP * sizeof(float));
+    int x1 = -(P[X] - 1) / 2, x2 = P[X] / 2 - 1;
+    if (P[X] == 1 && periodic[X] == 0) x2 = 0;
+    int y1 = -(P[Y] - 1) / 2, y2 = P[Y] / 2 - 1;
+    int z1 = -(P[Z] - 1) / 2, z2 = P[Z] / 2 - 1;
+    x1 *= periodic[X] + 1; x2 *= periodic[X] + 1;
+    y1 *= periodic[Y] + 1; y2 *= periodic[Y] + 1;
+    z1 *= periodic[Z] + 1; z2 *= periodic[Z] + 1;
+    for (int s = 0; s < 3; s++)
+        for (int x = x1; x <= x2; x++) {
+            int xw = wrapi(x, P[X]);
+#pragma omp parallel for schedule(static)
+            for (int y = y1; y <= y2; y++) {
+                int yw = wrapi(y, P[Y]);
+                for (int z = z1; z <= z2; z++) {
+                    int zw = wrapi(z, P[Z]);
+                    float R[3] = {(float)x * cell[X], (float)y * cell[Y], (float)z * cell[Z]};
+                    float B[3];
+                    face_integral_f32(B, R, cell, s, accuracy);
+                    size_t idx = ((size_t)xw * P[Y] + yw) * P[Z] + zw;
+                    for (int d = s; d < 3; d++) K[KERN_IDX[s][d] * NP + idx] += B[d];
+                }
+            }
+        }
+}
+
+void or_add_exch6_kernel(const int P[3], const or_real cell[3], or_real* K) {
+    /* exchange.go:44-69 builds a fresh 6NGBR kernel and init.go:256-267 adds it
+     * element-wise.  Writes happen in the reference's order, so a padded size of 1
+     * maps wrap(+-1,1)=0 onto the centre and the last write wins. */
+    size_t NP = (size_t)P[0] * P[1] * P[2];
+    or_real* E = (or_real*)xcalloc(NP, sizeof(or_real));
+    or_real hx = cell[X] * cell[X], hy = cell[Y] * cell[Y], hz = cell[Z] * cell[Z];
+#define EI(a, b, c) (((size_t)wrapi(a, P[X]) * P[Y] + wrapi(b, P[Y])) * P[Z] + wrapi(c, P[Z]))
+    E[EI(0, 0, 0)] = -2 / hx - 2 / hy - 2 / hz;
+    E[EI(1, 0, 0)] = 1 / hx;
+    E[EI(-1, 0, 0)] = 1 / hx;
+    E[EI(0, 1, 0)] = 1 / hy;
+    E[EI(0, -1, 0)] = 1 / hy;
+    E[EI(0, 0, 1)] = 1 / hz;
+    E[EI(0, 0, -1)] = 1 / hz;
+#undef EI
+    for (int s = 0; s < 3; s++) {
+        or_real* k = K + KERN_IDX[s][s] * NP;
+        for (size_t i = 0; i < NP; i++) k[i] += E[i];
+    }
+    free(E);
+}
+
+/* ====================================================================== */
+/* FFT: mixed-radix recursive decimation in time, double twiddles          */
+/* ====================================================================== */
+typedef struct { or_real re, im; } cpx;
+
+typedef struct {
+    int n, nf;
+    int f[64];
+    cpx* tw; /* tw[k] = exp(-2 pi i k / n) */
+} fftplan;
+
+#define MAXPLANS 64
+static fftplan g_plans[MAXPLANS];
+static int g_nplans = 0;
+
+static const fftplan* get_plan(int n) {
+    const fftplan* found = NULL;
+#pragma omp critical(or_fft_plan)
+    {
+        for (int i = 0; i < g_nplans; i++)
+            if (g_plans[i].n == n) { found = &g_plans[i]; break; }
+        if (!found) {
+            if (g_nplans >= MAXPLANS) { fprintf(stderr, "oracle: too many FFT plans\n"); abort(); }
+            fftplan* p = &g_plans[g_nplans];
+            p->n = n;
+            p->nf = 0;
+            int m = n;
+            while (m % 4 == 0 && m > 4) { p->f[p->nf++] = 4; m /= 4; }
+            while (m % 2 == 0) { p->f[p->nf++] = 2; m /= 2; }
+            for (int q = 3; m > 1; q += 2)
+                while (m % q == 0) { p->f[p->nf++] = q; m /= q; }
+            p->tw = (cpx*)xcalloc((size_t)n, sizeof(cpx));
+            for (int k = 0; k < n; k++) {
+                /* exact reduction of the angle: k/n in [0,1) */
+                double a = -2.0 * 3.14159265358979323846 * (double)k / (double)n;
+                p->tw[k].re = (or_real)cos(a);
+                p->tw[k].im = (or_real)sin(a);
+            }
+            g_nplans++;
+            found = p;
+        }
+    }
+    return found;
+}
+
+static inline cpx cmul(cpx a, cpx b) {
+    cpx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+
+/* out[0..n) = DFT of in[0], in[is], ...; N = plan length, tws = N/n */
+static void fft_rec(const cpx* in, cpx* out, int n, size_t is, const int* f, const cpx* tw, int N, int tws,
+                    int sign) {
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = f[0], m = n / p;
+    for (int q = 0; q < p; q++) fft_rec(in + q * is, out + (size_t)q * m, m, is * p, f + 1, tw, N, tws * p, sign);
+    if (p == 2) {
+        for (int k = 0; k < m; k++) {
+            cpx w = tw[(size_t)k * tws];
+            if (sign > 0) w.im = -w.im;
+            cpx a = out[k], b = cmul(out[k + m], w);
+            out[k].re = a.re + b.re; out[k].im = a.im + b.im;
+            out[k + m].re = a.re - b.re; out[k + m].im = a.im - b.im;
+        }
+        return;
+    }
+    if (p == 4) {
+        for (int k = 0; k < m; k++) {
+            cpx y[4];
+            y[0] = out[k];
+            for (int r = 1; r < 4; r++) {
+                cpx w = tw[((size_t)r * k * tws) % N];
+                if (sign > 0) w.im = -w.im;
+                y[r] = cmul(out[k + r * m], w);
+            }
+            cpx t0 = {y[0].re + y[2].re, y[0].im + y[2].im};
+            cpx t1 = {y[0].re - y[2].re, y[0].im - y[2].im};
+            cpx t2 = {y[1].re + y[3].re, y[1].im + y[3].im};
+            cpx t3 = {y[1].re - y[3].re, y[1].im - y[3].im};
+            /* multiply t3 by -i (forward) or +i (inverse) */
+            cpx t3r = sign < 0 ? (cpx){t3.im, -t3.re} : (cpx){-t3.im, t3.re};
+            out[k].re = t0.re + t2.re; out[k].im = t0.im + t2.im;
+            out[k + m].re = t1.re + t3r.re; out[k + m].im = t1.im + t3r.im;
+            out[k + 2 * m].re = t0.re - t2.re; out[k + 2 * m].im = t0.im - t2.im;
+            out[k + 3 * m].re = t1.re - t3r.re; out[k + 3 * m].im = t1.im - t3r.im;
+        }
+        return;
+    }
+    cpx ystack[64];
+    cpx* y = p <= 64 ? ystack : (cpx*)xcalloc((size_t)p, sizeof(cpx));
+    cpx zstack[64];
+    cpx* zz = p <= 64 ? zstack : (cpx*)xcalloc((size_t)p, sizeof(cpx));
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++) {
+            cpx w = tw[((size_t)r * k * tws) % N];
+            if (sign > 0) w.im = -w.im;
+            y[r] = cmul(out[k + (size_t)r * m], w);
+        }
+        for (int s = 0; s < p; s++) {
+            cpx acc = {0, 0};
+            for (int r = 0; r < p; r++) {
+                cpx w = tw[((size_t)r * s * m * tws) % N];
+                if (sign > 0) w.im = -w.im;
+                cpx t = cmul(y[r], w);
+                acc.re += t.re; acc.im += t.im;
+            }
+            zz[s] = acc;
+        }
+        for (int s = 0; s < p; s++) out[k + (size_t)s * m] = zz[s];
+    }
+    if (p > 64) { free(y); free(zz); }
+}
+
+static void fft_line(cpx* line, cpx* tmp, const fftplan* pl, int sign) {
+    memcpy(tmp, line, (size_t)pl->n * sizeof(cpx));
+    fft_rec(tmp, line, pl->n, 1, pl->f, pl->tw, pl->n, 1, sign);
+}
+
+void or_fft_c2c(or_real* data, int n, int howmany, int sign) {
+    const fftplan* pl = get_plan(n);
+    cpx* d = (cpx*)data;
+#pragma omp parallel
+    {
+        cpx* tmp = (cpx*)xcalloc((size_t)n, sizeof(cpx));
+#pragma omp for schedule(static)
+        for (int l = 0; l < howmany; l++) fft_line(d + (size_t)l * n, tmp, pl, sign);
+        free(tmp);
+    }
+}
+
+/* FFT along one axis of a [n0][n1][n2] complex array, restricted to lines whose
+ * other two indices lie below lim (zero-skipping, gpu_fft6.cu:54-256 idea). */
+static void fft3_axis(cpx* d, const int n[3], int axis, const int lim[3], int sign) {
+    const fftplan* pl = get_plan(n[axis]);
+    int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    if (a1 > a2) { int t = a1; a1 = a2; a2 = t; }
+    size_t strides[3] = {(size_t)n[1] * n[2], (size_t)n[2], 1};
+    long nl = (long)lim[a1] * lim[a2];
+    int L = n[axis];
+#pragma omp parallel
+    {
+        cpx* line = (cpx*)xcalloc((size_t)L, sizeof(cpx));
+        cpx* tmp = (cpx*)xcalloc((size_t)L, sizeof(cpx));
+#pragma omp for schedule(static)
+        for (long q = 0; q < nl; q++) {
+            int i1 = (int)(q / lim[a2]), i2 = (int)(q % lim[a2]);
+            cpx* base = d + i1 * strides[a1] + i2 * strides[a2];
+            size_t st = strides[axis];
+            for (int i = 0; i < L; i++) line[i] = base[i * st];
+            fft_line(line, tmp, pl, sign);
+            for (int i = 0; i < L; i++) base[i * st] = line[i];
+        }
+        free(line);
+        free(tmp);
+    }
+}
+
+void or_fft3_c2c(or_real* data, const int n[3], int sign) {
+    for (int ax = 2; ax >= 0; ax--) fft3_axis((cpx*)data, n, ax, n, sign);
+}
+
+/* ====================================================================== */
+/* convolution: conv.go:57-167, gpu_fft0.cu:96-109, gpu_kernmul.cu:25-141  */
+/* ====================================================================== */
+struct or_conv {
+    int N[3], P[3];
+    size_t NP;
+    or_real* Kr; /* 6 * NP, real parts of FFT(K)/(P0 P1 P2) (conv.go:141-158) */
+    cpx* buf[3];
+};
+
+or_conv* or_conv_new(const int N[3], const int P[3], const or_real* K6) {
+    or_conv* c = (or_conv*)xcalloc(1, sizeof(or_conv));
+    for (int i = 0; i < 3; i++) { c->N[i] = N[i]; c->P[i] = P[i]; }
+    c->NP = (size_t)P[0] * P[1] * P[2];
+    c->Kr = (or_real*)xcalloc(6 * c->NP, sizeof(or_real));
+    for (int i = 0; i < 3; i++) c->buf[i] = (cpx*)xcalloc(c->NP, sizeof(cpx));
+    double norm = 1.0 / (double)c->NP;
+    for (int k = 0; k < 6; k++) {
+        cpx* b = c->buf[0];
+        for (size_t i = 0; i < c->NP; i++) { b[i].re = K6[k * c->NP + i]; b[i].im = 0; }
+        or_fft3_c2c((or_real*)b, P, -1);
+        for (size_t i = 0; i < c->NP; i++) c->Kr[k * c->NP + i] = (or_real)(b[i].re * norm);
+    }
+    return c;
+}
+
+void or_conv_free(or_conv* c) {
+    if (!c) return;
+    free(c->Kr);
+    for (int i = 0; i < 3; i++) free(c->buf[i]);
+    free(c);
+}
+
+void or_conv_kernel_spectrum(const or_conv* c, or_real* out) {
+    int Pzc = c->P[2] / 2 + 1;
+    for (int k = 0; k < 6; k++)
+        for (int x = 0; x < c->P[0]; x++)
+            for (int y = 0; y < c->P[1]; y++)
+                for (int z = 0; z < Pzc; z++)
+                    out[(((size_t)k * c->P[0] + x) * c->P[1] + y) * Pzc + z] =
+                        c->Kr[k * c->NP + ((size_t)x * c->P[1] + y) * c->P[2] + z];
+}
+
+static void pad_into(const or_conv* c, const or_real* in, cpx* b) {
+    memset(b, 0, c->NP * sizeof(cpx));
+    for (int x = 0; x < c->N[0]; x++)
+        for (int y = 0; y < c->N[1]; y++)
+            for (int z = 0; z < c->N[2]; z++)
+                b[((size_t)x * c->P[1] + y) * c->P[2] + z].re = in[((size_t)x * c->N[1] + y) * c->N[2] + z];
+}
+
+static void forward_padded(or_conv* c, cpx* b) {
+    /* Z lines: only x<Nx, y<Ny are non-zero; Y lines: only x<Nx; X lines: all */
+    int limZ[3] = {c->N[0], c->N[1], c->P[2]};
+    int limY[3] = {c->N[0], c->P[1], c->P[2]};
+    fft3_axis(b, c->P, 2, limZ, -1);
+    fft3_axis(b, c->P, 1, limY, -1);
+    fft3_axis(b, c->P, 0, c->P, -1);
+}
+
+static void inverse_padded(or_conv* c, cpx* b) {
+    int limY[3] = {c->N[0], c->P[1], c->P[2]};
+    int limZ[3] = {c->N[0], c->N[1], c->P[2]};
+    fft3_axis(b, c->P, 0, c->P, +1);
+    fft3_axis(b, c->P, 1, limY, +1);
+    fft3_axis(b, c->P, 2, limZ, +1);
+}
+
+void or_conv_forward1(or_conv* c, const or_real* in, or_real* out) {
+    cpx* b = c->buf[0];
+    pad_into(c, in, b);
+    forward_padded(c, b);
+    int Pzc = c->P[2] / 2 + 1;
+    for (int x = 0; x < c->P[0]; x++)
+        for (int y = 0; y < c->P[1]; y++)
+            for (int z = 0; z < Pzc; z++) {
+                cpx v = b[((size_t)x * c->P[1] + y) * c->P[2] + z];
+                size_t o = (((size_t)x * c->P[1] + y) * Pzc + z) * 2;
+                out[o] = v.re;
+                out[o + 1] = v.im;
+            }
+}
+
+void or_conv_apply(or_conv* c, const or_real* m, or_real* h) {
+    size_t N = (size_t)c->N[0] * c->N[1] * c->N[2];
+    for (int i = 0; i < 3; i++) {
+        pad_into(c, m + i * N, c->buf[i]);
+        forward_padded(c, c->buf[i]);
+    }
+    const or_real *Kxx = c->Kr, *Kyy = c->Kr + c->NP, *Kzz = c->Kr + 2 * c->NP;
+    const or_real *Kyz = c->Kr + 3 * c->NP, *Kxz = c->Kr + 4 * c->NP, *Kxy = c->Kr + 5 * c->NP;
+    cpx *bx = c->buf[0], *by = c->buf[1], *bz = c->buf[2];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)c->NP; i++) {
+        cpx mx = bx[i], my = by[i], mz = bz[i];
+        bx[i].re = mx.re * Kxx[i] + my.re * Kxy[i] + mz.re * Kxz[i];
+        bx[i].im = mx.im * Kxx[i] + my.im * Kxy[i] + mz.im * Kxz[i];
+        by[i].re = mx.re * Kxy[i] + my.re * Kyy[i] + mz.re * Kyz[i];
+        by[i].im = mx.im * Kxy[i] + my.im * Kyy[i] + mz.im * Kyz[i];
+        bz[i].re = mx.re * Kxz[i] + my.re * Kyz[i] + mz.re * Kzz[i];
+        bz[i].im = mx.im * Kxz[i] + my.im * Kyz[i] + mz.im * Kzz[i];
+    }
+    for (int i = 0; i < 3; i++) {
+        inverse_padded(c, c->buf[i]);
+        for (int x = 0; x < c->N[0]; x++)
+            for (int y = 0; y < c->N[1]; y++)
+                for (int z = 0; z < c->N[2]; z++)
+                    h[i * N + ((size_t)x * c->N[1] + y) * c->N[2] + z] =
+                        c->buf[i][((size_t)x * c->P[1] + y) * c->P[2] + z].re;
+    }
+}
+
+void or_conv_direct(const int N[3], const int P[3], const or_real* K6, const or_real* m, or_real* h) {
+    size_t n = (size_t)N[0] * N[1] * N[2], NP = (size_t)P[0] * P[1] * P[2];
+    memset(h, 0, 3 * n * sizeof(or_real));
+    for (int x = 0; x < N[0]; x++)
+        for (int y = 0; y < N[1]; y++)
+            for (int z = 0; z < N[2]; z++) {
+                size_t o = ((size_t)x * N[1] + y) * N[2] + z;
+                for (int xs = 0; xs < N[0]; xs++)
+                    for (int ys = 0; ys < N[1]; ys++)
+                        for (int zs = 0; zs < N[2]; zs++) {
+                            size_t src = ((size_t)xs * N[1] + ys) * N[2] + zs;
+                            size_t kid = ((size_t)wrapi(x - xs, P[0]) * P[1] + wrapi(y - ys, P[1])) * P[2] +
+                                         wrapi(z - zs, P[2]);
+                            for (int d = 0; d < 3; d++)
+                                for (int s = 0; s < 3; s++)
+                                    h[d * n + o] += K6[KERN_IDX[s][d] * NP + kid] * m[s * n + src];
+                        }
+            }
+}
+
+/* ====================================================================== */
+/* pointwise: gpu_local_contr3.cu:21-50,238-256; gpu_torque.cu:20-50;      */
+/* gpu_normalize.cu:20-30; gpu_linalg.cu:121-132; gpu_reduction.cu:576-630 */
+/* ====================================================================== */
+void or_local_fields(const or_real* m, or_real* h, int N, const or_real hext[3], int anisType, or_real k,
+                     const or_real u[3]) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        or_real mu = 0;
+        if (anisType == OR_ANIS_UNIAXIAL) mu = m[i] * u[0] + m[N + i] * u[1] + m[2 * N + i] * u[2];
+        for (int c = 0; c < 3; c++) {
+            or_real a = anisType == OR_ANIS_UNIAXIAL ? k * mu * u[c] : 0;
+            h[c * N + i] += hext[c] + a;
+        }
+    }
+}
+
+void or_local_fields_phi(const or_real* m, or_real* h, or_real* phi, int N, const or_real hext[3], int anisType,
+                         or_real k, const or_real u[3]) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        or_real mx = m[i], my = m[N + i], mz = m[2 * N + i];
+        or_real mu = 0;
+        if (anisType == OR_ANIS_UNIAXIAL) mu = mx * u[0] + my * u[1] + mz * u[2];
+        for (int c = 0; c < 3; c++) {
+            or_real a = anisType == OR_ANIS_UNIAXIAL ? k * mu * u[c] : 0;
+            h[c * N + i] += hext[c] + a;
+        }
+        phi[i] -= mx * hext[0] + my * hext[1] + mz * hext[2] + (anisType == OR_ANIS_UNIAXIAL ? 0.5 * k * mu * mu : 0);
+    }
+}
+
+void or_deltaM(const or_real* m, or_real* h, or_real alpha, or_real dtgilb, int N) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        or_real Mx = m[i], My = m[N + i], Mz = m[2 * N + i];
+        or_real Hx = h[i], Hy = h[N + i], Hz = h[2 * N + i];
+        or_real cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx; /* m x H */
+        or_real dx = My * cz - Mz * cy, dy = Mz * cx - Mx * cz, dz = Mx * cy - My * cx; /* m x (m x H) */
+        h[i] = dtgilb * (cx - alpha * dx);
+        h[N + i] = dtgilb * (cy - alpha * dy);
+        h[2 * N + i] = dtgilb * (cz - alpha * dz);
+    }
+}
+
+void or_normalize(or_real* m, int N) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        double n2 = (double)m[i] * m[i] + (double)m[N + i] * m[N + i] + (double)m[2 * N + i] * m[2 * N + i];
+        double inv = 1.0 / sqrt(n2);
+        m[i] = (or_real)(m[i] * inv);
+        m[N + i] = (or_real)(m[N + i] * inv);
+        m[2 * N + i] = (or_real)(m[2 * N + i] * inv);
+    }
+}
+
+void or_scale_dot(or_real* r, const or_real* a, const or_real* b, or_real s, int N) {
+    for (int i = 0; i < N; i++) r[i] = s * (a[i] * b[i] + a[N + i] * b[N + i] + a[2 * N + i] * b[2 * N + i]);
+}
+
+double or_reduce(int op, const or_real* x, int n) {
+    double acc;
+    switch (op) {
+    case OR_REDUCE_ADD:
+        acc = 0;
+        for (int i = 0; i < n; i++) acc += x[i];
+        return acc;
+    case OR_REDUCE_SUMABS:
+        acc = 0;
+        for (int i = 0; i < n; i++) acc += fabs((double)x[i]);
+        return acc;
+    case OR_REDUCE_MAX:
+        acc = x[0];
+        for (int i = 0; i < n; i++) if (x[i] > acc) acc = x[i];
+        return acc;
+    case OR_REDUCE_MIN:
+        acc = x[0];
+        for (int i = 0; i < n; i++) if (x[i] < acc) acc = x[i];
+        return acc;
+    case OR_REDUCE_MAXABS:
+        acc = 0;
+        for (int i = 0; i < n; i++) if (fabs((double)x[i]) > acc) acc = fabs((double)x[i]);
+        return acc;
+    }
+    fprintf(stderr, "oracle: bad reduce op %d\n", op);
+    abort();
+}
+
+void or_add_exch6_stencil(const or_real* m, or_real* h, const int size[3], const int periodic[3],
+                          const or_real cell[3]) {
+    /* gpu_exch1.cu:21-301: per component h += c0 m + sum_d c_d m(neighbour), a missing
+     * neighbour counts as zero (free boundary) unless the direction is periodic. */
+    int n0 = size[0], n1 = size[1], n2 = size[2];
+    size_t N = (size_t)n0 * n1 * n2;
+    or_real cx = 1 / (cell[X] * cell[X]), cy = 1 / (cell[Y] * cell[Y]), cz = 1 / (cell[Z] * cell[Z]);
+    or_real c0 = n0 == 1 ? -2 * cy - 2 * cz : -2 * cx - 2 * cy - 2 * cz;
+    for (int c = 0; c < 3; c++) {
+        const or_real* mc = m + c * N;
+        or_real* hc = h + c * N;
+#pragma omp parallel for schedule(static)
+        for (int x = 0; x < n0; x++)
+            for (int y = 0; y < n1; y++)
+                for (int z = 0; z < n2; z++) {
+                    size_t i = ((size_t)x * n1 + y) * n2 + z;
+                    or_real r = c0 * mc[i];
+#define NB(xx, yy, zz, coef)                                                                            \
+    do {                                                                                                \
+        int a_ = (xx), b_ = (yy), d_ = (zz);                                                            \
+        if (periodic[0]) a_ = (a_ + n0) % n0;                                                           \
+        if (periodic[1]) b_ = (b_ + n1) % n1;                                                           \
+        if (periodic[2]) d_ = (d_ + n2) % n2;                                                           \
+        if (a_ >= 0 && a_ < n0 && b_ >= 0 && b_ < n1 && d_ >= 0 && d_ < n2)                             \
+            r += (coef) * mc[((size_t)a_ * n1 + b_) * n2 + d_];                                         \
+    } while (0)
+                    if (n0 > 1) { NB(x - 1, y, z, cx); NB(x + 1, y, z, cx); }
+                    NB(x, y - 1, z, cy);
+                    NB(x, y + 1, z, cy);
+                    NB(x, y, z - 1, cz);
+                    NB(x, y, z + 1, cz);
+#undef NB
+                    hc[i] += r;
+                }
+    }
+}
+
+/* ====================================================================== */
+/* simulation: sim.go, init.go:80-287, field.go, torque.go, rk.go, simrun  */
+/* ====================================================================== */
+#define MAX_STAGES 7
+#define MAX_STEP_TRIALS 10 /* rk.go:277 */
+
+struct or_sim {
+    or_params p;
+    double* fieldT;
+    double* fieldB;
+    int fieldLastIdx; /* pointwiseField.lastIdx: only moves forward (simfield.go:126-133) */
+    int N;
+    double units[5];
+    or_real cell[3];
+    or_real kInt, axis[3];
+    or_conv* conv;
+    or_real *m, *m1, *mbackup, *k[MAX_STAGES], *phi, *tmp;
+    int stages, fsal, fsal_initiated, has_b2;
+    double errororder;
+    double a[MAX_STAGES][MAX_STAGES], b[MAX_STAGES], c[MAX_STAGES], b2[MAX_STAGES];
+    double time, dt;
+    long steps, trials_total, rejects_total;
+    double lastErr, maxTorque, energy, hextSI[3];
+};
+
+static void set_tableau(or_sim* s) {
+    /* coefficients as rationals (rk.go:62-212); the reference rounds them to float32 */
+    memset(s->a, 0, sizeof(s->a));
+    memset(s->b, 0, sizeof(s->b));
+    memset(s->c, 0, sizeof(s->c));
+    memset(s->b2, 0, sizeof(s->b2));
+    s->fsal = 0;
+    s->has_b2 = 0;
+    switch (s->p.solver) {
+    case OR_RK2:
+    case OR_RK12:
+        s->stages = 2;
+        s->c[1] = 1;
+        s->a[1][0] = 1;
+        s->b[0] = .5; s->b[1] = .5;
+        if (s->p.solver == OR_RK12) { s->has_b2 = 1; s->errororder = 1; s->b2[0] = 1; s->b2[1] = 0; }
+        break;
+    case OR_RK3:
+        s->stages = 3;
+        s->c[1] = .5; s->c[2] = 1;
+        s->a[1][0] = .5; s->a[2][0] = -1; s->a[2][1] = 2;
+        s->b[0] = 1. / 6; s->b[1] = 2. / 3; s->b[2] = 1. / 6;
+        break;
+    case OR_RK23:
+        s->stages = 4;
+        s->fsal = 1;
+        s->c[1] = .5; s->c[2] = .75; s->c[3] = 1;
+        s->a[1][0] = .5;
+        s->a[2][1] = .75;
+        s->a[3][0] = 2. / 9; s->a[3][1] = 1. / 3; s->a[3][2] = 4. / 9;
+        s->b[0] = 2. / 9; s->b[1] = 1. / 3; s->b[2] = 4. / 9;
+        s->has_b2 = 1; s->errororder = 2;
+        s->b2[0] = 7. / 24; s->b2[1] = .25; s->b2[2] = 1. / 3; s->b2[3] = 1. / 8;
+        break;
+    case OR_RK4:
+        s->stages = 4;
+        s->c[1] = .5; s->c[2] = .5; s->c[3] = 1;
+        s->a[1][0] = .5; s->a[2][1] = .5; s->a[3][2] = 1;
+        s->b[0] = 1. / 6; s->b[1] = 1. / 3; s->b[2] = 1. / 3; s->b[3] = 1. / 6;
+        break;
+    case OR_RKCK:
+        s->stages = 6;
+        s->c[1] = .2; s->c[2] = .3; s->c[3] = .6; s->c[4] = 1; s->c[5] = 7. / 8;
+        s->a[1][0] = .2;
+        s->a[2][0] = 3. / 40; s->a[2][1] = 9. / 40;
+        s->a[3][0] = .3; s->a[3][1] = -.9; s->a[3][2] = 1.2;
+        s->a[4][0] = -11. / 54; s->a[4][1] = 2.5; s->a[4][2] = -70. / 27; s->a[4][3] = 35. / 27;
+        s->a[5][0] = 1631. / 55296; s->a[5][1] = 175. / 512; s->a[5][2] = 575. / 13824;
+        s->a[5][3] = 44275. / 110592; s->a[5][4] = 253. / 4096;
+        s->b[0] = 37. / 378; s->b[2] = 250. / 621; s->b[3] = 125. / 594; s->b[5] = 512. / 1771;
+        s->has_b2 = 1; s->errororder = 4;
+        s->b2[0] = 2825. / 27648; s->b2[2] = 18575. / 48384; s->b2[3] = 13525. / 55296;
+        s->b2[4] = 277. / 14336; s->b2[5] = .25;
+        break;
+    case OR_RKDP:
+        s->stages = 7;
+        s->fsal = 1;
+        s->c[1] = .2; s->c[2] = .3; s->c[3] = .8; s->c[4] = 8. / 9; s->c[5] = 1; s->c[6] = 1;
+        s->a[1][0] = .2;
+        s->a[2][0] = 3. / 40; s->a[2][1] = 9. / 40;
+        s->a[3][0] = 44. / 45; s->a[3][1] = -56. / 15; s->a[3][2] = 32. / 9;
+        s->a[4][0] = 19372. / 6561; s->a[4][1] = -25360. / 2187; s->a[4][2] = 64448. / 6561; s->a[4][3] = -212. / 729;
+        s->a[5][0] = 9017. / 3168; s->a[5][1] = -355. / 33; s->a[5][2] = 46732. / 5247; s->a[5][3] = 49. / 176;
+        s->a[5][4] = -5103. / 18656;
+        s->a[6][0] = 35. / 384; s->a[6][2] = 500. / 1113; s->a[6][3] = 125. / 192; s->a[6][4] = -2187. / 6784;
+        s->a[6][5] = 11. / 84;
+        s->b[0] = 35. / 384; s->b[2] = 500. / 1113; s->b[3] = 125. / 192; s->b[4] = -2187. / 6784; s->b[5] = 11. / 84;
+        s->has_b2 = 1; s->errororder = 4;
+        s->b2[0] = 5179. / 57600; s->b2[2] = 7571. / 16695; s->b2[3] = 393. / 640; s->b2[4] = -92097. / 339200;
+        s->b2[5] = 187. / 2100; s->b2[6] = 1. / 40;
+        break;
+    default:
+        fprintf(stderr, "oracle: unknown solver %d\n", s->p.solver);
+        abort();
+    }
+}
+
+/* applied field in tesla at SI time t (simfield.go:87-153) */
+static void applied_field(or_sim* s, double t, double B[3]) {
+    B[0] = B[1] = B[2] = 0;
+    if (s->p.nField == 0) return;
+    if (s->p.nField == 1) { for (int i = 0; i < 3; i++) B[i] = s->fieldB[i]; return; }
+    for (; s->fieldLastIdx < s->p.nField; s->fieldLastIdx++)
+        if (s->fieldT[s->fieldLastIdx] >= t) break;
+    int i = s->fieldLastIdx;
+    if (i - 1 < 0 || i >= s->p.nField) return;
+    double dt = s->fieldT[i] - s->fieldT[i - 1];
+    double f = (t - s->fieldT[i - 1]) / dt;
+    for (int c = 0; c < 3; c++) B[c] = s->fieldB[3 * (i - 1) + c] + f * (s->fieldB[3 * i + c] - s->fieldB[3 * (i - 1) + c]);
+}
+
+or_sim* or_sim_new(const or_params* p, const float* m0) {
+    or_sim* s = (or_sim*)xcalloc(1, sizeof(or_sim));
+    s->p = *p;
+    if (p->nField > 0) {
+        s->fieldT = (double*)xcalloc((size_t)p->nField, sizeof(double));
+        s->fieldB = (double*)xcalloc((size_t)p->nField * 3, sizeof(double));
+        memcpy(s->fieldT, p->fieldT, (size_t)p->nField * sizeof(double));
+        memcpy(s->fieldB, p->fieldB, (size_t)p->nField * 3 * sizeof(double));
+    }
+    s->p.fieldT = s->fieldT;
+    s->p.fieldB = s->fieldB;
+    s->N = p->size[0] * p->size[1] * p->size[2];
+    or_units(p->msat, p->aexch, s->units);
+    for (int i = 0; i < 3; i++) s->cell[i] = (or_real)(p->cellsize[i] / s->units[0]); /* init.go:139-148 */
+    s->kInt = (or_real)(p->k1 / s->units[4]);                                          /* init.go:103-108 */
+    for (int i = 0; i < 3; i++) s->axis[i] = (or_real)p->anisAxis[i];
+    size_t n3 = 3 * (size_t)s->N;
+    s->m = (or_real*)xcalloc(n3, sizeof(or_real));
+    s->m1 = (or_real*)xcalloc(n3, sizeof(or_real));
+    s->mbackup = (or_real*)xcalloc(n3, sizeof(or_real));
+    s->tmp = (or_real*)xcalloc(n3, sizeof(or_real));
+    s->phi = (or_real*)xcalloc((size_t)s->N, sizeof(or_real));
+    for (int i = 0; i < MAX_STAGES; i++) s->k[i] = (or_real*)xcalloc(n3, sizeof(or_real));
+    for (size_t i = 0; i < n3; i++) s->m[i] = m0[i];
+    /* initConv (init.go:234-274) */
+    if (p->wantDemag) {
+        int P[3];
+        or_pad_size(p->size, p->periodic, P);
+        size_t NP = (size_t)P[0] * P[1] * P[2];
+        or_real* K = (or_real*)xcalloc(6 * NP, sizeof(or_real));
+        or_facekernel6(P, s->cell, p->demagAccuracy > 0 ? p->demagAccuracy : 8, p->periodic, K);
+        if (p->exchInConv && p->exchType == 6) or_add_exch6_kernel(P, s->cell, K);
+        s->conv = or_conv_new(p->size, P, K);
+        free(K);
+    }
+    /* initSolver (init.go:277-287) */
+    s->dt = p->dt / s->units[1];
+    if (s->dt == 0) s->dt = 1e-4; /* DEFAULT_DT_INTERNAL, sim.go:30 */
+    set_tableau(s);
+    return s;
+}
+
+void or_sim_free(or_sim* s) {
+    if (!s) return;
+    or_conv_free(s->conv);
+    free(s->m); free(s->m1); free(s->mbackup); free(s->tmp); free(s->phi);
+    for (int i = 0; i < MAX_STAGES; i++) free(s->k[i]);
+    free(s->fieldT); free(s->fieldB);
+    free(s);
+}
+
+static void update_hext(or_sim* s, or_real hint[3]) { /* field.go:50-61 */
+    applied_field(s, s->time * s->units[1], s->hextSI);
+    for (int i = 0; i < 3; i++) hint[i] = (or_real)(s->hextSI[i] / s->units[2]);
+}
+
+/* calcHeff (field.go:74-80) / calcHeffEnergy (field.go:83-87) */
+static void calc_heff(or_sim* s, const or_real* m, or_real* h, int energy) {
+    int N = s->N;
+    if (s->p.wantDemag) or_conv_apply(s->conv, m, h);
+    else memset(h, 0, 3 * (size_t)N * sizeof(or_real));
+    if (!s->p.wantDemag || !s->p.exchInConv)
+        if (s->p.exchType == 6) or_add_exch6_stencil(m, h, s->p.size, s->p.periodic, s->cell);
+    or_real hint[3];
+    update_hext(s, hint);
+    if (energy) {
+        or_scale_dot(s->phi, m, h, -0.5, N);
+        or_local_fields_phi(m, h, s->phi, N, hint, s->p.anisType, s->kInt, s->axis);
+        double vol = (double)s->cell[0] * s->cell[1] * s->cell[2];
+        s->energy = or_reduce(OR_REDUCE_ADD, s->phi, N) * vol;
+    } else {
+        or_local_fields(m, h, N, hint, s->p.anisType, s->kInt, s->axis);
+    }
+}
+
+static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53,98-102 */
+    double a = s->p.alpha;
+    or_deltaM(m, h, (or_real)a, (or_real)(1.0 / (1.0 + a * a)), s->N);
+}
+
+static void madd(or_real* a, double c, const or_real* b, size_t n) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; i++) a[i] += (or_real)(c * b[i]);
+}
+
+/* one RK step, rk.go:279-445; replay != NULL overrides dt and acceptance per trial */
+static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_trial) {
+    int order = s->stages;
+    size_t n3 = 3 * (size_t)s->N;
+    double time0 = s->time;
+    or_real *m = s->m, *m1 = s->m1;
+    memcpy(s->mbackup, m, n3 * sizeof(or_real));
+    int goodstep = 0, trials = 0;
+    double prevEnergy = s->energy == 0 ? 9999999999.0 : s->energy;
+    while (!goodstep && trials < MAX_STEP_TRIALS) {
+        int t = trials;
+        trials++;
+        s->trials_total++;
+        for (int i = 0; i < order; i++) {
+            if (s->fsal && i == 0 && s->fsal_initiated) {
+                memcpy(s->k[0], s->k[order - 1], n3 * sizeof(or_real));
+            } else {
+                s->time = time0 + s->c[i] * s->dt;
+                memcpy(m1, m, n3 * sizeof(or_real));
+                for (int j = 0; j < order; j++)
+                    if (s->a[i][j] != 0) madd(m1, s->dt * s->a[i][j], s->k[j], n3);
+                int wantE = s->p.wantEnergy && (s->fsal ? i == order - 1 : i == 0);
+                calc_heff(s, m1, s->k[i], wantE);
+                torque(s, m1, s->k[i]);
+                s->fsal_initiated = 1;
+            }
+            if (i == 0) { /* rk.go:348-373 (T=0 always here) */
+                if (!s->has_b2) s->dt = s->p.dt / s->units[1];
+                double maxTorque = or_reduce(OR_REDUCE_MAXABS, s->k[0], (int)n3);
+                s->maxTorque = maxTorque;
+                double dm = s->dt * maxTorque;
+                if (dm < s->p.minDm) s->dt = s->p.minDm / maxTorque;
+                if (s->dt * s->units[1] < s->p.minDt) s->dt = s->p.minDt / s->units[1];
+                if (s->p.maxDm != 0 && dm > s->p.maxDm) s->dt = s->p.maxDm / maxTorque;
+                if (s->p.maxDt != 0 && s->dt * s->units[1] > s->p.maxDt) s->dt = s->p.maxDt / s->units[1];
+                if (dt_trial && t < n_trial) s->dt = dt_trial[t];
+                if (isnan(s->dt)) { fprintf(stderr, "oracle: dt = NaN\n"); return -1; }
+            }
+        }
+        s->time = time0 + s->dt; /* rk.go:376 */
+        if (s->has_b2) {
+            memcpy(m1, m, n3 * sizeof(or_real));
+            for (int i = 0; i < order; i++)
+                if (s->b2[i] != 0) madd(m1, s->b2[i] * s->dt, s->k[i], n3);
+        }
+        for (int i = 0; i < order; i++)
+            if (s->b[i] != 0) madd(m, s->b[i] * s->dt, s->k[i], n3);
+        if (s->has_b2) {
+            madd(m1, -1, m, n3);
+            double err = or_reduce(OR_REDUCE_MAXABS, m1, (int)n3);
+            s->lastErr = err;
+            double factor = pow(s->p.maxError / err, 1.0 / s->errororder);
+            if (factor > 2) factor = 2;
+            if (factor < 0.01) factor = 0.01;
+            s->dt = s->dt * factor;
+            if (isnan(s->dt)) { fprintf(stderr, "oracle: dt = NaN\n"); return -1; }
+            int bad = err > 2 * s->p.maxError || (s->p.wantEnergy && s->energy > prevEnergy);
+            if (dt_trial && t < n_trial) bad = !accept[t];
+            if (bad) {
+                memcpy(m, s->mbackup, n3 * sizeof(or_real));
+                goodstep = 0;
+                s->rejects_total++;
+            } else {
+                goodstep = 1;
+                if (s->dt * s->units[1] < s->p.minDt) s->dt = s->p.minDt / s->units[1];
+                if (s->p.maxDt != 0 && s->dt * s->units[1] > s->p.maxDt) s->dt = s->p.maxDt / s->units[1];
+            }
+        } else {
+            goodstep = 1;
+        }
+    }
+    or_normalize(m, s->N); /* rk.go:444 */
+    return trials;
+}
+
+int or_sim_step(or_sim* s) {
+    int r = rk_step(s, NULL, NULL, 0);
+    s->steps++;
+    return r;
+}
+
+int or_sim_step_replay(or_sim* s, const double* dt_trial, const int* accept, int n_trial) {
+    int r = rk_step(s, dt_trial, accept, n_trial);
+    s->steps++;
+    return r;
+}
+
+int or_sim_run(or_sim* s, double seconds) { /* simrun.go:19-57 */
+    double stop = s->time + seconds / s->units[1];
+    or_normalize(s->m, s->N);
+    int n = 0;
+    while (s->time < stop) {
+        if (or_sim_step(s) < 0) return -1;
+        n++;
+        if (isnan(s->time) || isinf(s->time)) return -1;
+    }
+    return n;
+}
+
+void or_sim_get_m(const or_sim* s, double* m) {
+    for (size_t i = 0; i < 3 * (size_t)s->N; i++) m[i] = s->m[i];
+}
+
+void or_sim_set_m(or_sim* s, const double* m) {
+    for (size_t i = 0; i < 3 * (size_t)s->N; i++) s->m[i] = (or_real)m[i];
+}
+
+void or_sim_state(const or_sim* s, double* st) {
+    st[0] = s->time; st[1] = s->dt; st[2] = (double)s->steps; st[3] = s->lastErr; st[4] = s->maxTorque;
+    st[5] = (double)s->trials_total; st[6] = (double)s->rejects_total; st[7] = s->energy; st[8] = s->units[1];
+}
+
+void or_sim_heff(or_sim* s, const double* m, double* h) {
+    size_t n3 = 3 * (size_t)s->N;
+    for (size_t i = 0; i < n3; i++) s->tmp[i] = (or_real)m[i];
+    calc_heff(s, s->tmp, s->m1, 0);
+    for (size_t i = 0; i < n3; i++) h[i] = s->m1[i];
+}
+
+void or_sim_torque(or_sim* s, const double* m, double* k) {
+    size_t n3 = 3 * (size_t)s->N;
+    for (size_t i = 0; i < n3; i++) s->tmp[i] = (or_real)m[i];
+    calc_heff(s, s->tmp, s->m1, 0);
+    torque(s, s->tmp, s->m1);
+    for (size_t i = 0; i < n3; i++) k[i] = s->m1[i];
+}
+
+double or_sim_energy(or_sim* s) {
+    calc_heff(s, s->m, s->m1, 1);
+    return s->energy;
+}

@@ -1,0 +1,167 @@
+"""GPU parity of the device-resident engine (include/mumax_engine.h) against
+the FP64 oracle.
+
+Tolerances: the engine computes in FP32 (like the reference); fields and
+torques are compared relative to their maximum (FP32 FFT round-off of a few
+1e-7 x log2 P), trajectories by max |dm| with the oracle replaying the GPU's
+accepted-dt schedule (SURVEY: parity at the same dt sequence), bar 1e-5.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def random_m(size, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal((3,) + size)
+    return (m / np.linalg.norm(m, axis=0)).astype(np.float32)
+
+
+def smooth_m(size, seed):
+    """A smooth, exchange-friendly state (random low-order modes)."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = size
+    x, y, z = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+    m = np.zeros((3,) + size)
+    for c in range(3):
+        a = rng.standard_normal(4)
+        m[c] = a[0] + a[1] * np.cos(2 * np.pi * y / Y + a[3]) + a[2] * np.sin(2 * np.pi * z / Z)
+    m[2] += 2.0
+    return (m / np.linalg.norm(m, axis=0)).astype(np.float32)
+
+
+def engine(mm, size, cell, mode="fused", solver="rkdp", dt=0.0, maxerror=1e-5, alpha=0.05, k1=0.0, axis=None,
+           field=None, m0=None, msat=8e5, aexch=1.3e-11):
+    s = mm.Sim(mode)
+    X, Y, Z = size
+    s.gridsize(Z, Y, X)
+    s.cellsize(cell[2], cell[1], cell[0])
+    s.msat(msat)
+    s.aexch(aexch)
+    s.alpha(alpha)
+    if axis is not None:
+        s.anisuniaxial(axis[2], axis[1], axis[0])
+        s.k1(k1)
+    s.solvertype(solver)
+    s.maxerror(maxerror)
+    if dt:
+        s.dt(dt)
+    if field is not None:
+        s.applystatic("field", field[2], field[1], field[0])
+    s.setm_array(m0)
+    return s
+
+
+def oracle(orc, size, cell, m0, solver="rkdp", dt=0.0, maxerror=1e-5, alpha=0.05, k1=0.0, axis=None, field=None,
+           msat=8e5, aexch=1.3e-11):
+    kw = {}
+    if axis is not None:
+        kw.update(k1=k1, anis_axis=axis)
+    if field is not None:
+        kw["field"] = (np.zeros(1), np.asarray(field, float))
+    return orc.sim(m0, size=size, cellsize=cell, msat=msat, aexch=aexch, alpha=alpha, solver=solver, dt=dt,
+                   maxerror=maxerror, **kw)
+
+
+CASES = [
+    ((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9)),   # film, 2-D path
+    ((4, 16, 32), (4e-9, 4e-9, 4e-9)),                # cuboid, 3-D path
+    ((3, 12, 20), (2e-9, 3e-9, 2.5e-9)),              # non-power-of-two data sizes
+    ((1, 24, 40), (2e-9, 2e-9, 2e-9)),
+]
+
+
+@pytest.mark.parametrize("size,cell", CASES)
+def test_kernel_spectrum_vs_oracle(mm, orc, size, cell):
+    m0 = random_m(size, 1)
+    s = engine(mm, size, cell, m0=m0)
+    st = s.state()
+    P = (int(st["px"]), int(st["py"]), int(st["pz"]))
+    got = s.kernel_spectrum()
+    u = orc.units(8e5, 1.3e-11)
+    ci = np.asarray(cell) / u["length"]
+    K = orc.facekernel6(P, ci)
+    orc.add_exch6_kernel(K, ci)
+    ref = orc.conv_new(size, P, K).kernel_spectrum()
+    nK = got.shape[0]
+    ref = ref[:nK]
+    assert np.abs(got - ref).max() < 2e-6 * np.abs(ref).max()
+
+
+def _pow2(n):
+    return n & (n - 1) == 0
+
+
+@pytest.mark.parametrize("mode", ["fused", "abi"])
+@pytest.mark.parametrize("size,cell", CASES)
+def test_heff_and_torque_vs_oracle(mm, orc, size, cell, mode):
+    if mode == "abi" and not all(_pow2(n) for n in size):
+        pytest.skip("the drop-in FFT plan handles power-of-two padded sizes in this build")
+    m0 = random_m(size, 2)
+    field = (0.01, -0.02, 0.03)
+    axis = (0.0, 0.6, 0.8)
+    s = engine(mm, size, cell, mode=mode, m0=m0, field=field, axis=axis, k1=3e5)
+    o = oracle(orc, size, cell, m0, field=field, axis=axis, k1=3e5)
+    h = s.heff(m0)
+    href = o.heff(m0).reshape(h.shape)
+    assert np.abs(h - href).max() < 5e-6 * np.abs(href).max()
+    k = s.torque(m0)
+    kref = o.torque(m0).reshape(k.shape)
+    assert np.abs(k - kref).max() < 5e-6 * np.abs(href).max()
+
+
+@pytest.mark.parametrize("size,cell", CASES[:2])
+def test_rk4_steps_vs_oracle(mm, orc, size, cell):
+    m0 = smooth_m(size, 3)
+    dt = 2e-14
+    axis = (1.0, 0.0, 0.0)
+    s = engine(mm, size, cell, solver="rk4", dt=dt, m0=m0, axis=axis, k1=5e5, alpha=0.1, msat=1e6, aexch=1.5e-11)
+    o = oracle(orc, size, cell, m0, solver="rk4", dt=dt, axis=axis, k1=5e5, alpha=0.1, msat=1e6, aexch=1.5e-11)
+    s.steps(20)
+    for _ in range(20):
+        o.step()
+    got = s.m
+    ref = o.m.reshape(got.shape)
+    assert np.abs(got - ref).max() < 1e-5
+    st = s.state()
+    assert st["steps"] == 20
+    assert st["time"] == pytest.approx(o.state()["time"], rel=1e-6)
+
+
+@pytest.mark.parametrize("mode", ["fused", "abi"])
+def test_dp_run_replayed_in_oracle(mm, orc, mode):
+    size, cell = (1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9)
+    cfg = mm.workloads.film_sp4_like(seed=1)
+    s = mm.workloads.make_sim(cfg, mode)
+    n = s.run(0.05e-9)
+    assert n > 5
+    dts, acc = s.trial_log()
+    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    # Run() normalises first (simrun.go:24); replay the GPU's trials step by step
+    m3 = o.m.reshape(3, -1)
+    o.m = (m3 / np.linalg.norm(m3, axis=0)).reshape(-1)
+    i = 0
+    while i < len(dts):
+        j = i
+        while not acc[j] and j - i < 9:
+            j += 1
+        o.step_replay(dts[i:j + 1], acc[i:j + 1])
+        i = j + 1
+    got = s.m
+    ref = o.m.reshape(got.shape)
+    assert np.abs(got - ref).max() < 1e-5
+    assert s.state()["steps"] == o.state()["steps"]
+
+
+def test_unit_norm_and_damping_decreases_energy(mm):
+    size, cell = (4, 32, 32), (4e-9, 4e-9, 4e-9)
+    m0 = random_m(size, 4)
+    s = engine(mm, size, cell, solver="rk4", dt=1e-14, m0=m0, alpha=0.5, msat=1e6, aexch=1.5e-11)
+    s.steps(1)
+    e0 = s.energy()
+    s.steps(40)
+    e1 = s.energy()
+    m = s.m
+    assert np.abs(np.linalg.norm(m, axis=0) - 1).max() < 1e-6
+    assert e1 < e0

@@ -1,0 +1,314 @@
+"""Pins of the CPU oracle (oracle/oracle.c) without the reference binary.
+
+Each test checks the oracle against something it does not restate: the
+reference's own known answers (redux_test.go, fft_test.go), closed forms
+(exact demag factors, dipole far field, spin-wave dispersion, analytic
+macrospin precession and damping), brute force (direct DFT, direct
+convolution) or an independent library (numpy.fft).  A plausible slip in the
+oracle (a dropped term, a sign, a transposed index) fails at least one.
+"""
+import numpy as np
+import pytest
+
+from oracle import REDUCE_ADD, REDUCE_MAX, REDUCE_MAXABS, REDUCE_MIN, REDUCE_SUMABS
+
+
+# ------------------------------------------------------------------ units (material.go:36-110)
+def test_units_closed_form(orc):
+    u = orc.units(8.0e5, 1.3e-11)
+    mu0 = 4e-7 * np.pi
+    assert u["length"] == pytest.approx(np.sqrt(2 * 1.3e-11 / (mu0 * 8e5 ** 2)), rel=1e-15)
+    assert u["length"] == pytest.approx(5.6858e-9, rel=1e-4)   # exchange length of permalloy-like film
+    assert u["time"] == pytest.approx(1 / (2.211e5 * 8e5), rel=1e-15)
+    assert u["time"] == pytest.approx(5.6535e-12, rel=1e-4)
+    assert u["field"] == pytest.approx(mu0 * 8e5, rel=1e-15)
+    assert u["edens"] == pytest.approx(mu0 * 8e5 ** 2 / 2, rel=1e-12)  # A/L^2 = mu0 Ms^2 / 2
+
+
+# ------------------------------------------------------------------ FaceKernel6 (demag.go:26-183)
+def test_facekernel_cube_self_term(orc):
+    K = orc.facekernel6((2, 2, 2), (1.0, 1.0, 1.0))
+    diag = [K[i, 0, 0, 0] for i in range(3)]
+    # exact demag factor of a cube is 1/3; 8x8 point charges per face err by < 1 %
+    for d in diag:
+        assert d == pytest.approx(-1 / 3, rel=1e-2)
+    assert diag[0] == diag[1] == diag[2]
+    # off-diagonal self terms vanish by symmetry
+    for i in (3, 4, 5):
+        assert abs(K[i, 0, 0, 0]) < 1e-15
+
+
+def test_facekernel_quadrature_converges_to_exact_cube(orc):
+    # the face-charge sum converges to the exact self-demag of a cube as n grows
+    errs = []
+    for n in (2, 4, 8, 16):
+        K = orc.facekernel6((2, 2, 2), (1.0, 1.0, 1.0), accuracy=n)
+        errs.append(abs(K[0, 0, 0, 0] + 1 / 3))
+    assert errs[0] > errs[1] > errs[2] > errs[3]
+
+
+def test_facekernel_thin_film_limit(orc):
+    # film of 128x128 cubic cells, one cell thick: the field at the centre of a
+    # uniformly magnetised (255x255) platelet tends to -M along the normal (internal X)
+    K = orc.facekernel6((1, 256, 256), (1.0, 1.0, 1.0))
+    assert K[0].sum() == pytest.approx(-1.0, abs=5e-3)
+    assert abs(K[1].sum()) < 5e-3 and abs(K[2].sum()) < 5e-3
+    # in-plane and normal factors add up to the (quadrature) trace of a closed body
+    assert K[0].sum() + K[1].sum() + K[2].sum() == pytest.approx(-1.0, abs=5e-3)
+
+
+def test_facekernel_far_field_is_point_dipole(orc):
+    P = (2, 2, 64)
+    cell = (1.0, 1.0, 1.0)
+    K = orc.facekernel6(P, cell)
+    for d in (10, 20, 30):
+        # dipole of volume 1 along z at distance d on the z axis: Bz = 2/(4 pi d^3), Bx = By = 0
+        assert K[2, 0, 0, d] == pytest.approx(2 / (4 * np.pi * d ** 3), rel=2e-3)
+        # transverse: Bx from an x dipole = -1/(4 pi d^3)
+        assert K[0, 0, 0, d] == pytest.approx(-1 / (4 * np.pi * d ** 3), rel=2e-3)
+
+
+def test_facekernel_symmetry(orc):
+    P = (4, 6, 8)
+    K = orc.facekernel6(P, (0.7, 1.0, 1.3))
+
+    def mirror(a, axes):
+        for ax in axes:
+            a = np.roll(np.flip(a, ax), 1, ax)
+        return a
+
+    # diagonal comps even in every axis; xy odd in x,y; xz odd in x,z; yz odd in y,z
+    parity = {0: (1, 1, 1), 1: (1, 1, 1), 2: (1, 1, 1), 3: (1, -1, -1), 4: (-1, 1, -1), 5: (-1, -1, 1)}
+    for c, par in parity.items():
+        for ax in range(3):
+            m = mirror(K[c], [ax])
+            # the row at index P/2 carries no displacement and stays zero
+            sl = [slice(None)] * 3
+            sl[ax] = P[ax] // 2
+            assert np.all(K[c][tuple(sl)] == 0)
+            assert np.allclose(m, par[ax] * K[c], atol=1e-14)
+
+
+def test_facekernel_float32_twin(orc):
+    P = (2, 8, 8)
+    cell = (0.5, 0.7, 0.7)
+    K64 = orc.facekernel6(P, cell)
+    K32 = orc.facekernel6_f32(P, np.asarray(cell, np.float32))
+    assert np.abs(K32 - K64).max() <= 2e-6 * np.abs(K64).max()
+
+
+# ------------------------------------------------------------------ exchange (exchange.go:44-69)
+def test_exchange_kernel_spin_wave(orc):
+    # 3-D interior cells: h = -(2/hz^2)(1 - cos k) m for a spin wave along z
+    N = (4, 4, 64)
+    P = (8, 8, 128)
+    cell = (1.0, 0.8, 0.5)
+    K = np.zeros((6,) + P)
+    orc.add_exch6_kernel(K, cell)
+    c = orc.conv_new(N, P, K)
+    k = 2 * np.pi * 5 / 64
+    z = np.arange(N[2])
+    m = np.zeros((3,) + N)
+    m[0] = np.cos(k * z)[None, None, :]
+    h = c.apply(m)
+    expect = -(2 / cell[2] ** 2) * (1 - np.cos(k)) * m[0]
+    sl = (slice(1, 3), slice(1, 3), slice(1, N[2] - 1))
+    assert np.allclose(h[0][sl], expect[sl], atol=1e-10)
+
+
+def test_exchange_2d_centre_quirk(orc):
+    P = (1, 8, 8)
+    K = np.zeros((6,) + P)
+    orc.add_exch6_kernel(K, (2.0, 1.0, 1.0))
+    # wrap(+-1, 1) = 0 overwrites the centre with 1/hx^2 (exchange.go:58-61)
+    assert K[0, 0, 0, 0] == pytest.approx(1 / 4.0)
+    assert K[0, 0, 1, 0] == 1.0 and K[0, 0, 0, 1] == 1.0
+    assert K[3:].sum() == 0
+
+
+def test_conv_exchange_equals_stencil_3d(orc):
+    N = (3, 5, 6)
+    P = (6, 10, 12)
+    cell = (0.9, 1.1, 1.3)
+    K = np.zeros((6,) + P)
+    orc.add_exch6_kernel(K, cell)
+    rng = np.random.default_rng(0)
+    m = rng.standard_normal((3,) + N)
+    h_conv = orc.conv_new(N, P, K).apply(m)
+    h_sten = orc.exch6_stencil(m, np.zeros_like(m), N, cell)
+    assert np.allclose(h_conv, h_sten, atol=1e-11)
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 11, 12, 13, 16, 30, 64, 96, 100])
+def test_fft_brute_force(orc, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    j = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(j, j) / n)
+    assert np.allclose(orc.fft(x.copy()), x @ W.T, atol=1e-10 * n)
+    assert np.allclose(orc.fft(x.copy(), +1), x @ W.conj().T, atol=1e-10 * n)
+
+
+def test_fft_round_trip_parseval_impulse(orc):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((3, 256)) + 1j * rng.standard_normal((3, 256))
+    X = orc.fft(x.copy())
+    assert np.allclose(orc.fft(X.copy(), +1) / 256, x, atol=1e-12)
+    assert np.allclose((np.abs(X) ** 2).sum(axis=1) / 256, (np.abs(x) ** 2).sum(axis=1))
+    d = np.zeros((1, 64), complex)
+    d[0, 0] = 1
+    assert np.allclose(orc.fft(d), 1.0)
+
+
+def test_fft3_matches_numpy(orc):
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((4, 6, 10)) + 1j * rng.standard_normal((4, 6, 10))
+    assert np.allclose(orc.fft3(x.copy()), np.fft.fftn(x), atol=1e-11)
+
+
+# ------------------------------------------------------------------ convolution (conv.go)
+def _symmetric(K):
+    R = K.copy()
+    for ax in (1, 2, 3):
+        R = np.roll(np.flip(R, ax), 1, ax)
+    return 0.5 * (K + R)
+
+
+@pytest.mark.parametrize("N,P", [((2, 3, 4), (4, 6, 8)), ((1, 5, 7), (1, 10, 14)), ((3, 4, 5), (6, 8, 10))])
+def test_conv_matches_direct(orc, N, P):
+    rng = np.random.default_rng(3)
+    K = _symmetric(rng.standard_normal((6,) + P))
+    m = rng.standard_normal((3,) + N)
+    h1 = orc.conv_new(N, P, K).apply(m)
+    h2 = orc.conv_direct(N, P, K, m)
+    assert np.allclose(h1, h2, atol=1e-11)
+
+
+def test_conv_unit_and_zero_kernel(orc):
+    N, P = (2, 4, 4), (4, 8, 8)
+    m = np.random.default_rng(4).standard_normal((3,) + N)
+    K = np.zeros((6,) + P)
+    assert np.abs(orc.conv_new(N, P, K).apply(m)).max() == 0
+    for c in range(3):
+        K[c, 0, 0, 0] = 1.0  # UnitKernel6 (kernel.go:47-58)
+    assert np.allclose(orc.conv_new(N, P, K).apply(m), m, atol=1e-14)
+
+
+def test_padded_forward_equals_manual_padding(orc):
+    # fft_test.go:79-165: padded transform == transform of the manually padded array
+    N, P = (1, 16, 16), (1, 32, 32)
+    rng = np.random.default_rng(5)
+    x = rng.random(N)
+    c = orc.conv_new(N, P, np.zeros((6,) + P))
+    got = c.forward1(x)
+    xp = np.zeros(P)
+    xp[: N[0], : N[1], : N[2]] = x
+    ref = np.fft.fftn(xp)[:, :, : P[2] // 2 + 1]
+    assert np.allclose(got, ref, atol=1e-12)
+
+
+def test_kernel_spectrum_real_and_symmetric(orc):
+    P = (4, 16, 16)
+    K = orc.facekernel6(P, (1.0, 1.0, 1.0))
+    c = orc.conv_new((2, 8, 8), P, K)
+    S = c.kernel_spectrum()
+    full = np.fft.fftn(K, axes=(1, 2, 3)).real / np.prod(P)
+    assert np.allclose(S, full[..., : P[2] // 2 + 1], atol=1e-14)
+    assert np.abs(np.fft.fftn(K, axes=(1, 2, 3)).imag).max() < 1e-12
+
+
+# ------------------------------------------------------------------ pointwise
+def test_local_fields_closed_form(orc):
+    rng = np.random.default_rng(6)
+    m = orc.normalize(rng.standard_normal((3, 10)))
+    u = np.array([0.0, 0.6, 0.8])
+    h = orc.local_fields(m, np.zeros_like(m), np.array([0.1, -0.2, 0.3]), 1, 0.7, u)
+    mu = (m * u[:, None]).sum(0)
+    assert np.allclose(h, np.array([0.1, -0.2, 0.3])[:, None] + 0.7 * mu * u[:, None])
+
+
+def test_torque_closed_form(orc):
+    rng = np.random.default_rng(7)
+    m = orc.normalize(rng.standard_normal((3, 50)))
+    h = rng.standard_normal((3, 50))
+    a = 0.3
+    k = orc.deltaM(m, h, a, 1 / (1 + a * a))
+    mxh = np.cross(m.T, h.T).T
+    ref = (mxh - a * np.cross(m.T, mxh.T).T) / (1 + a * a)
+    assert np.allclose(k, ref)
+    assert np.abs((m * k).sum(0)).max() < 1e-14  # m . k = 0
+    # |k| = |m x h| sqrt(1 + a^2) / (1 + a^2) for |m| = 1
+    assert np.allclose(np.linalg.norm(k, axis=0), np.linalg.norm(mxh, axis=0) / np.sqrt(1 + a * a))
+
+
+def test_normalize(orc):
+    m = np.random.default_rng(8).standard_normal((3, 40)) * 3
+    assert np.allclose(np.linalg.norm(orc.normalize(m), axis=0), 1.0)
+
+
+# ------------------------------------------------------------------ reductions (redux_test.go:13-102)
+NS = [2, 8, 16, 31, 33, 128, 255, 511, 1023, 1024, 1025, 20000]
+
+
+@pytest.mark.parametrize("N", NS)
+def test_reduce_known_answers(orc, N):
+    assert orc.reduce(REDUCE_ADD, np.ones(N)) == N
+    assert orc.reduce(REDUCE_SUMABS, -np.ones(N)) == N
+    x = np.ones(N)
+    x[12349 % N] = 10.0
+    assert orc.reduce(REDUCE_MAX, x) == 10.0
+    x[12349 % N] = -10.0
+    assert orc.reduce(REDUCE_MAXABS, x) == 10.0
+    assert orc.reduce(REDUCE_MIN, x) == -10.0
+
+
+# ------------------------------------------------------------------ RK stepping (rk.go)
+def _macrospin(orc, solver, dt, alpha=0.1, B=0.1, t_end_internal=50.0, maxerror=1e-6):
+    msat = 8e5
+    u = orc.units(msat, 1.3e-11)
+    th0 = 0.3
+    m0 = np.zeros((3, 1, 1, 2), np.float32)
+    m0[0], m0[1], m0[2] = np.sin(th0), 0.0, np.cos(th0)
+    s = orc.sim(m0, size=(1, 1, 2), cellsize=(1e-9, 1e-9, 1e-9), msat=msat, aexch=1.3e-11, alpha=alpha,
+                solver=solver, dt=dt * u["time"], maxerror=maxerror, want_demag=False, exch_type=0,
+                field=(np.zeros(1), np.array([0.0, 0.0, B])))
+    s.run(t_end_internal * u["time"])
+    st = s.state()
+    t = st["time"]
+    H = B / u["field"]
+    dtg = 1 / (1 + alpha ** 2)
+    # internal-coordinate LL: mz = tanh(a dtg H t + atanh(mz0)), phi = phi0 - dtg H t
+    mz = np.tanh(alpha * dtg * H * t + np.arctanh(np.cos(th0)))
+    phi = -dtg * H * t
+    r = np.sqrt(1 - mz ** 2)
+    exact = np.array([r * np.cos(phi), r * np.sin(phi), mz])
+    got = s.m.reshape(3, 2)[:, 0]
+    return np.abs(got - exact).max(), st
+
+
+def test_rk4_macrospin_fourth_order(orc):
+    e1, _ = _macrospin(orc, "rk4", dt=2.0)
+    e2, _ = _macrospin(orc, "rk4", dt=1.0)
+    assert e2 < 1e-6
+    assert 10 < e1 / e2 < 24  # 2^4 = 16
+
+
+def test_rk2_macrospin_second_order(orc):
+    e1, _ = _macrospin(orc, "rk2", dt=0.5)
+    e2, _ = _macrospin(orc, "rk2", dt=0.25)
+    assert 3 < e1 / e2 < 5.5
+
+
+@pytest.mark.parametrize("solver", ["rkdp", "rkck", "rk23", "rk12"])
+def test_adaptive_macrospin(orc, solver):
+    err, st = _macrospin(orc, solver, dt=0.0, maxerror=1e-7)
+    assert err < 2e-5
+    assert st["steps"] > 3
+
+
+def test_forced_reject_path(orc):
+    # a huge first dt with a tiny maxError must be rejected and retried (rk.go:425-428)
+    err, st = _macrospin(orc, "rkdp", dt=40.0, maxerror=1e-9, t_end_internal=50.0)
+    assert st["rejects"] >= 1
+    assert err < 1e-5
