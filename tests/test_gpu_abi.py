@@ -306,10 +306,10 @@ def test_fft_plan_padded(lib, size):
 
 
 # ------------------------------------------------------------------ semi-analytical step, STT, noise
-def _rk_macrospin(m, h, alpha, dt, n=4000):
+def _rk_macrospin(m, h, alpha, dt, n=4000, precession=+1):
     def f(m):
         mxh = np.cross(m.T, h.T).T
-        return (mxh - alpha * np.cross(m.T, mxh.T).T) / (1 + alpha * alpha)
+        return (precession * mxh - alpha * np.cross(m.T, mxh.T).T) / (1 + alpha * alpha)
 
     d = dt / n
     for _ in range(n):
@@ -330,7 +330,10 @@ def test_anal_fw_step_is_exact_constant_field_solution(lib, orc):
     alpha, dt = 0.1, 0.8
     dm, dh, do = Dev(lib, m), Dev(lib, h), Dev(lib, n=3 * N)
     lib.gpu_anal_fw_step(F32(dt), F32(alpha), N, dm.p, do.p, dh.p)
-    ref = _rk_macrospin(m, h, alpha, dt)
+    # gpu_anal.cu:63-81 rotates (e2,e3) by +|h| qt in the right-handed frame (h, e2, e3):
+    # dm/dt = -m x h - alpha m x (m x h), the opposite precession sense to gpu_deltaM's
+    # internal-coordinate torque (gpu_torque.cu:40-52)
+    ref = _rk_macrospin(m, h, alpha, dt, precession=-1)
     assert np.abs(do.get().reshape(3, N) - ref).max() < 2e-5
 
 
