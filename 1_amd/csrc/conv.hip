@@ -320,13 +320,20 @@ __global__ __launch_bounds__(kNT) void k_zfwd(StageIn si, float2* __restrict__ B
 #pragma unroll
     for (int j = 0; j < 6; j++) coef[j] = dt * si.a[j];
     const int cnt = nrows * Nz;
-    for (int e = threadIdx.x; e < cnt; e += kNT) {
-        float v = si.m[base + e];
-        for (int j = 0; j < si.nk; j++) v = fmaf(coef[j], si.k[j][base + e], v);
-        if (si.mout) si.mout[base + e] = v;
-        const int row = e / Nz, col = e - row * Nz;
-        reinterpret_cast<float*>(lds + row * LS)[col] = v;
-    }
+    const int nk = si.nk;
+    float* mout = si.mout;
+    batched_copy<float, kNT>(
+        cnt,
+        [&](int e) {
+            float v = si.m[base + e];
+            for (int j = 0; j < nk; j++) v = fmaf(coef[j], si.k[j][base + e], v);
+            return v;
+        },
+        [&](int e, float v) {
+            if (mout) mout[base + e] = v;
+            const int row = e / Nz, col = e - row * Nz;
+            reinterpret_cast<float*>(lds + row * LS)[col] = v;
+        });
     if (Nz & 1)
         for (int row = threadIdx.x; row < nrows; row += kNT) reinterpret_cast<float*>(lds + row * LS)[Nz] = 0.f;
     __syncthreads();
@@ -353,10 +360,13 @@ __global__ __launch_bounds__(kNT) void k_zinv(const float2* __restrict__ B1, flo
     const int c = blockIdx.z, x = blockIdx.y, y0 = blockIdx.x * TR;
     const int nrows = min(TR, Ny - y0);
     const float2* src = B1 + ((long)c * Nx + x) * Q * Ny + y0;
-    for (int t = threadIdx.x; t < Q * TR; t += kNT) {
-        const int row = t % TR, kz = t / TR;
-        if (row < nrows) lds[row * LS + kz] = src[(long)kz * Ny + row];
-    }
+    batched_copy<float2, kNT>(
+        Q * TR,
+        [&](int t) {
+            const int row = t % TR, kz = t / TR;
+            return row < nrows ? src[(long)kz * Ny + row] : make_float2(0.f, 0.f);
+        },
+        [&](int t, float2 v) { lds[(t % TR) * LS + t / TR] = v; });
     __syncthreads();
     float2 y[NY];
 #pragma unroll
@@ -394,10 +404,16 @@ __global__ __launch_bounds__(kNT) void k_yfwd(const float2* __restrict__ B1, flo
     if (st && !st->trial_active) return;
     const int c = blockIdx.z, x0 = blockIdx.y * XC, kz = blockIdx.x;
     const int nx = min(XC, Nx - x0);
-    for (int t = threadIdx.x; t < nx * Ny; t += kNT) {
-        const int xi = t / Ny, y = t - xi * Ny;
-        lds[xi * LS + y] = B1[(((long)c * Nx + x0 + xi) * Q + kz) * Ny + y];
-    }
+    batched_copy<float2, kNT>(
+        nx * Ny,
+        [&](int t) {
+            const int xi = t / Ny, y = t - xi * Ny;
+            return B1[(((long)c * Nx + x0 + xi) * Q + kz) * Ny + y];
+        },
+        [&](int t, float2 v) {
+            const int xi = t / Ny, y = t - xi * Ny;
+            lds[xi * LS + y] = v;
+        });
     __syncthreads();
     fft_lds<float, PY, -1, XC, kNT>(lds, LS, tw, Ny);
     float2* dst = B2 + (((long)c * Q + kz) * PY) * Nx + x0;
@@ -418,10 +434,9 @@ __global__ __launch_bounds__(kNT) void k_yinv(const float2* __restrict__ B2, flo
     const int c = blockIdx.z, x0 = blockIdx.y * XC, kz = blockIdx.x;
     const int nx = min(XC, Nx - x0);
     const float2* src = B2 + (((long)c * Q + kz) * PY) * Nx + x0;
-    for (int t = threadIdx.x; t < PY * nx; t += kNT) {
-        const int xi = t % nx, ky = t / nx;
-        lds[xi * LS + ky] = src[(long)ky * Nx + xi];
-    }
+    batched_copy<float2, kNT>(
+        PY * nx, [&](int t) { return src[(long)(t / nx) * Nx + (t % nx)]; },
+        [&](int t, float2 v) { lds[(t % nx) * LS + t / nx] = v; });
     __syncthreads();
     fft_lds<float, PY, +1, XC, kNT>(lds, LS, tw, PY);
     for (int t = threadIdx.x; t < nx * Ny; t += kNT) {
@@ -442,13 +457,18 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
     const long p0 = (long)blockIdx.x * NP;
     const int np = (int)min((long)NP, npairs - p0);
     const long cstride = npairs * Nx;
-    for (int c = 0; c < 3; c++) {
-        const float2* src = B2 + c * cstride + p0 * Nx;
-        for (int t = threadIdx.x; t < np * Nx; t += kNT) {
-            const int pi = t / Nx, x = t - pi * Nx;
-            lds[(c * NP + pi) * LS + x] = src[t];
-        }
-    }
+    const int per = np * Nx;
+    batched_copy<float2, kNT>(
+        3 * per,
+        [&](int t) {
+            const int c = t / per, r = t - c * per;
+            return B2[c * cstride + p0 * Nx + r];
+        },
+        [&](int t, float2 v) {
+            const int c = t / per, r = t - c * per;
+            const int pi = r / Nx, x = r - pi * Nx;
+            lds[(c * NP + pi) * LS + x] = v;
+        });
     __syncthreads();
     fft_lds<float, PX, -1, 3 * NP, kNT>(lds, LS, tw, Nx);
     const int hx = PX / 2 + 1, hy = PY / 2 + 1;
@@ -494,13 +514,18 @@ __global__ __launch_bounds__(kNT) void k_y2d(float2* __restrict__ B1, const floa
     const int kz0 = blockIdx.x * NKZ;
     const int nkz = min(NKZ, Q - kz0);
     const long cstride = (long)Q * Ny;
-    for (int c = 0; c < 3; c++) {
-        const float2* src = B1 + c * cstride + (long)kz0 * Ny;
-        for (int t = threadIdx.x; t < nkz * Ny; t += kNT) {
-            const int j = t / Ny, y = t - j * Ny;
-            lds[(c * NKZ + j) * LS + y] = src[t];
-        }
-    }
+    const int per = nkz * Ny;
+    batched_copy<float2, kNT>(
+        3 * per,
+        [&](int t) {
+            const int c = t / per, r = t - c * per;
+            return B1[c * cstride + (long)kz0 * Ny + r];
+        },
+        [&](int t, float2 v) {
+            const int c = t / per, r = t - c * per;
+            const int j = r / Ny, y = r - j * Ny;
+            lds[(c * NKZ + j) * LS + y] = v;
+        });
     __syncthreads();
     fft_lds<float, PY, -1, 3 * NKZ, kNT>(lds, LS, tw, Ny);
     const int hy = PY / 2 + 1;

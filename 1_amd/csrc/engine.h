@@ -27,9 +27,6 @@ struct EngineState {
     float dt_used;      // dt of the current trial
     float max_torque;   // maxabs(k0) of the current step (rk.go:353)
     float err;          // error estimate of the last trial (rk.go:403)
-    unsigned int red_k;    // atomicMax(float bits) of |k| of the stage being evaluated
-    unsigned int red_k6;   // same for the last stage of an FSAL tableau (k0 of the next step)
-    unsigned int red_err;  // atomicMax(float bits) of |m_low - m_high|
     int trials;         // trials of the current step (MAX_STEP_TRIALS = 10)
     int accept;         // last trial accepted
     int step_end;       // last trial ended the step (accepted or out of trials)
@@ -134,7 +131,7 @@ struct TorqueArgs {
     float unitField;        // tesla per internal field unit
     double unitTime;        // seconds per internal time unit
     float c;                // stage node c_i
-    int red6;               // accumulate |k| into red_k6 instead of red_k
+    float* part;            // per-block max |k| (null: no reduction at this stage)
     int exch;               // add the 6-neighbour exchange stencil here
     int n[3];
     float cexch[4];         // cx, cy, cz, c0
@@ -146,8 +143,13 @@ struct FinalArgs {
     int S;
     float b[7], b2[7];
     int adaptive;
+    float* part_err;        // per-block max |m_low - m_high| (adaptive)
 };
-// high/low order solutions; adaptive: mtmp = m_high, red_err = max|m_low-m_high|;
+
+// Pointwise kernels run a fixed grid of kRedBlocks blocks (grid-stride), so
+// every per-block partial buffer has exactly kRedBlocks entries.
+constexpr int kRedBlocks = 1024;
+// high/low order solutions; adaptive: mtmp = m_high, part_err = per-block max|m_low-m_high|;
 // fixed step: m = normalize(m_high) in place.
 void launch_rk_final(float* m, const float* const k[7], float* mtmp, EngineState* st, const FinalArgs& a, int N,
                      hipStream_t s);
@@ -161,6 +163,10 @@ struct CtrlParams {
     double* log_dt;
     int* log_accept;
     int log_cap;
+    const float* part_k0;   // per-block max |k0| (stage 0 torque)
+    const float* part_k6;   // per-block max |k_last| (FSAL tableau)
+    const float* part_err;  // per-block error (adaptive)
+    int fold_k0;            // ctrl_end also records max |k0| (no separate after-k0 control)
 };
 void launch_ctrl_begin(EngineState* st, hipStream_t s);
 void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s);

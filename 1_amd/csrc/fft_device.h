@@ -231,6 +231,28 @@ __device__ __forceinline__ void fft_lds(C2<T>* lds, int ls, const C2<T>* tw, int
     }
 }
 
+// Load phases issue kUnroll independent global loads per lane before any of
+// them is consumed, so a block keeps up to 8 x NT loads in flight instead of
+// waiting one memory latency per element (loop trip counts are runtime values,
+// so the compiler would not batch them by itself).
+constexpr int kUnroll = 8;
+template <typename T, int NT, typename Ld, typename St>
+__device__ __forceinline__ void batched_copy(int cnt, Ld ld, St st) {
+    for (int base = threadIdx.x; base < cnt; base += NT * kUnroll) {
+        T v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const int i = base + u * NT;
+            if (i < cnt) v[u] = ld(i);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const int i = base + u * NT;
+            if (i < cnt) st(i, v[u]);
+        }
+    }
+}
+
 // Real-to-complex split: given Z = FFT_M(x[2n] + i x[2n+1]) (M = P/2), returns
 // X[k] of the length-P real transform, k in [0, M]:  X = E + W^k O with
 // E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2i,  W = exp(-2 pi i / P).

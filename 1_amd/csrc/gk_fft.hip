@@ -34,18 +34,28 @@ __global__ __launch_bounds__(NT) void k_fft_lines(C2<T>* __restrict__ data, cons
     __shared__ C2<T> lds[NL * LS];
     const long l0 = (long)blockIdx.x * NL;
     const bool col = estride != 1;
-    for (int t = threadIdx.x; t < NL * nvalid; t += NT) {
-        int i, e;
+    auto pos = [&](int t, int n, int& i, int& e) {
         if (col) {
             i = t % NL;
             e = t / NL;
         } else {
-            i = t / nvalid;
-            e = t % nvalid;
+            i = t / n;
+            e = t % n;
         }
-        const long l = l0 + i;
-        if (l < nlines) lds[i * LS + e] = data[(l / n_inner) * outer + (l % n_inner) * inner + e * estride];
-    }
+    };
+    batched_copy<C2<T>, NT>(
+        NL * nvalid,
+        [&](int t) {
+            int i, e;
+            pos(t, nvalid, i, e);
+            const long l = l0 + i;
+            return l < nlines ? data[(l / n_inner) * outer + (l % n_inner) * inner + e * estride] : mk<T>(0, 0);
+        },
+        [&](int t, C2<T> v) {
+            int i, e;
+            pos(t, nvalid, i, e);
+            lds[i * LS + e] = v;
+        });
     __syncthreads();
     fft_lds<T, L, DIR, NL, NT>(lds, LS, tw, nvalid);
     for (int t = threadIdx.x; t < NL * nkeep; t += NT) {
@@ -72,17 +82,20 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float* __restrict__ in, f
     __shared__ float2 lds[NL * LS];
     const long r0 = (long)blockIdx.x * NL;
     const int npair = (n_real + 1) / 2;
-    for (int t = threadIdx.x; t < NL * npair; t += NT) {
-        const int i = t / npair, e = t % npair;
-        const long r = r0 + i;
-        float2 v = make_float2(0.f, 0.f);
-        if (r < nrows) {
-            const float* src = in + (r / n_inner) * in_outer + (r % n_inner) * in_inner;
-            v.x = src[2 * e];
-            v.y = (2 * e + 1 < n_real) ? src[2 * e + 1] : 0.f;
-        }
-        lds[i * LS + e] = v;
-    }
+    batched_copy<float2, NT>(
+        NL * npair,
+        [&](int t) {
+            const int i = t / npair, e = t % npair;
+            const long r = r0 + i;
+            float2 v = make_float2(0.f, 0.f);
+            if (r < nrows) {
+                const float* src = in + (r / n_inner) * in_outer + (r % n_inner) * in_inner;
+                v.x = src[2 * e];
+                v.y = (2 * e + 1 < n_real) ? src[2 * e + 1] : 0.f;
+            }
+            return v;
+        },
+        [&](int t, float2 v) { lds[(t / npair) * LS + t % npair] = v; });
     __syncthreads();
     fft_lds<float, M, -1, NL, NT>(lds, LS, twM, npair);
     for (int t = threadIdx.x; t < NL * (M + 1); t += NT) {
@@ -105,11 +118,14 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2* __restrict__ in, 
     constexpr int NY = (NL * M + NT - 1) / NT;
     __shared__ float2 lds[NL * LS];
     const long r0 = (long)blockIdx.x * NL;
-    for (int t = threadIdx.x; t < NL * (M + 1); t += NT) {
-        const int i = t / (M + 1), k = t % (M + 1);
-        const long r = r0 + i;
-        if (r < nrows) lds[i * LS + k] = in[(r / n_inner) * in_outer + (r % n_inner) * in_inner + k];
-    }
+    batched_copy<float2, NT>(
+        NL * (M + 1),
+        [&](int t) {
+            const int i = t / (M + 1), k = t % (M + 1);
+            const long r = r0 + i;
+            return r < nrows ? in[(r / n_inner) * in_outer + (r % n_inner) * in_inner + k] : make_float2(0.f, 0.f);
+        },
+        [&](int t, float2 v) { lds[(t / (M + 1)) * LS + t % (M + 1)] = v; });
     __syncthreads();
     float2 y[NY];
 #pragma unroll

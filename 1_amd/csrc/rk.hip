@@ -63,8 +63,10 @@ __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, flo
         hk[2 * N + i] = kz;
         lmax = fmaxf(lmax, fmaxf(fabsf(kx), fmaxf(fabsf(ky), fabsf(kz))));
     }
-    const float bm = block_max_nonneg(lmax);
-    if (threadIdx.x == 0) atomicMax(a.red6 ? &st->red_k6 : &st->red_k, __float_as_uint(bm));
+    if (a.part) {
+        const float bm = block_max_nonneg(lmax);
+        if (threadIdx.x == 0) a.part[blockIdx.x] = bm;
+    }
 }
 
 struct KPtrs {
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(256) void k_rk_final(float* __restrict__ m, KPtrs k
     }
     if (a.adaptive) {
         const float bm = block_max_nonneg(lmax);
-        if (threadIdx.x == 0) atomicMax(&st->red_err, __float_as_uint(bm));
+        if (threadIdx.x == 0) a.part_err[blockIdx.x] = bm;
     }
 }
 
@@ -158,16 +160,22 @@ __global__ void k_ctrl_begin(EngineState* st) {
         st->new_step = 0;
     }
     st->dt_used = st->dt;
-    st->red_k = 0u;
-    st->red_err = 0u;
 }
 
-// after k0 is known (rk.go:348-373)
+// one wave reduces kRedBlocks per-block maxima; result valid in every lane
+__device__ __forceinline__ float wave_max_partials(const float* part) {
+    float v = 0.f;
+    for (int i = threadIdx.x; i < kRedBlocks; i += 64) v = fmaxf(v, part[i]);
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// after k0 is known (rk.go:348-373); launched with one wave
 __global__ void k_ctrl_after_k0(EngineState* st, int fsal_copy, CtrlParams p) {
     if (!st->trial_active) return;
-    if (fsal_copy) st->red_k = st->red_k6;
+    const float maxTorque = wave_max_partials(fsal_copy ? p.part_k6 : p.part_k0);
+    if (threadIdx.x != 0) return;
     if (!p.adaptive) st->dt = p.inputDt;
-    const float maxTorque = __uint_as_float(st->red_k);
     st->max_torque = maxTorque;
     float dt = st->dt;
     const float dm = dt * maxTorque;
@@ -183,6 +191,13 @@ __global__ void k_ctrl_after_k0(EngineState* st, int fsal_copy, CtrlParams p) {
 // after the high/low solutions (rk.go:376-440)
 __global__ void k_ctrl_end(EngineState* st, CtrlParams p) {
     if (!st->trial_active) return;
+    const float err_red = p.adaptive ? wave_max_partials(p.part_err) : 0.f;
+    const float k0_red = p.fold_k0 ? wave_max_partials(p.part_k0) : 0.f;
+    if (threadIdx.x != 0) return;
+    if (p.fold_k0) {
+        st->max_torque = k0_red;
+        if (!p.adaptive) st->dt = p.inputDt;
+    }
     const float dtu = st->dt_used;
     st->time = st->time0 + (double)dtu;
     st->trials++;
@@ -190,7 +205,7 @@ __global__ void k_ctrl_end(EngineState* st, CtrlParams p) {
     st->fsal_init = 1;
     int bad = 0;
     if (p.adaptive) {
-        const float err = __uint_as_float(st->red_err);
+        const float err = err_red;
         st->err = err;
         float factor = (float)pow((double)(p.maxError / err), 1.0 / (double)p.errorOrder);
         if (factor > 2.f) factor = 2.f;
@@ -236,7 +251,7 @@ __global__ __launch_bounds__(256) void k_sum_comp(const float* __restrict__ v, i
 
 static dim3 gridN(long n) {
     long g = (n + 255) / 256;
-    if (g > 2048) g = 2048;
+    if (g > kRedBlocks) g = kRedBlocks;
     if (g < 1) g = 1;
     return dim3((unsigned)g);
 }
@@ -284,7 +299,8 @@ void launch_set_target(EngineState* st, long long nsteps, double stop, hipStream
 
 
 void launch_torque(const float* mi, float* hk, EngineState* st, const TorqueArgs& a, int N, hipStream_t s) {
-    k_torque<<<gridN(N), 256, 0, s>>>(mi, hk, st, a, N);
+    // a partial buffer has kRedBlocks entries: launch exactly that many blocks when reducing
+    k_torque<<<a.part ? dim3(kRedBlocks) : gridN(N), 256, 0, s>>>(mi, hk, st, a, N);
     GK_LAUNCH_CHECK();
 }
 
@@ -292,7 +308,7 @@ void launch_rk_final(float* m, const float* const k[7], float* mtmp, EngineState
                      hipStream_t s) {
     KPtrs kp;
     for (int i = 0; i < 7; i++) kp.k[i] = k[i];
-    k_rk_final<<<gridN(N), 256, 0, s>>>(m, kp, mtmp, st, a, N);
+    k_rk_final<<<a.adaptive ? dim3(kRedBlocks) : gridN(N), 256, 0, s>>>(m, kp, mtmp, st, a, N);
     GK_LAUNCH_CHECK();
 }
 
@@ -306,11 +322,11 @@ void launch_ctrl_begin(EngineState* st, hipStream_t s) {
     GK_LAUNCH_CHECK();
 }
 void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s) {
-    k_ctrl_after_k0<<<1, 1, 0, s>>>(st, fsal_copy, p);
+    k_ctrl_after_k0<<<1, 64, 0, s>>>(st, fsal_copy, p);
     GK_LAUNCH_CHECK();
 }
 void launch_ctrl_end(EngineState* st, const CtrlParams& p, hipStream_t s) {
-    k_ctrl_end<<<1, 1, 0, s>>>(st, p);
+    k_ctrl_end<<<1, 64, 0, s>>>(st, p);
     GK_LAUNCH_CHECK();
 }
 void launch_commit(float* m, const float* mtmp, EngineState* st, int N, hipStream_t s) {

@@ -202,6 +202,7 @@ struct mmSim {
     int* log_acc = nullptr;
     int log_cap = 1 << 16;
     double* dsum = nullptr;
+    float* part = nullptr;  // 3 x kRedBlocks: |k0|, |k_last|, error
     double* fT = nullptr;
     float* fB = nullptr;
     FieldTable field{};
@@ -261,8 +262,18 @@ static EngineState read_state(mmSim* s) {
     return h;
 }
 
+static bool needs_after_k0(const mmSim* s) {
+    // the dt clamps of rk.go:356-371 need max|k0| before stage 1; FSAL tableaux take
+    // max|k0| from the previous trial's last stage
+    return s->tb.fsal || s->p.minDm != 0 || s->p.maxDm != 0 || s->p.minDt != 0 || s->p.maxDt != 0;
+}
+
 static CtrlParams ctrl_params(const mmSim* s) {
     CtrlParams c{};
+    c.part_k0 = s->part;
+    c.part_k6 = s->part + kRedBlocks;
+    c.part_err = s->part + 2 * kRedBlocks;
+    c.fold_k0 = !needs_after_k0(s);
     c.adaptive = s->tb.adaptive;
     c.maxError = (float)s->p.maxError;
     c.errorOrder = s->tb.order;
@@ -277,7 +288,7 @@ static CtrlParams ctrl_params(const mmSim* s) {
     return c;
 }
 
-static TorqueArgs torque_args(const mmSim* s, float c, int red6) {
+static TorqueArgs torque_args(const mmSim* s, float c, float* part) {
     TorqueArgs a{};
     a.alpha = (float)s->p.alpha;
     a.dtg = (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha));
@@ -288,7 +299,7 @@ static TorqueArgs torque_args(const mmSim* s, float c, int red6) {
     a.unitField = (float)s->unitField;
     a.unitTime = s->unitTime;
     a.c = c;
-    a.red6 = red6;
+    a.part = part;
     return a;
 }
 
@@ -320,10 +331,11 @@ static void enqueue_trial(mmSim* s) {
         si.mout = si.nk ? s->mi : nullptr;
         conv_apply(s->cp, si, s->k[i], s->s, &pf);
         pf.begin(5, s->s);
-        launch_torque(si.nk ? s->mi : s->m, s->k[i], s->st, torque_args(s, t.c[i], t.fsal && i == t.S - 1), s->N,
-                      s->s);
+        // max|k| is needed for k0 (rk.go:353) and, with FSAL, for the last stage (next k0)
+        float* part = i == 0 ? s->part : (t.fsal && i == t.S - 1) ? s->part + kRedBlocks : nullptr;
+        launch_torque(si.nk ? s->mi : s->m, s->k[i], s->st, torque_args(s, t.c[i], part), s->N, s->s);
         pf.end(s->s);
-        if (i == 0) {
+        if (i == 0 && needs_after_k0(s)) {
             pf.begin(7, s->s);
             launch_ctrl_after_k0(s->st, 0, cp, s->s);
             pf.end(s->s);
@@ -336,6 +348,7 @@ static void enqueue_trial(mmSim* s) {
         fa.b2[i] = t.b2[i];
     }
     fa.adaptive = t.adaptive;
+    fa.part_err = s->part + 2 * kRedBlocks;
     const float* kk[7];
     for (int i = 0; i < 7; i++) kk[i] = s->k[i] ? s->k[i] : s->k[0];
     pf.begin(6, s->s);
@@ -568,6 +581,8 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->log_dt = (double*)dmalloc(s, s->log_cap * sizeof(double));
     s->log_acc = (int*)dmalloc(s, s->log_cap * sizeof(int));
     s->dsum = (double*)dmalloc(s, 4 * sizeof(double));
+    s->part = (float*)dmalloc(s, 3 * kRedBlocks * sizeof(float));
+    GK_CHECK(hipMemset(s->part, 0, 3 * kRedBlocks * sizeof(float)));
     EngineState h{};
     h.dt = (float)(p->dt / s->unitTime);  // init.go:280-284
     if (h.dt == 0.f) h.dt = 1e-4f;
@@ -607,7 +622,7 @@ void mm_sim_free(mmSim* s) {
         if (s->k[i]) GK_CHECK(hipFree(s->k[i]));
     for (float* p : {s->m, s->mi, s->mtmp, s->fB})
         if (p) GK_CHECK(hipFree(p));
-    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT})
+    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT, (void*)s->part})
         if (p) GK_CHECK(hipFree(p));
     conv_plan_free(s->cp);
     if (s->plan) delete_gpuFFT3dPlan(s->plan);
