@@ -9,7 +9,7 @@
 // every FFT pass reads and writes whole contiguous lines (coalesced 64-lane
 // accesses) and no separate transpose or pad/unpad pass exists:
 //   m[c][x][y][z]  --Z r2c (stage build fused)-->  B1[c][x][kz][y]   (y fastest)
-//   3-D: B1 --Y fwd--> B2[c][kz][ky][x] --X fwd . K . X inv (in place)--> B2
+//   3-D: B1 --Y fwd--> B2[c][kz][x][ky] --X fwd . K . X inv (in place)--> B2
 //        --Y inv--> B1 --Z c2r--> h[c][x][y][z]
 //   2-D: B1 --Y fwd . K . Y inv (in place)--> B1 --Z c2r--> h
 // Only data lines that can be non-zero are transformed (rows y >= Ny and
@@ -177,7 +177,7 @@ void conv_plan_init(ConvPlan& cp, const int N[3]) {
     const long b2 = cp.P[0] == 1 ? 0 : 3L * cp.Q * cp.P[1] * N[0];
     GK_CHECK(hipMalloc(&cp.Kc, kc * sizeof(float)));
     GK_CHECK(hipMalloc(&cp.B1, b1 * sizeof(float2)));
-    cp.B2 = nullptr;
+    cp.B2 = nullptr;  // layout [3][Q][Nx][Py]
     if (b2) GK_CHECK(hipMalloc(&cp.B2, b2 * sizeof(float2)));
     cp.bytes = kc * sizeof(float) + (b1 + b2) * sizeof(float2);
 }
@@ -304,203 +304,272 @@ template <int L> constexpr int rows_per_block() {
     return p;
 }
 
-// Z forward: rows of m (stage build on the fly) -> B1[c][x][kz][y]
+// shuffle of a complex value within groups of W lanes
+template <int W> __device__ __forceinline__ float2 shfl2(float2 v, int src) {
+    return make_float2(__shfl(v.x, src, W), __shfl(v.y, src, W));
+}
+
+// Z forward.  TR rows (y0..y0+TR) of one (c, x) plane; each row is a group of G
+// lanes holding the packed complex row z[n] = x[2n] + i x[2n+1] (RK stage build
+// m + sum dt a_ij k_j applied while loading), transformed in registers; the
+// r2c split takes Z[M-k] from the partner lane by a shuffle, and the spectrum is
+// transposed through a padded LDS tile into B1[c][x][kz][y] (TR consecutive y).
 template <int M, int TR>
-__global__ __launch_bounds__(kNT) void k_zfwd(StageIn si, float2* __restrict__ B1, const float2* __restrict__ twM,
-                                              const float2* __restrict__ twP, int Nx, int Ny, int Nz) {
-    constexpr int LS = M + 1, Q = M + 1;
-    __shared__ float2 lds[TR * LS];
+__global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zfwd(StageIn si, float2* __restrict__ B1,
+                                                              const float2* __restrict__ twM,
+                                                              const float2* __restrict__ twP, int Nx, int Ny, int Nz) {
+    using F = RegFFT<M>;
+    constexpr int E = F::E, G = F::G, NT = TR * G, Q = M + 1, TS = TR + 1;
+    constexpr int FFT_LDS = TR * F::LDS, TILE_LDS = Q * TS;
+    __shared__ float2 lds[FFT_LDS > TILE_LDS ? FFT_LDS : TILE_LDS];
     if (si.st && !si.st->trial_active) return;
+    const int row = threadIdx.x / G, g = threadIdx.x % G;
     const int c = blockIdx.z, x = blockIdx.y, y0 = blockIdx.x * TR;
     const int nrows = min(TR, Ny - y0);
+    const bool ok = row < nrows;
     const long N = (long)Nx * Ny * Nz;
-    const long base = c * N + ((long)x * Ny + y0) * Nz;
+    const long rbase = c * N + ((long)x * Ny + y0 + row) * Nz;
     const float dt = si.st ? si.st->dt_used : 0.f;
     float coef[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) coef[j] = dt * si.a[j];
-    const int cnt = nrows * Nz;
     const int nk = si.nk;
-    float* mout = si.mout;
-    batched_copy<float, kNT>(
-        cnt,
-        [&](int e) {
-            float v = si.m[base + e];
-            for (int j = 0; j < nk; j++) v = fmaf(coef[j], si.k[j][base + e], v);
-            return v;
-        },
-        [&](int e, float v) {
-            if (mout) mout[base + e] = v;
-            const int row = e / Nz, col = e - row * Nz;
-            reinterpret_cast<float*>(lds + row * LS)[col] = v;
-        });
-    if (Nz & 1)
-        for (int row = threadIdx.x; row < nrows; row += kNT) reinterpret_cast<float*>(lds + row * LS)[Nz] = 0.f;
-    __syncthreads();
-    fft_lds<float, M, -1, TR, kNT>(lds, LS, twM, (Nz + 1) / 2);
-    float2* dst = B1 + ((long)c * Nx + x) * Q * Ny + y0;
-    for (int t = threadIdx.x; t < Q * TR; t += kNT) {
-        const int row = t % TR, kz = t / TR;
-        if (row < nrows) {
-            float2 zk = lds[row * LS + (kz % M)], zmk = lds[row * LS + ((M - kz) % M)];
-            dst[(long)kz * Ny + row] = r2c_split<float>(zk, zmk, twP[kz]);
+    float2 v[E];
+    if ((Nz & 1) == 0) {
+        // pairs (2n, 2n+1) as float2: rows start at an even float offset
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int n = g + G * e;
+            float2 z = make_float2(0.f, 0.f);
+            if (ok && 2 * n < Nz) {
+                z = *reinterpret_cast<const float2*>(si.m + rbase + 2 * n);
+                for (int j = 0; j < nk; j++) {
+                    const float2 kv = *reinterpret_cast<const float2*>(si.k[j] + rbase + 2 * n);
+                    z.x = fmaf(coef[j], kv.x, z.x);
+                    z.y = fmaf(coef[j], kv.y, z.y);
+                }
+                if (si.mout) *reinterpret_cast<float2*>(si.mout + rbase + 2 * n) = z;
+            }
+            v[e] = z;
         }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int n = g + G * e;
+            float z2[2] = {0.f, 0.f};
+            for (int h = 0; h < 2; h++) {
+                const int i = 2 * n + h;
+                if (ok && i < Nz) {
+                    float a = si.m[rbase + i];
+                    for (int j = 0; j < nk; j++) a = fmaf(coef[j], si.k[j][rbase + i], a);
+                    if (si.mout) si.mout[rbase + i] = a;
+                    z2[h] = a;
+                }
+            }
+            v[e] = make_float2(z2[0], z2[1]);
+        }
+    }
+    reg_fft<float, M, -1>(v, g, lds + row * F::LDS, twM);
+    // r2c split: lane g holds Z[k], k = g + G e; Z[M-k] is in lane (G-g) % G at e' = E-1-e
+    // (or, for g = 0, in this lane at e' = (E-e) % E)
+    const int partner = (G - g) % G;
+    float2 X[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const float2 t = shfl2<G>(v[E - 1 - e], partner);
+        const float2 zmk = g == 0 ? v[(E - e) % E] : t;
+        X[e] = r2c_split<float>(v[e], zmk, twP[g + G * e]);
+    }
+    const float2 nyq = make_float2(v[0].x - v[0].y, 0.f);  // X[M], valid in lane 0
+    __syncthreads();  // the FFT exchange buffer becomes the transpose tile
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[(g + G * e) * TS + row] = X[e];
+    if (g == 0) lds[M * TS + row] = nyq;
+    __syncthreads();
+    float2* dst = B1 + ((long)c * Nx + x) * Q * Ny + y0;
+    for (int t = threadIdx.x; t < Q * TR; t += NT) {
+        const int r = t % TR, kz = t / TR;
+        if (r < nrows) dst[(long)kz * Ny + r] = lds[kz * TS + r];
     }
 }
 
-// Z inverse: B1[c][x][kz][y] -> rows of h
+// Z inverse: B1[c][x][kz][y0..y0+TR) -> rows of h.  The spectrum is read through
+// a padded LDS tile (TR consecutive y per kz), merged (c2r) in registers and
+// transformed back; each lane writes its (2n, 2n+1) pairs as float2.
 template <int M, int TR>
-__global__ __launch_bounds__(kNT) void k_zinv(const float2* __restrict__ B1, float* __restrict__ h,
-                                              const float2* __restrict__ twM, const float2* __restrict__ twP,
-                                              const EngineState* st, int Nx, int Ny, int Nz) {
-    constexpr int LS = M + 1, Q = M + 1;
-    constexpr int NY = (TR * M + kNT - 1) / kNT;
-    __shared__ float2 lds[TR * LS];
+__global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zinv(const float2* __restrict__ B1, float* __restrict__ h,
+                                                              const float2* __restrict__ twM,
+                                                              const float2* __restrict__ twP, const EngineState* st,
+                                                              int Nx, int Ny, int Nz) {
+    using F = RegFFT<M>;
+    constexpr int E = F::E, G = F::G, NT = TR * G, Q = M + 1, TS = TR + 1;
+    constexpr int FFT_LDS = TR * F::LDS, TILE_LDS = Q * TS;
+    __shared__ float2 lds[FFT_LDS > TILE_LDS ? FFT_LDS : TILE_LDS];
     if (st && !st->trial_active) return;
+    const int row = threadIdx.x / G, g = threadIdx.x % G;
     const int c = blockIdx.z, x = blockIdx.y, y0 = blockIdx.x * TR;
     const int nrows = min(TR, Ny - y0);
     const float2* src = B1 + ((long)c * Nx + x) * Q * Ny + y0;
-    batched_copy<float2, kNT>(
+    batched_copy<float2, NT>(
         Q * TR,
         [&](int t) {
-            const int row = t % TR, kz = t / TR;
-            return row < nrows ? src[(long)kz * Ny + row] : make_float2(0.f, 0.f);
+            const int r = t % TR, kz = t / TR;
+            return r < nrows ? src[(long)kz * Ny + r] : make_float2(0.f, 0.f);
         },
-        [&](int t, float2 v) { lds[(t % TR) * LS + t / TR] = v; });
+        [&](int t, float2 val) { lds[(t / TR) * TS + t % TR] = val; });
     __syncthreads();
-    float2 y[NY];
+    float2 v[E];
 #pragma unroll
-    for (int q = 0; q < NY; q++) {
-        const int t = threadIdx.x + q * kNT;
-        if (t < TR * M) {
-            const int row = t / M, k = t % M;
-            y[q] = c2r_merge<float>(lds[row * LS + k], lds[row * LS + (M - k)], twP[k]);
+    for (int e = 0; e < E; e++) {
+        const int k = g + G * e;
+        v[e] = c2r_merge<float>(lds[k * TS + row], lds[(M - k) * TS + row], twP[k]);
+    }
+    __syncthreads();  // the tile becomes the FFT exchange buffer
+    reg_fft<float, M, +1>(v, g, lds + row * F::LDS, twM);
+    if (row < nrows) {
+        const long N = (long)Nx * Ny * Nz;
+        float* dst = h + c * N + ((long)x * Ny + y0 + row) * Nz;
+        if ((Nz & 1) == 0) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int n = g + G * e;
+                if (2 * n < Nz) *reinterpret_cast<float2*>(dst + 2 * n) = v[e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int n = g + G * e;
+                if (2 * n < Nz) dst[2 * n] = v[e].x;
+                if (2 * n + 1 < Nz) dst[2 * n + 1] = v[e].y;
+            }
         }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NY; q++) {
-        const int t = threadIdx.x + q * kNT;
-        if (t < TR * M) lds[(t / M) * LS + (t % M)] = y[q];
-    }
-    __syncthreads();
-    fft_lds<float, M, +1, TR, kNT>(lds, LS, twM, M);
-    const long N = (long)Nx * Ny * Nz;
-    float* dst = h + c * N + ((long)x * Ny + y0) * Nz;
-    const int cnt = nrows * Nz;
-    for (int e = threadIdx.x; e < cnt; e += kNT) {
-        const int row = e / Nz, col = e - row * Nz;
-        dst[e] = reinterpret_cast<const float*>(lds + row * LS)[col];
     }
 }
 
-// 3-D Y forward: B1[c][x][kz][0..Ny) -> B2[c][kz][ky][x]
-template <int PY, int XC>
+// 3-D Y forward: one line (c, kz, x) per lane group, B1[c][x][kz][0..Ny) -> B2[c][kz][x][0..Py).
+// Lines are numbered with x fastest, so a block writes consecutive B2 lines.
+template <int PY>
 __global__ __launch_bounds__(kNT) void k_yfwd(const float2* __restrict__ B1, float2* __restrict__ B2,
                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int Ny,
                                               int Q) {
-    constexpr int LS = PY + 1;
-    __shared__ float2 lds[XC * LS];
+    using F = RegFFT<PY>;
+    constexpr int LPB = kNT / F::G;
+    __shared__ float2 lds[LPB * F::LDS];
     if (st && !st->trial_active) return;
-    const int c = blockIdx.z, x0 = blockIdx.y * XC, kz = blockIdx.x;
-    const int nx = min(XC, Nx - x0);
-    batched_copy<float2, kNT>(
-        nx * Ny,
-        [&](int t) {
-            const int xi = t / Ny, y = t - xi * Ny;
-            return B1[(((long)c * Nx + x0 + xi) * Q + kz) * Ny + y];
-        },
-        [&](int t, float2 v) {
-            const int xi = t / Ny, y = t - xi * Ny;
-            lds[xi * LS + y] = v;
-        });
-    __syncthreads();
-    fft_lds<float, PY, -1, XC, kNT>(lds, LS, tw, Ny);
-    float2* dst = B2 + (((long)c * Q + kz) * PY) * Nx + x0;
-    for (int t = threadIdx.x; t < PY * nx; t += kNT) {
-        const int xi = t % nx, ky = t / nx;
-        dst[(long)ky * Nx + xi] = lds[xi * LS + ky];
+    const int li = threadIdx.x / F::G, g = threadIdx.x % F::G;
+    const long line = (long)blockIdx.x * LPB + li;
+    const bool ok = line < 3L * Nx * Q;
+    const int x = (int)(line % Nx);
+    const long r = line / Nx;
+    const int kz = (int)(r % Q), c = (int)(r / Q);
+    const float2* src = B1 + (((long)c * Nx + x) * Q + kz) * Ny;
+    float2 v[F::E];
+#pragma unroll
+    for (int e = 0; e < F::E; e++) {
+        const int i = g + F::G * e;
+        v[e] = (ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
+    }
+    reg_fft<float, PY, -1>(v, g, lds + li * F::LDS, tw);
+    if (ok) {
+        float2* dst = B2 + (((long)c * Q + kz) * Nx + x) * PY;
+#pragma unroll
+        for (int e = 0; e < F::E; e++) dst[g + F::G * e] = v[e];
     }
 }
 
-// 3-D Y inverse: B2[c][kz][ky][x] -> B1[c][x][kz][0..Ny)
-template <int PY, int XC>
+// 3-D Y inverse: B2[c][kz][x][0..Py) -> B1[c][x][kz][0..Ny) (rows >= Ny are dropped)
+template <int PY>
 __global__ __launch_bounds__(kNT) void k_yinv(const float2* __restrict__ B2, float2* __restrict__ B1,
                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int Ny,
                                               int Q) {
-    constexpr int LS = PY + 1;
-    __shared__ float2 lds[XC * LS];
+    using F = RegFFT<PY>;
+    constexpr int LPB = kNT / F::G;
+    __shared__ float2 lds[LPB * F::LDS];
     if (st && !st->trial_active) return;
-    const int c = blockIdx.z, x0 = blockIdx.y * XC, kz = blockIdx.x;
-    const int nx = min(XC, Nx - x0);
-    const float2* src = B2 + (((long)c * Q + kz) * PY) * Nx + x0;
-    batched_copy<float2, kNT>(
-        PY * nx, [&](int t) { return src[(long)(t / nx) * Nx + (t % nx)]; },
-        [&](int t, float2 v) { lds[(t % nx) * LS + t / nx] = v; });
-    __syncthreads();
-    fft_lds<float, PY, +1, XC, kNT>(lds, LS, tw, PY);
-    for (int t = threadIdx.x; t < nx * Ny; t += kNT) {
-        const int xi = t / Ny, y = t - xi * Ny;
-        B1[(((long)c * Nx + x0 + xi) * Q + kz) * Ny + y] = lds[xi * LS + y];
+    const int li = threadIdx.x / F::G, g = threadIdx.x % F::G;
+    const long line = (long)blockIdx.x * LPB + li;
+    const bool ok = line < 3L * Nx * Q;
+    const int x = (int)(line % Nx);
+    const long r = line / Nx;
+    const int kz = (int)(r % Q), c = (int)(r / Q);
+    const float2* src = B2 + (((long)c * Q + kz) * Nx + x) * PY;
+    float2 v[F::E];
+#pragma unroll
+    for (int e = 0; e < F::E; e++) v[e] = ok ? src[g + F::G * e] : make_float2(0.f, 0.f);
+    reg_fft<float, PY, +1>(v, g, lds + li * F::LDS, tw);
+    if (ok) {
+        float2* dst = B1 + (((long)c * Nx + x) * Q + kz) * Ny;
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int i = g + F::G * e;
+            if (i < Ny) dst[i] = v[e];
+        }
     }
 }
 
-// 3-D X forward . kernel . X inverse on NP consecutive (kz,ky) pairs, all 3 components
-template <int PX, int NP>
+// 3-D X forward . kernel . X inverse.  One (kz, ky) column per group of G lanes,
+// all 3 components in registers (lane-strided x = g + G e): the loads and stores
+// touch G rows of consecutive ky per instruction, the 6-component symmetric
+// multiply is lane-local, and each transform needs a single LDS exchange.
+template <int PX>
 __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const float* __restrict__ Kc,
                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int PY,
                                               int Q) {
-    constexpr int LS = PX + 1;
-    __shared__ float2 lds[3 * NP * LS];
+    using F = RegFFT<PX>;
+    constexpr int LPB = kNT / F::G;
+    __shared__ float2 lds[LPB * F::LDS];
     if (st && !st->trial_active) return;
-    const long npairs = (long)Q * PY;
-    const long p0 = (long)blockIdx.x * NP;
-    const int np = (int)min((long)NP, npairs - p0);
-    const long cstride = npairs * Nx;
-    const int per = np * Nx;
-    batched_copy<float2, kNT>(
-        3 * per,
-        [&](int t) {
-            const int c = t / per, r = t - c * per;
-            return B2[c * cstride + p0 * Nx + r];
-        },
-        [&](int t, float2 v) {
-            const int c = t / per, r = t - c * per;
-            const int pi = r / Nx, x = r - pi * Nx;
-            lds[(c * NP + pi) * LS + x] = v;
-        });
-    __syncthreads();
-    fft_lds<float, PX, -1, 3 * NP, kNT>(lds, LS, tw, Nx);
-    const int hx = PX / 2 + 1, hy = PY / 2 + 1;
-    const long cs = (long)Q * hy * hx;
-    for (int t = threadIdx.x; t < np * PX; t += kNT) {
-        const int pi = t / PX, kx = t - pi * PX;
-        const long p = p0 + pi;
-        const int kz = (int)(p / PY), ky = (int)(p - (long)kz * PY);
-        const bool fx = kx > PX / 2, fy = ky > PY / 2;
-        const int kxp = fx ? PX - kx : kx, kyp = fy ? PY - ky : ky;
-        const long b = ((long)kz * hy + kyp) * hx + kxp;
-        const float kxx = Kc[b], kyy = Kc[cs + b], kzz = Kc[2 * cs + b];
-        float kyz = Kc[3 * cs + b], kxz = Kc[4 * cs + b], kxy = Kc[5 * cs + b];
-        if (fy) kyz = -kyz;
-        if (fx) kxz = -kxz;
-        if (fx != fy) kxy = -kxy;
-        float2* px = &lds[(0 * NP + pi) * LS + kx];
-        float2* py = &lds[(1 * NP + pi) * LS + kx];
-        float2* pz = &lds[(2 * NP + pi) * LS + kx];
-        const float2 a = *px, bb = *py, cc = *pz;
-        *px = make_float2(a.x * kxx + bb.x * kxy + cc.x * kxz, a.y * kxx + bb.y * kxy + cc.y * kxz);
-        *py = make_float2(a.x * kxy + bb.x * kyy + cc.x * kyz, a.y * kxy + bb.y * kyy + cc.y * kyz);
-        *pz = make_float2(a.x * kxz + bb.x * kyz + cc.x * kzz, a.y * kxz + bb.y * kyz + cc.y * kzz);
-    }
-    __syncthreads();
-    fft_lds<float, PX, +1, 3 * NP, kNT>(lds, LS, tw, PX);
-    for (int c = 0; c < 3; c++) {
-        float2* dst = B2 + c * cstride + p0 * Nx;
-        for (int t = threadIdx.x; t < np * Nx; t += kNT) {
-            const int pi = t / Nx, x = t - pi * Nx;
-            dst[t] = lds[(c * NP + pi) * LS + x];
+    const int li = threadIdx.x / F::G, g = threadIdx.x % F::G;
+    // lines numbered with ky fastest: consecutive lanes groups read consecutive ky
+    const int lpr = LPB < PY ? LPB : PY;  // lines per block row of ky
+    const int rows = (PY + lpr - 1) / lpr;
+    const int kz = blockIdx.x / rows;
+    const int ky = (blockIdx.x % rows) * lpr + li;
+    const bool ok = li < lpr && ky < PY && kz < Q;
+    const long cstride = (long)Q * Nx * PY;
+    float2* base = B2 + (long)kz * Nx * PY + ky;
+    float2 v[3][F::E];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int x = g + F::G * e;
+            v[c][e] = (ok && x < Nx) ? base[c * cstride + (long)x * PY] : make_float2(0.f, 0.f);
         }
+#pragma unroll
+    for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, tw);
+    if (ok) {
+        const int hx = PX / 2 + 1, hy = PY / 2 + 1;
+        const long cs = (long)Q * hy * hx;
+        const bool fy = ky > PY / 2;
+        const int kyp = fy ? PY - ky : ky;
+        const float* kb = Kc + ((long)kz * hy + kyp) * hx;
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int kx = g + F::G * e;
+            const bool fx = kx > PX / 2;
+            const int kxp = fx ? PX - kx : kx;
+            const float kxx = kb[kxp], kyy = kb[cs + kxp], kzz = kb[2 * cs + kxp];
+            float kyz = kb[3 * cs + kxp], kxz = kb[4 * cs + kxp], kxy = kb[5 * cs + kxp];
+            if (fy) kyz = -kyz;
+            if (fx) kxz = -kxz;
+            if (fx != fy) kxy = -kxy;
+            const float2 a = v[0][e], b = v[1][e], c = v[2][e];
+            v[0][e] = make_float2(a.x * kxx + b.x * kxy + c.x * kxz, a.y * kxx + b.y * kxy + c.y * kxz);
+            v[1][e] = make_float2(a.x * kxy + b.x * kyy + c.x * kyz, a.y * kxy + b.y * kyy + c.y * kyz);
+            v[2][e] = make_float2(a.x * kxz + b.x * kyz + c.x * kzz, a.y * kxz + b.y * kyz + c.y * kzz);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, tw);
+    if (ok) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int e = 0; e < F::E; e++) {
+                const int x = g + F::G * e;
+                if (x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
+            }
     }
 }
 
@@ -571,6 +640,15 @@ template <int L> constexpr int groups_per_block(int budget_lines_bytes) {
     return p;
 }
 
+// rows per Z block: enough consecutive y for 64-128 B transposed segments, at most 1024 threads
+template <int M> constexpr int z_rows() {
+    constexpr int G = RegFFT<M>::G;
+    int tr = 1024 / G;
+    if (tr > 16) tr = 16;
+    while (tr > 1 && tr * (M + 1) * 8 > 72 * 1024) tr /= 2;
+    return tr;
+}
+
 static void launch_zfwd(const ConvPlan& cp, const StageIn& si, hipStream_t s) {
     const int M = cp.M, Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
     const float2* twM = twiddle_f32(M);
@@ -578,9 +656,9 @@ static void launch_zfwd(const ConvPlan& cp, const StageIn& si, hipStream_t s) {
     switch (M) {
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
-        constexpr int TR = rows_per_block<n>();                                                 \
+        constexpr int TR = z_rows<n>();                                                         \
         dim3 g(div_up(Ny, TR), Nx, 3);                                                          \
-        k_zfwd<n, TR><<<g, kNT, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz);                       \
+        k_zfwd<n, TR><<<g, TR * RegFFT<n>::G, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz);         \
         break;                                                                                  \
     }
         GK_POW2_1_2048(GK_CASE)
@@ -597,9 +675,9 @@ static void launch_zinv(const ConvPlan& cp, float* h, const EngineState* st, hip
     switch (M) {
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
-        constexpr int TR = rows_per_block<n>();                                                 \
+        constexpr int TR = z_rows<n>();                                                         \
         dim3 g(div_up(Ny, TR), Nx, 3);                                                          \
-        k_zinv<n, TR><<<g, kNT, 0, s>>>(cp.B1, h, twM, twP, st, Nx, Ny, Nz);                    \
+        k_zinv<n, TR><<<g, TR * RegFFT<n>::G, 0, s>>>(cp.B1, h, twM, twP, st, Nx, Ny, Nz);      \
         break;                                                                                  \
     }
         GK_POW2_1_2048(GK_CASE)
@@ -612,13 +690,14 @@ static void launch_zinv(const ConvPlan& cp, float* h, const EngineState* st, hip
 static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipStream_t s) {
     const int PY = cp.P[1], Nx = cp.N[0], Ny = cp.N[1], Q = cp.Q;
     const float2* tw = twiddle_f32(PY);
+    const long nlines = 3L * Nx * Q;
     switch (PY) {
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
-        constexpr int XC = rows_per_block<n>();                                                 \
-        dim3 g(Q, div_up(Nx, XC), 3);                                                           \
-        if (fwd) k_yfwd<n, XC><<<g, kNT, 0, s>>>(cp.B1, cp.B2, tw, st, Nx, Ny, Q);              \
-        else k_yinv<n, XC><<<g, kNT, 0, s>>>(cp.B2, cp.B1, tw, st, Nx, Ny, Q);                  \
+        constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
+        const int g = div_up(nlines, LPB);                                                      \
+        if (fwd) k_yfwd<n><<<g, kNT, 0, s>>>(cp.B1, cp.B2, tw, st, Nx, Ny, Q);                  \
+        else k_yinv<n><<<g, kNT, 0, s>>>(cp.B2, cp.B1, tw, st, Nx, Ny, Q);                      \
         break;                                                                                  \
     }
         GK_POW2_2_4096(GK_CASE)
@@ -631,12 +710,13 @@ static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipS
 static void launch_xmul(const ConvPlan& cp, const EngineState* st, hipStream_t s) {
     const int PX = cp.P[0], PY = cp.P[1], Nx = cp.N[0], Q = cp.Q;
     const float2* tw = twiddle_f32(PX);
-    const long npairs = (long)Q * PY;
     switch (PX) {
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
-        constexpr int NP = groups_per_block<n>(49152);                                          \
-        k_xmul<n, NP><<<div_up(npairs, NP), kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, Nx, PY, Q);      \
+        constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
+        const int lpr = LPB < PY ? LPB : PY;                                                    \
+        const int rows = (PY + lpr - 1) / lpr;                                                  \
+        k_xmul<n><<<Q * rows, kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, Nx, PY, Q);                    \
         break;                                                                                  \
     }
         GK_POW2_2_1024(GK_CASE)

@@ -231,6 +231,63 @@ __device__ __forceinline__ void fft_lds(C2<T>* lds, int ls, const C2<T>* tw, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register FFT: one line of length L is held by G consecutive lanes, E = L/G
+// elements per lane in the lane-strided layout v[e] = x[g + G e] (g = lane
+// within its group).  In that layout every butterfly of a Stockham pass finds
+// all its inputs in one lane, and the last pass writes its outputs back into
+// the same layout, so a whole transform needs only (passes - 1) exchanges
+// through LDS, and loading / storing the line from / to memory is coalesced.
+// `lds` is this line's exchange buffer (RegFFT<L>::LDS complex), `tw` the table of L twiddles.
+template <int L> struct RegFFT {
+    static constexpr int E = L <= 512 ? (L < 8 ? L : 8) : L / 64;  // elements per lane
+    static constexpr int G = L / E;                                // lanes per line
+    // exchange buffer per line: one float2 of padding per 8, so the stride-8
+    // scatter of a radix-8 pass and neighbouring lines spread over all 64 banks
+    static constexpr int LDS = L + L / 8;
+};
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
+
+template <typename T, int L, int DIR, int NS = 1>
+__device__ __forceinline__ void reg_fft(C2<T>* v, int g, C2<T>* lds, const C2<T>* tw) {
+    if constexpr (NS < L) {
+        constexpr int R = next_radix(L / NS);
+        constexpr int E = RegFFT<L>::E, G = RegFFT<L>::G;
+        static_assert(E % R == 0, "radix must divide the elements per lane");
+        constexpr int NB = E / R;  // butterflies per lane
+        constexpr bool last = NS * R == L;
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int j = g + G * q;
+            const int k = j % NS;
+            C2<T> b[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) b[r] = v[q + r * NB];
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) b[r] = cmul<T>(b[r], twid<T, DIR>(tw, r * k * (L / (NS * R))));
+            }
+            dft<T, R, DIR>(b);
+            if (last) {
+                // output index j + r L/R = g + G (q + r NB): back in the lane-strided slots
+#pragma unroll
+                for (int r = 0; r < R; r++) v[q + r * NB] = b[r];
+            } else {
+                const int base = (j / NS) * NS * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[lds_pad(base + r * NS)] = b[r];
+            }
+        }
+        if constexpr (!last) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = lds[lds_pad(g + G * e)];
+            __syncthreads();
+            reg_fft<T, L, DIR, NS * R>(v, g, lds, tw);
+        }
+    }
+}
+
 // Load phases issue kUnroll independent global loads per lane before any of
 // them is consumed, so a block keeps up to 8 x NT loads in flight instead of
 // waiting one memory latency per element (loop trip counts are runtime values,

@@ -116,11 +116,12 @@ def main():
 
     prof_ms = np.zeros(8)
     prof_n = np.zeros(8, np.int32)
-    lib.mm_sim_profile_steps(sim._ptr, args.profile_steps, prof_ms.ctypes.data, prof_n.ctypes.data)
+    if args.profile_steps > 0:
+        lib.mm_sim_profile_steps(sim._ptr, args.profile_steps, prof_ms.ctypes.data, prof_n.ctypes.data)
     avg = {CLASSES[i]: (prof_ms[i] / prof_n[i] if prof_n[i] else 0.0) for i in range(8)}
     per, b_conv, b_step = model_bytes(N, P, cfg["size"][0], cfg["size"][1], 6 if P[0] > 1 else 4)
     dominant = max(per, key=lambda k: prof_ms[CLASSES.index(k)])
-    ach = per[dominant] / (avg[dominant] / 1e3) / 1e9
+    ach = per[dominant] / (avg[dominant] / 1e3) / 1e9 if avg[dominant] > 0 else 0.0
     step_gbs = b_step / (ms_per_step / 1e3) / 1e9
 
     m = sim.m
