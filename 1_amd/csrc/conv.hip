@@ -446,6 +446,196 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zinv(const float2* __rest
     }
 }
 
+// Fused Z inverse + RK stage epilogue.  One block: TR rows (y0..y0+TR) of one x
+// plane for all 3 components (3 x TR lane groups).  The 3 spectra are read
+// through transposed LDS tiles, merged and transformed back in registers, and
+// the fields are exchanged through LDS so that every thread sees h of all 3
+// components for its cells.  The epilogue (EpiArgs) rebuilds the stage input
+// m_i from m and the earlier k_j (the same fmaf sequence as the forward pass's
+// stage build), evaluates the torque, and stores k_i (with the per-block max
+// |k_i|) or, for the last stage of a fixed-step tableau, the normalised new m.
+// h itself never leaves the chip.  NU bounds the number of earlier stages read
+// (the launcher points unused slots at m with zero coefficients, which leaves
+// every fmaf chain unchanged), so all loads are unconditional.
+template <int M, int TR, int NU, bool FIN>
+__global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void k_zinv_epi(const float2* __restrict__ B1, EpiArgs ea,
+                                                                      const float2* __restrict__ twM,
+                                                                      const float2* __restrict__ twP,
+                                                                      const EngineState* st, int Nx, int Ny, int Nz) {
+    using F = RegFFT<M>;
+    constexpr int E = F::E, G = F::G, NR = TR * G, NT = 3 * NR, Q = M + 1, TS = TR + 1;
+    constexpr int HS = M / 2 > 0 ? M / 2 : 1;  // float2 per h row (z pairs); Nz <= M
+    constexpr int TILE = 3 * Q * TS, FFTB = 3 * TR * F::LDS, HB = 3 * TR * HS;
+    constexpr int LN = TILE > FFTB ? (TILE > HB ? TILE : HB) : (FFTB > HB ? FFTB : HB);
+    __shared__ float2 lds[LN];
+    __shared__ float hext_s[3];
+    __shared__ float red[NT];
+    if (!st->trial_active) return;
+    const int c = threadIdx.x / NR, rem = threadIdx.x - c * NR, row = rem / G, g = rem % G;
+    const int x = blockIdx.y, y0 = blockIdx.x * TR;
+    const int nrows = min(TR, Ny - y0);
+    if (threadIdx.x == 0) stage_field(ea.t, st, hext_s);
+    const long cst = (long)Nx * Q * Ny;
+    const float2* src = B1 + (long)x * Q * Ny + y0;
+    batched_copy<float2, NT>(
+        3 * Q * TR,
+        [&](int t) {
+            const int cc = t / (Q * TR), u = t - cc * (Q * TR), kz = u / TR, r = u - kz * TR;
+            return r < nrows ? src[cc * cst + (long)kz * Ny + r] : make_float2(0.f, 0.f);
+        },
+        [&](int t, float2 val) {
+            const int cc = t / (Q * TR), u = t - cc * (Q * TR), kz = u / TR, r = u - kz * TR;
+            lds[(cc * Q + kz) * TS + r] = val;
+        });
+    __syncthreads();
+    float2 v[E];
+    {
+        const float2* tile = lds + c * Q * TS;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = g + G * e;
+            v[e] = c2r_merge<float>(tile[k * TS + row], tile[(M - k) * TS + row], twP[k]);
+        }
+    }
+    __syncthreads();  // the tiles become the FFT exchange buffers
+    reg_fft<float, M, +1>(v, g, lds + (c * TR + row) * F::LDS, twM);
+    __syncthreads();  // ... and then the h rows
+    const int nh = (Nz + 1) >> 1;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int n = g + G * e;
+        if (n < nh) lds[(c * TR + row) * HS + n] = v[e];
+    }
+    __syncthreads();
+
+    const float hext[3] = {hext_s[0], hext_s[1], hext_s[2]};
+    const float dt = st->dt_used;
+    float ca[NU > 0 ? NU : 1], cb[NU > 0 ? NU : 1];
+#pragma unroll
+    for (int j = 0; j < NU; j++) {
+        ca[j] = dt * ea.a[j];
+        cb[j] = dt * ea.b[j];
+    }
+    const float cbs = dt * ea.bs;
+    const long N = (long)Nx * Ny * Nz;
+    float lmax = 0.f;
+    // one cell from the accumulated stage input mi (and final sum hi) and the field h
+    auto finish = [&](const float mi[3], const float hi[3], const float h[3], float o[3]) {
+        float k[3];
+        llg_torque_cell(ea.t, hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
+        lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
+        if constexpr (FIN) {
+            float f[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) f[q] = fmaf(cbs, k[q], hi[q]);
+            const float sc = rsqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+#pragma unroll
+            for (int q = 0; q < 3; q++) o[q] = f[q] * sc;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) o[q] = k[q];
+        }
+    };
+    const long rowbase = ((long)x * Ny + y0) * Nz;
+    if ((Nz & 1) == 0) {
+        // cell pairs (2n, 2n+1) as float2
+        constexpr int IT = (TR * HS + NT - 1) / NT;
+        const int nzh = Nz >> 1, P = nrows * nzh;
+        float2 ai[IT][3], af[IT][3];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int p = threadIdx.x + it * NT;
+            if (p < P) {
+                const int r = p / nzh, n = p - r * nzh;
+                const long gi = rowbase + (long)r * Nz + 2 * n;
+#pragma unroll
+                for (int q = 0; q < 3; q++) ai[it][q] = af[it][q] = *reinterpret_cast<const float2*>(ea.m + q * N + gi);
+#pragma unroll
+                for (int j = 0; j < NU; j++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const float2 kv = *reinterpret_cast<const float2*>(ea.k[j] + q * N + gi);
+                        ai[it][q].x = fmaf(ca[j], kv.x, ai[it][q].x);
+                        ai[it][q].y = fmaf(ca[j], kv.y, ai[it][q].y);
+                        if constexpr (FIN) {
+                            af[it][q].x = fmaf(cb[j], kv.x, af[it][q].x);
+                            af[it][q].y = fmaf(cb[j], kv.y, af[it][q].y);
+                        }
+                    }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int p = threadIdx.x + it * NT;
+            if (p < P) {
+                const int r = p / nzh, n = p - r * nzh;
+                const long gi = rowbase + (long)r * Nz + 2 * n;
+                float2 hv[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) hv[q] = lds[(q * TR + r) * HS + n];
+                float mi[3], hi[3], h[3], o0[3], o1[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    mi[q] = ai[it][q].x;
+                    hi[q] = af[it][q].x;
+                    h[q] = hv[q].x;
+                }
+                finish(mi, hi, h, o0);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    mi[q] = ai[it][q].y;
+                    hi[q] = af[it][q].y;
+                    h[q] = hv[q].y;
+                }
+                finish(mi, hi, h, o1);
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    *reinterpret_cast<float2*>(ea.out + q * N + gi) = make_float2(o0[q], o1[q]);
+            }
+        }
+    } else {
+        const float* hf = reinterpret_cast<const float*>(lds);
+        const int P = nrows * Nz;
+        for (int p = threadIdx.x; p < P; p += NT) {
+            const int r = p / Nz, z = p - r * Nz;
+            const long gi = rowbase + (long)r * Nz + z;
+            float mi[3], hi[3], h[3], o[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                mi[q] = hi[q] = ea.m[q * N + gi];
+                h[q] = hf[(q * TR + r) * 2 * HS + z];
+#pragma unroll
+                for (int j = 0; j < NU; j++) {
+                    const float kv = ea.k[j][q * N + gi];
+                    mi[q] = fmaf(ca[j], kv, mi[q]);
+                    hi[q] = fmaf(cb[j], kv, hi[q]);
+                }
+            }
+            finish(mi, hi, h, o);
+#pragma unroll
+            for (int q = 0; q < 3; q++) ea.out[q * N + gi] = o[q];
+        }
+    }
+    if (ea.t.part) {
+        red[threadIdx.x] = lmax;
+        __syncthreads();
+        if constexpr (NT >= 64) {
+            if (threadIdx.x < 64) {
+                float vm = 0.f;
+                for (int i = threadIdx.x; i < NT; i += 64) vm = fmaxf(vm, red[i]);
+                for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+                if (threadIdx.x == 0) ea.t.part[blockIdx.y * gridDim.x + blockIdx.x] = vm;
+            }
+        } else {
+            if (threadIdx.x == 0) {
+                float vm = 0.f;
+                for (int i = 0; i < NT; i++) vm = fmaxf(vm, red[i]);
+                ea.t.part[blockIdx.y * gridDim.x + blockIdx.x] = vm;
+            }
+        }
+    }
+}
+
 // 3-D Y forward: one line (c, kz, x) per lane group, B1[c][x][kz][0..Ny) -> B2[c][kz][x][0..Py).
 // Lines are numbered with x fastest, so a block writes consecutive B2 lines.
 template <int PY>
@@ -687,6 +877,67 @@ static void launch_zinv(const ConvPlan& cp, float* h, const EngineState* st, hip
     GK_LAUNCH_CHECK();
 }
 
+// rows per block of the fused Z inverse: 3 x TR lane groups <= 1024 threads, the
+// 3 transposed tiles of (M+1) x (TR+1) complex within 96 KB of LDS
+template <int M> constexpr int epi_rows() {
+    constexpr int G = RegFFT<M>::G;
+    int tr = 1024 / (3 * G);
+    if (tr > 16) tr = 16;
+    int p = 1;
+    while (p * 2 <= tr) p *= 2;
+    tr = p;
+    while (tr > 1 && 3 * (M + 1) * (tr + 1) * 8 > 96 * 1024) tr /= 2;
+    return tr;
+}
+
+static int epi_rows_rt(int M) {
+    switch (M) {
+#define GK_CASE(n) \
+    case n: return epi_rows<n>();
+        GK_POW2_1_2048(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+}
+
+int conv_epi_blocks(const ConvPlan& cp) { return div_up(cp.N[1], epi_rows_rt(cp.M)) * cp.N[0]; }
+
+static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineState* st, hipStream_t s) {
+    const int M = cp.M, Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
+    GK_ASSERT(st != nullptr && ea.nu >= 0 && ea.nu <= 6 && (!ea.fin || ea.nu <= 3), "fused Z inverse: bad epilogue");
+    // pad the list of earlier stages to the kernel's bound with (m, 0, 0): fmaf(0, m, z) == z
+    const int nu = ea.nu;
+    EpiArgs e = ea;
+    for (int j = nu; j < 6; j++) {
+        e.k[j] = ea.m;
+        e.a[j] = 0.f;
+        e.b[j] = 0.f;
+    }
+    const float2* twM = twiddle_f32(M);
+    const float2* twP = twiddle_f32(cp.P[2]);
+    switch (M) {
+#define GK_CASE(n)                                                                               \
+    case n: {                                                                                    \
+        constexpr int TR = epi_rows<n>();                                                        \
+        const dim3 g(div_up(Ny, TR), Nx), b(3 * TR * RegFFT<n>::G);                              \
+        if (e.fin) {                                                                             \
+            if (nu <= 1) k_zinv_epi<n, TR, 1, true><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else k_zinv_epi<n, TR, 3, true><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);         \
+        } else {                                                                                 \
+            if (nu == 0) k_zinv_epi<n, TR, 0, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else if (nu == 1) k_zinv_epi<n, TR, 1, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else if (nu <= 3) k_zinv_epi<n, TR, 3, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else k_zinv_epi<n, TR, 6, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);            \
+        }                                                                                        \
+        break;                                                                                   \
+    }
+        GK_POW2_1_2048(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+
 static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipStream_t s) {
     const int PY = cp.P[1], Nx = cp.N[0], Ny = cp.N[1], Q = cp.Q;
     const float2* tw = twiddle_f32(PY);
@@ -775,7 +1026,7 @@ Prof::~Prof() {
     for (auto e : pool) (void)hipEventDestroy(e);
 }
 
-void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof) {
+void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof, const EpiArgs* epi) {
     Prof dummy;
     Prof& p = prof ? *prof : dummy;
     p.begin(0, s);
@@ -797,7 +1048,10 @@ void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, 
         p.end(s);
     }
     p.begin(4, s);
-    launch_zinv(cp, h, in.st, s);
+    if (epi)
+        launch_zinv_epi(cp, *epi, in.st, s);
+    else
+        launch_zinv(cp, h, in.st, s);
     p.end(s);
 }
 

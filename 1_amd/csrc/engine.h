@@ -105,7 +105,8 @@ double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool 
 // real-space demag (+exchange) kernel over the padded grid P, 6 x P floats to the host
 void conv_realspace_kernel(const int P[3], const double cell[3], int accuracy, bool exch, float* out_host);
 // Optional per-kernel-class event timing (bench/profiling).  Classes:
-// 0 Z fwd, 1 Y fwd, 2 X.K.X (3-D) or Y.K.Y (2-D), 3 Y inv, 4 Z inv, 5 torque, 6 RK final, 7 control/copy.
+// 0 Z fwd, 1 Y fwd, 2 X.K.X (3-D) or Y.K.Y (2-D), 3 Y inv, 4 Z inv (+ torque epilogue),
+// 5 separate torque, 6 separate RK final (adaptive tableaux), 7 control/copy.
 struct Prof {
     static constexpr int kClasses = 8;
     bool on = false;
@@ -118,8 +119,14 @@ struct Prof {
     void reset() { marks.clear(); used = 0; }
     ~Prof();
 };
+struct EpiArgs;
 // h = K * m (planar 3N each); in = stage build description, out = 3N floats.
-void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof = nullptr);
+// With epi != null the Z inverse pass ends in the torque / final-combination
+// epilogue described by *epi instead of writing h.
+void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof = nullptr,
+                const EpiArgs* epi = nullptr);
+// number of blocks (= per-block partial maxima) of the fused Z inverse + torque pass
+int conv_epi_blocks(const ConvPlan& cp);
 
 // ---------------------------------------------------------------- RK kernels
 struct TorqueArgs {
@@ -138,6 +145,51 @@ struct TorqueArgs {
 };
 // k = torque(m_i, H + local fields (+ exchange)); in place over hk.
 void launch_torque(const float* mi, float* hk, EngineState* st, const TorqueArgs& a, int N, hipStream_t s);
+
+// Applied field in internal units at the stage node: t = time0 + float64(c_i dt)
+// (rk.go:316), B(t) in tesla / B_unit (field.go:50-61).
+__device__ __forceinline__ void stage_field(const TorqueArgs& a, const EngineState* st, float hext[3]) {
+    const double t = st->time0 + (double)(a.c * st->dt_used);
+    float b[3];
+    applied_field(a.field, t * a.unitTime, b);
+    for (int i = 0; i < 3; i++) hext[i] = b[i] / a.unitField;
+}
+
+// LL torque of one cell: h = H + B_ext + uniaxial anisotropy of m
+// (gpu_local_contr3.cu), k = dtg (m x h - alpha m x (m x h)) (gpu_torque.cu:20-50).
+__device__ __forceinline__ void llg_torque_cell(const TorqueArgs& a, const float hext[3], float Mx, float My,
+                                                float Mz, float Hx, float Hy, float Hz, float k[3]) {
+    Hx += hext[0];
+    Hy += hext[1];
+    Hz += hext[2];
+    if (a.anisType == 1) {
+        const float mu = Mx * a.u[0] + My * a.u[1] + Mz * a.u[2];
+        Hx += a.kAnis * mu * a.u[0];
+        Hy += a.kAnis * mu * a.u[1];
+        Hz += a.kAnis * mu * a.u[2];
+    }
+    const float cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx;
+    const float dx = My * cz - Mz * cy, dy = Mz * cx - Mx * cz, dz = Mx * cy - My * cx;
+    k[0] = a.dtg * (cx - a.alpha * dx);
+    k[1] = a.dtg * (cy - a.alpha * dy);
+    k[2] = a.dtg * (cz - a.alpha * dz);
+}
+
+// Epilogue of the fused Z inverse pass (one RK stage):
+//   m_i = m + dt sum_j a_j k_j               (the stage input, rebuilt from m and the k_j)
+//   k_i = torque(m_i, h)                     -> out (with per-block max |k_i| if t.part)
+// or, when fin (last stage of a fixed-step tableau, rk.go:381-390 + the final normalise),
+//   m <- normalise(m + dt (sum_j b_j k_j + bs k_i))   (out == m, k_i is not stored)
+struct EpiArgs {
+    const float* m;
+    float* out;
+    const float* k[6];
+    float a[6], b[6];
+    float bs;
+    int nu;
+    int fin;
+    TorqueArgs t;
+};
 
 struct FinalArgs {
     int S;
@@ -166,6 +218,7 @@ struct CtrlParams {
     const float* part_k0;   // per-block max |k0| (stage 0 torque)
     const float* part_k6;   // per-block max |k_last| (FSAL tableau)
     const float* part_err;  // per-block error (adaptive)
+    int n_k, n_err;         // entries of part_k0/part_k6 and of part_err
     int fold_k0;            // ctrl_end also records max |k0| (no separate after-k0 control)
 };
 void launch_ctrl_begin(EngineState* st, hipStream_t s);

@@ -33,35 +33,19 @@ __device__ __forceinline__ float block_max_nonneg(float v) {
 // k = dtg (m x h - alpha m x (m x h)) with h = H_conv + B_ext(t_i) + anisotropy (+ exchange)
 __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, float* __restrict__ hk,
                                                 EngineState* st, TorqueArgs a, int N) {
-    __shared__ float hext[3];
+    __shared__ float hext_s[3];
     if (!st->trial_active) return;
-    if (threadIdx.x == 0) {
-        // stage time: time0 + float64(c_i * dt) (rk.go:316), field in tesla -> internal (field.go:50-61)
-        const double t = st->time0 + (double)(a.c * st->dt_used);
-        float b[3];
-        applied_field(a.field, t * a.unitTime, b);
-        for (int i = 0; i < 3; i++) hext[i] = b[i] / a.unitField;
-    }
+    if (threadIdx.x == 0) stage_field(a, st, hext_s);
     __syncthreads();
-    const float ex = hext[0], ey = hext[1], ez = hext[2];
+    const float hext[3] = {hext_s[0], hext_s[1], hext_s[2]};
     float lmax = 0.f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const float Mx = m[i], My = m[N + i], Mz = m[2 * N + i];
-        float Hx = hk[i] + ex, Hy = hk[N + i] + ey, Hz = hk[2 * N + i] + ez;
-        if (a.anisType == 1) {
-            const float mu = Mx * a.u[0] + My * a.u[1] + Mz * a.u[2];
-            Hx += a.kAnis * mu * a.u[0];
-            Hy += a.kAnis * mu * a.u[1];
-            Hz += a.kAnis * mu * a.u[2];
-        }
-        const float cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx;
-        const float dx = My * cz - Mz * cy, dy = Mz * cx - Mx * cz, dz = Mx * cy - My * cx;
-        const float kx = a.dtg * (cx - a.alpha * dx), ky = a.dtg * (cy - a.alpha * dy),
-                    kz = a.dtg * (cz - a.alpha * dz);
-        hk[i] = kx;
-        hk[N + i] = ky;
-        hk[2 * N + i] = kz;
-        lmax = fmaxf(lmax, fmaxf(fabsf(kx), fmaxf(fabsf(ky), fabsf(kz))));
+        float k[3];
+        llg_torque_cell(a, hext, m[i], m[N + i], m[2 * N + i], hk[i], hk[N + i], hk[2 * N + i], k);
+        hk[i] = k[0];
+        hk[N + i] = k[1];
+        hk[2 * N + i] = k[2];
+        lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
     }
     if (a.part) {
         const float bm = block_max_nonneg(lmax);
@@ -162,10 +146,10 @@ __global__ void k_ctrl_begin(EngineState* st) {
     st->dt_used = st->dt;
 }
 
-// one wave reduces kRedBlocks per-block maxima; result valid in every lane
-__device__ __forceinline__ float wave_max_partials(const float* part) {
+// one wave reduces n per-block maxima; result valid in every lane
+__device__ __forceinline__ float wave_max_partials(const float* part, int n) {
     float v = 0.f;
-    for (int i = threadIdx.x; i < kRedBlocks; i += 64) v = fmaxf(v, part[i]);
+    for (int i = threadIdx.x; i < n; i += 64) v = fmaxf(v, part[i]);
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
     return v;
 }
@@ -173,7 +157,7 @@ __device__ __forceinline__ float wave_max_partials(const float* part) {
 // after k0 is known (rk.go:348-373); launched with one wave
 __global__ void k_ctrl_after_k0(EngineState* st, int fsal_copy, CtrlParams p) {
     if (!st->trial_active) return;
-    const float maxTorque = wave_max_partials(fsal_copy ? p.part_k6 : p.part_k0);
+    const float maxTorque = wave_max_partials(fsal_copy ? p.part_k6 : p.part_k0, p.n_k);
     if (threadIdx.x != 0) return;
     if (!p.adaptive) st->dt = p.inputDt;
     st->max_torque = maxTorque;
@@ -191,8 +175,8 @@ __global__ void k_ctrl_after_k0(EngineState* st, int fsal_copy, CtrlParams p) {
 // after the high/low solutions (rk.go:376-440)
 __global__ void k_ctrl_end(EngineState* st, CtrlParams p) {
     if (!st->trial_active) return;
-    const float err_red = p.adaptive ? wave_max_partials(p.part_err) : 0.f;
-    const float k0_red = p.fold_k0 ? wave_max_partials(p.part_k0) : 0.f;
+    const float err_red = p.adaptive ? wave_max_partials(p.part_err, p.n_err) : 0.f;
+    const float k0_red = p.fold_k0 ? wave_max_partials(p.part_k0, p.n_k) : 0.f;
     if (threadIdx.x != 0) return;
     if (p.fold_k0) {
         st->max_torque = k0_red;

@@ -202,7 +202,9 @@ struct mmSim {
     int* log_acc = nullptr;
     int log_cap = 1 << 16;
     double* dsum = nullptr;
-    float* part = nullptr;  // 3 x kRedBlocks: |k0|, |k_last|, error
+    float* part = nullptr;  // 3 x npart: |k0|, |k_last|, error
+    int npart = 0;          // max(kRedBlocks, blocks of the fused Z inverse + torque pass)
+    int nk_part = 0;        // partial maxima written per torque evaluation
     double* fT = nullptr;
     float* fB = nullptr;
     FieldTable field{};
@@ -271,8 +273,10 @@ static bool needs_after_k0(const mmSim* s) {
 static CtrlParams ctrl_params(const mmSim* s) {
     CtrlParams c{};
     c.part_k0 = s->part;
-    c.part_k6 = s->part + kRedBlocks;
-    c.part_err = s->part + 2 * kRedBlocks;
+    c.part_k6 = s->part + s->npart;
+    c.part_err = s->part + 2 * s->npart;
+    c.n_k = s->nk_part;
+    c.n_err = kRedBlocks;
     c.fold_k0 = !needs_after_k0(s);
     c.adaptive = s->tb.adaptive;
     c.maxError = (float)s->p.maxError;
@@ -328,13 +332,26 @@ static void enqueue_trial(mmSim* s) {
                 si.a[si.nk] = t.a[i][j];
                 si.nk++;
             }
-        si.mout = si.nk ? s->mi : nullptr;
-        conv_apply(s->cp, si, s->k[i], s->s, &pf);
-        pf.begin(5, s->s);
+        // the torque runs in the epilogue of the Z inverse pass, which rebuilds m_i
+        // from m and the k_j; the last stage of a fixed-step tableau also forms
+        // and normalises the new m there (rk.go:381-390)
+        const bool fin = !t.adaptive && i == t.S - 1;
+        EpiArgs ea{};
+        ea.m = s->m;
+        for (int j = 0; j < i; j++)
+            if (t.a[i][j] != 0.f || (fin && t.b[j] != 0.f)) {
+                ea.k[ea.nu] = s->k[j];
+                ea.a[ea.nu] = t.a[i][j];
+                ea.b[ea.nu] = fin ? t.b[j] : 0.f;
+                ea.nu++;
+            }
+        ea.fin = fin;
+        ea.bs = fin ? t.b[i] : 0.f;
+        ea.out = fin ? s->m : s->k[i];
         // max|k| is needed for k0 (rk.go:353) and, with FSAL, for the last stage (next k0)
-        float* part = i == 0 ? s->part : (t.fsal && i == t.S - 1) ? s->part + kRedBlocks : nullptr;
-        launch_torque(si.nk ? s->mi : s->m, s->k[i], s->st, torque_args(s, t.c[i], part), s->N, s->s);
-        pf.end(s->s);
+        float* part = i == 0 ? s->part : (t.fsal && i == t.S - 1) ? s->part + s->npart : nullptr;
+        ea.t = torque_args(s, t.c[i], part);
+        conv_apply(s->cp, si, nullptr, s->s, &pf, &ea);
         if (i == 0 && needs_after_k0(s)) {
             pf.begin(7, s->s);
             launch_ctrl_after_k0(s->st, 0, cp, s->s);
@@ -348,12 +365,14 @@ static void enqueue_trial(mmSim* s) {
         fa.b2[i] = t.b2[i];
     }
     fa.adaptive = t.adaptive;
-    fa.part_err = s->part + 2 * kRedBlocks;
+    fa.part_err = s->part + 2 * s->npart;
     const float* kk[7];
     for (int i = 0; i < 7; i++) kk[i] = s->k[i] ? s->k[i] : s->k[0];
-    pf.begin(6, s->s);
-    launch_rk_final(s->m, kk, s->mtmp, s->st, fa, s->N, s->s);
-    pf.end(s->s);
+    if (t.adaptive) {
+        pf.begin(6, s->s);
+        launch_rk_final(s->m, kk, s->mtmp, s->st, fa, s->N, s->s);
+        pf.end(s->s);
+    }
     pf.begin(7, s->s);
     launch_ctrl_end(s->st, cp, s->s);
     if (t.adaptive) launch_commit(s->m, s->mtmp, s->st, s->N, s->s);
@@ -581,8 +600,6 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->log_dt = (double*)dmalloc(s, s->log_cap * sizeof(double));
     s->log_acc = (int*)dmalloc(s, s->log_cap * sizeof(int));
     s->dsum = (double*)dmalloc(s, 4 * sizeof(double));
-    s->part = (float*)dmalloc(s, 3 * kRedBlocks * sizeof(float));
-    GK_CHECK(hipMemset(s->part, 0, 3 * kRedBlocks * sizeof(float)));
     EngineState h{};
     h.dt = (float)(p->dt / s->unitTime);  // init.go:280-284
     if (h.dt == 0.f) h.dt = 1e-4f;
@@ -610,6 +627,10 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     if (s->kernelImagRatio > 1e-4)
         fprintf(stderr, "Warning: FFT Kernel max imaginary part ratio = %g\n", s->kernelImagRatio);
     s->conv = true;
+    s->nk_part = conv_epi_blocks(s->cp);
+    s->npart = s->nk_part > kRedBlocks ? s->nk_part : kRedBlocks;
+    s->part = (float*)dmalloc(s, 3 * (size_t)s->npart * sizeof(float));
+    GK_CHECK(hipMemset(s->part, 0, 3 * (size_t)s->npart * sizeof(float)));
     if (mode == MM_MODE_ABI) abi_init(s, s->cell);
     GK_CHECK(hipDeviceSynchronize());
     return s;

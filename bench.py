@@ -25,12 +25,15 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-CLASSES = ["z_fwd_r2c", "y_fwd", "x_fwd_kmul_x_inv", "y_inv", "z_inv_c2r", "torque", "rk_final", "control"]
+CLASSES = ["z_fwd_r2c", "y_fwd", "x_fwd_kmul_x_inv", "y_inv", "z_inv_c2r_torque", "torque", "rk_final", "control"]
 
 
 def model_bytes(N, P, Nx, Ny, nK):
     """Algorithmic bytes of one field evaluation, per kernel class, and of one
-    RK4 step (B_step = 4 B_conv + 12 N 27)."""
+    RK4 step (B_step = 4 B_conv + 12 N 27).  The pointwise RK work (12 N 27 per
+    step: stage builds, torques, final combination) runs in the epilogue of the
+    Z inverse pass, so that class carries a quarter of it per evaluation and
+    the classes of the 4 evaluations add up to B_step."""
     Q = P[2] // 2 + 1
     A1 = Nx * Ny * Q
     A2 = Nx * P[1] * Q
@@ -40,8 +43,7 @@ def model_bytes(N, P, Nx, Ny, nK):
         "y_fwd": 3 * (8 * A1 + 8 * A2),
         "x_fwd_kmul_x_inv": 3 * 16 * A2 + 4 * nK * A3,
         "y_inv": 3 * (8 * A2 + 8 * A1),
-        "z_inv_c2r": 3 * (8 * A1 + 4 * N),
-        "torque": 12 * N * 3,
+        "z_inv_c2r_torque": 3 * (8 * A1 + 4 * N) + 12 * N * 27 // 4,
     }
     b_conv = 3 * (8 * N + 32 * A1 + 32 * A2) + 4 * nK * A3
     b_step = 4 * b_conv + 12 * N * 27
@@ -87,6 +89,19 @@ def cpu_baseline(cfg, seconds):
     }
 
 
+def calibrate_copy(lib, n=32 << 20, reps=3):
+    """A streaming kernel of known traffic for scaling the FETCH_SIZE /
+    WRITE_SIZE counters: gpu_linear_combination(a, b, 1, 2, n) reads a and b
+    and writes a, i.e. 8n bytes fetched and 4n bytes written per launch."""
+    a = lib.new_gpu_array(n)
+    b = lib.new_gpu_array(n)
+    for _ in range(reps):
+        lib.gpu_linear_combination(a, b, 1.0, 2.0, n)
+    lib.free_gpu_array(a)
+    lib.free_gpu_array(b)
+    return {"kernel": "gpu_linear_combination", "launches": reps, "fetch_bytes": 8 * n, "write_bytes": 4 * n}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +110,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also launch a streaming kernel of known traffic (for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs each simulation on one MI355X; runs on %d GPUs are not part of this build "
@@ -166,6 +183,8 @@ def main():
         "norm_max_dev": norm_err,
     }
     sim.free()
+    if args.calibrate:
+        out["calibration"] = calibrate_copy(lib)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(out), flush=True)
