@@ -19,7 +19,9 @@
 // The kernel spectrum is real and each component is even or odd along every
 // axis (Kxy odd in x and y, Kxz odd in x, Kyz odd in y, diagonal even), so
 // only kx <= Px/2, ky <= Py/2 is stored (Kc), a quarter of the reference's
-// 6 x Px x Py x (Pz/2+1) floats; signs are restored on the fly.
+// 6 x Px x Py x (Pz/2+1) floats; signs are restored on the fly.  3-D: Kc is
+// [kz][ky][kx][6] (the 6 components of one frequency are 24 contiguous bytes,
+// read as 3 float2 by the lane that needs them); 2-D: [4][kz][ky].
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -127,7 +129,7 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* a, double 
 
 // spectrum -> compressed kernel (and optionally the reference's full layout)
 __global__ void k_extract_kernel(const double2* __restrict__ W, int P0, int P1, int P2, int Q, double norm,
-                                 float* __restrict__ Kc_comp, float* __restrict__ full_comp,
+                                 float* __restrict__ Kc, int comp, int nK, float* __restrict__ full_comp,
                                  unsigned long long* maxre, unsigned long long* maxim) {
     const long total = (long)P0 * P1 * Q;
     const int hx = P0 / 2 + 1, hy = P1 / 2 + 1;
@@ -142,8 +144,8 @@ __global__ void k_extract_kernel(const double2* __restrict__ W, int P0, int P1, 
         lim = fmax(lim, fabs(im));
         if (full_comp) full_comp[t] = (float)re;
         if (kx < hx && ky < hy) {
-            if (P0 == 1) Kc_comp[(long)kz * hy + ky] = (float)re;
-            else Kc_comp[((long)kz * hy + ky) * hx + kx] = (float)re;
+            if (P0 == 1) Kc[((long)comp * Q + kz) * hy + ky] = (float)re;
+            else Kc[(((long)kz * hy + ky) * hx + kx) * nK + comp] = (float)re;
         }
     }
     atomic_max_nonneg(maxre, lre);
@@ -268,14 +270,13 @@ double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool 
     GK_CHECK(hipMalloc(&mx, 2 * sizeof(unsigned long long)));
     GK_CHECK(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned long long), s));
     const int hx = P0 / 2 + 1, hy = P1 / 2 + 1;
-    const long kcs = P0 == 1 ? (long)cp.Q * hy : (long)cp.Q * hy * hx;
     const long fulls = (long)P0 * P1 * cp.Q;
     const double norm = 1.0 / (double)NP;
     // components stored: 3-D xx yy zz yz xz xy; 2-D xx yy zz yz
     for (int k = 0; k < cp.nK; k++) {
         k_real_to_complex<<<grid1d(NP), kBlock, 0, s>>>(K + k * NP, W, NP);
         fft3_c2c_f64(W, cp.P, -1, s);
-        k_extract_kernel<<<grid1d(fulls), kBlock, 0, s>>>(W, P0, P1, P2, cp.Q, norm, cp.Kc + k * kcs,
+        k_extract_kernel<<<grid1d(fulls), kBlock, 0, s>>>(W, P0, P1, P2, cp.Q, norm, cp.Kc, k, cp.nK,
                                                             full_dev ? full_dev + k * fulls : nullptr, mx, mx + 1);
         GK_LAUNCH_CHECK();
     }
@@ -295,6 +296,16 @@ double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool 
 
 // =============================================================== passes
 constexpr int kNT = 256;
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// and each XCD has its own L2.  Renumber so that every XCD gets one contiguous
+// range of logical blocks: neighbouring blocks, which share cache lines (and
+// mirrored kernel rows), then meet in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int kXcd = 8;
+    if (nb % kXcd) return b;
+    return (b % kXcd) * (nb / kXcd) + b / kXcd;
+}
 
 template <int L> constexpr int rows_per_block() {
     int n = 71680 / ((L + 1) * 8);
@@ -341,7 +352,7 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zfwd(StageIn si, float2* 
         for (int e = 0; e < E; e++) {
             const int n = g + G * e;
             float2 z = make_float2(0.f, 0.f);
-            if (ok && 2 * n < Nz) {
+            if (!pad_half<M>(e) && ok && 2 * n < Nz) {
                 z = *reinterpret_cast<const float2*>(si.m + rbase + 2 * n);
                 for (int j = 0; j < nk; j++) {
                     const float2 kv = *reinterpret_cast<const float2*>(si.k[j] + rbase + 2 * n);
@@ -359,7 +370,7 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zfwd(StageIn si, float2* 
             float z2[2] = {0.f, 0.f};
             for (int h = 0; h < 2; h++) {
                 const int i = 2 * n + h;
-                if (ok && i < Nz) {
+                if (!pad_half<M>(e) && ok && i < Nz) {
                     float a = si.m[rbase + i];
                     for (int j = 0; j < nk; j++) a = fmaf(coef[j], si.k[j][rbase + i], a);
                     if (si.mout) si.mout[rbase + i] = a;
@@ -433,14 +444,14 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zinv(const float2* __rest
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int n = g + G * e;
-                if (2 * n < Nz) *reinterpret_cast<float2*>(dst + 2 * n) = v[e];
+                if (!pad_half<M>(e) && 2 * n < Nz) *reinterpret_cast<float2*>(dst + 2 * n) = v[e];
             }
         } else {
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int n = g + G * e;
-                if (2 * n < Nz) dst[2 * n] = v[e].x;
-                if (2 * n + 1 < Nz) dst[2 * n + 1] = v[e].y;
+                if (!pad_half<M>(e) && 2 * n < Nz) dst[2 * n] = v[e].x;
+                if (!pad_half<M>(e) && 2 * n + 1 < Nz) dst[2 * n + 1] = v[e].y;
             }
         }
     }
@@ -472,7 +483,8 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void k_zinv_epi(const float2
     __shared__ float red[NT];
     if (!st->trial_active) return;
     const int c = threadIdx.x / NR, rem = threadIdx.x - c * NR, row = rem / G, g = rem % G;
-    const int x = blockIdx.y, y0 = blockIdx.x * TR;
+    const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int x = lb / gridDim.x, y0 = (lb % gridDim.x) * TR;
     const int nrows = min(TR, Ny - y0);
     if (threadIdx.x == 0) stage_field(ea.t, st, hext_s);
     const long cst = (long)Nx * Q * Ny;
@@ -504,7 +516,7 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void k_zinv_epi(const float2
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int n = g + G * e;
-        if (n < nh) lds[(c * TR + row) * HS + n] = v[e];
+        if (!pad_half<M>(e) && n < nh) lds[(c * TR + row) * HS + n] = v[e];
     }
     __syncthreads();
 
@@ -657,7 +669,7 @@ __global__ __launch_bounds__(kNT) void k_yfwd(const float2* __restrict__ B1, flo
 #pragma unroll
     for (int e = 0; e < F::E; e++) {
         const int i = g + F::G * e;
-        v[e] = (ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
+        v[e] = (!pad_half<PY>(e) && ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
     }
     reg_fft<float, PY, -1>(v, g, lds + li * F::LDS, tw);
     if (ok) {
@@ -692,7 +704,7 @@ __global__ __launch_bounds__(kNT) void k_yinv(const float2* __restrict__ B2, flo
 #pragma unroll
         for (int e = 0; e < F::E; e++) {
             const int i = g + F::G * e;
-            if (i < Ny) dst[i] = v[e];
+            if (!pad_half<PY>(e) && i < Ny) dst[i] = v[e];
         }
     }
 }
@@ -713,8 +725,9 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
     // lines numbered with ky fastest: consecutive lanes groups read consecutive ky
     const int lpr = LPB < PY ? LPB : PY;  // lines per block row of ky
     const int rows = (PY + lpr - 1) / lpr;
-    const int kz = blockIdx.x / rows;
-    const int ky = (blockIdx.x % rows) * lpr + li;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int kz = b / rows;
+    const int ky = (b % rows) * lpr + li;
     const bool ok = li < lpr && ky < PY && kz < Q;
     const long cstride = (long)Q * Nx * PY;
     float2* base = B2 + (long)kz * Nx * PY + ky;
@@ -724,23 +737,24 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
 #pragma unroll
         for (int e = 0; e < F::E; e++) {
             const int x = g + F::G * e;
-            v[c][e] = (ok && x < Nx) ? base[c * cstride + (long)x * PY] : make_float2(0.f, 0.f);
+            v[c][e] = (!pad_half<PX>(e) && ok && x < Nx) ? base[c * cstride + (long)x * PY] : make_float2(0.f, 0.f);
         }
 #pragma unroll
     for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, tw);
     if (ok) {
         const int hx = PX / 2 + 1, hy = PY / 2 + 1;
-        const long cs = (long)Q * hy * hx;
         const bool fy = ky > PY / 2;
         const int kyp = fy ? PY - ky : ky;
-        const float* kb = Kc + ((long)kz * hy + kyp) * hx;
+        const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
 #pragma unroll
         for (int e = 0; e < F::E; e++) {
             const int kx = g + F::G * e;
             const bool fx = kx > PX / 2;
             const int kxp = fx ? PX - kx : kx;
-            const float kxx = kb[kxp], kyy = kb[cs + kxp], kzz = kb[2 * cs + kxp];
-            float kyz = kb[3 * cs + kxp], kxz = kb[4 * cs + kxp], kxy = kb[5 * cs + kxp];
+            const float2* kk = reinterpret_cast<const float2*>(kb + kxp * 6);
+            const float2 k01 = kk[0], k23 = kk[1], k45 = kk[2];
+            const float kxx = k01.x, kyy = k01.y, kzz = k23.x;
+            float kyz = k23.y, kxz = k45.x, kxy = k45.y;
             if (fy) kyz = -kyz;
             if (fx) kxz = -kxz;
             if (fx != fy) kxy = -kxy;
@@ -758,7 +772,7 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
 #pragma unroll
             for (int e = 0; e < F::E; e++) {
                 const int x = g + F::G * e;
-                if (x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
+                if (!pad_half<PX>(e) && x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
             }
     }
 }

@@ -248,6 +248,13 @@ template <int L> struct RegFFT {
 };
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
 
+// A padded length L >= 2 satisfies L >= 2n - 1 and is even, so only indices
+// < L/2 can hold data (forward inputs) or be kept (inverse outputs).  In the
+// lane-strided layout (index g + G e) these are exactly the elements e < E/2;
+// the others are compile-time zeros / dead, and the compiler prunes the first
+// and last passes accordingly.
+template <int L> __device__ constexpr bool pad_half(int e) { return RegFFT<L>::E >= 2 && e >= RegFFT<L>::E / 2; }
+
 template <typename T, int L, int DIR, int NS = 1>
 __device__ __forceinline__ void reg_fft(C2<T>* v, int g, C2<T>* lds, const C2<T>* tw) {
     if constexpr (NS < L) {
