@@ -648,63 +648,84 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void k_zinv_epi(const float2
     }
 }
 
-// 3-D Y forward: one line (c, kz, x) per lane group, B1[c][x][kz][0..Ny) -> B2[c][kz][x][0..Py).
-// Lines are numbered with x fastest, so a block writes consecutive B2 lines.
-template <int PY>
-__global__ __launch_bounds__(kNT) void k_yfwd(const float2* __restrict__ B1, float2* __restrict__ B2,
-                                              const float2* __restrict__ tw, const EngineState* st, int Nx, int Ny,
-                                              int Q) {
+// 3-D Y passes, software-pipelined.  One line per lane group and iteration:
+//   forward  B1[c][x][kz][0..Ny) -> B2[c][kz][x][0..Py)   (zero padded)
+//   inverse  B2[c][kz][x][0..Py) -> B1[c][x][kz][0..Ny)   (rows >= Ny dropped)
+// Lines are numbered with x fastest (a block touches consecutive B2 lines).
+// Each lane group walks `nit` lines strided by the grid and loads line i+1
+// before transforming line i, so the memory of one line overlaps the FFT of
+// the previous one; twiddles come from LDS, so no global load sits between a
+// prefetch and its use.
+template <int PY, bool FWD>
+__global__ __launch_bounds__(kNT) void k_ypass(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int Ny,
+                                               int Q, int nit) {
     using F = RegFFT<PY>;
-    constexpr int LPB = kNT / F::G;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    constexpr bool TWL = PY <= 1024;  // twiddle table in LDS (longer lines keep it in global memory)
     __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[TWL ? PY : 1];
     if (st && !st->trial_active) return;
-    const int li = threadIdx.x / F::G, g = threadIdx.x % F::G;
-    const long line = (long)blockIdx.x * LPB + li;
-    const bool ok = line < 3L * Nx * Q;
-    const int x = (int)(line % Nx);
-    const long r = line / Nx;
-    const int kz = (int)(r % Q), c = (int)(r / Q);
-    const float2* src = B1 + (((long)c * Nx + x) * Q + kz) * Ny;
-    float2 v[F::E];
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const long nlines = 3L * Nx * Q, stride = (long)gridDim.x * LPB;
+    long line = (long)blockIdx.x * LPB + li;
+    // B1 line (c, x, kz) and B2 line (c, kz, x) of a line number (x fastest)
+    auto b1 = [&](long ln) {
+        const int x = (int)(ln % Nx);
+        const long r = ln / Nx;
+        const int kz = (int)(r % Q), c = (int)(r / Q);
+        return (((long)c * Nx + x) * Q + kz) * Ny;
+    };
+    auto b2 = [&](long ln) {
+        const int x = (int)(ln % Nx);
+        const long r = ln / Nx;
+        const int kz = (int)(r % Q), c = (int)(r / Q);
+        return (((long)c * Q + kz) * Nx + x) * PY;
+    };
+    auto load = [&](float2* v, long ln) {
+        const bool ok = ln < nlines;
+        if constexpr (FWD) {
+            const float2* p = src + b1(ok ? ln : 0);
 #pragma unroll
-    for (int e = 0; e < F::E; e++) {
-        const int i = g + F::G * e;
-        v[e] = (!pad_half<PY>(e) && ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
-    }
-    reg_fft<float, PY, -1>(v, g, lds + li * F::LDS, tw);
-    if (ok) {
-        float2* dst = B2 + (((long)c * Q + kz) * Nx + x) * PY;
+            for (int e = 0; e < E; e++) {
+                const int i = g + G * e;
+                v[e] = (!pad_half<PY>(e) && ok && i < Ny) ? p[i] : make_float2(0.f, 0.f);
+            }
+        } else {
+            const float2* p = src + b2(ok ? ln : 0);
 #pragma unroll
-        for (int e = 0; e < F::E; e++) dst[g + F::G * e] = v[e];
-    }
-}
-
-// 3-D Y inverse: B2[c][kz][x][0..Py) -> B1[c][x][kz][0..Ny) (rows >= Ny are dropped)
-template <int PY>
-__global__ __launch_bounds__(kNT) void k_yinv(const float2* __restrict__ B2, float2* __restrict__ B1,
-                                              const float2* __restrict__ tw, const EngineState* st, int Nx, int Ny,
-                                              int Q) {
-    using F = RegFFT<PY>;
-    constexpr int LPB = kNT / F::G;
-    __shared__ float2 lds[LPB * F::LDS];
-    if (st && !st->trial_active) return;
-    const int li = threadIdx.x / F::G, g = threadIdx.x % F::G;
-    const long line = (long)blockIdx.x * LPB + li;
-    const bool ok = line < 3L * Nx * Q;
-    const int x = (int)(line % Nx);
-    const long r = line / Nx;
-    const int kz = (int)(r % Q), c = (int)(r / Q);
-    const float2* src = B2 + (((long)c * Q + kz) * Nx + x) * PY;
-    float2 v[F::E];
+            for (int e = 0; e < E; e++) v[e] = ok ? p[g + G * e] : make_float2(0.f, 0.f);
+        }
+    };
+    constexpr bool PIPE = E <= 16;  // registers for a second line (longer lines load in place)
+    float2 nxt[PIPE ? E : 1];
+    if constexpr (PIPE) load(nxt, line);
+    __syncthreads();  // twiddle table
+    for (int it = 0; it < nit; it++, line += stride) {
+        float2 v[E];
+        if constexpr (PIPE) {
 #pragma unroll
-    for (int e = 0; e < F::E; e++) v[e] = ok ? src[g + F::G * e] : make_float2(0.f, 0.f);
-    reg_fft<float, PY, +1>(v, g, lds + li * F::LDS, tw);
-    if (ok) {
-        float2* dst = B1 + (((long)c * Nx + x) * Q + kz) * Ny;
+            for (int e = 0; e < E; e++) v[e] = nxt[e];
+            if (it + 1 < nit) load(nxt, line + stride);
+        } else {
+            load(v, line);
+        }
+        reg_fft<float, PY, FWD ? -1 : +1>(v, g, lds + li * F::LDS, TWL ? twl : tw);
+        if (line < nlines) {
+            if constexpr (FWD) {
+                float2* p = dst + b2(line);
 #pragma unroll
-        for (int e = 0; e < F::E; e++) {
-            const int i = g + F::G * e;
-            if (!pad_half<PY>(e) && i < Ny) dst[i] = v[e];
+                for (int e = 0; e < E; e++) p[g + G * e] = v[e];
+            } else {
+                float2* p = dst + b1(line);
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int i = g + G * e;
+                    if (!pad_half<PY>(e) && i < Ny) p[i] = v[e];
+                }
+            }
         }
     }
 }
@@ -952,6 +973,31 @@ static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineS
     GK_LAUNCH_CHECK();
 }
 
+// Blocks per CU of the pipelined Y passes (each walks several lines).
+#ifndef GK_Y_BLOCKS_PER_CU
+#define GK_Y_BLOCKS_PER_CU 8
+#endif
+constexpr int kYBlocksPerCU = GK_Y_BLOCKS_PER_CU;
+
+static int cu_count() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        GK_CHECK(hipGetDevice(&dev));
+        GK_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    return n;
+}
+
+// Split `items` units of work (lines per block row) over at most `blocks` blocks
+// so that every block runs the same number of iterations: returns the iteration
+// count and sets `grid`.
+static int pipeline_split(long items, long blocks, int* grid) {
+    const long nit = std::max(1L, (items + blocks - 1) / blocks);
+    *grid = (int)((items + nit - 1) / nit);
+    return (int)nit;
+}
+
 static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipStream_t s) {
     const int PY = cp.P[1], Nx = cp.N[0], Ny = cp.N[1], Q = cp.Q;
     const float2* tw = twiddle_f32(PY);
@@ -960,9 +1006,10 @@ static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipS
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
         constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
-        const int g = div_up(nlines, LPB);                                                      \
-        if (fwd) k_yfwd<n><<<g, kNT, 0, s>>>(cp.B1, cp.B2, tw, st, Nx, Ny, Q);                  \
-        else k_yinv<n><<<g, kNT, 0, s>>>(cp.B2, cp.B1, tw, st, Nx, Ny, Q);                      \
+        int g = 0;                                                                              \
+        const int nit = pipeline_split(div_up(nlines, LPB), (long)kYBlocksPerCU * cu_count(), &g); \
+        if (fwd) k_ypass<n, true><<<g, kNT, 0, s>>>(cp.B1, cp.B2, tw, st, Nx, Ny, Q, nit);      \
+        else k_ypass<n, false><<<g, kNT, 0, s>>>(cp.B2, cp.B1, tw, st, Nx, Ny, Q, nit);          \
         break;                                                                                  \
     }
         GK_POW2_2_4096(GK_CASE)
