@@ -730,71 +730,100 @@ __global__ __launch_bounds__(kNT) void k_ypass(const float2* __restrict__ src, f
     }
 }
 
-// 3-D X forward . kernel . X inverse.  One (kz, ky) column per group of G lanes,
-// all 3 components in registers (lane-strided x = g + G e): the loads and stores
-// touch G rows of consecutive ky per instruction, the 6-component symmetric
-// multiply is lane-local, and each transform needs a single LDS exchange.
+// 3-D X forward . kernel . X inverse, software-pipelined.  One (kz, ky) column
+// per group of G lanes, all 3 components in registers (lane-strided x = g + G e):
+// the loads and stores touch G rows of consecutive ky per instruction, the
+// 6-component symmetric multiply is lane-local, and each transform needs one
+// LDS exchange per radix pass.  A block item is one kz and a run of lpr
+// consecutive ky; a block walks `nit` consecutive items, loading the columns of
+// the next item before transforming the current one.  Items are ordered so that
+// the runs ky and Py - ky (which read the same kernel rows) follow each other.
 template <int PX>
 __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const float* __restrict__ Kc,
                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int PY,
-                                              int Q) {
+                                              int Q, int nit) {
     using F = RegFFT<PX>;
-    constexpr int LPB = kNT / F::G;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
     __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[PX];
     if (st && !st->trial_active) return;
-    const int li = threadIdx.x / F::G, g = threadIdx.x % F::G;
-    // lines numbered with ky fastest: consecutive lanes groups read consecutive ky
-    const int lpr = LPB < PY ? LPB : PY;  // lines per block row of ky
-    const int rows = (PY + lpr - 1) / lpr;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int kz = b / rows;
-    const int ky = (b % rows) * lpr + li;
-    const bool ok = li < lpr && ky < PY && kz < Q;
+    for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const int lpr = LPB < PY ? LPB : PY;  // columns per item
+    const int rows = PY / lpr;            // items per kz plane
+    const long items = (long)Q * rows;
     const long cstride = (long)Q * Nx * PY;
-    float2* base = B2 + (long)kz * Nx * PY + ky;
-    float2 v[3][F::E];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int e = 0; e < F::E; e++) {
-            const int x = g + F::G * e;
-            v[c][e] = (!pad_half<PX>(e) && ok && x < Nx) ? base[c * cstride + (long)x * PY] : make_float2(0.f, 0.f);
-        }
-#pragma unroll
-    for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, tw);
-    if (ok) {
-        const int hx = PX / 2 + 1, hy = PY / 2 + 1;
-        const bool fy = ky > PY / 2;
-        const int kyp = fy ? PY - ky : ky;
-        const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
-#pragma unroll
-        for (int e = 0; e < F::E; e++) {
-            const int kx = g + F::G * e;
-            const bool fx = kx > PX / 2;
-            const int kxp = fx ? PX - kx : kx;
-            const float2* kk = reinterpret_cast<const float2*>(kb + kxp * 6);
-            const float2 k01 = kk[0], k23 = kk[1], k45 = kk[2];
-            const float kxx = k01.x, kyy = k01.y, kzz = k23.x;
-            float kyz = k23.y, kxz = k45.x, kxy = k45.y;
-            if (fy) kyz = -kyz;
-            if (fx) kxz = -kxz;
-            if (fx != fy) kxy = -kxy;
-            const float2 a = v[0][e], b = v[1][e], c = v[2][e];
-            v[0][e] = make_float2(a.x * kxx + b.x * kxy + c.x * kxz, a.y * kxx + b.y * kxy + c.y * kxz);
-            v[1][e] = make_float2(a.x * kxy + b.x * kyy + c.x * kyz, a.y * kxy + b.y * kyy + c.y * kyz);
-            v[2][e] = make_float2(a.x * kxz + b.x * kyz + c.x * kzz, a.y * kxz + b.y * kyz + c.y * kzz);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, tw);
-    if (ok) {
+    const int hx = PX / 2 + 1, hy = PY / 2 + 1;
+    // column of item t for this lane group (ky < 0: no column)
+    auto column = [&](long t, int& kz, int& ky) {
+        kz = (int)(t / rows);
+        const int rr = (int)(t % rows);
+        const int r = (rr & 1) ? rows - 1 - (rr >> 1) : (rr >> 1);
+        ky = (t < items && li < lpr) ? r * lpr + li : -1;
+    };
+    auto load = [&](float2 (*v)[E], long t) {
+        int kz, ky;
+        column(t, kz, ky);
+        const float2* base = B2 + (long)kz * Nx * PY + (ky < 0 ? 0 : ky);
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
-            for (int e = 0; e < F::E; e++) {
-                const int x = g + F::G * e;
-                if (!pad_half<PX>(e) && x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
+            for (int e = 0; e < E; e++) {
+                const int x = g + G * e;
+                v[c][e] = (!pad_half<PX>(e) && ky >= 0 && x < Nx) ? base[c * cstride + (long)x * PY]
+                                                                   : make_float2(0.f, 0.f);
             }
+    };
+    const long t0 = (long)blockIdx.x * nit;
+    float2 nxt[3][E];
+    load(nxt, t0);
+    __syncthreads();  // twiddle table
+    for (int it = 0; it < nit; it++) {
+        const long t = t0 + it;
+        float2 v[3][E];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) v[c][e] = nxt[c][e];
+        if (it + 1 < nit) load(nxt, t + 1);
+        int kz, ky;
+        column(t, kz, ky);
+#pragma unroll
+        for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, twl);
+        if (ky >= 0) {
+            const bool fy = ky > PY / 2;
+            const int kyp = fy ? PY - ky : ky;
+            const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int kx = g + G * e;
+                const bool fx = kx > PX / 2;
+                const int kxp = fx ? PX - kx : kx;
+                const float2* kk = reinterpret_cast<const float2*>(kb + kxp * 6);
+                const float2 k01 = kk[0], k23 = kk[1], k45 = kk[2];
+                const float kxx = k01.x, kyy = k01.y, kzz = k23.x;
+                float kyz = k23.y, kxz = k45.x, kxy = k45.y;
+                if (fy) kyz = -kyz;
+                if (fx) kxz = -kxz;
+                if (fx != fy) kxy = -kxy;
+                const float2 a = v[0][e], b = v[1][e], c = v[2][e];
+                v[0][e] = make_float2(a.x * kxx + b.x * kxy + c.x * kxz, a.y * kxx + b.y * kxy + c.y * kxz);
+                v[1][e] = make_float2(a.x * kxy + b.x * kyy + c.x * kyz, a.y * kxy + b.y * kyy + c.y * kyz);
+                v[2][e] = make_float2(a.x * kxz + b.x * kyz + c.x * kzz, a.y * kxz + b.y * kyz + c.y * kzz);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, twl);
+        if (ky >= 0) {
+            float2* base = B2 + (long)kz * Nx * PY + ky;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int x = g + G * e;
+                    if (!pad_half<PX>(e) && x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
+                }
+        }
     }
 }
 
@@ -978,6 +1007,11 @@ static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineS
 #define GK_Y_BLOCKS_PER_CU 8
 #endif
 constexpr int kYBlocksPerCU = GK_Y_BLOCKS_PER_CU;
+// ... and of the pipelined X . K . X pass
+#ifndef GK_X_BLOCKS_PER_CU
+#define GK_X_BLOCKS_PER_CU 4
+#endif
+constexpr int kXBlocksPerCU = GK_X_BLOCKS_PER_CU;
 
 static int cu_count() {
     static int n = 0;
@@ -1027,8 +1061,9 @@ static void launch_xmul(const ConvPlan& cp, const EngineState* st, hipStream_t s
     case n: {                                                                                   \
         constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
         const int lpr = LPB < PY ? LPB : PY;                                                    \
-        const int rows = (PY + lpr - 1) / lpr;                                                  \
-        k_xmul<n><<<Q * rows, kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, Nx, PY, Q);                    \
+        int g = 0;                                                                              \
+        const int nit = pipeline_split((long)Q * (PY / lpr), (long)kXBlocksPerCU * cu_count(), &g); \
+        k_xmul<n><<<g, kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, Nx, PY, Q, nit);                      \
         break;                                                                                  \
     }
         GK_POW2_2_1024(GK_CASE)
