@@ -1122,6 +1122,18 @@ Prof::~Prof() {
     for (auto e : pool) (void)hipEventDestroy(e);
 }
 
+void conv_run_pass(const ConvPlan& cp, int cls, const StageIn& in, float* h, const EpiArgs* epi, hipStream_t s) {
+    const bool d3 = cp.P[0] > 1;
+    switch (cls) {
+    case 0: launch_zfwd(cp, in, s); break;
+    case 1: GK_ASSERT(d3, "no Y forward pass in 2-D"); launch_y3d(cp, in.st, true, s); break;
+    case 2: if (d3) launch_xmul(cp, in.st, s); else launch_y2d(cp, in.st, s); break;
+    case 3: GK_ASSERT(d3, "no Y inverse pass in 2-D"); launch_y3d(cp, in.st, false, s); break;
+    case 4: if (epi) launch_zinv_epi(cp, *epi, in.st, s); else launch_zinv(cp, h, in.st, s); break;
+    default: GK_ASSERT(false, "pass class %d", cls);
+    }
+}
+
 void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof, const EpiArgs* epi) {
     Prof dummy;
     Prof& p = prof ? *prof : dummy;

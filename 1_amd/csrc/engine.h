@@ -127,6 +127,10 @@ void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, 
                 const EpiArgs* epi = nullptr);
 // number of blocks (= per-block partial maxima) of the fused Z inverse + torque pass
 int conv_epi_blocks(const ConvPlan& cp);
+// one pass of class cls (0 Z fwd, 1 Y fwd, 2 X.K.X / Y.K.Y, 3 Y inv, 4 Z inv), for timing
+void conv_run_pass(const ConvPlan& cp, int cls, const StageIn& in, float* h, const EpiArgs* epi, hipStream_t s);
+// st->trial_active = 1 (stage kernels do their work until the next trial start)
+void launch_set_active(EngineState* st, hipStream_t s);
 
 // ---------------------------------------------------------------- RK kernels
 struct TorqueArgs {

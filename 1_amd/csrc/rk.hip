@@ -276,6 +276,11 @@ void launch_energy(const float* m, const float* h, const float hext[3], int anis
     k_energy<<<gridN(N), 256, 0, s>>>(m, h, hext[0], hext[1], hext[2], anisType, kAnis, u[0], u[1], u[2], N, out);
     GK_LAUNCH_CHECK();
 }
+__global__ void k_set_active(EngineState* st) { st->trial_active = 1; }
+void launch_set_active(EngineState* st, hipStream_t s) {
+    k_set_active<<<1, 1, 0, s>>>(st);
+    GK_LAUNCH_CHECK();
+}
 void launch_set_target(EngineState* st, long long nsteps, double stop, hipStream_t s) {
     k_set_target<<<1, 1, 0, s>>>(st, nsteps, stop);
     GK_LAUNCH_CHECK();

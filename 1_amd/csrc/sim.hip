@@ -842,6 +842,30 @@ double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts) {
     return total;
 }
 
+double mm_sim_time_pass(mmSim* s, int cls, int n) {
+    GK_ASSERT(s->mode == MM_MODE_FUSED && n > 0 && cls >= 0 && cls <= 4, "time_pass: fused mode, class 0..4");
+    StageIn si{};
+    si.m = s->m;
+    si.st = s->st;
+    EpiArgs ea{};
+    ea.m = s->m;
+    ea.out = s->k[0];
+    ea.t = torque_args(s, s->tb.c[0], s->part);
+    launch_set_active(s->st, s->s);
+    hipEvent_t e0, e1;
+    GK_CHECK(hipEventCreate(&e0));
+    GK_CHECK(hipEventCreate(&e1));
+    GK_CHECK(hipEventRecord(e0, s->s));
+    for (int i = 0; i < n; i++) conv_run_pass(s->cp, cls, si, nullptr, &ea, s->s);
+    GK_CHECK(hipEventRecord(e1, s->s));
+    GK_CHECK(hipEventSynchronize(e1));
+    float total = 0.f;
+    GK_CHECK(hipEventElapsedTime(&total, e0, e1));
+    GK_CHECK(hipEventDestroy(e0));
+    GK_CHECK(hipEventDestroy(e1));
+    return total;
+}
+
 double mm_sim_time_steps(mmSim* s, int n) {
     GK_ASSERT(s->mode == MM_MODE_FUSED && !s->tb.adaptive, "timing needs the fused mode and a fixed-step solver");
     hipEvent_t e0, e1;

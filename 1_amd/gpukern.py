@@ -83,6 +83,7 @@ ENGINE = {
     "mm_sim_device_bytes": (C.c_ulonglong, [P]),
     "mm_sim_time_steps": (C.c_double, [P, I]),
     "mm_sim_profile_steps": (C.c_double, [P, I, P, P]),
+    "mm_sim_time_pass": (C.c_double, [P, I, I]),
 }
 
 

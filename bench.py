@@ -8,11 +8,16 @@ One "step" is one RK4 step (4 field evaluations) of the whole grid, with the
 state resident in HBM before the timed region starts.
 
 Prints ONE JSON line.  `value` is Mcell-steps/s (cells x steps / s) of the
-timed region, measured with HIP events on the engine's own stream; the
-`roofline` object reports the dominant kernel's algorithmic bytes per launch
-over its average launch time (events around each launch in a separate
-profiled pass); `cpu_baseline` is the FP64 CPU oracle timed on this host on a
-bounded sample of the same workload.
+timed region, measured with HIP events on the engine's own stream.  The
+dominant kernel class is the one with the largest total time in a profiled
+pass (events around every launch); the `roofline` object reports its
+algorithmic bytes per launch over its average launch time, measured as n
+back-to-back launches of that pass between two HIP events on the engine's
+stream.  `traffic` is the HBM bytes per launch of that kernel from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload (scaled
+by the copy-kernel calibration recorded there), or null if none matches.
+`cpu_baseline` is the FP64 CPU oracle timed on this host on a bounded sample
+of the same workload.
 """
 import argparse
 import ctypes as C
@@ -25,6 +30,24 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def committed_traffic(kernel_class, workload):
+    """HBM bytes per launch from profiles/*traffic*.md: a line
+    `traffic_bytes_per_launch <class> <workload> <bytes>`."""
+    d = os.path.join(HERE, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for fn in sorted(os.listdir(d)):
+        if "traffic" not in fn:
+            continue
+        with open(os.path.join(d, fn)) as f:
+            for line in f:
+                t = line.strip().strip("`").split()
+                if len(t) == 4 and t[0] == "traffic_bytes_per_launch" and t[1] == kernel_class and t[2] == workload:
+                    return int(t[3])
+    return None
 CLASSES = ["z_fwd_r2c", "y_fwd", "x_fwd_kmul_x_inv", "y_inv", "z_inv_c2r_torque", "torque", "rk_final", "control"]
 
 
@@ -108,6 +131,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--pass-launches", type=int, default=200,
+                    help="back-to-back launches of the dominant pass for its average duration")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -138,7 +163,9 @@ def main():
     avg = {CLASSES[i]: (prof_ms[i] / prof_n[i] if prof_n[i] else 0.0) for i in range(8)}
     per, b_conv, b_step = model_bytes(N, P, cfg["size"][0], cfg["size"][1], 6 if P[0] > 1 else 4)
     dominant = max(per, key=lambda k: prof_ms[CLASSES.index(k)])
-    ach = per[dominant] / (avg[dominant] / 1e3) / 1e9 if avg[dominant] > 0 else 0.0
+    dom_ms = lib.mm_sim_time_pass(sim._ptr, CLASSES.index(dominant), args.pass_launches) / args.pass_launches
+    ach = per[dominant] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = committed_traffic(dominant, cfg["name"])
     step_gbs = b_step / (ms_per_step / 1e3) / 1e9
 
     m = sim.m
@@ -175,9 +202,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": per[dominant],
-            "avg_launch_ms": round(avg[dominant], 5),
+            "avg_launch_ms": round(dom_ms, 5),
+            "launches_timed": args.pass_launches,
         },
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
         "norm_max_dev": norm_err,

@@ -108,6 +108,12 @@ unsigned long long mm_sim_device_bytes(mmSim* s);
  * counts in ms[8], counts[8]. */
 double mm_sim_time_steps(mmSim* s, int n);
 double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts);
+/* Fused mode.  n back-to-back launches of one convolution pass class (0..4 as
+ * above; the Z inverse with the stage-1 torque epilogue) on the engine's
+ * stream between two HIP events; returns the elapsed ms.  The passes work on
+ * the engine's scratch spectra and its first stage buffer, so the state m is
+ * unchanged; the next step recomputes everything they touch. */
+double mm_sim_time_pass(mmSim* s, int cls, int n);
 
 #ifdef __cplusplus
 }
