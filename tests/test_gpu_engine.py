@@ -129,39 +129,65 @@ def test_rk4_steps_vs_oracle(mm, orc, size, cell):
     assert st["time"] == pytest.approx(o.state()["time"], rel=1e-6)
 
 
-@pytest.mark.parametrize("mode", ["fused", "abi"])
-def test_dp_run_replayed_in_oracle(mm, orc, mode):
-    size, cell = (1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9)
-    cfg = mm.workloads.film_sp4_like(seed=1)
-    s = mm.workloads.make_sim(cfg, mode)
-    n = s.run(0.05e-9)
-    assert n > 5
+def replay_in_oracle(mm, orc, cfg, s):
+    """Replay the GPU's trial log (dt, accepted) step by step in the oracle `orc`
+    started from the same m0; returns the oracle sim."""
     dts, acc = s.trial_log()
     o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
-    # Run() normalises first (simrun.go:24); replay the GPU's trials step by step
+    # Run() normalises first (simrun.go:24)
     m3 = o.m.reshape(3, -1)
     o.m = (m3 / np.linalg.norm(m3, axis=0)).reshape(-1)
     i = 0
     while i < len(dts):
         j = i
-        while not acc[j] and j - i < 9:
+        while not acc[j] and j - i < 9:  # a step: rejected trials, then the accepted one (MAX_STEP_TRIALS 10)
             j += 1
         o.step_replay(dts[i:j + 1], acc[i:j + 1])
         i = j + 1
+    return o
+
+
+@pytest.mark.parametrize("mode", ["fused", "abi"])
+def test_dp_run_replayed_in_oracle(mm, orc, mode):
+    cfg = mm.workloads.film_sp4_like(seed=1)
+    s = mm.workloads.make_sim(cfg, mode)
+    n = s.run(0.05e-9)
+    assert n > 5
+    o = replay_in_oracle(mm, orc, cfg, s)
     got = s.m
     ref = o.m.reshape(got.shape)
     assert np.abs(got - ref).max() < 1e-5
     assert s.state()["steps"] == o.state()["steps"]
 
 
-def test_unit_norm_and_damping_decreases_energy(mm):
-    size, cell = (4, 32, 32), (4e-9, 4e-9, 4e-9)
-    m0 = random_m(size, 4)
-    s = engine(mm, size, cell, solver="rk4", dt=1e-14, m0=m0, alpha=0.5, msat=1e6, aexch=1.5e-11)
-    s.steps(1)
-    e0 = s.energy()
-    s.steps(40)
-    e1 = s.energy()
-    m = s.m
-    assert np.abs(np.linalg.norm(m, axis=0) - 1).max() < 1e-6
-    assert e1 < e0
+def test_film_1ns_dp_inside_fp32_envelope_of_fp64_oracle(mm, orc):
+    """The 128x32x1 film (Dormand-Prince, maxError 1e-5, B = (-24.6, 4.3, 0) mT)
+    for the full 1 ns; the FP64 oracle and its FP32 twin (same algorithm,
+    float32 arithmetic) replay the GPU's accepted-dt schedule.
+
+    The film reverses during this nanosecond, and the reversal amplifies
+    rounding: the FP32 twin itself drifts from FP64 by ~3e-6 at 0.05 ns and
+    ~1e-2 at 1 ns, so no float32 engine meets 1e-5 after 1 ns on this
+    trajectory (the 0.05 ns replay test holds the 1e-5 bar).  Bar here: the
+    GPU's max |dm| and max |d<m>| against FP64 stay within twice the FP32
+    twin's (or 1e-5, whichever is larger)."""
+    from oracle import Oracle
+
+    cfg = mm.workloads.film_sp4_like(seed=1)
+    s = mm.workloads.make_sim(cfg, "fused")
+    s.run(1e-9)
+    st = s.state()
+    assert st["time"] * st["unit_time"] >= 1e-9 * (1 - 1e-6)  # internal time units
+    o64 = replay_in_oracle(mm, orc, cfg, s)
+    o32 = replay_in_oracle(mm, Oracle("f32"), cfg, s)
+    got = s.m.reshape(3, -1).astype(np.float64)
+    r64 = o64.m.reshape(3, -1)
+    r32 = o32.m.reshape(3, -1)
+    e_gpu, e_32 = float(np.abs(got - r64).max()), float(np.abs(r32 - r64).max())
+    a_gpu = float(np.abs(got.mean(axis=1) - r64.mean(axis=1)).max())
+    a_32 = float(np.abs(r32.mean(axis=1) - r64.mean(axis=1)).max())
+    print("film 1 ns: steps %d, rejects %d; vs FP64 oracle: GPU max|dm| %.3g, max|d<m>| %.3g; "
+          "FP32 twin max|dm| %.3g, max|d<m>| %.3g" % (st["steps"], st["rejects"], e_gpu, a_gpu, e_32, a_32))
+    assert st["steps"] == o64.state()["steps"] == o32.state()["steps"]
+    assert e_gpu <= max(1e-5, 2 * e_32)
+    assert a_gpu <= max(1e-5, 2 * a_32)
