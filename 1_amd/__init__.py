@@ -7,4 +7,4 @@ seeded workloads (``workloads``).
 """
 from . import gpukern, workloads  # noqa: F401
 from .gpukern import LIB_PATH, load  # noqa: F401
-from .sim import InputErr, Sim  # noqa: F401
+from .sim import Batch, InputErr, Sim  # noqa: F401

@@ -1,0 +1,681 @@
+// Batched engine: many independent simulations (same Runge-Kutta tableau, any
+// sizes and materials) advanced together, one launch per pass and length
+// class per RK stage for the whole batch.
+//
+// Each simulation keeps its own device state exactly as in the single-simulation
+// engine (sim.hip): m, k_i, spectra B1/B2, kernel spectrum, EngineState, trial
+// log.  A batched kernel covers every simulation whose padded length matches its
+// template: blocks are numbered per simulation and prefix tables map a block to
+// (simulation, local block), after which the kernel runs the same device body
+// as the single-simulation kernel (passes.h, rk_dev.h).  So a simulation gets
+// bit-identical arithmetic alone and in a batch; only its scheduling differs.
+//
+// Step control stays per simulation on the device (own dt, error, accept /
+// reject, FSAL state, stop time), so a batched trial is a fixed launch
+// sequence, captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "../../include/mumax_engine.h"
+#include "engine.h"
+#include "passes.h"
+#include "rk_dev.h"
+#include "sim.h"
+
+namespace gk {
+
+// one simulation of a batch, as the batched kernels see it (device table)
+struct BSim {
+    int Nx, Ny, Nz, Q, PY;
+    long N;
+    float* m;
+    float* k[7];
+    float* mtmp;
+    float2* B1;
+    float2* B2;
+    const float* Kc;
+    EngineState* st;
+    float* part;  // 3 x npart: |k0|, |k_last|, error
+    int npart;
+    int nfin;     // blocks of the RK final (error partials)
+    TorqueArgs t; // c and part are set per stage
+    CtrlParams cp;
+};
+
+// one RK stage, the same for every simulation of the batch
+struct BStage {
+    int i;
+    int fsal0;  // stage 0 of an FSAL tableau: simulations holding an FSAL k skip the convolution
+    int nk;
+    int kj[6];
+    float a[6];
+    int nu;
+    int uj[6];
+    float ua[6], ub[6];
+    float bs;
+    int fin;
+    float c;
+    int partslot;  // -1: none, 0: |k0|, 1: |k_last|
+};
+
+// largest j with start[j] <= b (start[0] = 0, increasing)
+__device__ __forceinline__ int bfind(const int* __restrict__ start, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// the simulation of this block and the block's index within it
+struct BBlock {
+    const BSim* d;
+    int lb;
+};
+__device__ __forceinline__ BBlock bblock(const BSim* sims, const int* idx, const int* start, int n) {
+    const int j = bfind(start, n, blockIdx.x);
+    return BBlock{sims + idx[j], (int)blockIdx.x - start[j]};
+}
+__device__ __forceinline__ bool bskip(const BSim& d, const BStage& sg) {
+    return !d.st->trial_active || (sg.fsal0 && d.st->fsal_init);
+}
+
+// ------------------------------------------------------------------ passes
+template <int M, int TR>
+__global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, const int* idx, const int* start,
+                                                               int n, BStage sg, const float2* __restrict__ twM,
+                                                               const float2* __restrict__ twP) {
+    __shared__ float2 lds[zfwd_lds<M, TR>()];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    const int nyb = (d.Ny + TR - 1) / TR;
+    const int yb = bb.lb % nyb, r = bb.lb / nyb, x = r % d.Nx, c = r / d.Nx;
+    StageIn si{};
+    si.m = d.m;
+    si.st = d.st;
+    si.nk = sg.nk;
+    for (int j = 0; j < 6; j++) {
+        si.k[j] = j < sg.nk ? d.k[sg.kj[j]] : d.m;
+        si.a[j] = sg.a[j];
+    }
+    zfwd_tile<M, TR>(si, d.B1, twM, twP, d.Nx, d.Ny, d.Nz, c, x, yb * TR, lds);
+}
+
+template <int PY, bool FWD>
+__global__ __launch_bounds__(kNT) void kb_ypass(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+                                                const float2* __restrict__ tw) {
+    using F = RegFFT<PY>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    constexpr bool TWL = PY <= 1024;
+    __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[TWL ? PY : 1];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const long nlines = 3L * d.Nx * d.Q;
+    const long line = (long)bb.lb * LPB + li;
+    float2 v[E];
+    ypass_load<PY, FWD>(v, FWD ? d.B1 : d.B2, line, line < nlines, d.Nx, d.Ny, d.Q, g);
+    __syncthreads();  // twiddle table
+    reg_fft<float, PY, FWD ? -1 : +1>(v, g, lds + li * F::LDS, TWL ? twl : tw);
+    if (line < nlines) ypass_store<PY, FWD>(v, FWD ? d.B2 : d.B1, line, d.Nx, d.Ny, d.Q, g);
+}
+
+template <int PX>
+__global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+                                               const float2* __restrict__ tw) {
+    using F = RegFFT<PX>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[PX];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const int lpr = LPB < d.PY ? LPB : d.PY;
+    const int rows = d.PY / lpr;
+    int kz, ky;
+    xmul_column(bb.lb, (long)d.Q * rows, rows, lpr, li, kz, ky);
+    float2 v[3][E];
+    xmul_load<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, twl);
+    if (ky >= 0) xmul_kernel<PX>(v, d.Kc, kz, ky, d.PY, g);
+#pragma unroll
+    for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, twl);
+    if (ky >= 0) xmul_store<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
+}
+
+template <int PY, int NKZ>
+__global__ __launch_bounds__(kNT) void kb_y2d(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+                                              const float2* __restrict__ tw) {
+    __shared__ float2 lds[3 * NKZ * (PY + 1)];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    y2d_tile<PY, NKZ>(d.B1, d.Kc, tw, d.Ny, d.Q, bb.lb * NKZ, lds);
+}
+
+template <int M, int TR, int NU, bool FIN>
+__global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim* sims, const int* idx,
+                                                                       const int* start, int n, BStage sg,
+                                                                       const float2* __restrict__ twM,
+                                                                       const float2* __restrict__ twP) {
+    __shared__ float2 lds[zinv_epi_lds<M, TR>()];
+    __shared__ float hext_s[3];
+    __shared__ float red[3 * TR * RegFFT<M>::G];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    const int nyb = (d.Ny + TR - 1) / TR;
+    const int yb = bb.lb % nyb, x = bb.lb / nyb;
+    EpiArgs ea{};
+    ea.m = d.m;
+    ea.out = FIN ? d.m : d.k[sg.i];
+    for (int j = 0; j < 6; j++) {
+        const bool u = j < sg.nu;
+        ea.k[j] = u ? d.k[sg.uj[j]] : d.m;  // unused slots: m with zero coefficients
+        ea.a[j] = u ? sg.ua[j] : 0.f;
+        ea.b[j] = u ? sg.ub[j] : 0.f;
+    }
+    ea.bs = sg.bs;
+    ea.nu = sg.nu;
+    ea.fin = sg.fin;
+    ea.t = d.t;
+    ea.t.c = sg.c;
+    ea.t.part = sg.partslot < 0 ? nullptr : d.part + (long)sg.partslot * d.npart;
+    zinv_epi_tile<M, TR, NU, FIN>(d.B1, ea, twM, twP, d.st, d.Nx, d.Ny, d.Nz, x, yb * TR, bb.lb, lds, hext_s, red);
+}
+
+// ------------------------------------------------------------------ RK pointwise and control
+// FSAL: k_0 <- k_{S-1} for simulations that hold one (rk.go:311-315)
+__global__ __launch_bounds__(256) void kb_fsal_copy(const BSim* sims, const int* idx, const int* start, int n,
+                                                    int last) {
+    const int j = bfind(start, n, blockIdx.x);
+    const BSim& d = sims[idx[j]];
+    if (!d.st->trial_active || !d.st->fsal_init) return;
+    copy_body(d.k[0], d.k[last], 3 * d.N, (int)blockIdx.x - start[j], start[j + 1] - start[j]);
+}
+
+__global__ __launch_bounds__(256) void kb_rk_final(const BSim* sims, const int* idx, const int* start, int n,
+                                                   FinalArgs fa) {
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (!d.st->trial_active) return;
+    KPtrs kp;
+    for (int s = 0; s < 7; s++) kp.k[s] = d.k[s] ? d.k[s] : d.k[0];
+    FinalArgs a = fa;
+    a.part_err = d.part + 2L * d.npart;
+    rk_final_body(d.m, kp, d.mtmp, d.st, a, (int)d.N, bb.lb, d.nfin);
+}
+
+__global__ __launch_bounds__(256) void kb_commit(const BSim* sims, const int* idx, const int* start, int n) {
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    commit_body(d.m, d.mtmp, d.st, (int)d.N, bb.lb, d.nfin);
+}
+
+// one thread per simulation
+__global__ void kb_ctrl_begin(const BSim* sims, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) ctrl_begin_body(sims[j].st);
+}
+// one wave per simulation
+__global__ __launch_bounds__(64) void kb_ctrl_after_k0(const BSim* sims, int fsal) {
+    const BSim& d = sims[blockIdx.x];
+    if (!d.st->trial_active || d.cp.fold_k0) return;
+    ctrl_after_k0_body(d.st, fsal && d.st->fsal_init, d.cp);
+}
+__global__ __launch_bounds__(64) void kb_ctrl_end(const BSim* sims) {
+    const BSim& d = sims[blockIdx.x];
+    ctrl_end_body(d.st, d.cp);
+}
+__global__ void kb_done(const BSim* sims, int n, int* done) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) done[j] = sims[j].st->done;
+}
+
+}  // namespace gk
+
+using namespace gk;
+
+// ------------------------------------------------------------------ host
+namespace {
+
+// blocks of one pass for one simulation, per padded length class
+struct LaunchSet {
+    int L = 0;       // template length of the class
+    int cnt = 0;     // simulations
+    int blocks = 0;  // total blocks
+    int* idx = nullptr;
+    int* start = nullptr;
+};
+
+int epi_rows_host(int M);
+int z_rows_host(int M);
+int y2d_nkz_host(int PY);
+
+}  // namespace
+
+struct mmBatch {
+    std::vector<mmSim*> sims;
+    Tableau tb;
+    hipStream_t s = nullptr;
+    BSim* dsims = nullptr;
+    int* ddone = nullptr;
+    std::vector<void*> owned;  // device allocations of the tables
+    // per pass: one LaunchSet per length class
+    std::vector<LaunchSet> zf, yf, xm, yi, y2, zi, fin, all256;
+    int useGraph = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long long trials = 0;
+};
+
+namespace {
+
+// rows per Z block (same choice as the single-simulation launchers)
+template <int M> constexpr int z_rows_c() {
+    constexpr int G = RegFFT<M>::G;
+    int tr = 1024 / G;
+    if (tr > 16) tr = 16;
+    while (tr > 1 && tr * (M + 1) * 8 > 72 * 1024) tr /= 2;
+    return tr;
+}
+template <int M> constexpr int epi_rows_c() {
+    constexpr int G = RegFFT<M>::G;
+    int tr = 1024 / (3 * G);
+    if (tr > 16) tr = 16;
+    int p = 1;
+    while (p * 2 <= tr) p *= 2;
+    tr = p;
+    while (tr > 1 && 3 * (M + 1) * (tr + 1) * 8 > 96 * 1024) tr /= 2;
+    return tr;
+}
+template <int L> constexpr int y2d_nkz_c() {
+    int n = 49152 / (3 * (L + 1) * 8);
+    n = n < 1 ? 1 : (n > 32 ? 32 : n);
+    int p = 1;
+    while (p * 2 <= n) p *= 2;
+    return p;
+}
+
+#define GK_B_POW2_1_2048(X) X(1) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+#define GK_B_POW2_2_4096(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+#define GK_B_POW2_2_1024(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+
+int z_rows_host(int M) {
+    switch (M) {
+#define GK_CASE(n) \
+    case n: return z_rows_c<n>();
+        GK_B_POW2_1_2048(GK_CASE)
+#undef GK_CASE
+    }
+    abort();
+}
+int epi_rows_host(int M) {
+    switch (M) {
+#define GK_CASE(n) \
+    case n: return epi_rows_c<n>();
+        GK_B_POW2_1_2048(GK_CASE)
+#undef GK_CASE
+    }
+    abort();
+}
+int y2d_nkz_host(int PY) {
+    switch (PY) {
+#define GK_CASE(n) \
+    case n: return y2d_nkz_c<n>();
+        GK_B_POW2_2_4096(GK_CASE)
+#undef GK_CASE
+    }
+    abort();
+}
+int lanes_host(int L) {  // RegFFT<L>::G
+    const int E = L <= 512 ? (L < 8 ? L : 8) : L / 64;
+    return L / E;
+}
+
+void* balloc(mmBatch* b, size_t bytes) {
+    void* p = nullptr;
+    GK_CHECK(hipMalloc(&p, bytes));
+    b->owned.push_back(p);
+    return p;
+}
+
+// group simulations by a length key; blocks(sim) gives each simulation's block count
+template <typename Key, typename Blocks>
+std::vector<LaunchSet> make_sets(mmBatch* b, const std::vector<int>& members, Key key, Blocks blocks) {
+    std::map<int, std::vector<int>> groups;
+    for (int j : members) groups[key(b->sims[j])].push_back(j);
+    std::vector<LaunchSet> out;
+    for (auto& kv : groups) {
+        LaunchSet ls;
+        ls.L = kv.first;
+        ls.cnt = (int)kv.second.size();
+        std::vector<int> st(ls.cnt + 1, 0);
+        for (int i = 0; i < ls.cnt; i++) st[i + 1] = st[i] + blocks(b->sims[kv.second[i]]);
+        ls.blocks = st[ls.cnt];
+        ls.idx = (int*)balloc(b, ls.cnt * sizeof(int));
+        ls.start = (int*)balloc(b, (ls.cnt + 1) * sizeof(int));
+        GK_CHECK(hipMemcpy(ls.idx, kv.second.data(), ls.cnt * sizeof(int), hipMemcpyHostToDevice));
+        GK_CHECK(hipMemcpy(ls.start, st.data(), (ls.cnt + 1) * sizeof(int), hipMemcpyHostToDevice));
+        if (ls.blocks > 0) out.push_back(ls);
+    }
+    return out;
+}
+
+int fin_blocks(const mmSim* s) { return std::max(1, std::min(kRedBlocks, div_up(s->N, 256))); }
+
+// ---------------------------------------------------------------- launchers (one length class each)
+void bl_zfwd(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    const float2* twM = twiddle_f32(ls.L);
+    const float2* twP = twiddle_f32(2 * ls.L);
+    switch (ls.L) {
+#define GK_CASE(n)                                                                                  \
+    case n:                                                                                         \
+        kb_zfwd<n, z_rows_c<n>()><<<ls.blocks, z_rows_c<n>() * RegFFT<n>::G, 0, b->s>>>(b->dsims, ls.idx, \
+                                                                                      ls.start, ls.cnt, sg, twM, twP); \
+        break;
+        GK_B_POW2_1_2048(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+void bl_ypass(mmBatch* b, const LaunchSet& ls, const BStage& sg, bool fwd) {
+    const float2* tw = twiddle_f32(ls.L);
+    switch (ls.L) {
+#define GK_CASE(n)                                                                                       \
+    case n:                                                                                              \
+        if (fwd) kb_ypass<n, true><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw);  \
+        else kb_ypass<n, false><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw);     \
+        break;
+        GK_B_POW2_2_4096(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+void bl_xmul(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    const float2* tw = twiddle_f32(ls.L);
+    switch (ls.L) {
+#define GK_CASE(n)                                                                            \
+    case n: kb_xmul<n><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw); break;
+        GK_B_POW2_2_1024(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+void bl_y2d(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    const float2* tw = twiddle_f32(ls.L);
+    switch (ls.L) {
+#define GK_CASE(n)                                                                                           \
+    case n: kb_y2d<n, y2d_nkz_c<n>()><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw); \
+        break;
+        GK_B_POW2_2_4096(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+void bl_zinv_epi(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    const float2* twM = twiddle_f32(ls.L);
+    const float2* twP = twiddle_f32(2 * ls.L);
+    switch (ls.L) {
+#define GK_CASE(n)                                                                                       \
+    case n: {                                                                                            \
+        constexpr int TR = epi_rows_c<n>();                                                              \
+        const int th = 3 * TR * RegFFT<n>::G;                                                            \
+        if (sg.fin) {                                                                                    \
+            if (sg.nu <= 1) kb_zinv_epi<n, TR, 1, true><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else kb_zinv_epi<n, TR, 3, true><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);         \
+        } else {                                                                                         \
+            if (sg.nu == 0) kb_zinv_epi<n, TR, 0, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else if (sg.nu == 1) kb_zinv_epi<n, TR, 1, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else if (sg.nu <= 3) kb_zinv_epi<n, TR, 3, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else kb_zinv_epi<n, TR, 6, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);            \
+        }                                                                                                \
+        break;                                                                                           \
+    }
+        GK_B_POW2_1_2048(GK_CASE)
+#undef GK_CASE
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- one batched trial
+void enqueue_btrial(mmBatch* b) {
+    const Tableau& t = b->tb;
+    const int n = (int)b->sims.size();
+    kb_ctrl_begin<<<div_up(n, 64), 64, 0, b->s>>>(b->dsims, n);
+    GK_LAUNCH_CHECK();
+    for (int i = 0; i < t.S; i++) {
+        BStage sg{};
+        sg.i = i;
+        sg.fsal0 = (i == 0 && t.fsal) ? 1 : 0;
+        if (sg.fsal0)
+            for (auto& ls : b->all256) {
+                kb_fsal_copy<<<ls.blocks, 256, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, t.S - 1);
+                GK_LAUNCH_CHECK();
+            }
+        for (int j = 0; j < t.S; j++)
+            if (t.a[i][j] != 0.f) {
+                sg.kj[sg.nk] = j;
+                sg.a[sg.nk] = t.a[i][j];
+                sg.nk++;
+            }
+        const bool fin = !t.adaptive && i == t.S - 1;
+        for (int j = 0; j < i; j++)
+            if (t.a[i][j] != 0.f || (fin && t.b[j] != 0.f)) {
+                sg.uj[sg.nu] = j;
+                sg.ua[sg.nu] = t.a[i][j];
+                sg.ub[sg.nu] = fin ? t.b[j] : 0.f;
+                sg.nu++;
+            }
+        sg.fin = fin;
+        sg.bs = fin ? t.b[i] : 0.f;
+        sg.c = t.c[i];
+        sg.partslot = i == 0 ? 0 : (t.fsal && i == t.S - 1) ? 1 : -1;
+        for (auto& ls : b->zf) bl_zfwd(b, ls, sg);
+        for (auto& ls : b->yf) bl_ypass(b, ls, sg, true);
+        for (auto& ls : b->xm) bl_xmul(b, ls, sg);
+        for (auto& ls : b->yi) bl_ypass(b, ls, sg, false);
+        for (auto& ls : b->y2) bl_y2d(b, ls, sg);
+        for (auto& ls : b->zi) bl_zinv_epi(b, ls, sg);
+        if (i == 0) {
+            kb_ctrl_after_k0<<<n, 64, 0, b->s>>>(b->dsims, t.fsal);
+            GK_LAUNCH_CHECK();
+        }
+    }
+    if (t.adaptive) {
+        FinalArgs fa{};
+        fa.S = t.S;
+        for (int i = 0; i < 7; i++) {
+            fa.b[i] = t.b[i];
+            fa.b2[i] = t.b2[i];
+        }
+        fa.adaptive = 1;
+        for (auto& ls : b->fin) {
+            kb_rk_final<<<ls.blocks, 256, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, fa);
+            GK_LAUNCH_CHECK();
+        }
+    }
+    kb_ctrl_end<<<n, 64, 0, b->s>>>(b->dsims);
+    GK_LAUNCH_CHECK();
+    if (t.adaptive)
+        for (auto& ls : b->fin) {
+            kb_commit<<<ls.blocks, 256, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt);
+            GK_LAUNCH_CHECK();
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+mmBatch* mm_batch_new(mmSim* const* sims, int n) {
+    GK_ASSERT(n > 0 && sims, "mm_batch_new: empty batch");
+    mmBatch* b = new mmBatch();
+    b->sims.assign(sims, sims + n);
+    b->tb = sims[0]->tb;
+    for (int j = 0; j < n; j++) {
+        const mmSim* s = sims[j];
+        GK_ASSERT(s->mode == MM_MODE_FUSED && s->conv, "mm_batch_new: simulation %d is not a fused-mode engine", j);
+        GK_ASSERT(s->p.solver == sims[0]->p.solver, "mm_batch_new: simulations must share the solver");
+    }
+    GK_CHECK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+    // simulation table
+    std::vector<BSim> h(n);
+    for (int j = 0; j < n; j++) {
+        mmSim* s = sims[j];
+        BSim& d = h[j];
+        d = BSim{};
+        d.Nx = s->cp.N[0];
+        d.Ny = s->cp.N[1];
+        d.Nz = s->cp.N[2];
+        d.Q = s->cp.Q;
+        d.PY = s->cp.P[1];
+        d.N = s->N;
+        d.m = s->m;
+        for (int i = 0; i < 7; i++) d.k[i] = s->k[i];
+        d.mtmp = s->mtmp;
+        d.B1 = s->cp.B1;
+        d.B2 = s->cp.B2;
+        d.Kc = s->cp.Kc;
+        d.st = s->st;
+        d.part = s->part;
+        d.npart = s->npart;
+        d.nfin = fin_blocks(s);
+        d.t = torque_args(s, 0.f, nullptr);
+        d.cp = ctrl_params(s);
+        d.cp.n_err = d.nfin;
+    }
+    b->dsims = (BSim*)balloc(b, n * sizeof(BSim));
+    GK_CHECK(hipMemcpy(b->dsims, h.data(), n * sizeof(BSim), hipMemcpyHostToDevice));
+    b->ddone = (int*)balloc(b, n * sizeof(int));
+    // launch sets per pass and length class
+    std::vector<int> all(n), d3, d2;
+    for (int j = 0; j < n; j++) {
+        all[j] = j;
+        (sims[j]->cp.P[0] > 1 ? d3 : d2).push_back(j);
+    }
+    auto M = [](const mmSim* s) { return s->cp.M; };
+    auto PY = [](const mmSim* s) { return s->cp.P[1]; };
+    auto PX = [](const mmSim* s) { return s->cp.P[0]; };
+    b->zf = make_sets(b, all, M, [](const mmSim* s) { return div_up(s->cp.N[1], z_rows_host(s->cp.M)) * s->cp.N[0] * 3; });
+    b->zi = make_sets(b, all, M, [](const mmSim* s) { return div_up(s->cp.N[1], epi_rows_host(s->cp.M)) * s->cp.N[0]; });
+    auto ylines = [](const mmSim* s) { return div_up(3L * s->cp.N[0] * s->cp.Q, 256 / lanes_host(s->cp.P[1])); };
+    b->yf = make_sets(b, d3, PY, ylines);
+    b->yi = make_sets(b, d3, PY, ylines);
+    b->xm = make_sets(b, d3, PX, [](const mmSim* s) {
+        const int lpb = 256 / lanes_host(s->cp.P[0]);
+        const int lpr = std::min(lpb, s->cp.P[1]);
+        return s->cp.Q * (s->cp.P[1] / lpr);
+    });
+    b->y2 = make_sets(b, d2, PY, [](const mmSim* s) { return div_up(s->cp.Q, y2d_nkz_host(s->cp.P[1])); });
+    b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });
+    b->all256 = make_sets(b, all, [](const mmSim*) { return 0; },
+                          [](const mmSim* s) { return std::max(1, std::min(1024, div_up(3L * s->N, 256 * 4))); });
+    // twiddle tables are created on first use: make them now, outside any graph capture
+    for (int j = 0; j < n; j++) {
+        const ConvPlan& cp = sims[j]->cp;
+        (void)twiddle_f32(cp.M);
+        (void)twiddle_f32(cp.P[2]);
+        if (cp.P[1] > 1) (void)twiddle_f32(cp.P[1]);
+        if (cp.P[0] > 1) (void)twiddle_f32(cp.P[0]);
+    }
+    GK_CHECK(hipDeviceSynchronize());
+    return b;
+}
+
+void mm_batch_free(mmBatch* b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void* p : b->owned) (void)hipFree(p);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
+void mm_batch_set_graph(mmBatch* b, int on) { b->useGraph = on; }
+
+int mm_batch_size(mmBatch* b) { return (int)b->sims.size(); }
+
+// Run(): every simulation j gets the stop time time_j + seconds[j] and is
+// normalised first (simrun.go:19-24); Steps() (seconds null): steps[j] steps,
+// no normalisation (simrun.go:85-92).
+void mm_batch_start(mmBatch* b, const double* seconds, const long long* steps) {
+    for (size_t j = 0; j < b->sims.size(); j++) {
+        mmSim* s = b->sims[j];
+        GK_CHECK(hipStreamSynchronize(s->s));
+        EngineState h = read_state(s);
+        const double stop = seconds ? h.time + seconds[j] / s->unitTime : INFINITY;
+        if (seconds) launch_normalize_active(s->m, nullptr, s->N, b->s);
+        launch_set_target(s->st, steps ? steps[j] : -1, stop, b->s);
+    }
+    GK_CHECK(hipStreamSynchronize(b->s));
+}
+
+// enqueue n batched trials and wait; returns the number of simulations not done
+int mm_batch_trials(mmBatch* b, int n) {
+    if (b->useGraph) {
+        if (!b->exec) {
+            GK_CHECK(hipStreamBeginCapture(b->s, hipStreamCaptureModeRelaxed));
+            enqueue_btrial(b);
+            GK_CHECK(hipStreamEndCapture(b->s, &b->graph));
+            GK_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < n; i++) GK_CHECK(hipGraphLaunch(b->exec, b->s));
+    } else {
+        for (int i = 0; i < n; i++) enqueue_btrial(b);
+    }
+    b->trials += n;
+    const int ns = (int)b->sims.size();
+    kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone);
+    GK_LAUNCH_CHECK();
+    std::vector<int> done(ns);
+    GK_CHECK(hipMemcpyAsync(done.data(), b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));
+    GK_CHECK(hipStreamSynchronize(b->s));
+    int active = 0;
+    for (int d : done) active += !d;
+    return active;
+}
+
+// done flags of every simulation (1 = reached its stop time / step target)
+void mm_batch_done(mmBatch* b, int* done) {
+    const int ns = (int)b->sims.size();
+    kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone);
+    GK_LAUNCH_CHECK();
+    GK_CHECK(hipMemcpyAsync(done, b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));
+    GK_CHECK(hipStreamSynchronize(b->s));
+}
+
+// run until every simulation is done, polling every `poll` trials; returns batched trials
+long long mm_batch_run(mmBatch* b, const double* seconds, int poll) {
+    mm_batch_start(b, seconds, nullptr);
+    long long n = 0;
+    if (poll < 1) poll = 16;
+    while (true) {
+        const int active = mm_batch_trials(b, poll);
+        n += poll;
+        if (!active) break;
+    }
+    return n;
+}
+
+}  // extern "C"

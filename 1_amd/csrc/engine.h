@@ -95,6 +95,8 @@ struct ConvPlan {
 // Choose padded sizes, allocate buffers.  Kc is filled by conv_build_kernel.
 void conv_plan_init(ConvPlan& cp, const int N[3]);
 void conv_plan_free(ConvPlan& cp);
+// bytes conv_plan_init allocates for data size N, and the blocks of the fused Z inverse pass
+size_t conv_plan_footprint(const int N[3], int* epi_blocks);
 // FaceKernel6 (accuracy x accuracy charges per face) in FP64 on the device,
 // optionally plus the 6-neighbour exchange kernel, FFT in FP64, stored
 // compressed in cp.Kc.  If full != null, also writes nK x [Px][Py][Q] floats

@@ -1,0 +1,660 @@
+// Device code of the convolution passes, shared by the single-simulation
+// kernels (conv.hip) and the batched kernels (batch.hip): each pass is a
+// __device__ body over one tile / line / column set, wrapped by kernels that
+// only decide which tile a block works on.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "fft_device.h"
+
+namespace gk {
+
+// =============================================================== passes
+constexpr int kNT = 256;
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// and each XCD has its own L2.  Renumber so that every XCD gets one contiguous
+// range of logical blocks: neighbouring blocks, which share cache lines (and
+// mirrored kernel rows), then meet in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int kXcd = 8;
+    if (nb % kXcd) return b;
+    return (b % kXcd) * (nb / kXcd) + b / kXcd;
+}
+
+template <int L> constexpr int rows_per_block() {
+    int n = 71680 / ((L + 1) * 8);
+    n = n < 1 ? 1 : (n > 16 ? 16 : n);
+    int p = 1;
+    while (p * 2 <= n) p *= 2;
+    return p;
+}
+
+// shuffle of a complex value within groups of W lanes
+template <int W> __device__ __forceinline__ float2 shfl2(float2 v, int src) {
+    return make_float2(__shfl(v.x, src, W), __shfl(v.y, src, W));
+}
+
+// Z forward.  TR rows (y0..y0+TR) of one (c, x) plane; each row is a group of G
+// lanes holding the packed complex row z[n] = x[2n] + i x[2n+1] (RK stage build
+// m + sum dt a_ij k_j applied while loading), transformed in registers; the
+// r2c split takes Z[M-k] from the partner lane by a shuffle, and the spectrum is
+// transposed through a padded LDS tile into B1[c][x][kz][y] (TR consecutive y).
+template <int M, int TR> constexpr int zfwd_lds() {
+    constexpr int FFT_LDS = TR * RegFFT<M>::LDS, TILE_LDS = (M + 1) * (TR + 1);
+    return FFT_LDS > TILE_LDS ? FFT_LDS : TILE_LDS;
+}
+
+// body: component c, plane x, rows y0..y0+TR; lds holds zfwd_lds<M,TR>() complex
+template <int M, int TR>
+__device__ __forceinline__ void zfwd_tile(const StageIn& si, float2* __restrict__ B1, const float2* __restrict__ twM,
+                                          const float2* __restrict__ twP, int Nx, int Ny, int Nz, int c, int x,
+                                          int y0, float2* lds) {
+    using F = RegFFT<M>;
+    constexpr int E = F::E, G = F::G, NT = TR * G, Q = M + 1, TS = TR + 1;
+    const int row = threadIdx.x / G, g = threadIdx.x % G;
+    const int nrows = min(TR, Ny - y0);
+    const bool ok = row < nrows;
+    const long N = (long)Nx * Ny * Nz;
+    const long rbase = c * N + ((long)x * Ny + y0 + row) * Nz;
+    const float dt = si.st ? si.st->dt_used : 0.f;
+    float coef[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) coef[j] = dt * si.a[j];
+    const int nk = si.nk;
+    float2 v[E];
+    if ((Nz & 1) == 0) {
+        // pairs (2n, 2n+1) as float2: rows start at an even float offset
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int n = g + G * e;
+            float2 z = make_float2(0.f, 0.f);
+            if (!pad_half<M>(e) && ok && 2 * n < Nz) {
+                z = *reinterpret_cast<const float2*>(si.m + rbase + 2 * n);
+                for (int j = 0; j < nk; j++) {
+                    const float2 kv = *reinterpret_cast<const float2*>(si.k[j] + rbase + 2 * n);
+                    z.x = fmaf(coef[j], kv.x, z.x);
+                    z.y = fmaf(coef[j], kv.y, z.y);
+                }
+                if (si.mout) *reinterpret_cast<float2*>(si.mout + rbase + 2 * n) = z;
+            }
+            v[e] = z;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int n = g + G * e;
+            float z2[2] = {0.f, 0.f};
+            for (int h = 0; h < 2; h++) {
+                const int i = 2 * n + h;
+                if (!pad_half<M>(e) && ok && i < Nz) {
+                    float a = si.m[rbase + i];
+                    for (int j = 0; j < nk; j++) a = fmaf(coef[j], si.k[j][rbase + i], a);
+                    if (si.mout) si.mout[rbase + i] = a;
+                    z2[h] = a;
+                }
+            }
+            v[e] = make_float2(z2[0], z2[1]);
+        }
+    }
+    reg_fft<float, M, -1>(v, g, lds + row * F::LDS, twM);
+    // r2c split: lane g holds Z[k], k = g + G e; Z[M-k] is in lane (G-g) % G at e' = E-1-e
+    // (or, for g = 0, in this lane at e' = (E-e) % E)
+    const int partner = (G - g) % G;
+    float2 X[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const float2 t = shfl2<G>(v[E - 1 - e], partner);
+        const float2 zmk = g == 0 ? v[(E - e) % E] : t;
+        X[e] = r2c_split<float>(v[e], zmk, twP[g + G * e]);
+    }
+    const float2 nyq = make_float2(v[0].x - v[0].y, 0.f);  // X[M], valid in lane 0
+    __syncthreads();  // the FFT exchange buffer becomes the transpose tile
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[(g + G * e) * TS + row] = X[e];
+    if (g == 0) lds[M * TS + row] = nyq;
+    __syncthreads();
+    float2* dst = B1 + ((long)c * Nx + x) * Q * Ny + y0;
+    for (int t = threadIdx.x; t < Q * TR; t += NT) {
+        const int r = t % TR, kz = t / TR;
+        if (r < nrows) dst[(long)kz * Ny + r] = lds[kz * TS + r];
+    }
+}
+
+template <int M, int TR>
+__global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zfwd(StageIn si, float2* __restrict__ B1,
+                                                              const float2* __restrict__ twM,
+                                                              const float2* __restrict__ twP, int Nx, int Ny, int Nz) {
+    __shared__ float2 lds[zfwd_lds<M, TR>()];
+    if (si.st && !si.st->trial_active) return;
+    zfwd_tile<M, TR>(si, B1, twM, twP, Nx, Ny, Nz, blockIdx.z, blockIdx.y, blockIdx.x * TR, lds);
+}
+
+// Z inverse: B1[c][x][kz][y0..y0+TR) -> rows of h.  The spectrum is read through
+// a padded LDS tile (TR consecutive y per kz), merged (c2r) in registers and
+// transformed back; each lane writes its (2n, 2n+1) pairs as float2.
+template <int M, int TR>
+__global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zinv(const float2* __restrict__ B1, float* __restrict__ h,
+                                                              const float2* __restrict__ twM,
+                                                              const float2* __restrict__ twP, const EngineState* st,
+                                                              int Nx, int Ny, int Nz) {
+    using F = RegFFT<M>;
+    constexpr int E = F::E, G = F::G, NT = TR * G, Q = M + 1, TS = TR + 1;
+    __shared__ float2 lds[zfwd_lds<M, TR>()];
+    if (st && !st->trial_active) return;
+    const int row = threadIdx.x / G, g = threadIdx.x % G;
+    const int c = blockIdx.z, x = blockIdx.y, y0 = blockIdx.x * TR;
+    const int nrows = min(TR, Ny - y0);
+    const float2* src = B1 + ((long)c * Nx + x) * Q * Ny + y0;
+    batched_copy<float2, NT>(
+        Q * TR,
+        [&](int t) {
+            const int r = t % TR, kz = t / TR;
+            return r < nrows ? src[(long)kz * Ny + r] : make_float2(0.f, 0.f);
+        },
+        [&](int t, float2 val) { lds[(t / TR) * TS + t % TR] = val; });
+    __syncthreads();
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int k = g + G * e;
+        v[e] = c2r_merge<float>(lds[k * TS + row], lds[(M - k) * TS + row], twP[k]);
+    }
+    __syncthreads();  // the tile becomes the FFT exchange buffer
+    reg_fft<float, M, +1>(v, g, lds + row * F::LDS, twM);
+    if (row < nrows) {
+        const long N = (long)Nx * Ny * Nz;
+        float* dst = h + c * N + ((long)x * Ny + y0 + row) * Nz;
+        if ((Nz & 1) == 0) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int n = g + G * e;
+                if (!pad_half<M>(e) && 2 * n < Nz) *reinterpret_cast<float2*>(dst + 2 * n) = v[e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int n = g + G * e;
+                if (!pad_half<M>(e) && 2 * n < Nz) dst[2 * n] = v[e].x;
+                if (!pad_half<M>(e) && 2 * n + 1 < Nz) dst[2 * n + 1] = v[e].y;
+            }
+        }
+    }
+}
+
+// Fused Z inverse + RK stage epilogue.  One block: TR rows (y0..y0+TR) of one x
+// plane for all 3 components (3 x TR lane groups).  The 3 spectra are read
+// through transposed LDS tiles, merged and transformed back in registers, and
+// the fields are exchanged through LDS so that every thread sees h of all 3
+// components for its cells.  The epilogue (EpiArgs) rebuilds the stage input
+// m_i from m and the earlier k_j (the same fmaf sequence as the forward pass's
+// stage build), evaluates the torque, and stores k_i (with the per-block max
+// |k_i|) or, for the last stage of a fixed-step tableau, the normalised new m.
+// h itself never leaves the chip.  NU bounds the number of earlier stages read
+// (the launcher points unused slots at m with zero coefficients, which leaves
+// every fmaf chain unchanged), so all loads are unconditional.
+template <int M, int TR> constexpr int zinv_epi_lds() {
+    constexpr int Q = M + 1, TS = TR + 1, HS = M / 2 > 0 ? M / 2 : 1;
+    constexpr int TILE = 3 * Q * TS, FFTB = 3 * TR * RegFFT<M>::LDS, HB = 3 * TR * HS;
+    return TILE > FFTB ? (TILE > HB ? TILE : HB) : (FFTB > HB ? FFTB : HB);
+}
+
+// body: plane x, rows y0..y0+TR, all 3 components; the per-block max |k| goes to
+// ea.t.part[pidx]; lds / hext_s / red: zinv_epi_lds<M,TR>() complex, 3 and
+// 3 TR G floats
+template <int M, int TR, int NU, bool FIN>
+__device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, const EpiArgs& ea,
+                                              const float2* __restrict__ twM, const float2* __restrict__ twP,
+                                              const EngineState* st, int Nx, int Ny, int Nz, int x, int y0, int pidx,
+                                              float2* lds, float* hext_s, float* red) {
+    using F = RegFFT<M>;
+    constexpr int E = F::E, G = F::G, NR = TR * G, NT = 3 * NR, Q = M + 1, TS = TR + 1;
+    constexpr int HS = M / 2 > 0 ? M / 2 : 1;  // float2 per h row (z pairs); Nz <= M
+    const int c = threadIdx.x / NR, rem = threadIdx.x - c * NR, row = rem / G, g = rem % G;
+    const int nrows = min(TR, Ny - y0);
+    if (threadIdx.x == 0) stage_field(ea.t, st, hext_s);
+    const long cst = (long)Nx * Q * Ny;
+    const float2* src = B1 + (long)x * Q * Ny + y0;
+    batched_copy<float2, NT>(
+        3 * Q * TR,
+        [&](int t) {
+            const int cc = t / (Q * TR), u = t - cc * (Q * TR), kz = u / TR, r = u - kz * TR;
+            return r < nrows ? src[cc * cst + (long)kz * Ny + r] : make_float2(0.f, 0.f);
+        },
+        [&](int t, float2 val) {
+            const int cc = t / (Q * TR), u = t - cc * (Q * TR), kz = u / TR, r = u - kz * TR;
+            lds[(cc * Q + kz) * TS + r] = val;
+        });
+    __syncthreads();
+    float2 v[E];
+    {
+        const float2* tile = lds + c * Q * TS;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = g + G * e;
+            v[e] = c2r_merge<float>(tile[k * TS + row], tile[(M - k) * TS + row], twP[k]);
+        }
+    }
+    __syncthreads();  // the tiles become the FFT exchange buffers
+    reg_fft<float, M, +1>(v, g, lds + (c * TR + row) * F::LDS, twM);
+    __syncthreads();  // ... and then the h rows
+    const int nh = (Nz + 1) >> 1;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int n = g + G * e;
+        if (!pad_half<M>(e) && n < nh) lds[(c * TR + row) * HS + n] = v[e];
+    }
+    __syncthreads();
+
+    const float hext[3] = {hext_s[0], hext_s[1], hext_s[2]};
+    const float dt = st->dt_used;
+    float ca[NU > 0 ? NU : 1], cb[NU > 0 ? NU : 1];
+#pragma unroll
+    for (int j = 0; j < NU; j++) {
+        ca[j] = dt * ea.a[j];
+        cb[j] = dt * ea.b[j];
+    }
+    const float cbs = dt * ea.bs;
+    const long N = (long)Nx * Ny * Nz;
+    float lmax = 0.f;
+    // one cell from the accumulated stage input mi (and final sum hi) and the field h
+    auto finish = [&](const float mi[3], const float hi[3], const float h[3], float o[3]) {
+        float k[3];
+        llg_torque_cell(ea.t, hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
+        lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
+        if constexpr (FIN) {
+            float f[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) f[q] = fmaf(cbs, k[q], hi[q]);
+            const float sc = rsqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+#pragma unroll
+            for (int q = 0; q < 3; q++) o[q] = f[q] * sc;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) o[q] = k[q];
+        }
+    };
+    const long rowbase = ((long)x * Ny + y0) * Nz;
+    if ((Nz & 1) == 0) {
+        // cell pairs (2n, 2n+1) as float2
+        constexpr int IT = (TR * HS + NT - 1) / NT;
+        const int nzh = Nz >> 1, P = nrows * nzh;
+        float2 ai[IT][3], af[IT][3];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int p = threadIdx.x + it * NT;
+            if (p < P) {
+                const int r = p / nzh, n = p - r * nzh;
+                const long gi = rowbase + (long)r * Nz + 2 * n;
+#pragma unroll
+                for (int q = 0; q < 3; q++) ai[it][q] = af[it][q] = *reinterpret_cast<const float2*>(ea.m + q * N + gi);
+#pragma unroll
+                for (int j = 0; j < NU; j++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const float2 kv = *reinterpret_cast<const float2*>(ea.k[j] + q * N + gi);
+                        ai[it][q].x = fmaf(ca[j], kv.x, ai[it][q].x);
+                        ai[it][q].y = fmaf(ca[j], kv.y, ai[it][q].y);
+                        if constexpr (FIN) {
+                            af[it][q].x = fmaf(cb[j], kv.x, af[it][q].x);
+                            af[it][q].y = fmaf(cb[j], kv.y, af[it][q].y);
+                        }
+                    }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int p = threadIdx.x + it * NT;
+            if (p < P) {
+                const int r = p / nzh, n = p - r * nzh;
+                const long gi = rowbase + (long)r * Nz + 2 * n;
+                float2 hv[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) hv[q] = lds[(q * TR + r) * HS + n];
+                float mi[3], hi[3], h[3], o0[3], o1[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    mi[q] = ai[it][q].x;
+                    hi[q] = af[it][q].x;
+                    h[q] = hv[q].x;
+                }
+                finish(mi, hi, h, o0);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    mi[q] = ai[it][q].y;
+                    hi[q] = af[it][q].y;
+                    h[q] = hv[q].y;
+                }
+                finish(mi, hi, h, o1);
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+                    *reinterpret_cast<float2*>(ea.out + q * N + gi) = make_float2(o0[q], o1[q]);
+            }
+        }
+    } else {
+        const float* hf = reinterpret_cast<const float*>(lds);
+        const int P = nrows * Nz;
+        for (int p = threadIdx.x; p < P; p += NT) {
+            const int r = p / Nz, z = p - r * Nz;
+            const long gi = rowbase + (long)r * Nz + z;
+            float mi[3], hi[3], h[3], o[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                mi[q] = hi[q] = ea.m[q * N + gi];
+                h[q] = hf[(q * TR + r) * 2 * HS + z];
+#pragma unroll
+                for (int j = 0; j < NU; j++) {
+                    const float kv = ea.k[j][q * N + gi];
+                    mi[q] = fmaf(ca[j], kv, mi[q]);
+                    hi[q] = fmaf(cb[j], kv, hi[q]);
+                }
+            }
+            finish(mi, hi, h, o);
+#pragma unroll
+            for (int q = 0; q < 3; q++) ea.out[q * N + gi] = o[q];
+        }
+    }
+    if (ea.t.part) {
+        red[threadIdx.x] = lmax;
+        __syncthreads();
+        if constexpr (NT >= 64) {
+            if (threadIdx.x < 64) {
+                float vm = 0.f;
+                for (int i = threadIdx.x; i < NT; i += 64) vm = fmaxf(vm, red[i]);
+                for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+                if (threadIdx.x == 0) ea.t.part[pidx] = vm;
+            }
+        } else {
+            if (threadIdx.x == 0) {
+                float vm = 0.f;
+                for (int i = 0; i < NT; i++) vm = fmaxf(vm, red[i]);
+                ea.t.part[pidx] = vm;
+            }
+        }
+    }
+}
+
+template <int M, int TR, int NU, bool FIN>
+__global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void k_zinv_epi(const float2* __restrict__ B1, EpiArgs ea,
+                                                                      const float2* __restrict__ twM,
+                                                                      const float2* __restrict__ twP,
+                                                                      const EngineState* st, int Nx, int Ny, int Nz) {
+    __shared__ float2 lds[zinv_epi_lds<M, TR>()];
+    __shared__ float hext_s[3];
+    __shared__ float red[3 * TR * RegFFT<M>::G];
+    if (!st->trial_active) return;
+    const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    zinv_epi_tile<M, TR, NU, FIN>(B1, ea, twM, twP, st, Nx, Ny, Nz, lb / gridDim.x, (lb % gridDim.x) * TR,
+                                  blockIdx.y * gridDim.x + blockIdx.x, lds, hext_s, red);
+}
+
+// 3-D Y passes, software-pipelined.  One line per lane group and iteration:
+//   forward  B1[c][x][kz][0..Ny) -> B2[c][kz][x][0..Py)   (zero padded)
+//   inverse  B2[c][kz][x][0..Py) -> B1[c][x][kz][0..Ny)   (rows >= Ny dropped)
+// Lines are numbered with x fastest (a block touches consecutive B2 lines).
+// Each lane group walks `nit` lines strided by the grid and loads line i+1
+// before transforming line i, so the memory of one line overlaps the FFT of
+// the previous one; twiddles come from LDS, so no global load sits between a
+// prefetch and its use.
+// B1 line (c, x, kz) and B2 line (c, kz, x) of a Y-pass line number (x fastest)
+__device__ __forceinline__ long ypass_b1(long ln, int Nx, int Ny, int Q) {
+    const int x = (int)(ln % Nx);
+    const long r = ln / Nx;
+    const int kz = (int)(r % Q), c = (int)(r / Q);
+    return (((long)c * Nx + x) * Q + kz) * Ny;
+}
+__device__ __forceinline__ long ypass_b2(long ln, int Nx, int Q, int PY) {
+    const int x = (int)(ln % Nx);
+    const long r = ln / Nx;
+    const int kz = (int)(r % Q), c = (int)(r / Q);
+    return (((long)c * Q + kz) * Nx + x) * PY;
+}
+// one line into the lane-strided registers of lane g (zeros when !ok)
+template <int PY, bool FWD>
+__device__ __forceinline__ void ypass_load(float2* v, const float2* __restrict__ src, long ln, bool ok, int Nx,
+                                           int Ny, int Q, int g) {
+    using F = RegFFT<PY>;
+    if constexpr (FWD) {
+        const float2* p = src + ypass_b1(ok ? ln : 0, Nx, Ny, Q);
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int i = g + F::G * e;
+            v[e] = (!pad_half<PY>(e) && ok && i < Ny) ? p[i] : make_float2(0.f, 0.f);
+        }
+    } else {
+        const float2* p = src + ypass_b2(ok ? ln : 0, Nx, Q, PY);
+#pragma unroll
+        for (int e = 0; e < F::E; e++) v[e] = ok ? p[g + F::G * e] : make_float2(0.f, 0.f);
+    }
+}
+template <int PY, bool FWD>
+__device__ __forceinline__ void ypass_store(const float2* v, float2* __restrict__ dst, long ln, int Nx, int Ny,
+                                            int Q, int g) {
+    using F = RegFFT<PY>;
+    if constexpr (FWD) {
+        float2* p = dst + ypass_b2(ln, Nx, Q, PY);
+#pragma unroll
+        for (int e = 0; e < F::E; e++) p[g + F::G * e] = v[e];
+    } else {
+        float2* p = dst + ypass_b1(ln, Nx, Ny, Q);
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int i = g + F::G * e;
+            if (!pad_half<PY>(e) && i < Ny) p[i] = v[e];
+        }
+    }
+}
+
+template <int PY, bool FWD>
+__global__ __launch_bounds__(kNT) void k_ypass(const float2* __restrict__ src, float2* __restrict__ dst,
+                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int Ny,
+                                               int Q, int nit) {
+    using F = RegFFT<PY>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    constexpr bool TWL = PY <= 1024;  // twiddle table in LDS (longer lines keep it in global memory)
+    __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[TWL ? PY : 1];
+    if (st && !st->trial_active) return;
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const long nlines = 3L * Nx * Q, stride = (long)gridDim.x * LPB;
+    long line = (long)blockIdx.x * LPB + li;
+    constexpr bool PIPE = E <= 16;  // registers for a second line (longer lines load in place)
+    float2 nxt[PIPE ? E : 1];
+    if constexpr (PIPE) ypass_load<PY, FWD>(nxt, src, line, line < nlines, Nx, Ny, Q, g);
+    __syncthreads();  // twiddle table
+    for (int it = 0; it < nit; it++, line += stride) {
+        float2 v[E];
+        if constexpr (PIPE) {
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = nxt[e];
+            if (it + 1 < nit) ypass_load<PY, FWD>(nxt, src, line + stride, line + stride < nlines, Nx, Ny, Q, g);
+        } else {
+            ypass_load<PY, FWD>(v, src, line, line < nlines, Nx, Ny, Q, g);
+        }
+        reg_fft<float, PY, FWD ? -1 : +1>(v, g, lds + li * F::LDS, TWL ? twl : tw);
+        if (line < nlines) ypass_store<PY, FWD>(v, dst, line, Nx, Ny, Q, g);
+    }
+}
+
+// 3-D X forward . kernel . X inverse, software-pipelined.  One (kz, ky) column
+// per group of G lanes, all 3 components in registers (lane-strided x = g + G e):
+// the loads and stores touch G rows of consecutive ky per instruction, the
+// 6-component symmetric multiply is lane-local, and each transform needs one
+// LDS exchange per radix pass.  A block item is one kz and a run of lpr
+// consecutive ky; a block walks `nit` consecutive items, loading the columns of
+// the next item before transforming the current one.  Items are ordered so that
+// the runs ky and Py - ky (which read the same kernel rows) follow each other.
+// the 3 components of column (kz, ky) into lane g's registers (ky < 0: zeros)
+template <int PX>
+__device__ __forceinline__ void xmul_load(float2 (*v)[RegFFT<PX>::E], const float2* __restrict__ B2, int kz, int ky,
+                                          int Nx, int PY, int Q, int g) {
+    using F = RegFFT<PX>;
+    const long cstride = (long)Q * Nx * PY;
+    const float2* base = B2 + (long)kz * Nx * PY + (ky < 0 ? 0 : ky);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int x = g + F::G * e;
+            v[c][e] = (!pad_half<PX>(e) && ky >= 0 && x < Nx) ? base[c * cstride + (long)x * PY] : make_float2(0.f, 0.f);
+        }
+}
+template <int PX>
+__device__ __forceinline__ void xmul_store(float2 (*v)[RegFFT<PX>::E], float2* __restrict__ B2, int kz, int ky,
+                                           int Nx, int PY, int Q, int g) {
+    using F = RegFFT<PX>;
+    const long cstride = (long)Q * Nx * PY;
+    float2* base = B2 + (long)kz * Nx * PY + ky;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int x = g + F::G * e;
+            if (!pad_half<PX>(e) && x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
+        }
+}
+// H = K M on the spectrum of column (kz, ky): the symmetric 6-component kernel,
+// stored for kx <= PX/2, ky <= PY/2, with the parity signs restored
+template <int PX>
+__device__ __forceinline__ void xmul_kernel(float2 (*v)[RegFFT<PX>::E], const float* __restrict__ Kc, int kz, int ky,
+                                            int PY, int g) {
+    using F = RegFFT<PX>;
+    const int hx = PX / 2 + 1, hy = PY / 2 + 1;
+    const bool fy = ky > PY / 2;
+    const int kyp = fy ? PY - ky : ky;
+    const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
+#pragma unroll
+    for (int e = 0; e < F::E; e++) {
+        const int kx = g + F::G * e;
+        const bool fx = kx > PX / 2;
+        const int kxp = fx ? PX - kx : kx;
+        const float2* kk = reinterpret_cast<const float2*>(kb + kxp * 6);
+        const float2 k01 = kk[0], k23 = kk[1], k45 = kk[2];
+        const float kxx = k01.x, kyy = k01.y, kzz = k23.x;
+        float kyz = k23.y, kxz = k45.x, kxy = k45.y;
+        if (fy) kyz = -kyz;
+        if (fx) kxz = -kxz;
+        if (fx != fy) kxy = -kxy;
+        const float2 a = v[0][e], b = v[1][e], c = v[2][e];
+        v[0][e] = make_float2(a.x * kxx + b.x * kxy + c.x * kxz, a.y * kxx + b.y * kxy + c.y * kxz);
+        v[1][e] = make_float2(a.x * kxy + b.x * kyy + c.x * kyz, a.y * kxy + b.y * kyy + c.y * kyz);
+        v[2][e] = make_float2(a.x * kxz + b.x * kyz + c.x * kzz, a.y * kxz + b.y * kyz + c.y * kzz);
+    }
+}
+// column of item t for lane group li: items are (kz, run of lpr ky), with the
+// runs ky and PY - ky adjacent; ky < 0 when the lane group has no column
+__device__ __forceinline__ void xmul_column(long t, long items, int rows, int lpr, int li, int& kz, int& ky) {
+    kz = (int)(t / rows);
+    const int rr = (int)(t % rows);
+    const int r = (rr & 1) ? rows - 1 - (rr >> 1) : (rr >> 1);
+    ky = (t < items && li < lpr) ? r * lpr + li : -1;
+}
+
+template <int PX>
+__global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const float* __restrict__ Kc,
+                                              const float2* __restrict__ tw, const EngineState* st, int Nx, int PY,
+                                              int Q, int nit) {
+    using F = RegFFT<PX>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[PX];
+    if (st && !st->trial_active) return;
+    for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const int lpr = LPB < PY ? LPB : PY;  // columns per item
+    const int rows = PY / lpr;            // items per kz plane
+    const long items = (long)Q * rows;
+    const long t0 = (long)blockIdx.x * nit;
+    float2 nxt[3][E];
+    {
+        int kz, ky;
+        xmul_column(t0, items, rows, lpr, li, kz, ky);
+        xmul_load<PX>(nxt, B2, kz, ky, Nx, PY, Q, g);
+    }
+    __syncthreads();  // twiddle table
+    for (int it = 0; it < nit; it++) {
+        const long t = t0 + it;
+        float2 v[3][E];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int e = 0; e < E; e++) v[c][e] = nxt[c][e];
+        if (it + 1 < nit) {
+            int kz1, ky1;
+            xmul_column(t + 1, items, rows, lpr, li, kz1, ky1);
+            xmul_load<PX>(nxt, B2, kz1, ky1, Nx, PY, Q, g);
+        }
+        int kz, ky;
+        xmul_column(t, items, rows, lpr, li, kz, ky);
+#pragma unroll
+        for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, twl);
+        if (ky >= 0) xmul_kernel<PX>(v, Kc, kz, ky, PY, g);
+#pragma unroll
+        for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, twl);
+        if (ky >= 0) xmul_store<PX>(v, B2, kz, ky, Nx, PY, Q, g);
+    }
+}
+
+// 2-D: Y forward . kernel . Y inverse on NKZ consecutive kz rows, all 3 components, in place in B1
+template <int PY, int NKZ>
+__device__ __forceinline__ void y2d_tile(float2* __restrict__ B1, const float* __restrict__ Kc,
+                                         const float2* __restrict__ tw, int Ny, int Q, int kz0, float2* lds) {
+    constexpr int LS = PY + 1;
+    const int nkz = min(NKZ, Q - kz0);
+    const long cstride = (long)Q * Ny;
+    const int per = nkz * Ny;
+    batched_copy<float2, kNT>(
+        3 * per,
+        [&](int t) {
+            const int c = t / per, r = t - c * per;
+            return B1[c * cstride + (long)kz0 * Ny + r];
+        },
+        [&](int t, float2 v) {
+            const int c = t / per, r = t - c * per;
+            const int j = r / Ny, y = r - j * Ny;
+            lds[(c * NKZ + j) * LS + y] = v;
+        });
+    __syncthreads();
+    fft_lds<float, PY, -1, 3 * NKZ, kNT>(lds, LS, tw, Ny);
+    const int hy = PY / 2 + 1;
+    const long cs = (long)Q * hy;
+    for (int t = threadIdx.x; t < nkz * PY; t += kNT) {
+        const int j = t / PY, ky = t - j * PY;
+        const int kz = kz0 + j;
+        const bool fy = ky > PY / 2;
+        const int kyp = fy ? PY - ky : ky;
+        const long b = (long)kz * hy + kyp;
+        const float kxx = Kc[b], kyy = Kc[cs + b], kzz = Kc[2 * cs + b];
+        const float kyz = fy ? -Kc[3 * cs + b] : Kc[3 * cs + b];
+        float2* px = &lds[(0 * NKZ + j) * LS + ky];
+        float2* py = &lds[(1 * NKZ + j) * LS + ky];
+        float2* pz = &lds[(2 * NKZ + j) * LS + ky];
+        const float2 a = *px, bb = *py, cc = *pz;
+        *px = make_float2(a.x * kxx, a.y * kxx);
+        *py = make_float2(bb.x * kyy + cc.x * kyz, bb.y * kyy + cc.y * kyz);
+        *pz = make_float2(bb.x * kyz + cc.x * kzz, bb.y * kyz + cc.y * kzz);
+    }
+    __syncthreads();
+    fft_lds<float, PY, +1, 3 * NKZ, kNT>(lds, LS, tw, PY);
+    for (int c = 0; c < 3; c++) {
+        float2* dst = B1 + c * cstride + (long)kz0 * Ny;
+        for (int t = threadIdx.x; t < nkz * Ny; t += kNT) {
+            const int j = t / Ny, y = t - j * Ny;
+            dst[t] = lds[(c * NKZ + j) * LS + y];
+        }
+    }
+}
+
+template <int PY, int NKZ>
+__global__ __launch_bounds__(kNT) void k_y2d(float2* __restrict__ B1, const float* __restrict__ Kc,
+                                             const float2* __restrict__ tw, const EngineState* st, int Ny, int Q) {
+    __shared__ float2 lds[3 * NKZ * (PY + 1)];
+    if (st && !st->trial_active) return;
+    y2d_tile<PY, NKZ>(B1, Kc, tw, Ny, Q, blockIdx.x * NKZ, lds);
+}
+
+
+}  // namespace gk

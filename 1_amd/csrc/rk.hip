@@ -15,20 +15,9 @@
 #include <cmath>
 
 #include "engine.h"
+#include "rk_dev.h"
 
 namespace gk {
-
-__device__ __forceinline__ float block_max_nonneg(float v) {
-    __shared__ float wmax[16];
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) wmax[wave] = v;
-    __syncthreads();
-    float r = 0.f;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = fmaxf(r, wmax[w]);
-    return r;  // valid in thread 0
-}
 
 // k = dtg (m x h - alpha m x (m x h)) with h = H_conv + B_ext(t_i) + anisotropy (+ exchange)
 __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, float* __restrict__ hk,
@@ -53,74 +42,22 @@ __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, flo
     }
 }
 
-struct KPtrs {
-    const float* k[7];
-};
-
-// m_high = m + sum b_i dt k_i, m_low = m + sum b2_i dt k_i (rk.go:381-403)
+// m_high and error partials (adaptive) or m = normalise(m_high) (fixed step)
 __global__ __launch_bounds__(256) void k_rk_final(float* __restrict__ m, KPtrs kp, float* __restrict__ mtmp,
                                                   EngineState* st, FinalArgs a, int N) {
     if (!st->trial_active) return;
-    const float dt = st->dt_used;
-    float cb[7], cb2[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) {
-        cb[s] = a.b[s] * dt;
-        cb2[s] = a.b2[s] * dt;
-    }
-    float lmax = 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        float hi[3], lo[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float m0 = m[c * N + i];
-            float h = m0, l = m0;
-            for (int s = 0; s < a.S; s++) {
-                const float kv = (a.b[s] != 0.f || a.b2[s] != 0.f) ? kp.k[s][c * N + i] : 0.f;
-                if (a.b[s] != 0.f) h = fmaf(cb[s], kv, h);
-                if (a.adaptive && a.b2[s] != 0.f) l = fmaf(cb2[s], kv, l);
-            }
-            hi[c] = h;
-            lo[c] = l;
-        }
-        if (a.adaptive) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                mtmp[c * N + i] = hi[c];
-                lmax = fmaxf(lmax, fabsf(lo[c] - hi[c]));
-            }
-        } else {
-            const float s = rsqrtf(hi[0] * hi[0] + hi[1] * hi[1] + hi[2] * hi[2]);
-#pragma unroll
-            for (int c = 0; c < 3; c++) m[c * N + i] = hi[c] * s;
-        }
-    }
-    if (a.adaptive) {
-        const float bm = block_max_nonneg(lmax);
-        if (threadIdx.x == 0) a.part_err[blockIdx.x] = bm;
-    }
+    rk_final_body(m, kp, mtmp, st, a, N, blockIdx.x, gridDim.x);
 }
 
 __global__ void k_copy_if_active(float* __restrict__ dst, const float* __restrict__ src, long n,
                                  const EngineState* st) {
     if (!st->trial_active) return;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    copy_body(dst, src, n, blockIdx.x, gridDim.x);
 }
 
-// accepted: m = normalize(m_high); step ended after the last rejected trial:
-// m (restored, i.e. untouched) is normalised (rk.go:426-444)
 __global__ __launch_bounds__(256) void k_commit(float* __restrict__ m, const float* __restrict__ mtmp,
                                                 const EngineState* st, int N) {
-    if (!st->trial_active || !st->step_end) return;
-    const float* src = st->accept ? mtmp : m;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const float x = src[i], y = src[N + i], z = src[2 * N + i];
-        const float s = rsqrtf(x * x + y * y + z * z);
-        m[i] = x * s;
-        m[N + i] = y * s;
-        m[2 * N + i] = z * s;
-    }
+    commit_body(m, mtmp, st, N, blockIdx.x, gridDim.x);
 }
 
 __global__ void k_normalize_active(float* __restrict__ m, const EngineState* st, int N) {
@@ -134,89 +71,10 @@ __global__ void k_normalize_active(float* __restrict__ m, const EngineState* st,
     }
 }
 
-// ---------------------------------------------------------------- control (single thread)
-__global__ void k_ctrl_begin(EngineState* st) {
-    st->trial_active = !st->done;
-    if (!st->trial_active) return;
-    if (st->new_step) {
-        st->time0 = st->time;
-        st->trials = 0;
-        st->new_step = 0;
-    }
-    st->dt_used = st->dt;
-}
-
-// one wave reduces n per-block maxima; result valid in every lane
-__device__ __forceinline__ float wave_max_partials(const float* part, int n) {
-    float v = 0.f;
-    for (int i = threadIdx.x; i < n; i += 64) v = fmaxf(v, part[i]);
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// after k0 is known (rk.go:348-373); launched with one wave
-__global__ void k_ctrl_after_k0(EngineState* st, int fsal_copy, CtrlParams p) {
-    if (!st->trial_active) return;
-    const float maxTorque = wave_max_partials(fsal_copy ? p.part_k6 : p.part_k0, p.n_k);
-    if (threadIdx.x != 0) return;
-    if (!p.adaptive) st->dt = p.inputDt;
-    st->max_torque = maxTorque;
-    float dt = st->dt;
-    const float dm = dt * maxTorque;
-    if (dm < p.minDm) dt = p.minDm / maxTorque;
-    if (dt < p.minDt) dt = p.minDt;
-    if (p.maxDm != 0.f && dm > p.maxDm) dt = p.maxDm / maxTorque;
-    if (p.maxDt != 0.f && dt > p.maxDt) dt = p.maxDt;
-    if (isnan(dt)) st->nan_dt = 1;
-    st->dt = dt;
-    st->dt_used = dt;
-}
-
-// after the high/low solutions (rk.go:376-440)
-__global__ void k_ctrl_end(EngineState* st, CtrlParams p) {
-    if (!st->trial_active) return;
-    const float err_red = p.adaptive ? wave_max_partials(p.part_err, p.n_err) : 0.f;
-    const float k0_red = p.fold_k0 ? wave_max_partials(p.part_k0, p.n_k) : 0.f;
-    if (threadIdx.x != 0) return;
-    if (p.fold_k0) {
-        st->max_torque = k0_red;
-        if (!p.adaptive) st->dt = p.inputDt;
-    }
-    const float dtu = st->dt_used;
-    st->time = st->time0 + (double)dtu;
-    st->trials++;
-    st->trials_total++;
-    st->fsal_init = 1;
-    int bad = 0;
-    if (p.adaptive) {
-        const float err = err_red;
-        st->err = err;
-        float factor = (float)pow((double)(p.maxError / err), 1.0 / (double)p.errorOrder);
-        if (factor > 2.f) factor = 2.f;
-        if (factor < 0.01f) factor = 0.01f;
-        float dt = dtu * factor;
-        if (isnan(dt)) st->nan_dt = 1;
-        bad = err > 2.f * p.maxError;
-        if (!bad) {
-            if (dt < p.minDt) dt = p.minDt;
-            if (p.maxDt != 0.f && dt > p.maxDt) dt = p.maxDt;
-        }
-        st->dt = dt;
-    }
-    if (bad) st->rejects_total++;
-    st->accept = !bad;
-    st->step_end = !bad || st->trials >= kMaxStepTrials;
-    if (st->log_n < p.log_cap) {
-        p.log_dt[st->log_n] = (double)dtu;
-        p.log_accept[st->log_n] = !bad;
-    }
-    st->log_n++;
-    if (st->step_end) {
-        st->steps++;
-        st->new_step = 1;
-        if (st->time >= st->stop || st->steps >= st->steps_target || st->nan_dt) st->done = 1;
-    }
-}
+// ---------------------------------------------------------------- control (one thread / one wave)
+__global__ void k_ctrl_begin(EngineState* st) { ctrl_begin_body(st); }
+__global__ void k_ctrl_after_k0(EngineState* st, int fsal_copy, CtrlParams p) { ctrl_after_k0_body(st, fsal_copy, p); }
+__global__ void k_ctrl_end(EngineState* st, CtrlParams p) { ctrl_end_body(st, p); }
 
 // ---------------------------------------------------------------- FP64 output reductions
 __global__ __launch_bounds__(256) void k_sum_comp(const float* __restrict__ v, int N, int ncomp,

@@ -1,0 +1,92 @@
+// Host-side simulation state of the engine, shared by the single-simulation
+// driver (sim.hip) and the batched driver (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/gpukern.h"
+#include "../../include/mumax_engine.h"
+#include "engine.h"
+
+namespace gk {
+
+struct Tableau {
+    int S = 0, fsal = 0, adaptive = 0;
+    float order = 0;
+    float a[7][7] = {}, b[7] = {}, b2[7] = {}, c[7] = {};
+};
+
+}  // namespace gk
+
+using gk::ConvPlan;
+using gk::EngineState;
+using gk::FieldTable;
+using gk::Prof;
+using gk::Tableau;
+
+struct mmSim {
+    mmParams p;
+    int mode = MM_MODE_FUSED;
+    int N = 0;
+    long N3 = 0;
+    double unitLength = 0, unitTime = 0, unitField = 0, unitEnergy = 0, unitEdens = 0;
+    double cell[3] = {};  // internal units
+    float kAnis = 0;
+    float u[3] = {};
+    Tableau tb;
+    std::vector<double> fieldT;
+    std::vector<float> fieldB;
+    hipStream_t s = nullptr;
+    // device
+    float* m = nullptr;
+    float* k[7] = {};
+    float* mi = nullptr;
+    float* mtmp = nullptr;
+    EngineState* st = nullptr;
+    double* log_dt = nullptr;
+    int* log_acc = nullptr;
+    int log_cap = 1 << 16;
+    double* dsum = nullptr;
+    float* part = nullptr;  // 3 x npart: |k0|, |k_last|, error
+    int npart = 0;          // max(kRedBlocks, blocks of the fused Z inverse + torque pass)
+    int nk_part = 0;        // partial maxima written per torque evaluation
+    double* fT = nullptr;
+    float* fB = nullptr;
+    FieldTable field{};
+    ConvPlan cp{};
+    bool conv = false;
+    double kernelImagRatio = 0;
+    bool fsalHost = false;
+    unsigned long long bytes = 0;
+    Prof prof;
+    // ABI mode
+    gpuFFT3dPlan* plan = nullptr;
+    int Pabi[3] = {};
+    float* abuf[3] = {};
+    float* akern[6] = {};
+    float* am1 = nullptr;
+    float* abackup = nullptr;
+    float* redDev = nullptr;
+    std::vector<float> redHost;
+    int redBlocks = 0, redThreads = 0;
+    // host mirror of the reference's RK state for ABI mode
+    double htime = 0;
+    float hdt = 0;
+    long long hsteps = 0, htrials = 0, hrejects = 0;
+    float herr = 0, htorque = 0;
+    bool hfsal = false;
+    std::vector<double> hlog_dt;
+    std::vector<int> hlog_acc;
+};
+
+
+namespace gk {
+EngineState read_state(mmSim* s);
+// the dt clamps of rk.go:356-371 need max|k0| before stage 1; FSAL tableaux take
+// max|k0| from the previous trial's last stage
+bool needs_after_k0(const mmSim* s);
+CtrlParams ctrl_params(const mmSim* s);
+TorqueArgs torque_args(const mmSim* s, float c, float* part);
+void* dmalloc(mmSim* s, size_t bytes);
+}  // namespace gk

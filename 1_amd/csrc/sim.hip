@@ -19,16 +19,11 @@
 #include "../../include/gpukern.h"
 #include "../../include/mumax_engine.h"
 #include "engine.h"
+#include "sim.h"
 
 using namespace gk;
 
 namespace {
-
-struct Tableau {
-    int S = 0, fsal = 0, adaptive = 0;
-    float order = 0;
-    float a[7][7] = {}, b[7] = {}, b2[7] = {}, c[7] = {};
-};
 
 // rk.go:62-212, coefficients rounded to float32 as in the reference
 Tableau make_tableau(int solver) {
@@ -179,62 +174,7 @@ const double kGamma0 = 2.211e5;
 
 }  // namespace
 
-struct mmSim {
-    mmParams p;
-    int mode = MM_MODE_FUSED;
-    int N = 0;
-    long N3 = 0;
-    double unitLength = 0, unitTime = 0, unitField = 0, unitEnergy = 0, unitEdens = 0;
-    double cell[3] = {};  // internal units
-    float kAnis = 0;
-    float u[3] = {};
-    Tableau tb;
-    std::vector<double> fieldT;
-    std::vector<float> fieldB;
-    hipStream_t s = nullptr;
-    // device
-    float* m = nullptr;
-    float* k[7] = {};
-    float* mi = nullptr;
-    float* mtmp = nullptr;
-    EngineState* st = nullptr;
-    double* log_dt = nullptr;
-    int* log_acc = nullptr;
-    int log_cap = 1 << 16;
-    double* dsum = nullptr;
-    float* part = nullptr;  // 3 x npart: |k0|, |k_last|, error
-    int npart = 0;          // max(kRedBlocks, blocks of the fused Z inverse + torque pass)
-    int nk_part = 0;        // partial maxima written per torque evaluation
-    double* fT = nullptr;
-    float* fB = nullptr;
-    FieldTable field{};
-    ConvPlan cp{};
-    bool conv = false;
-    double kernelImagRatio = 0;
-    bool fsalHost = false;
-    unsigned long long bytes = 0;
-    Prof prof;
-    // ABI mode
-    gpuFFT3dPlan* plan = nullptr;
-    int Pabi[3] = {};
-    float* abuf[3] = {};
-    float* akern[6] = {};
-    float* am1 = nullptr;
-    float* abackup = nullptr;
-    float* redDev = nullptr;
-    std::vector<float> redHost;
-    int redBlocks = 0, redThreads = 0;
-    // host mirror of the reference's RK state for ABI mode
-    double htime = 0;
-    float hdt = 0;
-    long long hsteps = 0, htrials = 0, hrejects = 0;
-    float herr = 0, htorque = 0;
-    bool hfsal = false;
-    std::vector<double> hlog_dt;
-    std::vector<int> hlog_acc;
-};
-
-static void* dmalloc(mmSim* s, size_t bytes) {
+void* gk::dmalloc(mmSim* s, size_t bytes) {
     void* p = nullptr;
     GK_CHECK(hipMalloc(&p, bytes));
     s->bytes += bytes;
@@ -257,20 +197,18 @@ static void applied_field_host(const mmSim* s, double t_si, float out[3]) {
     for (int c = 0; c < 3; c++) out[c] = s->fieldB[3 * (i - 1) + c] + f * (s->fieldB[3 * i + c] - s->fieldB[3 * (i - 1) + c]);
 }
 
-static EngineState read_state(mmSim* s) {
+EngineState gk::read_state(mmSim* s) {
     EngineState h;
     GK_CHECK(hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, s->s));
     GK_CHECK(hipStreamSynchronize(s->s));
     return h;
 }
 
-static bool needs_after_k0(const mmSim* s) {
-    // the dt clamps of rk.go:356-371 need max|k0| before stage 1; FSAL tableaux take
-    // max|k0| from the previous trial's last stage
+bool gk::needs_after_k0(const mmSim* s) {
     return s->tb.fsal || s->p.minDm != 0 || s->p.maxDm != 0 || s->p.minDt != 0 || s->p.maxDt != 0;
 }
 
-static CtrlParams ctrl_params(const mmSim* s) {
+CtrlParams gk::ctrl_params(const mmSim* s) {
     CtrlParams c{};
     c.part_k0 = s->part;
     c.part_k6 = s->part + s->npart;
@@ -292,7 +230,7 @@ static CtrlParams ctrl_params(const mmSim* s) {
     return c;
 }
 
-static TorqueArgs torque_args(const mmSim* s, float c, float* part) {
+TorqueArgs gk::torque_args(const mmSim* s, float c, float* part) {
     TorqueArgs a{};
     a.alpha = (float)s->p.alpha;
     a.dtg = (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha));
@@ -840,6 +778,28 @@ double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts) {
     GK_CHECK(hipEventDestroy(e0));
     GK_CHECK(hipEventDestroy(e1));
     return total;
+}
+
+unsigned long long mm_sim_footprint(const mmParams* p) {
+    const long N3 = 3L * p->size[0] * p->size[1] * p->size[2];
+    const Tableau t = make_tableau(p->solver);
+    int epi = 0;
+    size_t b = conv_plan_footprint(p->size, &epi);
+    const int npart = std::max(kRedBlocks, epi);
+    b += N3 * sizeof(float) * (size_t)(1 + t.S + 2);  // m, k_0..k_{S-1}, m_i, m_tmp
+    b += sizeof(EngineState) + (size_t)(1 << 16) * (sizeof(double) + sizeof(int)) + 4 * sizeof(double);
+    b += 3 * (size_t)npart * sizeof(float);
+    if (p->nField >= 2) b += (size_t)p->nField * (sizeof(double) + 3 * sizeof(float));
+    return b;
+}
+
+void mm_sim_begin_run(mmSim* s, double seconds) {
+    GK_ASSERT(s->mode == MM_MODE_FUSED, "mm_sim_begin_run needs the fused mode");
+    EngineState h = read_state(s);
+    const double stop = h.time + seconds / s->unitTime;
+    launch_normalize_active(s->m, nullptr, s->N, s->s);  // simrun.go:24
+    launch_set_target(s->st, -1, stop, s->s);
+    GK_CHECK(hipStreamSynchronize(s->s));
 }
 
 double mm_sim_time_pass(mmSim* s, int cls, int n) {

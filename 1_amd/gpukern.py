@@ -84,6 +84,16 @@ ENGINE = {
     "mm_sim_time_steps": (C.c_double, [P, I]),
     "mm_sim_profile_steps": (C.c_double, [P, I, P, P]),
     "mm_sim_time_pass": (C.c_double, [P, I, I]),
+    "mm_sim_footprint": (C.c_ulonglong, [P]),
+    "mm_sim_begin_run": (None, [P, C.c_double]),
+    "mm_batch_new": (P, [P, I]),
+    "mm_batch_free": (None, [P]),
+    "mm_batch_set_graph": (None, [P, I]),
+    "mm_batch_size": (I, [P]),
+    "mm_batch_start": (None, [P, P, P]),
+    "mm_batch_trials": (I, [P, I]),
+    "mm_batch_done": (None, [P, P]),
+    "mm_batch_run": (C.c_longlong, [P, P, I]),
 }
 
 

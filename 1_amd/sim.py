@@ -316,3 +316,66 @@ class Sim:
         self._ptr = self._lib.mm_sim_new(C.byref(p), m0.ctypes.data, MODES[self.mode])
         if not self._ptr:
             raise RuntimeError("mm_sim_new failed")
+
+
+class Batch:
+    """Many fused-mode simulations advanced together (mm_batch_*): one launch
+    per pass and padded length per RK stage for the whole batch, per-simulation
+    step control on the device, one captured HIP graph per batched trial.
+
+    The simulations stay usable alone; a simulation computes the same numbers
+    in a batch as alone.  `run(seconds)` is Run() for every member (seconds:
+    one value or one per simulation)."""
+
+    def __init__(self, sims, graph=True):
+        if not sims:
+            raise InputErr("empty batch")
+        self.sims = list(sims)
+        for s in self.sims:
+            if s.mode != "fused":
+                raise InputErr("a batch needs fused-mode simulations")
+            s._init()
+        self._lib = self.sims[0]._lib
+        ptrs = (C.c_void_p * len(self.sims))(*[s._ptr for s in self.sims])
+        self._ptr = self._lib.mm_batch_new(ptrs, len(self.sims))
+        if not self._ptr:
+            raise RuntimeError("mm_batch_new failed")
+        self._lib.mm_batch_set_graph(self._ptr, int(bool(graph)))
+
+    def __len__(self):
+        return len(self.sims)
+
+    def _seconds(self, seconds):
+        v = np.broadcast_to(np.asarray(seconds, np.float64), (len(self.sims),))
+        return np.ascontiguousarray(v)
+
+    def start(self, seconds=None, steps=None):
+        sec = None if seconds is None else self._seconds(seconds)
+        st = None if steps is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(steps, np.int64), (len(self.sims),)))
+        self._lib.mm_batch_start(self._ptr, None if sec is None else sec.ctypes.data,
+                                 None if st is None else st.ctypes.data)
+
+    def trials(self, n):
+        """Enqueue n batched trials and wait; returns the simulations not yet done."""
+        return int(self._lib.mm_batch_trials(self._ptr, int(n)))
+
+    def done(self):
+        out = np.zeros(len(self.sims), np.int32)
+        self._lib.mm_batch_done(self._ptr, out.ctypes.data)
+        return out.astype(bool)
+
+    def run(self, seconds, poll=16):
+        sec = self._seconds(seconds)
+        return int(self._lib.mm_batch_run(self._ptr, sec.ctypes.data, int(poll)))
+
+    def free(self):
+        if self._ptr:
+            self._lib.mm_batch_free(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

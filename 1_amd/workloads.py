@@ -115,6 +115,49 @@ def film_vortex_ramp(seed=3, size=(1, 1024, 1024), ramp_time=2e-9, bmax=50e-3, c
     )
 
 
+SWEEP_SIZES = tuple(range(32, 257, 16))  # user Nx, Ny
+SWEEP_THICK = (1, 2, 4)                  # user Nz
+
+
+def sweep_member(seed, j, duration=0.5e-9, sizes=SWEEP_SIZES, thick=SWEEP_THICK, cell=3e-9):
+    """Simulation j of the batched sweep: user Nx, Ny uniform in `sizes`, Nz in
+    `thick`, cubes of `cell`; Ms ~ U[4e5, 1.2e6] A/m, A ~ U[0.8e-11, 2e-11] J/m,
+    alpha ~ U[0.01, 0.5], B_ext of isotropic direction with |B| ~ U[0, 100] mT;
+    m0 uniform on the sphere per cell; Dormand-Prince, maxError 1e-5.  Every
+    draw comes from SplitMix64 streams keyed by (seed, j)."""
+    key = (seed * 1000003 + j) & MASK64
+    u = uniform01(key, 10)
+    nx = sizes[int(u[0] * len(sizes)) % len(sizes)]
+    ny = sizes[int(u[1] * len(sizes)) % len(sizes)]
+    nz = thick[int(u[2] * len(thick)) % len(thick)]
+    msat = 4e5 + u[3] * 8e5
+    aexch = 0.8e-11 + u[4] * 1.2e-11
+    alpha = 0.01 + u[5] * 0.49
+    cz = 2 * u[6] - 1  # isotropic direction: cos(theta) uniform, phi uniform
+    ph = 2 * np.pi * u[7]
+    sz = np.sqrt(max(0.0, 1 - cz * cz))
+    b = u[8] * 100e-3
+    bx, by, bz = b * sz * np.cos(ph), b * sz * np.sin(ph), b * cz  # user axes
+    X, Y, Z = nz, ny, nx
+    N = X * Y * Z
+    m = normalize(normal(key ^ 0x5DEECE66D, 3 * N).reshape(3, X, Y, Z))
+    return dict(
+        name="sweep%d_%d" % (seed, j),
+        size=(X, Y, Z),
+        cellsize=(cell, cell, cell),
+        msat=msat, aexch=aexch, alpha=alpha,
+        solver="rkdp", maxerror=1e-5, dt=0.0,
+        field=("static", (bz, by, bx)),  # internal (X, Y, Z) = user (z, y, x)
+        duration=duration,
+        m0=m.astype(np.float32),
+    )
+
+
+def sweep(n=512, seed=4, duration=0.5e-9, **kw):
+    """The batched sweep: n independent simulations (see sweep_member)."""
+    return [sweep_member(seed, j, duration, **kw) for j in range(n)]
+
+
 def make_sim(cfg, mode="fused"):
     """Build a ``Sim`` from a workload dict using the reference's commands."""
     from .sim import Sim

@@ -125,6 +125,78 @@ def calibrate_copy(lib, n=32 << 20, reps=3):
     return {"kernel": "gpu_linear_combination", "launches": reps, "fetch_bytes": 8 * n, "write_bytes": 4 * n}
 
 
+def bench_sweep(args, mm):
+    """Batched sweep: all members in one batch (mm_batch_*) against members run
+    one at a time on the same GPU.  Setup (kernel spectra) is excluded from
+    both; latency = time from the start of the batch to each member's done flag
+    (members all submitted at t = 0)."""
+    cfgs = mm.workloads.sweep(args.sweep_sims, seed=4, duration=args.sweep_ns * 1e-9)
+    n = len(cfgs)
+    t0 = time.perf_counter()
+    sims = [mm.workloads.make_sim(c) for c in cfgs]
+    for s in sims:
+        s._init()
+    b = mm.Batch(sims)
+    setup = time.perf_counter() - t0
+    b.start(seconds=[c["duration"] for c in cfgs])
+    done_at = np.full(n, np.nan)
+    trials = 0
+    t1 = time.perf_counter()
+    while True:
+        active = b.trials(args.poll)
+        trials += args.poll
+        now = time.perf_counter() - t1
+        d = b.done()
+        done_at[d & np.isnan(done_at)] = now
+        if active == 0:
+            break
+    t_batch = time.perf_counter() - t1
+    steps = np.array([s.state()["steps"] for s in sims])
+    cells = np.array([np.prod(c["size"]) for c in cfgs])
+    b.free()
+    for s in sims:
+        s.free()
+    # serial: an evenly spaced sample of members, each alone
+    k = min(n, args.serial_sample)
+    pick = np.unique(np.linspace(0, n - 1, k).astype(int))
+    t_alone = []
+    for j in pick:
+        s = mm.workloads.make_sim(cfgs[j])
+        s._init()
+        s.sync()
+        ta = time.perf_counter()
+        s.run(cfgs[j]["duration"])
+        t_alone.append(time.perf_counter() - ta)
+        s.free()
+    t_alone = np.array(t_alone)
+    serial_rate = 1.0 / t_alone.mean()
+    rate = n / t_batch
+    return {
+        "metric": "sweep solves/s",
+        "value": round(rate, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded sweep (SplitMix64 seed 4), random m0 on the sphere",
+        "config": {"workload": "sweep_%d_sims_%gns_rkdp" % (n, args.sweep_ns), "sims": n,
+                   "cells_total": int(cells.sum()), "solver": "rkdp", "max_error": 1e-5},
+        "batch_seconds": round(t_batch, 4),
+        "batched_trials": trials,
+        "accepted_steps_total": int(steps.sum()),
+        "mcell_steps_per_s": round(float((steps * cells).sum()) / t_batch / 1e6, 2),
+        "latency_s": {"p50": round(float(np.percentile(done_at, 50)), 4),
+                      "p99": round(float(np.percentile(done_at, 99)), 4)},
+        "serial": {"sample": int(len(pick)), "mean_s_per_sim": round(float(t_alone.mean()), 5),
+                   "solves_per_s": round(serial_rate, 3),
+                   "latency_s": {"p50": round(float(np.percentile(t_alone, 50)), 4),
+                                 "p99": round(float(np.percentile(t_alone, 99)), 4)}},
+        "speedup_vs_serial": round(rate / serial_rate, 2),
+        "setup_seconds": round(setup, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +207,12 @@ def main():
                     help="back-to-back launches of the dominant pass for its average duration")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["cuboid", "sweep"], default="cuboid",
+                    help="cuboid: the single-simulation RK4 step (default); sweep: the batched sweep")
+    ap.add_argument("--sweep-sims", type=int, default=512)
+    ap.add_argument("--sweep-ns", type=float, default=0.5)
+    ap.add_argument("--poll", type=int, default=16, help="batched trials between done-flag polls")
+    ap.add_argument("--serial-sample", type=int, default=32, help="sweep members timed alone for the serial rate")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch a streaming kernel of known traffic (for HBM counter scaling)")
     args = ap.parse_args()
@@ -145,6 +223,9 @@ def main():
 
     mm = importlib.import_module("1_amd")
     lib = mm.load()
+    if args.workload == "sweep":
+        print(json.dumps(bench_sweep(args, mm)), flush=True)
+        return
     cfg = mm.workloads.cuboid_rk4()
     N = int(np.prod(cfg["size"]))
     sim = mm.workloads.make_sim(cfg)

@@ -115,6 +115,40 @@ double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts);
  * unchanged; the next step recomputes everything they touch. */
 double mm_sim_time_pass(mmSim* s, int cls, int n);
 
+/* device bytes mm_sim_new(p, ..., MM_MODE_FUSED) allocates (admission by HBM
+ * footprint); the kernel set-up needs transient FP64 buffers beyond this */
+unsigned long long mm_sim_footprint(const mmParams* p);
+/* Run()'s set-up without stepping (fused mode): m normalised, stop time =
+ * time + seconds (simrun.go:19-24); the trials are then driven by a batch
+ * (mm_batch_trials), which is how a running batch takes in a new request. */
+void mm_sim_begin_run(mmSim* s, double seconds);
+
+/* ---- batched engine ----------------------------------------------------
+ * n fused-mode simulations sharing the solver (any sizes, materials, fields)
+ * advanced together: every RK stage is one launch per pass and padded length
+ * for the whole batch, step control stays per simulation on the device, and a
+ * batched trial is captured once into a HIP graph and replayed.  The batch
+ * references the simulations (they stay owned by the caller and usable alone
+ * after mm_batch_free); a simulation's arithmetic is the same alone and in a
+ * batch. */
+typedef struct mmBatch mmBatch;
+mmBatch* mm_batch_new(mmSim* const* sims, int n);
+void mm_batch_free(mmBatch* b);
+/* 1 (default): replay a captured graph per trial; 0: enqueue the launches */
+void mm_batch_set_graph(mmBatch* b, int on);
+int mm_batch_size(mmBatch* b);
+/* Run() set-up: simulation j stops at time_j + seconds[j], m normalised first
+ * (simrun.go:19-24); or, seconds null, Steps(): steps[j] more steps, no
+ * normalisation (simrun.go:85-92).  steps null: no step target. */
+void mm_batch_start(mmBatch* b, const double* seconds, const long long* steps);
+/* enqueue n batched trials and wait; returns the simulations not yet done */
+int mm_batch_trials(mmBatch* b, int n);
+/* done[j] = 1 once simulation j reached its stop time / step target */
+void mm_batch_done(mmBatch* b, int* done);
+/* mm_batch_start(seconds) then trials until all are done (polling every
+ * `poll` trials); returns the batched trials enqueued */
+long long mm_batch_run(mmBatch* b, const double* seconds, int poll);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@ import re
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "1_amd", "libgpukern.so")
 
-DECL = re.compile(r"^\s*(?:unsigned\s+long\s+long\s+int|unsigned\s+long\s+long|[A-Za-z_][\w]*)\s*\*?\s*"
+DECL = re.compile(r"^\s*(?:unsigned\s+long\s+long\s+int|unsigned\s+long\s+long|long\s+long|[A-Za-z_][\w]*)\s*\*?\s*"
                   r"\**\s*([A-Za-z_]\w*)\s*\(", re.M)
 
 

@@ -129,21 +129,32 @@ def test_rk4_steps_vs_oracle(mm, orc, size, cell):
     assert st["time"] == pytest.approx(o.state()["time"], rel=1e-6)
 
 
-def replay_in_oracle(mm, orc, cfg, s):
-    """Replay the GPU's trial log (dt, accepted) step by step in the oracle `orc`
-    started from the same m0; returns the oracle sim."""
-    dts, acc = s.trial_log()
-    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
-    # Run() normalises first (simrun.go:24)
-    m3 = o.m.reshape(3, -1)
-    o.m = (m3 / np.linalg.norm(m3, axis=0)).reshape(-1)
-    i = 0
-    while i < len(dts):
+def replay_log(o, dts, acc, start=0, stop=None):
+    """Replay trials [start, stop) of a GPU trial log (dt, accepted) step by
+    step in oracle sim `o`; returns the index reached."""
+    i, stop = start, len(dts) if stop is None else stop
+    while i < stop:
         j = i
         while not acc[j] and j - i < 9:  # a step: rejected trials, then the accepted one (MAX_STEP_TRIALS 10)
             j += 1
         o.step_replay(dts[i:j + 1], acc[i:j + 1])
         i = j + 1
+    return i
+
+
+def normalise_oracle(o):
+    m3 = o.m.reshape(3, -1)
+    o.m = (m3 / np.linalg.norm(m3, axis=0)).reshape(-1)
+
+
+def replay_in_oracle(mm, orc, cfg, s):
+    """Replay the GPU's trial log step by step in the oracle `orc` started
+    from the same m0 (normalised, as Run() does first, simrun.go:24); returns
+    the oracle sim."""
+    dts, acc = s.trial_log()
+    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    normalise_oracle(o)
+    replay_log(o, dts, acc)
     return o
 
 
@@ -191,3 +202,103 @@ def test_film_1ns_dp_inside_fp32_envelope_of_fp64_oracle(mm, orc):
     assert st["steps"] == o64.state()["steps"] == o32.state()["steps"]
     assert e_gpu <= max(1e-5, 2 * e_32)
     assert a_gpu <= max(1e-5, 2 * a_32)
+
+
+def _crossing(ts, ys, level):
+    """First time the sampled curve ys(ts) reaches `level` (linear interpolation), or None."""
+    for i in range(1, len(ys)):
+        if (ys[i - 1] - level) * (ys[i] - level) <= 0 and ys[i] != ys[i - 1]:
+            return ts[i - 1] + (level - ys[i - 1]) / (ys[i] - ys[i - 1]) * (ts[i] - ts[i - 1])
+    return None
+
+
+def test_vortex_ramp_precision_vs_fp64_oracle(mm, orc):
+    """Precision study on a reduced vortex film (64x64x1 of 2 nm cubes, alpha 0.5,
+    Dormand-Prince 1e-5, in-plane field ramp 0 -> 50 mT over 0.5 ns; the same
+    physics as the 1024x1024 study at an oracle-sized grid).  The GPU runs in 40
+    chunks (each Run() renormalises first, simrun.go:24); the FP64 oracle replays
+    the same trials chunk by chunk.  Reported and checked at every chunk end:
+    max |dm|, the relative error of the FP64-reduced energy, <m>, and the
+    switching time (<m> along the field reaching 0.5)."""
+    T, nch = 0.5e-9, 40
+    cfg = mm.workloads.film_vortex_ramp(seed=3, size=(1, 64, 64), ramp_time=T)
+    s = mm.workloads.make_sim(cfg, "fused")
+    ts, marks, avg_g, e_g, m_g = [], [], [], [], []
+    for k in range(nch):
+        s.run(T / nch)
+        st = s.state()
+        ts.append(st["time"] * st["unit_time"])
+        marks.append(len(s.trial_log()[0]))
+        avg_g.append(s.average_m())
+        e_g.append(s.energy())
+        m_g.append(s.m.reshape(3, -1).astype(np.float64))
+    dts, acc = s.trial_log()
+    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    i, err, rel_e, d_avg = 0, 0.0, 0.0, 0.0
+    avg_o = []
+    for k in range(nch):
+        normalise_oracle(o)  # each Run() normalises first
+        i = replay_log(o, dts, acc, i, marks[k])
+        mo = o.m.reshape(3, -1)
+        avg_o.append(mo.mean(axis=1))
+        err = max(err, float(np.abs(m_g[k] - mo).max()))
+        eo = o.energy()
+        rel_e = max(rel_e, abs(e_g[k] - eo) / abs(eo))
+        d_avg = max(d_avg, float(np.abs(np.asarray(avg_g[k]) - avg_o[k]).max()))
+    tsw_g = _crossing(ts, [a[2] for a in avg_g], 0.5)
+    tsw_o = _crossing(ts, [a[2] for a in avg_o], 0.5)
+    print("vortex 64x64: trials %d; vs FP64 oracle: max|dm| %.3g, max rel. energy error %.3g, max|d<m>| %.3g; "
+          "switching time GPU %s ns, oracle %s ns" % (len(dts), err, rel_e, d_avg,
+                                                      tsw_g and "%.5f" % (tsw_g * 1e9), tsw_o and "%.5f" % (tsw_o * 1e9)))
+    assert err <= 1e-5
+    assert rel_e <= 1e-5
+    assert tsw_o is not None and tsw_g is not None
+    assert abs(tsw_g - tsw_o) <= 1e-3 * T
+
+
+def test_vortex_1024_precision_study(mm, orc):
+    """The 1024x1024x1 vortex film (2 nm cubes, alpha 0.5, Dormand-Prince 1e-5,
+    in-plane field ramp 0 -> 50 mT over 2 ns), at full size:
+    - the first 5 steps replayed in the FP64 oracle: max |dm| <= 1e-5;
+    - the 2 ns run in 20 chunks: <m> and energy from the FP64 device reductions,
+      the switching time (<m> along the field reaching 0.5), energy decreasing
+      at fixed field is not required (the field ramps), the state stays unit;
+    - the energy of the final state (zero field, t = 0) from the GPU pipeline
+      (FP32 fields, FP64 sum) against the FP64 oracle: relative error <= 1e-5."""
+    cfg = mm.workloads.film_vortex_ramp(seed=3)
+    T, nch = cfg["duration"], 20
+    s5 = mm.workloads.make_sim(cfg, "fused")
+    s5.steps(5)
+    m5 = s5.m.reshape(3, -1).astype(np.float64)
+    dts5, acc5 = s5.trial_log()
+    s5.free()
+    s = mm.workloads.make_sim(cfg, "fused")
+    ts, avg = [], []
+    for k in range(nch):
+        s.run(T / nch)
+        st = s.state()
+        ts.append(st["time"] * st["unit_time"])
+        avg.append(s.average_m())
+    mf = s.m
+    st = s.state()
+    assert np.abs(np.linalg.norm(mf, axis=0) - 1).max() < 1e-5
+    tsw = _crossing(ts, [a[2] for a in avg], 0.5)
+    s.free()
+    s2 = mm.workloads.make_sim(cfg, "fused")
+    s2.setm_array(mf)
+    e_gpu = s2.energy()
+    s2.free()
+    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    o.m = mf.reshape(-1).astype(np.float64)
+    e_o = o.energy()
+    rel_e = abs(e_gpu - e_o) / abs(e_o)
+    o.m = cfg["m0"].reshape(-1).astype(np.float64)
+    normalise_oracle(o)
+    replay_log(o, dts5, acc5)
+    err5 = float(np.abs(m5 - o.m.reshape(3, -1)).max())
+    print("vortex 1024x1024, 2 ns: steps %d, trials %d, rejects %d; switching time %s ns; <m> at 2 ns %s; "
+          "first 5 steps max|dm| vs FP64 oracle %.3g; final-state energy GPU %.9g vs FP64 %.9g (rel. %.3g)"
+          % (st["steps"], st["trials"], st["rejects"], tsw and "%.4f" % (tsw * 1e9), np.round(avg[-1], 4), err5,
+             e_gpu, e_o, rel_e))
+    assert err5 <= 1e-5
+    assert rel_e <= 1e-5
