@@ -197,6 +197,70 @@ def bench_sweep(args, mm):
     }
 
 
+def bench_serve(args, mm):
+    """Continuous batching of a Poisson request stream against one-at-a-time
+    service in arrival order.  The serial side runs `--serve-serial` requests
+    (all by default) one at a time; when that is a sample, every request's
+    service time is taken from the sample member with the nearest cells x
+    duration and the FIFO queue is computed from the arrivals (stated in
+    the output)."""
+    sv = importlib.import_module("1_amd.serve")
+    reqs = sv.request_stream(args.serve_requests, rate=args.serve_rate, seed=5)
+    n = len(reqs)
+    srv = sv.Server(poll=args.poll)
+    stats = srv.serve(reqs, max_seconds=args.serve_max_seconds)
+    lat = sv.latency_stats(reqs)
+    rate = n / stats["elapsed"]
+    work = np.array([np.prod(r.cfg["size"]) * r.duration for r in reqs])
+    k = n if args.serve_serial <= 0 else min(n, args.serve_serial)
+    pick = np.unique(np.linspace(0, n - 1, k).astype(int))
+    service = {}
+    for j in pick:
+        r = reqs[j]
+        s = mm.workloads.make_sim(r.cfg)
+        ta = time.perf_counter()
+        s._init()
+        s.run(r.duration)
+        s.sync()
+        service[j] = time.perf_counter() - ta
+        s.free()
+    svc = np.array([service[j] if j in service else service[min(service, key=lambda i: abs(np.log(work[i] / work[j])))]
+                    for j in range(n)])
+    t_free, fin = 0.0, np.zeros(n)
+    arr = np.array([r.arrival for r in reqs])
+    for j in np.argsort(arr, kind="stable"):
+        t_free = max(t_free, arr[j]) + svc[j]
+        fin[j] = t_free
+    slat = fin - arr
+    serial_rate = n / fin.max()
+    return {
+        "metric": "served solves/s",
+        "value": round(rate, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Poisson request stream (SplitMix64 seed 5)",
+        "config": {"workload": "serve_%d_requests_poisson_%g_per_s" % (n, args.serve_rate), "requests": n,
+                   "cells_total": int(sum(np.prod(r.cfg["size"]) for r in reqs)), "solver": "rkdp",
+                   "hbm_budget_bytes": sv.HBM_BYTES},
+        "elapsed_s": round(stats["elapsed"], 3),
+        "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4), "mean": round(lat["mean"], 4)},
+        "batch_rebuilds": stats["rebuilds"],
+        "max_concurrent": stats["max_active"],
+        "max_device_bytes": stats["max_bytes"],
+        "serial": {"measured": int(len(pick)), "of": n,
+                   "service_s_total": round(float(svc.sum()), 3),
+                   "solves_per_s": round(serial_rate, 3),
+                   "latency_s": {"p50": round(float(np.percentile(slat, 50)), 4),
+                                 "p99": round(float(np.percentile(slat, 99)), 4)},
+                   "note": "one request at a time in arrival order (FIFO)" +
+                           ("" if len(pick) == n else "; unmeasured service times from the nearest measured request")},
+        "speedup_vs_serial": round(rate / serial_rate, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,8 +271,13 @@ def main():
                     help="back-to-back launches of the dominant pass for its average duration")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["cuboid", "sweep"], default="cuboid",
-                    help="cuboid: the single-simulation RK4 step (default); sweep: the batched sweep")
+    ap.add_argument("--workload", choices=["cuboid", "sweep", "serve"], default="cuboid",
+                    help="cuboid: the single-simulation RK4 step (default); sweep: the batched sweep; "
+                         "serve: continuous batching of a request stream")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-serial", type=int, default=0, help="requests run alone for the serial side (0: all)")
+    ap.add_argument("--serve-max-seconds", type=float, default=900.0)
     ap.add_argument("--sweep-sims", type=int, default=512)
     ap.add_argument("--sweep-ns", type=float, default=0.5)
     ap.add_argument("--poll", type=int, default=16, help="batched trials between done-flag polls")
@@ -225,6 +294,9 @@ def main():
     lib = mm.load()
     if args.workload == "sweep":
         print(json.dumps(bench_sweep(args, mm)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args, mm)), flush=True)
         return
     cfg = mm.workloads.cuboid_rk4()
     N = int(np.prod(cfg["size"]))

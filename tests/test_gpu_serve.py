@@ -1,0 +1,41 @@
+"""Continuous batching (serve.Server): requests that arrive while others run
+are admitted into the running batch, and every result equals the same request
+run alone (bit for bit); admission respects the HBM budget."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sv():
+    return importlib.import_module("1_amd.serve")
+
+
+def small_stream(sv, n, seed):
+    return sv.request_stream(n, rate=400.0, seed=seed, max_cells=(96, 96, 4), dmin=0.004e-9, dmax=0.02e-9)
+
+
+def test_each_result_same_alone_and_in_the_mix(mm, sv):
+    reqs = small_stream(sv, 10, seed=21)
+    srv = sv.Server(poll=4, keep_results=True)
+    stats = srv.serve(reqs, max_seconds=300)
+    assert stats["max_active"] > 1  # requests overlapped on the device
+    assert all(r.finish is not None and r.finish >= r.arrival for r in reqs)
+    for r in reqs:
+        s = mm.workloads.make_sim(r.cfg)
+        s.run(r.duration)
+        assert np.array_equal(s.m, srv.results[r.rid]), r.rid
+        s.free()
+
+
+def test_admission_stays_within_the_hbm_budget(mm, sv):
+    reqs = small_stream(sv, 8, seed=22)
+    fp = sorted(sv.footprint(r.cfg) for r in reqs)
+    budget = fp[-1] + fp[-2]  # at most about two requests at a time
+    srv = sv.Server(hbm_bytes=budget, poll=4)
+    stats = srv.serve(reqs, max_seconds=300)
+    assert stats["max_bytes"] <= budget
+    assert all(r.finish is not None for r in reqs)
