@@ -24,6 +24,17 @@ void fft_rows_r2c(const float* in, float2* out, int P, long nrows, long n_inner,
 void fft_rows_c2r(const float2* in, float* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
                   long out_inner, long out_outer, int n_real, hipStream_t s);
 
+// The same for any length (gk_fft_any.hip): power-of-two lengths go to the
+// kernels above, lengths with factors 2, 3, 5, 7 to LDS Stockham passes with a
+// runtime radix list, others through Bluestein's chirp-z convolution.
+template <typename T, int DIR>
+void fft_lines_any(C2<T>* data, int L, long nlines, long n_inner, long inner, long outer, long estride, int nvalid,
+                   int nkeep, hipStream_t s);
+void fft_rows_r2c_any(const float* in, float2* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
+                      long out_inner, long out_outer, int n_real, hipStream_t s);
+void fft_rows_c2r_any(const float2* in, float* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
+                      long out_inner, long out_outer, int n_real, hipStream_t s);
+
 // in-place 3-D complex FFT in FP64 of [n0][n1][n2] (dir -1 forward, +1 inverse)
 void fft3_c2c_f64(double2* d, const int n[3], int dir, hipStream_t s);
 

@@ -280,16 +280,11 @@ struct gpuFFT3dPlan {
 extern "C" {
 
 gpuFFT3dPlan* new_gpuFFT3dPlan_padded(int* size, int* paddedSize) {
-    GK_ASSERT(paddedSize[0] > 0 && paddedSize[1] > 1 && paddedSize[2] > 1,
+    GK_ASSERT(paddedSize[0] > 0 && paddedSize[1] > 0 && paddedSize[2] > 1,
               "new_gpuFFT3dPlan_padded: padded size %d x %d x %d", paddedSize[0], paddedSize[1], paddedSize[2]);
     for (int i = 0; i < 3; i++)
         GK_ASSERT(size[i] > 0 && size[i] <= paddedSize[i], "new_gpuFFT3dPlan_padded: size[%d]=%d padded=%d", i,
                   size[i], paddedSize[i]);
-    GK_ASSERT(paddedSize[2] % 2 == 0, "padded Z size must be even (got %d)", paddedSize[2]);
-    for (int i = 0; i < 2; i++)
-        if (paddedSize[i] > 1)
-            GK_ASSERT(fft_length_supported(paddedSize[i]), "FFT length %d is not supported", paddedSize[i]);
-    GK_ASSERT(fft_length_supported(paddedSize[2] / 2), "real FFT length %d is not supported", paddedSize[2]);
     gpuFFT3dPlan* p = new gpuFFT3dPlan;
     for (int i = 0; i < 3; i++) {
         p->size[i] = size[i];
@@ -309,11 +304,11 @@ void gpuFFT3dPlan_forward(gpuFFT3dPlan* p, float* input, float* output) {
     float2* out = (float2*)output;
     hipStream_t s = stream();
     // Z: real rows (x<N0, y<N1) -> out[x][y][0..Q)
-    fft_rows_r2c(input, out, P2, (long)N0 * N1, N1, N2, (long)N1 * N2, Q, (long)P1 * Q, N2, s);
+    fft_rows_r2c_any(input, out, P2, (long)N0 * N1, N1, N2, (long)N1 * N2, Q, (long)P1 * Q, N2, s);
     // Y: columns (x<N0, kz) over y; rows y>=N1 are zero padding and not read
-    if (P1 > 1) fft_lines<float, -1>(out, P1, (long)N0 * Q, Q, 1, (long)P1 * Q, Q, N1, P1, s);
+    if (P1 > 1) fft_lines_any<float, -1>(out, P1, (long)N0 * Q, Q, 1, (long)P1 * Q, Q, N1, P1, s);
     // X: every (ky,kz) column; planes x>=N0 are zero padding and not read
-    if (P0 > 1) fft_lines<float, -1>(out, P0, (long)P1 * Q, (long)P1 * Q, 1, 0, (long)P1 * Q, N0, P0, s);
+    if (P0 > 1) fft_lines_any<float, -1>(out, P0, (long)P1 * Q, (long)P1 * Q, 1, 0, (long)P1 * Q, N0, P0, s);
     sync();
 }
 
@@ -323,11 +318,11 @@ void gpuFFT3dPlan_inverse(gpuFFT3dPlan* p, float* input, float* output) {
     float2* in = (float2*)input;
     hipStream_t s = stream();
     // X: all columns, only planes x<N0 are kept
-    if (P0 > 1) fft_lines<float, +1>(in, P0, (long)P1 * Q, (long)P1 * Q, 1, 0, (long)P1 * Q, P0, N0, s);
+    if (P0 > 1) fft_lines_any<float, +1>(in, P0, (long)P1 * Q, (long)P1 * Q, 1, 0, (long)P1 * Q, P0, N0, s);
     // Y: columns of the kept planes, only rows y<N1 are kept
-    if (P1 > 1) fft_lines<float, +1>(in, P1, (long)N0 * Q, Q, 1, (long)P1 * Q, Q, P1, N1, s);
+    if (P1 > 1) fft_lines_any<float, +1>(in, P1, (long)N0 * Q, Q, 1, (long)P1 * Q, Q, P1, N1, s);
     // Z: complex rows (x<N0, y<N1) -> first N2 reals
-    fft_rows_c2r(in, output, P2, (long)N0 * N1, N1, Q, (long)P1 * Q, N2, (long)N1 * N2, N2, s);
+    fft_rows_c2r_any(in, output, P2, (long)N0 * N1, N1, Q, (long)P1 * Q, N2, (long)N1 * N2, N2, s);
     sync();
 }
 

@@ -282,8 +282,14 @@ def test_zeropad_exact(lib):
 
 
 # ------------------------------------------------------------------ FFT plan (gpu_fft0.cu, fft_test.go)
-@pytest.mark.parametrize("size", [(1, 16, 16), (1, 32, 128), (4, 16, 32), (2, 8, 4), (16, 64, 64)])
+@pytest.mark.parametrize("size", [
+    (1, 16, 16), (1, 32, 128), (4, 16, 32), (2, 8, 4), (16, 64, 64),   # powers of two
+    (1, 96, 96), (3, 5, 12), (1, 160, 224), (2, 48, 80), (5, 7, 9),     # 2^a 3^b 5^c 7^d (fft_test.go's 32 i)
+    (1, 11, 13), (2, 37, 22), (3, 17, 40), (1, 101, 64),                 # prime factors: Bluestein
+])
 def test_fft_plan_padded(lib, size):
+    """The drop-in plan against numpy at the Go driver's padded sizes 2N
+    (kernel.go:80-90), any N."""
     P = tuple(2 * s if s > 1 else 1 for s in size)
     rng = np.random.default_rng(8)
     x = rng.random(size).astype(F32)
@@ -302,6 +308,28 @@ def test_fft_plan_padded(lib, size):
     norm = lib.gpuFFT3dPlan_normalization(plan)
     assert norm == np.prod(P)
     assert np.abs(back.get().reshape(size) / norm - x).max() < 1e-4  # fft_test.go tolerance
+    lib.delete_gpuFFT3dPlan(plan)
+
+
+@pytest.mark.parametrize("size,P", [((2, 9, 7), (2, 9, 7)), ((1, 6, 5), (1, 12, 5)), ((3, 4, 13), (6, 8, 13))])
+def test_fft_plan_periodic_odd_sizes(lib, size, P):
+    """Periodic axes keep their size (padSize, kernel.go:83-87), so the real
+    transform can have odd length."""
+    rng = np.random.default_rng(10)
+    x = rng.random(size).astype(F32)
+    plan = lib.new_gpuFFT3dPlan_padded(i3(*size), i3(*P))
+    Q = P[2] // 2 + 1
+    din = Dev(lib, x)
+    dout = Dev(lib, n=P[0] * P[1] * 2 * Q)
+    lib.gpuFFT3dPlan_forward(plan, din.p, dout.p)
+    got = dout.get().view(np.complex64).reshape(P[0], P[1], Q)
+    xp = np.zeros(P)
+    xp[: size[0], : size[1], : size[2]] = x
+    ref = np.fft.rfftn(xp)
+    assert np.abs(got - ref).max() < 2e-6 * np.abs(ref).max()
+    back = Dev(lib, n=x.size)
+    lib.gpuFFT3dPlan_inverse(plan, dout.p, back.p)
+    assert np.abs(back.get().reshape(size) / np.prod(P) - x).max() < 1e-4
     lib.delete_gpuFFT3dPlan(plan)
 
 

@@ -89,15 +89,9 @@ def test_kernel_spectrum_vs_oracle(mm, orc, size, cell):
     assert np.abs(got - ref).max() < 2e-6 * np.abs(ref).max()
 
 
-def _pow2(n):
-    return n & (n - 1) == 0
-
-
 @pytest.mark.parametrize("mode", ["fused", "abi"])
 @pytest.mark.parametrize("size,cell", CASES)
 def test_heff_and_torque_vs_oracle(mm, orc, size, cell, mode):
-    if mode == "abi" and not all(_pow2(n) for n in size):
-        pytest.skip("the drop-in FFT plan handles power-of-two padded sizes in this build")
     m0 = random_m(size, 2)
     field = (0.01, -0.02, 0.03)
     axis = (0.0, 0.6, 0.8)
