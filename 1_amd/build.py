@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpukern.so")
 
-SOURCES = ["gk_basic.hip", "gk_fft.hip", "gk_fft_any.hip", "conv.hip", "rk.hip", "sim.hip", "batch.hip"]
+SOURCES = ["gk_basic.hip", "gk_fft.hip", "gk_fft_any.hip", "conv.hip", "rk.hip", "sim.hip", "batch.hip", "io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

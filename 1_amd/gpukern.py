@@ -131,6 +131,23 @@ class Params(C.Structure):
 _lib = None
 
 
+CP = C.c_char_p
+PP = C.POINTER(C.c_char_p)
+
+IO = {
+    "mm_io_error": (CP, []),
+    "mm_ovf_info": (I, [CP, P]),
+    "mm_ovf_desc": (I, [CP, I, P, P, I]),
+    "mm_ovf_read": (I, [CP, P, P]),
+    "mm_ovf_write": (I, [CP, P, P, PP, PP, I]),
+    "mm_tensor_read": (I, [CP, P, P, P]),
+    "mm_tensor_write": (I, [CP, I, P, P]),
+    "mm_odt_open": (P, [CP, CP, I, PP, PP]),
+    "mm_odt_row": (I, [P, P]),
+    "mm_odt_close": (I, [P]),
+}
+
+
 def load(path=LIB_PATH):
     """Load libgpukern.so (once) and declare every entry point."""
     global _lib
@@ -140,7 +157,7 @@ def load(path=LIB_PATH):
         raise RuntimeError(
             "libgpukern.so is not built (%s); run `python 1_amd/build.py` or __graft_entry__.build()" % path)
     lib = C.CDLL(path)
-    for table in (GPUKERN, ENGINE):
+    for table in (GPUKERN, ENGINE, IO):
         for name, (res, args) in table.items():
             fn = getattr(lib, name)
             fn.restype = res
@@ -152,4 +169,4 @@ def load(path=LIB_PATH):
 def exported_symbols(path=LIB_PATH):
     """Names from both tables that the shared object exports (no GPU needed)."""
     lib = C.CDLL(path)
-    return {n for n in list(GPUKERN) + list(ENGINE) if hasattr(lib, n)}
+    return {n for n in list(GPUKERN) + list(ENGINE) + list(IO) if hasattr(lib, n)}

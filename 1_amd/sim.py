@@ -154,6 +154,77 @@ class Sim:
         m = np.ascontiguousarray(m, np.float32).reshape((3,) + self._need_size())
         self._m0 = m.copy()
 
+    def loadm(self, file):
+        """Initial magnetisation from an OVF 1.0 / 2.0 file of the grid size
+        (simconfig.go:291-297; the header's multiplier is not applied)."""
+        from . import ovf
+
+        m, meta = ovf.read_ovf(file)
+        if meta["size"] != tuple(self._need_size()):
+            raise InputErr("%s holds a %s grid, the simulation is %s" % (file, meta["size"], self._size))
+        self.setm_array(m)
+
+    # ------------------------------------------------------------ output (simoutput.go)
+    def field_at(self, t):
+        """Applied field in tesla (internal X, Y, Z) at time t (simfield.go:87-153)."""
+        if self._static is not None:
+            return np.array(self._static, float)
+        if not self._field:
+            return np.zeros(3)
+        ts = [f[0] for f in self._field]
+        i = int(np.searchsorted(ts, t, side="left"))
+        if i == 0 or i >= len(ts):
+            return np.zeros(3)
+        (t0, b0), (t1, b1) = self._field[i - 1], self._field[i]
+        w = (t - t0) / (t1 - t0)
+        return np.asarray(b0, float) + w * (np.asarray(b1, float) - np.asarray(b0, float))
+
+    def save(self, what, format="binary", outdir="."):
+        """Save(what, format) (simoutput.go:56-61, 90-117): "m" as
+        outdir/m%08d.omf in OVF 1.0 binary ("binary", "binary4", "omf"),
+        OVF 2.0 binary ("binary_2", "binary4_2", "omf_2") or OVF 1.0 text
+        ("text", "ascii"); "table" appends time, <m> and B (user x, y, z) to
+        outdir/datatable.odt."""
+        from . import ovf
+
+        import os
+
+        self._init()
+        if what == "m":
+            fmts = {"binary": (1, True), "binary4": (1, True), "omf": (1, True), "binary_2": (2, True),
+                    "binary4_2": (2, True), "omf_2": (2, True), "text": (1, False), "ascii": (1, False)}
+            if format not in fmts:
+                raise InputErr("unknown format %s. options are: %s" % (format, ", ".join(fmts)))
+            version, binary = fmts[format]
+            idx = getattr(self, "_autosave_idx", 0)
+            path = os.path.join(outdir, "m%08d.omf" % idx)
+            self._autosave_idx = idx + 1
+            st = self.state()
+            t = st["time"] * st["unit_time"]
+            desc = {"time": "%g" % t, "msat": "%g" % self._msat, "aexch": "%g" % self._aexch,
+                    "alpha": "%g" % self._alpha, "torque": "%g" % st["max_torque"]}
+            ovf.write_ovf(path, self.m, self._cell, version=version,
+                          binary=binary, multiplier=self._msat, desc=desc, total_time=t)
+            return path
+        if what == "table":
+            path = os.path.join(outdir, "datatable.odt")
+            st = self.state()
+            t = st["time"] * st["unit_time"]
+            m = self.average_m()
+            b = self.field_at(t)
+            if getattr(self, "_table", None) is None:
+                self._table = ovf.OdtTable(path, "mumax data", [("Time", "s"), ("Mx/Ms", ""), ("My/Ms", ""),
+                                                               ("Mz/Ms", ""), ("Bx", "T"), ("By", "T"),
+                                                               ("Bz", "T")])
+            self._table.row([t, m[2], m[1], m[0], b[2], b[1], b[0]])
+            return path
+        raise InputErr("unknown output quantity %s. options are: m, table" % what)
+
+    def close_table(self):
+        if getattr(self, "_table", None) is not None:
+            self._table.close()
+            self._table = None
+
     # ------------------------------------------------------------ run (simrun.go)
     def run(self, seconds):
         self._init()

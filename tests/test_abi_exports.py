@@ -10,7 +10,7 @@ import re
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "1_amd", "libgpukern.so")
 
-DECL = re.compile(r"^\s*(?:unsigned\s+long\s+long\s+int|unsigned\s+long\s+long|long\s+long|[A-Za-z_][\w]*)\s*\*?\s*"
+DECL = re.compile(r"^\s*(?:const\s+)?(?:unsigned\s+long\s+long\s+int|unsigned\s+long\s+long|long\s+long|[A-Za-z_][\w]*)\s*\*?\s*"
                   r"\**\s*([A-Za-z_]\w*)\s*\(", re.M)
 
 
@@ -53,5 +53,5 @@ def test_python_binding_tables_match_headers():
 
     gk = importlib.import_module("1_amd.gpukern")
     declared = declared_functions()
-    bound = set(gk.GPUKERN) | set(gk.ENGINE)
+    bound = set(gk.GPUKERN) | set(gk.ENGINE) | set(gk.IO)
     assert declared == bound, (declared - bound, bound - declared)

@@ -296,3 +296,29 @@ def test_vortex_1024_precision_study(mm, orc):
              e_gpu, e_o, rel_e))
     assert err5 <= 1e-5
     assert rel_e <= 1e-5
+
+
+def test_save_and_loadm_round_trip(mm, tmp_path):
+    """Save("m", ...) in every format and LoadM of the file give back m; the
+    table holds time, <m> and B in user order."""
+    from importlib import import_module
+
+    ovf = import_module("1_amd.ovf")
+    cfg = mm.workloads.film_sp4_like(seed=1)
+    s = mm.workloads.make_sim(cfg)
+    s.run(0.002e-9)
+    for fmt in ("binary", "omf_2", "text"):
+        p = s.save("m", fmt, outdir=str(tmp_path))
+        m, meta = ovf.read_ovf(p)
+        assert np.array_equal(m, s.m)
+        assert np.allclose(meta["step"], cfg["cellsize"]) and float(meta["desc"]["msat"]) == cfg["msat"]
+        s2 = mm.workloads.make_sim(cfg)
+        s2.loadm(p)
+        assert np.array_equal(s2.m, s.m)
+        s2.free()
+    t = s.save("table", outdir=str(tmp_path))
+    s.close_table()
+    row = [float(x) for x in open(t).read().splitlines()[5].split()]
+    avg = s.average_m()
+    assert row[1:4] == pytest.approx([avg[2], avg[1], avg[0]], abs=1e-6)
+    assert row[4:7] == pytest.approx([-24.6e-3, 4.3e-3, 0.0], abs=1e-9)
