@@ -564,6 +564,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         d.t = torque_args(s, 0.f, nullptr);
         d.cp = ctrl_params(s);
         d.cp.n_err = d.nfin;
+        d.cp.part_e = nullptr;  // the batched epilogue computes no energy
     }
     b->dsims = (BSim*)balloc(b, n * sizeof(BSim));
     GK_CHECK(hipMemcpy(b->dsims, h.data(), n * sizeof(BSim), hipMemcpyHostToDevice));

@@ -36,7 +36,10 @@ struct EngineState {
     int new_step;       // next trial starts a new step
     int log_n;          // trials logged
     int nan_dt;         // dt became NaN (rk.go:448-452)
-    double energy;
+    double energy;      // energy of the last evaluated trial (wantEnergy), internal units
+    double prev_energy; // energy at the start of the current step (rk.go:297-298)
+    int relax;          // Relax(): done once an accepted step raised the energy (relax.go:14-25)
+    double relax_prev;  // energy after the previous step of the relaxation
 };
 
 constexpr int kMaxStepTrials = 10;  // rk.go:277
@@ -133,6 +136,8 @@ int conv_epi_blocks(const ConvPlan& cp);
 void conv_run_pass(const ConvPlan& cp, int cls, const StageIn& in, float* h, const EpiArgs* epi, hipStream_t s);
 // st->trial_active = 1 (stage kernels do their work until the next trial start)
 void launch_set_active(EngineState* st, hipStream_t s);
+// st->relax = on (relax_prev = +inf): Relax() mode of the step control
+void launch_set_relax(EngineState* st, int on, hipStream_t s);
 
 // ---------------------------------------------------------------- RK kernels
 struct TorqueArgs {
@@ -195,6 +200,7 @@ struct EpiArgs {
     int nu;
     int fin;
     TorqueArgs t;
+    double* epart;  // non-null: per-block sums of the energy density of m_i (FP64)
 };
 
 struct FinalArgs {
@@ -226,6 +232,8 @@ struct CtrlParams {
     const float* part_err;  // per-block error (adaptive)
     int n_k, n_err;         // entries of part_k0/part_k6 and of part_err
     int fold_k0;            // ctrl_end also records max |k0| (no separate after-k0 control)
+    const double* part_e;   // non-null (wantEnergy): n_k per-block energy sums of this trial
+    double cell_volume;     // internal units
 };
 void launch_ctrl_begin(EngineState* st, hipStream_t s);
 void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s);

@@ -258,8 +258,21 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
     const float cbs = dt * ea.bs;
     const long N = (long)Nx * Ny * Nz;
     float lmax = 0.f;
+    double esum = 0.0;
+    const bool want_e = ea.epart != nullptr;
     // one cell from the accumulated stage input mi (and final sum hi) and the field h
     auto finish = [&](const float mi[3], const float hi[3], const float h[3], float o[3]) {
+        if (want_e) {
+            // phi = -1/2 m.H_conv - m.B_ext - 1/2 k (m.u)^2 (field.go:64-71, gpu_local_contr3.cu:238-256)
+            const double mx = mi[0], my = mi[1], mz = mi[2];
+            double phi = -0.5 * (mx * h[0] + my * h[1] + mz * h[2]);
+            phi -= mx * hext[0] + my * hext[1] + mz * hext[2];
+            if (ea.t.anisType == 1) {
+                const double mu = mx * ea.t.u[0] + my * ea.t.u[1] + mz * ea.t.u[2];
+                phi -= 0.5 * ea.t.kAnis * mu * mu;
+            }
+            esum += phi;
+        }
         float k[3];
         llg_torque_cell(ea.t, hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
         lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
@@ -353,6 +366,18 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
             finish(mi, hi, h, o);
 #pragma unroll
             for (int q = 0; q < 3; q++) ea.out[q * N + gi] = o[q];
+        }
+    }
+    if (want_e) {
+        // one slot per thread in the tile buffer (3 (M+1)(TR+1) >= NT complex; the h rows are done)
+        __syncthreads();
+        double* dred = reinterpret_cast<double*>(lds);
+        dred[threadIdx.x] = esum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double e = 0.0;
+            for (int i = 0; i < NT; i++) e += dred[i];
+            ea.epart[pidx] = e;
         }
     }
     if (ea.t.part) {

@@ -135,6 +135,14 @@ void launch_energy(const float* m, const float* h, const float hext[3], int anis
     GK_LAUNCH_CHECK();
 }
 __global__ void k_set_active(EngineState* st) { st->trial_active = 1; }
+__global__ void k_set_relax(EngineState* st, int on) {
+    st->relax = on;
+    st->relax_prev = INFINITY;
+}
+void launch_set_relax(EngineState* st, int on, hipStream_t s) {
+    k_set_relax<<<1, 1, 0, s>>>(st, on);
+    GK_LAUNCH_CHECK();
+}
 void launch_set_active(EngineState* st, hipStream_t s) {
     k_set_active<<<1, 1, 0, s>>>(st);
     GK_LAUNCH_CHECK();

@@ -103,6 +103,7 @@ __device__ __forceinline__ void ctrl_begin_body(EngineState* st) {
         st->time0 = st->time;
         st->trials = 0;
         st->new_step = 0;
+        st->prev_energy = st->energy == 0 ? 9999999999.0 : st->energy;  // rk.go:297-298
     }
     st->dt_used = st->dt;
 }
@@ -138,7 +139,13 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     if (!st->trial_active) return;
     const float err_red = p.adaptive ? wave_max_partials(p.part_err, p.n_err) : 0.f;
     const float k0_red = p.fold_k0 ? wave_max_partials(p.part_k0, p.n_k) : 0.f;
+    double e_red = 0.0;
+    if (p.part_e) {
+        for (int i = threadIdx.x; i < p.n_k; i += 64) e_red += p.part_e[i];
+        for (int off = 32; off > 0; off >>= 1) e_red += __shfl_xor(e_red, off, 64);
+    }
     if (threadIdx.x != 0) return;
+    if (p.part_e) st->energy = e_red * p.cell_volume;
     if (p.fold_k0) {
         st->max_torque = k0_red;
         if (!p.adaptive) st->dt = p.inputDt;
@@ -157,7 +164,8 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
         if (factor < 0.01f) factor = 0.01f;
         float dt = dtu * factor;
         if (isnan(dt)) st->nan_dt = 1;
-        bad = err > 2.f * p.maxError;
+        // wantEnergy: the energy must not rise within a step (rk.go:424-426)
+        bad = err > 2.f * p.maxError || (p.part_e && st->energy > st->prev_energy);
         if (!bad) {
             if (dt < p.minDt) dt = p.minDt;
             if (p.maxDt != 0.f && dt > p.maxDt) dt = p.maxDt;
@@ -175,6 +183,10 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     if (st->step_end) {
         st->steps++;
         st->new_step = 1;
+        if (st->relax) {  // relax.go:14-25: stop once a step raised the energy
+            if (st->energy > st->relax_prev) st->done = 1;
+            st->relax_prev = st->energy;
+        }
         if (st->time >= st->stop || st->steps >= st->steps_target || st->nan_dt) st->done = 1;
     }
 }

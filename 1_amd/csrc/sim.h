@@ -51,6 +51,10 @@ struct mmSim {
     float* part = nullptr;  // 3 x npart: |k0|, |k_last|, error
     int npart = 0;          // max(kRedBlocks, blocks of the fused Z inverse + torque pass)
     int nk_part = 0;        // partial maxima written per torque evaluation
+    double* epart = nullptr;  // per-block energy sums (npart), wantEnergy
+    bool wantEnergy = false;
+    int fieldCap = 0;                // points in fT / fB
+    std::vector<void*> retired;      // replaced field tables, freed with the simulation
     double* fT = nullptr;
     float* fB = nullptr;
     FieldTable field{};

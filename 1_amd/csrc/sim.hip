@@ -174,6 +174,34 @@ const double kGamma0 = 2.211e5;
 
 }  // namespace
 
+// device copy of the applied field table (static value in the table itself)
+static void upload_field(mmSim* s) {
+    const int n = s->p.nField;
+    if (s->fT) {
+        // a batch built earlier may still read the old table: retire it until
+        // the simulation is freed
+        s->retired.push_back(s->fT);
+        s->retired.push_back(s->fB);
+        s->fT = nullptr;
+        s->fB = nullptr;
+        s->fieldCap = 0;
+    }
+    s->field = FieldTable{};
+    s->field.n = n;
+    if (n == 1)
+        for (int c = 0; c < 3; c++) s->field.b0[c] = s->fieldB[c];
+    if (n >= 2) {
+        GK_CHECK(hipMalloc(&s->fT, n * sizeof(double)));
+        GK_CHECK(hipMalloc(&s->fB, 3 * n * sizeof(float)));
+        s->fieldCap = n;
+        s->bytes += n * (sizeof(double) + 3 * sizeof(float));
+        GK_CHECK(hipMemcpy(s->fT, s->fieldT.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        GK_CHECK(hipMemcpy(s->fB, s->fieldB.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice));
+        s->field.t = s->fT;
+        s->field.b = s->fB;
+    }
+}
+
 void* gk::dmalloc(mmSim* s, size_t bytes) {
     void* p = nullptr;
     GK_CHECK(hipMalloc(&p, bytes));
@@ -215,6 +243,8 @@ CtrlParams gk::ctrl_params(const mmSim* s) {
     c.part_err = s->part + 2 * s->npart;
     c.n_k = s->nk_part;
     c.n_err = kRedBlocks;
+    c.part_e = s->wantEnergy ? s->epart : nullptr;
+    c.cell_volume = s->cell[0] * s->cell[1] * s->cell[2];
     c.fold_k0 = !needs_after_k0(s);
     c.adaptive = s->tb.adaptive;
     c.maxError = (float)s->p.maxError;
@@ -289,6 +319,9 @@ static void enqueue_trial(mmSim* s) {
         // max|k| is needed for k0 (rk.go:353) and, with FSAL, for the last stage (next k0)
         float* part = i == 0 ? s->part : (t.fsal && i == t.S - 1) ? s->part + s->npart : nullptr;
         ea.t = torque_args(s, t.c[i], part);
+        // wantEnergy: the energy of the new state (FSAL, last stage) or of the step's start
+        // (rk.go:324-333)
+        if (s->wantEnergy && i == (t.fsal ? t.S - 1 : 0)) ea.epart = s->epart;
         conv_apply(s->cp, si, nullptr, s->s, &pf, &ea);
         if (i == 0 && needs_after_k0(s)) {
             pf.begin(7, s->s);
@@ -546,17 +579,7 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     h.done = 1;
     GK_CHECK(hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
     s->hdt = h.dt;
-    s->field.n = p->nField;
-    if (p->nField == 1)
-        for (int c = 0; c < 3; c++) s->field.b0[c] = s->fieldB[c];
-    if (p->nField >= 2) {
-        s->fT = (double*)dmalloc(s, p->nField * sizeof(double));
-        s->fB = (float*)dmalloc(s, 3 * p->nField * sizeof(float));
-        GK_CHECK(hipMemcpy(s->fT, s->fieldT.data(), p->nField * sizeof(double), hipMemcpyHostToDevice));
-        GK_CHECK(hipMemcpy(s->fB, s->fieldB.data(), 3 * p->nField * sizeof(float), hipMemcpyHostToDevice));
-        s->field.t = s->fT;
-        s->field.b = s->fB;
-    }
+    upload_field(s);
     // convolution (init.go:234-274)
     conv_plan_init(s->cp, p->size);
     s->bytes += s->cp.bytes;
@@ -569,6 +592,9 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->npart = s->nk_part > kRedBlocks ? s->nk_part : kRedBlocks;
     s->part = (float*)dmalloc(s, 3 * (size_t)s->npart * sizeof(float));
     GK_CHECK(hipMemset(s->part, 0, 3 * (size_t)s->npart * sizeof(float)));
+    s->epart = (double*)dmalloc(s, (size_t)s->npart * sizeof(double));
+    GK_CHECK(hipMemset(s->epart, 0, (size_t)s->npart * sizeof(double)));
+    s->wantEnergy = p->wantEnergy != 0;
     if (mode == MM_MODE_ABI) abi_init(s, s->cell);
     GK_CHECK(hipDeviceSynchronize());
     return s;
@@ -581,8 +607,10 @@ void mm_sim_free(mmSim* s) {
         if (s->k[i]) GK_CHECK(hipFree(s->k[i]));
     for (float* p : {s->m, s->mi, s->mtmp, s->fB})
         if (p) GK_CHECK(hipFree(p));
-    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT, (void*)s->part})
+    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT, (void*)s->part,
+                    (void*)s->epart})
         if (p) GK_CHECK(hipFree(p));
+    for (void* p : s->retired) GK_CHECK(hipFree(p));
     conv_plan_free(s->cp);
     if (s->plan) delete_gpuFFT3dPlan(s->plan);
     for (int c = 0; c < 3; c++)
@@ -788,9 +816,29 @@ unsigned long long mm_sim_footprint(const mmParams* p) {
     const int npart = std::max(kRedBlocks, epi);
     b += N3 * sizeof(float) * (size_t)(1 + t.S + 2);  // m, k_0..k_{S-1}, m_i, m_tmp
     b += sizeof(EngineState) + (size_t)(1 << 16) * (sizeof(double) + sizeof(int)) + 4 * sizeof(double);
-    b += 3 * (size_t)npart * sizeof(float);
+    b += 3 * (size_t)npart * sizeof(float) + (size_t)npart * sizeof(double);  // part, epart
     if (p->nField >= 2) b += (size_t)p->nField * (sizeof(double) + 3 * sizeof(float));
     return b;
+}
+
+void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b) {
+    GK_ASSERT(n >= 0 && (n <= 1 || t), "mm_sim_set_field: %d points", n);
+    GK_CHECK(hipStreamSynchronize(s->s));
+    s->p.nField = n;
+    s->fieldT.assign(t ? t : nullptr, t ? t + n : nullptr);
+    s->fieldB.resize(3 * n);
+    for (int i = 0; i < 3 * n; i++) s->fieldB[i] = (float)b[i];
+    upload_field(s);
+}
+
+long mm_sim_relax(mmSim* s, long max_steps) {
+    GK_ASSERT(s->mode == MM_MODE_FUSED, "mm_sim_relax needs the fused mode");
+    s->wantEnergy = true;  // relax.go:16
+    launch_set_relax(s->st, 1, s->s);
+    const long n = fused_run(s, max_steps > 0 ? max_steps : -1, INFINITY, true);
+    launch_set_relax(s->st, 0, s->s);
+    GK_CHECK(hipStreamSynchronize(s->s));
+    return n;
 }
 
 void mm_sim_begin_run(mmSim* s, double seconds) {

@@ -86,6 +86,8 @@ ENGINE = {
     "mm_sim_time_pass": (C.c_double, [P, I, I]),
     "mm_sim_footprint": (C.c_ulonglong, [P]),
     "mm_sim_begin_run": (None, [P, C.c_double]),
+    "mm_sim_relax": (C.c_long, [P, C.c_long]),
+    "mm_sim_set_field": (None, [P, I, P, P]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
     "mm_batch_set_graph": (None, [P, I]),

@@ -95,19 +95,39 @@ class Sim:
         self.cellsize(z / Z, y / Y, x / X)
 
     # ------------------------------------------------------------ fields (simfield.go)
+    # Applying a field does not invalidate a running simulation
+    # (simfield.go:87-110): the engine's field table is replaced in place.
     def applystatic(self, what, hz, hy, hx):
-        self._invalidate()
-        if what != "field":
+        if what.lower() not in ("field", "b"):
             raise InputErr("only 'field' can be applied")
         self._static = (hx, hy, hz)
         self._field = []
+        self._push_field()
 
     def applypointwise(self, what, t, hz, hy, hx):
-        self._invalidate()
-        if what != "field":
+        if what.lower() not in ("field", "b"):
             raise InputErr("only 'field' can be applied")
+        if self._static is not None:
+            self._field = []
         self._static = None
         self._field.append((float(t), (hx, hy, hz)))
+        self._push_field()
+
+    def _field_table(self):
+        if self._static is not None:
+            return np.zeros(1), np.ascontiguousarray(self._static, np.float64)
+        if self._field:
+            t = np.ascontiguousarray([f[0] for f in self._field], np.float64)
+            b = np.ascontiguousarray(np.asarray([f[1] for f in self._field], np.float64).reshape(-1))
+            return t, b
+        return None, None
+
+    def _push_field(self):
+        if self._ptr:
+            t, b = self._field_table()
+            n = 0 if t is None else len(t)
+            self._lib.mm_sim_set_field(self._ptr, n, None if t is None else t.ctypes.data,
+                                       None if b is None else b.ctypes.data)
 
     # ------------------------------------------------------------ solver (simsolver.go)
     def solvertype(self, s):
@@ -229,6 +249,24 @@ class Sim:
     def run(self, seconds):
         self._init()
         return int(self._lib.mm_sim_run(self._ptr, float(seconds)))
+
+    def relax(self, max_steps=0):
+        """Relax() (relax.go:14-25): steps with energy control until a step
+        raises the total energy (or max_steps > 0 steps were taken); returns
+        the number of steps taken."""
+        self._init()
+        return int(self._lib.mm_sim_relax(self._ptr, int(max_steps)))
+
+    def hysteresis(self, bz0, by0, bx0, bz1, by1, bx1, steps, save=True, outdir="."):
+        """Hysteresis() (simhysteresis.go:16-27): field scan in `steps`
+        intervals, relaxing and saving m (omf) and a table row at each point."""
+        for i in range(steps + 1):
+            f = i / steps
+            self.applystatic("field", bz0 + (bz1 - bz0) * f, by0 + (by1 - by0) * f, bx0 + (bx1 - bx0) * f)
+            self.relax()
+            if save:
+                self.save("m", "omf", outdir)
+                self.save("table", "ascii", outdir)
 
     def steps(self, n):
         self._init()
@@ -360,17 +398,8 @@ class Sim:
         p.solver = SOLVERS[self._solver]
         p.maxError, p.dt = self._maxerror, self._dt
         p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt
-        if self._static is not None:
-            t = np.zeros(1)
-            b = np.asarray(self._static, np.float64)
-        elif self._field:
-            t = np.asarray([f[0] for f in self._field], np.float64)
-            b = np.asarray([f[1] for f in self._field], np.float64).reshape(-1)
-        else:
-            t = b = None
+        t, b = self._field_table()
         if t is not None:
-            t = np.ascontiguousarray(t)
-            b = np.ascontiguousarray(b)
             self._keep = [t, b]
             p.nField = len(t)
             p.fieldT = t.ctypes.data_as(C.POINTER(C.c_double))

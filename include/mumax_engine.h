@@ -122,6 +122,17 @@ unsigned long long mm_sim_footprint(const mmParams* p);
  * time + seconds (simrun.go:19-24); the trials are then driven by a batch
  * (mm_batch_trials), which is how a running batch takes in a new request. */
 void mm_sim_begin_run(mmSim* s, double seconds);
+/* ApplyStatic / ApplyPointwise on a live simulation (simfield.go:87-153): n
+ * points (t[n] seconds, b[3n] tesla, internal order; n = 1: static, n = 0:
+ * none).  Takes effect at the next trial; a batch built before keeps the
+ * table it copied (the old table stays allocated until mm_sim_free). */
+void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b);
+/* Relax() (relax.go:14-25): wantEnergy on (trials that raise the energy are
+ * rejected, rk.go:424-426), steps until a step raises the energy, or until
+ * max_steps steps when max_steps > 0 (the reference has no bound); returns
+ * the steps taken.  Fused mode.  wantEnergy stays on afterwards, as in the
+ * reference. */
+long mm_sim_relax(mmSim* s, long max_steps);
 
 /* ---- batched engine ----------------------------------------------------
  * n fused-mode simulations sharing the solver (any sizes, materials, fields)

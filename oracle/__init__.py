@@ -102,6 +102,8 @@ class Oracle:
         L.or_sim_state.argtypes = [P, P]
         L.or_sim_heff.argtypes = [P, P, P]
         L.or_sim_torque.argtypes = [P, P, P]
+        L.or_sim_set_field.argtypes = [P, C.c_int, P, P]
+        L.or_sim_set_want_energy.argtypes = [P, C.c_int]
         L.or_sim_energy.argtypes = [P]
         L.or_sim_energy.restype = C.c_double
 
@@ -326,3 +328,23 @@ class Sim:
 
     def energy(self):
         return self.orc.lib.or_sim_energy(self.ptr)
+
+    def set_field(self, t, b):
+        """replace the applied field table (t: n times in s, b: n x 3 tesla)"""
+        self._fieldT = np.ascontiguousarray(t, np.float64)
+        self._fieldB = np.ascontiguousarray(b, np.float64).reshape(-1, 3)
+        self.orc.lib.or_sim_set_field(self.ptr, len(self._fieldT), self._fieldT.ctypes.data,
+                                      self._fieldB.ctypes.data)
+
+    def relax(self):
+        """Relax() (relax.go:14-25): with energy control on, step until a step
+        raises the energy; returns the number of steps."""
+        self.orc.lib.or_sim_set_want_energy(self.ptr, 1)
+        self.step()
+        n = 1
+        prev = self.state()["energy"] + 1
+        while self.state()["energy"] <= prev:
+            prev = self.state()["energy"]
+            self.step()
+            n += 1
+        return n

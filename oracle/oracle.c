@@ -971,6 +971,27 @@ void or_sim_torque(or_sim* s, const double* m, double* k) {
     for (size_t i = 0; i < n3; i++) k[i] = s->m1[i];
 }
 
+/* ApplyStatic / ApplyPointwise on a running simulation (simfield.go:87-110):
+ * replaces the field table; the pointwise search restarts at its first point */
+void or_sim_set_field(or_sim* s, int n, const double* t, const double* b) {
+    free(s->fieldT);
+    free(s->fieldB);
+    s->fieldT = s->fieldB = NULL;
+    if (n > 0) {
+        s->fieldT = (double*)xcalloc((size_t)n, sizeof(double));
+        s->fieldB = (double*)xcalloc((size_t)n * 3, sizeof(double));
+        if (t) memcpy(s->fieldT, t, (size_t)n * sizeof(double));
+        memcpy(s->fieldB, b, (size_t)n * 3 * sizeof(double));
+    }
+    s->p.nField = n;
+    s->p.fieldT = s->fieldT;
+    s->p.fieldB = s->fieldB;
+    s->fieldLastIdx = 0;
+}
+
+/* Sim.wantEnergy (relax.go:16): energy of each step, steps may not raise it */
+void or_sim_set_want_energy(or_sim* s, int on) { s->p.wantEnergy = on; }
+
 double or_sim_energy(or_sim* s) {
     calc_heff(s, s->m, s->m1, 1);
     return s->energy;

@@ -130,6 +130,10 @@ void or_sim_state(const or_sim* s, double* st);
 /* effective field and torque of the current m (calcHeff + Torque) */
 void or_sim_heff(or_sim* s, const double* m, double* h);
 void or_sim_torque(or_sim* s, const double* m, double* k);
+/* replace the applied-field table (n points, t[n] s, b[3n] T internal order) */
+void or_sim_set_field(or_sim* s, int n, const double* t, const double* b);
+/* energy-controlled stepping on/off (relax.go:16, rk.go:424-426) */
+void or_sim_set_want_energy(or_sim* s, int on);
 /* total energy (internal units * cell volume), energy.go:38-52 */
 double or_sim_energy(or_sim* s);
 

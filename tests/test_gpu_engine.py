@@ -322,3 +322,98 @@ def test_save_and_loadm_round_trip(mm, tmp_path):
     avg = s.average_m()
     assert row[1:4] == pytest.approx([avg[2], avg[1], avg[0]], abs=1e-6)
     assert row[4:7] == pytest.approx([-24.6e-3, 4.3e-3, 0.0], abs=1e-9)
+
+
+# ------------------------------------------------------------------ live field, Relax, Hysteresis
+def _relax_film(mm, alpha=1.0):
+    size, cell = (1, 32, 64), (3e-9, 3.90625e-9, 3.90625e-9)
+    m0 = np.zeros((3,) + size, np.float32)
+    m0[2], m0[1] = 1.0, 0.3
+    m0 /= np.linalg.norm(m0, axis=0)
+    s = engine(mm, size, cell, m0=m0, alpha=alpha, maxerror=1e-4)
+    return s, size, cell, m0
+
+
+def test_applystatic_on_running_sim_replayed_in_oracle(mm, orc):
+    """ApplyStatic between two Run()s replaces the field without re-initialising
+    (simfield.go:87-90); the oracle replays the same trials with the field
+    swapped at the same point."""
+    cfg = mm.workloads.film_sp4_like(seed=1)
+    s = mm.workloads.make_sim(cfg)
+    s.run(0.02e-9)
+    n1 = len(s.trial_log()[0])
+    s.applystatic("field", 10e-3, -5e-3, 2e-3)   # user (x, y, z) = internal (Z, Y, X)
+    s.run(0.02e-9)
+    dts, acc = s.trial_log()
+    assert len(dts) > n1 + 3
+    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    normalise_oracle(o)
+    assert replay_log(o, dts, acc, 0, n1) == n1
+    o.set_field(np.zeros(1), [[2e-3, -5e-3, 10e-3]])
+    normalise_oracle(o)
+    replay_log(o, dts, acc, n1)
+    got = s.m
+    assert np.abs(got - o.m.reshape(got.shape)).max() < 1e-5
+    assert s.state()["steps"] == o.state()["steps"]
+
+
+def test_relax_replayed_in_oracle(mm, orc):
+    """Relax() (relax.go:14-25): the GPU's energy-controlled trials replayed in
+    the FP64 oracle give the same m and energy; the GPU stopped by itself, at
+    an energy within the FP32/FP64 spread of the oracle's own Relax()."""
+    s, size, cell, m0 = _relax_film(mm)
+    n = s.relax(max_steps=20000)
+    assert 10 < n < 20000
+    st = s.state()
+    dts, acc = s.trial_log()
+    o = oracle(orc, size, cell, m0, alpha=1.0, maxerror=1e-4)
+    o.orc.lib.or_sim_set_want_energy(o.ptr, 1)
+    replay_log(o, dts, acc)
+    got = s.m
+    err = np.abs(got - o.m.reshape(got.shape)).max()
+    ost = o.state()
+    print("relax: %d steps, %d trials, E gpu %.9g oracle-replay %.9g, max|dm| %.2e" %
+          (n, len(dts), st["energy"], ost["energy"], err))
+    assert ost["steps"] == n
+    assert err < 1e-5
+    assert st["energy"] == pytest.approx(ost["energy"], rel=2e-6)
+    # the oracle's own Relax from the same start (its stopping step is set by
+    # FP64 rounding, the GPU's by FP32): same minimum to the FP32/FP64 spread
+    o2 = oracle(orc, size, cell, m0, alpha=1.0, maxerror=1e-4)
+    o2.relax()
+    assert st["energy"] == pytest.approx(o2.state()["energy"], rel=1e-4)
+    mo = o2.m.reshape(3, -1).mean(1)
+    assert np.abs(s.average_m() - mo).max() < 3e-2
+
+
+def test_hysteresis_replayed_in_oracle(mm, orc, tmp_path):
+    """Hysteresis() (simhysteresis.go:16-27): ApplyStatic + Relax per field
+    point on one live simulation, replayed in the oracle with the field
+    swapped at the same trials; Hysteresis() itself gives bit-identical m and
+    writes one m file and one table row per point."""
+    s, size, cell, m0 = _relax_film(mm)
+    pts = [(-20e-3, 5e-3, 0.0), (0.0, 2.5e-3, 0.0), (20e-3, 0.0, 0.0)]   # user (x, y, z)
+    marks = []
+    for bx, by, bz in pts:
+        s.applystatic("field", bx, by, bz)
+        s.relax(max_steps=20000)
+        marks.append(len(s.trial_log()[0]))
+    dts, acc = s.trial_log()
+    o = oracle(orc, size, cell, m0, alpha=1.0, maxerror=1e-4)
+    o.orc.lib.or_sim_set_want_energy(o.ptr, 1)
+    i = 0
+    for (bx, by, bz), mk in zip(pts, marks):
+        o.set_field(np.zeros(1), [[bz, by, bx]])
+        i = replay_log(o, dts, acc, i, mk)
+        assert i == mk
+    got = s.m
+    assert np.abs(got - o.m.reshape(got.shape)).max() < 1e-5
+
+    h, _, _, _ = _relax_film(mm)
+    h.hysteresis(*pts[0], *pts[2], 2, outdir=str(tmp_path))
+    h.close_table()
+    assert np.array_equal(h.m, got)
+    assert sorted(p.name for p in tmp_path.glob("m*.omf")) == ["m%08d.omf" % k for k in range(3)]
+    rows = [ln for ln in open(tmp_path / "datatable.odt").read().splitlines() if not ln.startswith("#")]
+    assert len(rows) == 3
+    assert float(rows[-1].split()[4]) == pytest.approx(20e-3, abs=1e-9)

@@ -312,3 +312,109 @@ def test_forced_reject_path(orc):
     err, st = _macrospin(orc, "rkdp", dt=40.0, maxerror=1e-9, t_end_internal=50.0)
     assert st["rejects"] >= 1
     assert err < 1e-5
+
+
+# ------------------------------------------------------------------ energy (energy.go, field.go:64-87)
+def _energy_sim(orc, m0, **kw):
+    cell = kw.pop("cellsize", (2e-9, 2e-9, 2e-9))
+    return orc.sim(np.asarray(m0, np.float32), size=m0.shape[1:], cellsize=cell, msat=8e5, aexch=1.3e-11,
+                   alpha=0.1, **kw)
+
+
+def test_energy_demag_is_half_m_dot_h(orc):
+    # phi = -m.H_demag / 2 per unit volume (calcEDensDemagExch, energy.go:20-23); H itself is
+    # pinned by the kernel and convolution tests above.  (A single cell, or a row one cell wide
+    # in y or z, has no demag kernel at all: FaceKernel6's loop is empty there, demag.go:42-46.)
+    rng = np.random.default_rng(5)
+    m0 = rng.standard_normal((3, 2, 3, 4))
+    m0 /= np.linalg.norm(m0, axis=0)
+    s = _energy_sim(orc, m0, exch_type=0, exch_in_conv=False)
+    h = s.heff(m0.astype(np.float32)).reshape(3, -1)
+    m = m0.astype(np.float32).astype(float).reshape(3, -1)
+    u = orc.units(8e5, 1.3e-11)
+    vol = (2e-9 / u["length"]) ** 3
+    assert abs(h).max() > 0.05
+    assert s.energy() == pytest.approx(-0.5 * (m * h).sum() * vol, rel=1e-9)
+
+
+def test_energy_zeeman_and_sign(orc):
+    # phi -= m.Hext (no factor 1/2), summed times the cell volume (energy.go:48-51)
+    m0 = np.zeros((3, 1, 2, 3))
+    m0[0], m0[2] = 0.6, 0.8
+    B = np.array([0.01, 0.02, -0.03])
+    s = _energy_sim(orc, m0, want_demag=False, exch_type=0, field=(np.zeros(1), B))
+    u = orc.units(8e5, 1.3e-11)
+    vol = (2e-9 / u["length"]) ** 3
+    mdotb = float(np.float32(0.6)) * B[0] + float(np.float32(0.8)) * B[2]
+    assert s.energy() == pytest.approx(-6 * vol * mdotb / u["field"], rel=1e-7)
+
+
+def test_energy_anisotropy_is_half_m_dot_h(orc):
+    # uniaxial: phi -= k (m.u)^2 / 2, i.e. -m.H_anis / 2 (gpu_local_contr2.cu:156)
+    rng = np.random.default_rng(3)
+    m0 = rng.standard_normal((3, 1, 1, 5))
+    m0 /= np.linalg.norm(m0, axis=0)
+    s = _energy_sim(orc, m0, want_demag=False, exch_type=0, k1=4e5, anis_axis=(0.0, 0.6, 0.8))
+    h = s.heff(m0.astype(np.float32)).reshape(3, -1)
+    m = m0.astype(np.float32).astype(float).reshape(3, -1)
+    u = orc.units(8e5, 1.3e-11)
+    vol = (2e-9 / u["length"]) ** 3
+    assert abs(h).max() > 0.1
+    assert s.energy() == pytest.approx(-0.5 * (m * h).sum() * vol, rel=1e-6)
+
+
+def test_energy_exchange_two_cells(orc):
+    # 6-neighbour stencil, two cells at angle t along Z, a film (no X term).  A missing
+    # neighbour counts as m = 0 (gpu_exch1.cu), so H_i = -(4/h^2) m_i + m_j / h^2 and
+    # E = vol sum_i -m_i.H_i / 2 = vol (4 - cos t) / h^2
+    t = 0.7
+    m0 = np.zeros((3, 1, 1, 2))
+    m0[2, 0, 0, 0] = 1.0
+    m0[2, 0, 0, 1], m0[1, 0, 0, 1] = np.cos(t), np.sin(t)
+    s = _energy_sim(orc, m0, want_demag=False, exch_type=6, exch_in_conv=False)
+    u = orc.units(8e5, 1.3e-11)
+    h = 2e-9 / u["length"]
+    m = m0.astype(np.float32).astype(float)
+    cos_t = (m[:, 0, 0, 0] * m[:, 0, 0, 1]).sum()
+    assert s.energy() == pytest.approx(h ** 3 * (4 - cos_t) / h ** 2, rel=1e-6)
+
+
+# ------------------------------------------------------------------ Relax / live field (relax.go, simfield.go)
+def _film(orc, field=None, alpha=1.0):
+    size = (1, 16, 32)
+    m0 = np.zeros((3,) + size, np.float32)
+    m0[2], m0[1] = 1.0, 0.3
+    m0 /= np.linalg.norm(m0, axis=0)
+    kw = {} if field is None else dict(field=(np.zeros(1), np.asarray(field, float)))
+    return orc.sim(m0, size=size, cellsize=(3e-9, 3.90625e-9, 3.90625e-9), msat=8e5, aexch=1.3e-11, alpha=alpha,
+                   solver="rkdp", maxerror=1e-4, **kw)
+
+
+def test_set_field_equals_field_at_creation(orc):
+    B = (0.0, 0.004, -0.02)
+    a = _film(orc, field=B)
+    b = _film(orc)
+    b.set_field(np.zeros(1), [B])
+    for _ in range(20):
+        a.step()
+        b.step()
+    assert np.array_equal(a.m, b.m)
+
+
+def test_relax_lowers_energy_and_stops(orc):
+    s = _film(orc)
+    e0 = s.energy()
+    n = s.relax()
+    st = s.state()
+    assert 5 < n < 2000 and st["steps"] == n
+    assert st["energy"] < e0
+    # every accepted step kept the energy from rising; the last one raised it
+    s2 = _film(orc)
+    s2.orc.lib.or_sim_set_want_energy(s2.ptr, 1)
+    es = []
+    for _ in range(n):
+        s2.step()
+        es.append(s2.state()["energy"])
+    assert np.array_equal(s2.m, s.m)
+    assert all(es[i + 1] <= es[i] for i in range(n - 2))
+    assert es[-1] > es[-2]
