@@ -58,7 +58,7 @@ struct BStage {
     float ua[6], ub[6];
     float bs;
     int fin;
-    float c;
+    float c, cmax;
     int partslot;  // -1: none, 0: |k0|, 1: |k_last|
 };
 
@@ -195,6 +195,7 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim*
     ea.fin = sg.fin;
     ea.t = d.t;
     ea.t.c = sg.c;
+    ea.t.cmax = sg.cmax;
     ea.t.part = sg.partslot < 0 ? nullptr : d.part + (long)sg.partslot * d.npart;
     zinv_epi_tile<M, TR, NU, FIN>(d.B1, ea, twM, twP, d.st, d.Nx, d.Ny, d.Nz, x, yb * TR, bb.lb, lds, hext_s, red);
 }
@@ -490,6 +491,7 @@ void enqueue_btrial(mmBatch* b) {
         sg.fin = fin;
         sg.bs = fin ? t.b[i] : 0.f;
         sg.c = t.c[i];
+        sg.cmax = t.cmax(i);
         sg.partslot = i == 0 ? 0 : (t.fsal && i == t.S - 1) ? 1 : -1;
         for (auto& ls : b->zf) bl_zfwd(b, ls, sg);
         for (auto& ls : b->yf) bl_ypass(b, ls, sg, true);

@@ -40,6 +40,7 @@ struct EngineState {
     double prev_energy; // energy at the start of the current step (rk.go:297-298)
     int relax;          // Relax(): done once an accepted step raised the energy (relax.go:14-25)
     double relax_prev;  // energy after the previous step of the relaxation
+    double field_tmax;  // latest stage time (internal) the applied field was evaluated at
 };
 
 constexpr int kMaxStepTrials = 10;  // rk.go:277
@@ -53,21 +54,31 @@ struct FieldTable {
     float b0[3];      // static value (n == 1)
 };
 
-__device__ __forceinline__ void applied_field(const FieldTable& f, double t_si, float out[3]) {
+// The reference's pointwise field keeps its search index and only moves it
+// forward (simfield.go:126-133): the interval used at time t is the one ending
+// at the first point at or past the latest time evaluated so far (t_search =
+// max of all evaluated times, t included), so a trial that steps back in time
+// after a rejection keeps the later interval (and a zero field past the end).
+__host__ __device__ inline void applied_field_at(int n, const double* T, const float* B, const float* b0, double t_si,
+                                                 double t_search_si, float out[3]) {
     out[0] = out[1] = out[2] = 0.f;
-    if (f.n == 0) return;
-    if (f.n == 1) {
-        out[0] = f.b0[0];
-        out[1] = f.b0[1];
-        out[2] = f.b0[2];
+    if (n == 0) return;
+    if (n == 1) {
+        out[0] = b0[0];
+        out[1] = b0[1];
+        out[2] = b0[2];
         return;
     }
     int i = 0;
-    while (i < f.n && f.t[i] < t_si) i++;  // first point at or past t
-    if (i - 1 < 0 || i >= f.n) return;    // out of range: zero field
-    double dt = f.t[i] - f.t[i - 1];
-    float s = (float)((t_si - f.t[i - 1]) / dt);
-    for (int c = 0; c < 3; c++) out[c] = f.b[3 * (i - 1) + c] + s * (f.b[3 * i + c] - f.b[3 * (i - 1) + c]);
+    while (i < n && T[i] < t_search_si) i++;  // first point at or past t_search
+    if (i - 1 < 0 || i >= n) return;          // out of range: zero field
+    const double dt = T[i] - T[i - 1];
+    const float s = (float)((t_si - T[i - 1]) / dt);
+    for (int c = 0; c < 3; c++) out[c] = B[3 * (i - 1) + c] + s * (B[3 * i + c] - B[3 * (i - 1) + c]);
+}
+
+__device__ __forceinline__ void applied_field(const FieldTable& f, double t_si, double t_search_si, float out[3]) {
+    applied_field_at(f.n, f.t, f.b, f.b0, t_si, t_search_si, out);
 }
 
 // ---------------------------------------------------------------- stage build
@@ -149,6 +160,7 @@ struct TorqueArgs {
     float unitField;        // tesla per internal field unit
     double unitTime;        // seconds per internal time unit
     float c;                // stage node c_i
+    float cmax;             // max c_j over the stages evaluated so far in this trial
     float* part;            // per-block max |k| (null: no reduction at this stage)
     int exch;               // add the 6-neighbour exchange stencil here
     int n[3];
@@ -161,8 +173,9 @@ void launch_torque(const float* mi, float* hk, EngineState* st, const TorqueArgs
 // (rk.go:316), B(t) in tesla / B_unit (field.go:50-61).
 __device__ __forceinline__ void stage_field(const TorqueArgs& a, const EngineState* st, float hext[3]) {
     const double t = st->time0 + (double)(a.c * st->dt_used);
+    const double ts = fmax(st->field_tmax, st->time0 + (double)(a.cmax * st->dt_used));
     float b[3];
-    applied_field(a.field, t * a.unitTime, b);
+    applied_field(a.field, t * a.unitTime, ts * a.unitTime, b);
     for (int i = 0; i < 3; i++) hext[i] = b[i] / a.unitField;
 }
 
@@ -234,6 +247,7 @@ struct CtrlParams {
     int fold_k0;            // ctrl_end also records max |k0| (no separate after-k0 control)
     const double* part_e;   // non-null (wantEnergy): n_k per-block energy sums of this trial
     double cell_volume;     // internal units
+    float cmax;             // largest stage node of the tableau (advances field_tmax)
 };
 void launch_ctrl_begin(EngineState* st, hipStream_t s);
 void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s);

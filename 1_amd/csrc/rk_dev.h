@@ -152,6 +152,7 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     }
     const float dtu = st->dt_used;
     st->time = st->time0 + (double)dtu;
+    st->field_tmax = fmax(st->field_tmax, st->time0 + (double)(p.cmax * dtu));
     st->trials++;
     st->trials_total++;
     st->fsal_init = 1;

@@ -15,6 +15,12 @@ struct Tableau {
     int S = 0, fsal = 0, adaptive = 0;
     float order = 0;
     float a[7][7] = {}, b[7] = {}, b2[7] = {}, c[7] = {};
+    // largest node among stages 0..i (the applied field's search time)
+    float cmax(int i) const {
+        float m = 0.f;
+        for (int j = 0; j <= i; j++) m = c[j] > m ? c[j] : m;
+        return m;
+    }
 };
 
 }  // namespace gk
@@ -54,6 +60,7 @@ struct mmSim {
     double* epart = nullptr;  // per-block energy sums (npart), wantEnergy
     bool wantEnergy = false;
     int fieldCap = 0;                // points in fT / fB
+    double fieldTmaxHost = -INFINITY;  // ABI mode: latest time the field was evaluated at
     std::vector<void*> retired;      // replaced field tables, freed with the simulation
     double* fT = nullptr;
     float* fB = nullptr;

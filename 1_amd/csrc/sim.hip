@@ -209,20 +209,11 @@ void* gk::dmalloc(mmSim* s, size_t bytes) {
     return p;
 }
 
-static void applied_field_host(const mmSim* s, double t_si, float out[3]) {
-    out[0] = out[1] = out[2] = 0.f;
-    int n = s->p.nField;
-    if (n == 0) return;
-    if (n == 1) {
-        for (int c = 0; c < 3; c++) out[c] = s->fieldB[c];
-        return;
-    }
-    int i = 0;
-    while (i < n && s->fieldT[i] < t_si) i++;
-    if (i - 1 < 0 || i >= n) return;
-    double dt = s->fieldT[i] - s->fieldT[i - 1];
-    float f = (float)((t_si - s->fieldT[i - 1]) / dt);
-    for (int c = 0; c < 3; c++) out[c] = s->fieldB[3 * (i - 1) + c] + f * (s->fieldB[3 * i + c] - s->fieldB[3 * (i - 1) + c]);
+// applied field at internal time t; t_search (internal) is the latest time the
+// field was evaluated at, t included (the forward-only search of simfield.go:126-133)
+static void applied_field_host(const mmSim* s, double t, double t_search, float out[3]) {
+    const float* b = s->fieldB.data();
+    applied_field_at(s->p.nField, s->fieldT.data(), b, b, t * s->unitTime, t_search * s->unitTime, out);
 }
 
 EngineState gk::read_state(mmSim* s) {
@@ -244,6 +235,7 @@ CtrlParams gk::ctrl_params(const mmSim* s) {
     c.n_k = s->nk_part;
     c.n_err = kRedBlocks;
     c.part_e = s->wantEnergy ? s->epart : nullptr;
+    c.cmax = s->tb.cmax(s->tb.S - 1);
     c.cell_volume = s->cell[0] * s->cell[1] * s->cell[2];
     c.fold_k0 = !needs_after_k0(s);
     c.adaptive = s->tb.adaptive;
@@ -319,6 +311,7 @@ static void enqueue_trial(mmSim* s) {
         // max|k| is needed for k0 (rk.go:353) and, with FSAL, for the last stage (next k0)
         float* part = i == 0 ? s->part : (t.fsal && i == t.S - 1) ? s->part + s->npart : nullptr;
         ea.t = torque_args(s, t.c[i], part);
+        ea.t.cmax = t.cmax(i);
         // wantEnergy: the energy of the new state (FSAL, last stage) or of the step's start
         // (rk.go:324-333)
         if (s->wantEnergy && i == (t.fsal ? t.S - 1 : 0)) ea.epart = s->epart;
@@ -390,7 +383,8 @@ static void abi_calc_heff(mmSim* s, float* m, float* h) {
     for (int c = 0; c < 3; c++) gpuFFT3dPlan_inverse(s->plan, s->abuf[c], h + c * N);
     // addLocalFields (field.go:36-61)
     float b[3];
-    applied_field_host(s, s->htime * s->unitTime, b);
+    s->fieldTmaxHost = std::max(s->fieldTmaxHost, s->htime);
+    applied_field_host(s, s->htime, s->fieldTmaxHost, b);
     float hext[3];
     for (int c = 0; c < 3; c++) hext[c] = b[c] / (float)s->unitField;
     float kk = s->kAnis;
@@ -577,6 +571,7 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     h.new_step = 1;
     h.stop = 0;
     h.done = 1;
+    h.field_tmax = -INFINITY;
     GK_CHECK(hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
     s->hdt = h.dt;
     upload_field(s);
@@ -727,7 +722,7 @@ static void eval_heff(mmSim* s, const float* m_host, float* h_dev) {
     conv_apply(s->cp, si, h_dev, s->s);
     EngineState h = read_state(s);
     float b[3], hext[3];
-    applied_field_host(s, h.time * s->unitTime, b);
+    applied_field_host(s, h.time, std::max(h.field_tmax, h.time), b);
     for (int c = 0; c < 3; c++) hext[c] = b[c] / (float)s->unitField;
     float kk = s->kAnis;
     float ax[3] = {s->u[0], s->u[1], s->u[2]};
@@ -750,9 +745,17 @@ double mm_sim_energy(mmSim* s) {
     StageIn si{};
     si.m = s->m;
     conv_apply(s->cp, si, s->mtmp, s->s);
-    double t = s->mode == MM_MODE_ABI ? s->htime : read_state(s).time;
+    double t, ts;
+    if (s->mode == MM_MODE_ABI) {
+        t = s->htime;
+        ts = std::max(s->fieldTmaxHost, t);
+    } else {
+        const EngineState h = read_state(s);
+        t = h.time;
+        ts = std::max(h.field_tmax, t);
+    }
     float b[3], hext[3];
-    applied_field_host(s, t * s->unitTime, b);
+    applied_field_host(s, t, ts, b);
     for (int c = 0; c < 3; c++) hext[c] = b[c] / (float)s->unitField;
     GK_CHECK(hipMemsetAsync(s->dsum, 0, sizeof(double), s->s));
     launch_energy(s->m, s->mtmp, hext, s->p.anisType, s->kAnis, s->u, s->N, s->dsum, s->s);
@@ -824,6 +827,13 @@ unsigned long long mm_sim_footprint(const mmParams* p) {
 void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b) {
     GK_ASSERT(n >= 0 && (n <= 1 || t), "mm_sim_set_field: %d points", n);
     GK_CHECK(hipStreamSynchronize(s->s));
+    if (n >= 2 && s->p.nField < 2) {
+        // a new pointwise field starts its search at the first point; one that
+        // grows an existing table keeps its position (simfield.go:101-110)
+        const double t0 = -INFINITY;
+        GK_CHECK(hipMemcpy(&s->st->field_tmax, &t0, sizeof(double), hipMemcpyHostToDevice));
+        s->fieldTmaxHost = t0;
+    }
     s->p.nField = n;
     s->fieldT.assign(t ? t : nullptr, t ? t + n : nullptr);
     s->fieldB.resize(3 * n);

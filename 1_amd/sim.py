@@ -123,7 +123,9 @@ class Sim:
         return None, None
 
     def _push_field(self):
-        if self._ptr:
+        # a pointwise field needs two points before it can be evaluated
+        # (simfield.go:123-125); the engine keeps the previous field until then
+        if self._ptr and (self._static is not None or len(self._field) != 1):
             t, b = self._field_table()
             n = 0 if t is None else len(t)
             self._lib.mm_sim_set_field(self._ptr, n, None if t is None else t.ctypes.data,
@@ -398,6 +400,8 @@ class Sim:
         p.solver = SOLVERS[self._solver]
         p.maxError, p.dt = self._maxerror, self._dt
         p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt
+        if self._static is None and len(self._field) == 1:
+            raise InputErr("Pointwise field/current needs at least two points")
         t, b = self._field_table()
         if t is not None:
             self._keep = [t, b]

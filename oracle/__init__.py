@@ -104,6 +104,8 @@ class Oracle:
         L.or_sim_torque.argtypes = [P, P, P]
         L.or_sim_set_field.argtypes = [P, C.c_int, P, P]
         L.or_sim_set_want_energy.argtypes = [P, C.c_int]
+        L.or_sim_trial_log.argtypes = [P, P, P, C.c_long]
+        L.or_sim_trial_log.restype = C.c_long
         L.or_sim_energy.argtypes = [P]
         L.or_sim_energy.restype = C.c_double
 
@@ -329,6 +331,14 @@ class Sim:
     def energy(self):
         return self.orc.lib.or_sim_energy(self.ptr)
 
+    def trial_log(self):
+        """(dt used, accepted) of every trial so far, like the GPU engine's log"""
+        n = self.orc.lib.or_sim_trial_log(self.ptr, None, None, 0)
+        dt = np.zeros(max(n, 1))
+        acc = np.zeros(max(n, 1), np.int32)
+        self.orc.lib.or_sim_trial_log(self.ptr, dt.ctypes.data, acc.ctypes.data, n)
+        return dt[:n], acc[:n]
+
     def set_field(self, t, b):
         """replace the applied field table (t: n times in s, b: n x 3 tesla)"""
         self._fieldT = np.ascontiguousarray(t, np.float64)
@@ -336,15 +346,20 @@ class Sim:
         self.orc.lib.or_sim_set_field(self.ptr, len(self._fieldT), self._fieldT.ctypes.data,
                                       self._fieldB.ctypes.data)
 
-    def relax(self):
+    def relax(self, on_step=None):
         """Relax() (relax.go:14-25): with energy control on, step until a step
-        raises the energy; returns the number of steps."""
+        raises the energy; returns the number of steps.  on_step() is called
+        after every step."""
         self.orc.lib.or_sim_set_want_energy(self.ptr, 1)
         self.step()
         n = 1
+        if on_step:
+            on_step()
         prev = self.state()["energy"] + 1
         while self.state()["energy"] <= prev:
             prev = self.state()["energy"]
             self.step()
             n += 1
+            if on_step:
+                on_step()
         return n

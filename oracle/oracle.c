@@ -19,7 +19,12 @@
  *   or_deltaM            m.k = 0; |k| = |m x h| sqrt(1+a^2)/(1+a^2); Larmor precession
  *   or_reduce            redux_test.go:13-102 known answers
  *   or_sim (RK)          tableau order on dm/dt=linear; analytic macrospin precession+damping;
- *                        forced reject path with a tiny maxError; SP4 <m>(t) vs solution3.txt (loose)
+ *                        forced reject path with a tiny maxError; SP4 <m>(t) vs solution3.txt (loose);
+ *                        float32-rounded coefficients (the reference's literals) vs exact rationals
+ *   energy (calcHeffEnergy) closed forms: Zeeman -m.B, anisotropy and demag -m.H/2, two-cell
+ *                        exchange vol (4 - cos t)/h^2 with the stencil's zero boundary
+ *   or_sim_set_field     identical to the same field given at creation
+ *   Relax (Python loop)  energy never rises over accepted steps, the stopping step raises it
  */
 #include "oracle.h"
 
@@ -666,7 +671,25 @@ struct or_sim {
     double time, dt;
     long steps, trials_total, rejects_total;
     double lastErr, maxTorque, energy, hextSI[3];
+    double* log_dt; /* per trial: dt used and accepted flag (the GPU engine's trial log) */
+    int* log_acc;
+    long log_n, log_cap;
 };
+
+static void log_trial(or_sim* s, double dt, int acc) {
+    if (s->log_n == s->log_cap) {
+        s->log_cap = s->log_cap ? 2 * s->log_cap : 1024;
+        s->log_dt = (double*)realloc(s->log_dt, (size_t)s->log_cap * sizeof(double));
+        s->log_acc = (int*)realloc(s->log_acc, (size_t)s->log_cap * sizeof(int));
+        if (!s->log_dt || !s->log_acc) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    }
+    s->log_dt[s->log_n] = dt;
+    s->log_acc[s->log_n] = acc;
+    s->log_n++;
+}
+
+static int g_tableau_f32 = 0;
+void or_set_tableau_f32(int on) { g_tableau_f32 = on; }
 
 static void set_tableau(or_sim* s) {
     /* coefficients as rationals (rk.go:62-212); the reference rounds them to float32 */
@@ -743,6 +766,14 @@ static void set_tableau(or_sim* s) {
         fprintf(stderr, "oracle: unknown solver %d\n", s->p.solver);
         abort();
     }
+    if (g_tableau_f32) { /* the reference's float32 coefficients (rk.go:62-212 are float32 literals) */
+        for (int r = 0; r < MAX_STAGES; r++) {
+            for (int q = 0; q < MAX_STAGES; q++) s->a[r][q] = (float)s->a[r][q];
+            s->b[r] = (float)s->b[r];
+            s->b2[r] = (float)s->b2[r];
+            s->c[r] = (float)s->c[r];
+        }
+    }
 }
 
 /* applied field in tesla at SI time t (simfield.go:87-153) */
@@ -807,6 +838,7 @@ void or_sim_free(or_sim* s) {
     free(s->m); free(s->m1); free(s->mbackup); free(s->tmp); free(s->phi);
     for (int i = 0; i < MAX_STAGES; i++) free(s->k[i]);
     free(s->fieldT); free(s->fieldB);
+    free(s->log_dt); free(s->log_acc);
     free(s);
 }
 
@@ -884,6 +916,7 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
             }
         }
         s->time = time0 + s->dt; /* rk.go:376 */
+        const double dt_used = s->dt;
         if (s->has_b2) {
             memcpy(m1, m, n3 * sizeof(or_real));
             for (int i = 0; i < order; i++)
@@ -902,6 +935,7 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
             if (isnan(s->dt)) { fprintf(stderr, "oracle: dt = NaN\n"); return -1; }
             int bad = err > 2 * s->p.maxError || (s->p.wantEnergy && s->energy > prevEnergy);
             if (dt_trial && t < n_trial) bad = !accept[t];
+            log_trial(s, dt_used, !bad);
             if (bad) {
                 memcpy(m, s->mbackup, n3 * sizeof(or_real));
                 goodstep = 0;
@@ -913,6 +947,7 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
             }
         } else {
             goodstep = 1;
+            log_trial(s, dt_used, 1);
         }
     }
     or_normalize(m, s->N); /* rk.go:444 */
@@ -987,6 +1022,15 @@ void or_sim_set_field(or_sim* s, int n, const double* t, const double* b) {
     s->p.fieldT = s->fieldT;
     s->p.fieldB = s->fieldB;
     s->fieldLastIdx = 0;
+}
+
+/* the first cap entries of the per-trial log (dt used, accepted); returns its length */
+long or_sim_trial_log(const or_sim* s, double* dt, int* acc, long cap) {
+    for (long i = 0; i < s->log_n && i < cap; i++) {
+        dt[i] = s->log_dt[i];
+        acc[i] = s->log_acc[i];
+    }
+    return s->log_n;
 }
 
 /* Sim.wantEnergy (relax.go:16): energy of each step, steps may not raise it */

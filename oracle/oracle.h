@@ -132,6 +132,10 @@ void or_sim_heff(or_sim* s, const double* m, double* h);
 void or_sim_torque(or_sim* s, const double* m, double* k);
 /* replace the applied-field table (n points, t[n] s, b[3n] T internal order) */
 void or_sim_set_field(or_sim* s, int n, const double* t, const double* b);
+/* 1: round the RK coefficients to float32 as the reference does (default 0: exact) */
+void or_set_tableau_f32(int on);
+/* per-trial log: dt used and accepted flag of every trial so far (first cap copied) */
+long or_sim_trial_log(const or_sim* s, double* dt, int* acc, long cap);
 /* energy-controlled stepping on/off (relax.go:16, rk.go:424-426) */
 void or_sim_set_want_energy(or_sim* s, int on);
 /* total energy (internal units * cell volume), energy.go:38-52 */

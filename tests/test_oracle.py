@@ -418,3 +418,21 @@ def test_relax_lowers_energy_and_stops(orc):
     assert np.array_equal(s2.m, s.m)
     assert all(es[i + 1] <= es[i] for i in range(n - 2))
     assert es[-1] > es[-2]
+
+
+def test_float32_coefficients_are_equivalent(orc):
+    # the reference's tableau literals are float32 (rk.go:62-212); the oracle's exact
+    # rationals give the same trajectory to well below the FP32 engine's own rounding
+    import ctypes
+
+    orc.lib.or_set_tableau_f32.argtypes = [ctypes.c_int]
+    ms = []
+    for f32 in (0, 1):
+        orc.lib.or_set_tableau_f32(f32)
+        s = _film(orc, field=(0.0, 0.004, -0.02), alpha=0.1)
+        for _ in range(40):
+            s.step()
+        ms.append(s.m)
+    orc.lib.or_set_tableau_f32(0)
+    d = np.abs(ms[0] - ms[1]).max()
+    assert 0 < d < 1e-6
