@@ -125,35 +125,70 @@ def calibrate_copy(lib, n=32 << 20, reps=3):
     return {"kernel": "gpu_linear_combination", "launches": reps, "fetch_bytes": 8 * n, "write_bytes": 4 * n}
 
 
+def dp_trial_bytes(size):
+    """Algorithmic bytes of one Dormand-Prince trial of one simulation with an
+    FSAL torque in hand: 6 field evaluations (B_conv each) plus the pointwise
+    RK work 12 N 58 (stage builds, torques, final and error combination,
+    normalisation)."""
+    X, Y, Z = size
+    N = X * Y * Z
+    P = [2 * X if X > 1 else 1, 2 * Y, 2 * Z]
+    Q = P[2] // 2 + 1
+    nK = 6 if X > 1 else 4
+    A1, A2, A3 = X * Y * Q, X * P[1] * Q, P[0] * P[1] * Q
+    if X > 1:
+        b_conv = 3 * (8 * N + 32 * A1 + 32 * A2) + 4 * nK * A3
+    else:
+        b_conv = 3 * (8 * N + 32 * A1) + 4 * nK * A3
+    return 6 * b_conv + 12 * N * 58
+
+
 def bench_sweep(args, mm):
     """Batched sweep: all members in one batch (mm_batch_*) against members run
     one at a time on the same GPU.  Setup (kernel spectra) is excluded from
     both; latency = time from the start of the batch to each member's done flag
-    (members all submitted at t = 0)."""
+    (members all submitted at t = 0).  When at most half of a batch is still
+    running it is rebuilt from the running members (the finished ones would
+    otherwise keep their length classes' launches alive)."""
     cfgs = mm.workloads.sweep(args.sweep_sims, seed=4, duration=args.sweep_ns * 1e-9)
     n = len(cfgs)
     t0 = time.perf_counter()
     sims = [mm.workloads.make_sim(c) for c in cfgs]
     for s in sims:
         s._init()
-    b = mm.Batch(sims)
+    b = mm.Batch(sims, graph=not args.no_graph)
     setup = time.perf_counter() - t0
     b.start(seconds=[c["duration"] for c in cfgs])
     done_at = np.full(n, np.nan)
-    trials = 0
+    members = np.arange(n)
+    trials = rebuilds = 0
     t1 = time.perf_counter()
+    b.trials(1)  # captures the batched trial
+    # full-batch trial time: the next `poll` trials, every member running
+    ta = time.perf_counter()
+    active = b.trials(args.poll)
+    t_full = (time.perf_counter() - ta) / args.poll
+    trials += 1 + args.poll
     while True:
-        active = b.trials(args.poll)
-        trials += args.poll
         now = time.perf_counter() - t1
         d = b.done()
-        done_at[d & np.isnan(done_at)] = now
+        fin = members[d]
+        done_at[fin[np.isnan(done_at[fin])]] = now
         if active == 0:
             break
+        run = members[~d]
+        if len(run) <= len(members) // 2 and len(run) < len(members):
+            b.free()
+            members = run
+            b = mm.Batch([sims[j] for j in members], graph=not args.no_graph)
+            rebuilds += 1
+        active = b.trials(args.poll)
+        trials += args.poll
     t_batch = time.perf_counter() - t1
+    b.free()
+    full_bytes = sum(dp_trial_bytes(c["size"]) for c in cfgs)
     steps = np.array([s.state()["steps"] for s in sims])
     cells = np.array([np.prod(c["size"]) for c in cfgs])
-    b.free()
     for s in sims:
         s.free()
     # serial: an evenly spaced sample of members, each alone
@@ -184,6 +219,9 @@ def bench_sweep(args, mm):
                    "cells_total": int(cells.sum()), "solver": "rkdp", "max_error": 1e-5},
         "batch_seconds": round(t_batch, 4),
         "batched_trials": trials,
+        "batch_rebuilds": rebuilds,
+        "full_batch_trial": {"ms": round(t_full * 1e3, 4), "model_bytes": int(full_bytes),
+                             "hbm_frac": round(full_bytes / t_full / (HBM_PEAK_GBS * 1e9), 4)},
         "accepted_steps_total": int(steps.sum()),
         "mcell_steps_per_s": round(float((steps * cells).sum()) / t_batch / 1e6, 2),
         "latency_s": {"p50": round(float(np.percentile(done_at, 50)), 4),
@@ -207,7 +245,7 @@ def bench_serve(args, mm):
     sv = importlib.import_module("1_amd.serve")
     reqs = sv.request_stream(args.serve_requests, rate=args.serve_rate, seed=5)
     n = len(reqs)
-    srv = sv.Server(poll=args.poll)
+    srv = sv.Server(poll=args.poll, graph=not args.no_graph)
     stats = srv.serve(reqs, max_seconds=args.serve_max_seconds)
     lat = sv.latency_stats(reqs)
     rate = n / stats["elapsed"]
@@ -281,6 +319,7 @@ def main():
     ap.add_argument("--sweep-sims", type=int, default=512)
     ap.add_argument("--sweep-ns", type=float, default=0.5)
     ap.add_argument("--poll", type=int, default=16, help="batched trials between done-flag polls")
+    ap.add_argument("--no-graph", action="store_true", help="batched trials as eager launches (no HIP graph)")
     ap.add_argument("--serial-sample", type=int, default=32, help="sweep members timed alone for the serial rate")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch a streaming kernel of known traffic (for HBM counter scaling)")
