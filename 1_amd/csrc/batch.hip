@@ -57,7 +57,8 @@ struct BStage {
     int uj[6];
     float ua[6], ub[6];
     float bs;
-    int fin;
+    int fin;             // epilogue mode (zinv_epi_tile's FIN)
+    double ce[6], ce_self;  // fin == 2: b2_j - b_j of the slots and of this stage
     float c, cmax;
     int partslot;  // -1: none, 0: |k0|, 1: |k_last|
 };
@@ -87,7 +88,7 @@ __device__ __forceinline__ bool bskip(const BSim& d, const BStage& sg) {
 }
 
 // ------------------------------------------------------------------ passes
-template <int M, int TR>
+template <int M, int TR, int NK>
 __global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, const int* idx, const int* start,
                                                                int n, BStage sg, const float2* __restrict__ twM,
                                                                const float2* __restrict__ twP) {
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, c
         si.k[j] = j < sg.nk ? d.k[sg.kj[j]] : d.m;
         si.a[j] = sg.a[j];
     }
-    zfwd_tile<M, TR>(si, d.B1, twM, twP, d.Nx, d.Ny, d.Nz, c, x, yb * TR, lds);
+    zfwd_tile<M, TR, NK>(si, d.B1, twM, twP, d.Nx, d.Ny, d.Nz, c, x, yb * TR, lds);
 }
 
 template <int PY, bool FWD>
@@ -144,9 +145,9 @@ __global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx,
     for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
     const int li = threadIdx.x / G, g = threadIdx.x % G;
     const int lpr = LPB < d.PY ? LPB : d.PY;
-    const int rows = d.PY / lpr;
+    const int rows = (d.PY + lpr - 1) / lpr;
     int kz, ky;
-    xmul_column(bb.lb, (long)d.Q * rows, rows, lpr, li, kz, ky);
+    xmul_column(bb.lb, (long)d.Q * rows, rows, lpr, li, d.PY, kz, ky);
     float2 v[3][E];
     xmul_load<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
     __syncthreads();  // twiddle table
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(kNT) void kb_y2d(const BSim* sims, const int* idx, 
     y2d_tile<PY, NKZ>(d.B1, d.Kc, tw, d.Ny, d.Q, bb.lb * NKZ, lds);
 }
 
-template <int M, int TR, int NU, bool FIN>
+template <int M, int TR, int NU, int FIN>
 __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim* sims, const int* idx,
                                                                        const int* start, int n, BStage sg,
                                                                        const float2* __restrict__ twM,
@@ -183,13 +184,17 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim*
     const int yb = bb.lb % nyb, x = bb.lb / nyb;
     EpiArgs ea{};
     ea.m = d.m;
-    ea.out = FIN ? d.m : d.k[sg.i];
+    ea.out = FIN == 1 ? d.m : d.k[sg.i];
     for (int j = 0; j < 6; j++) {
         const bool u = j < sg.nu;
         ea.k[j] = u ? d.k[sg.uj[j]] : d.m;  // unused slots: m with zero coefficients
         ea.a[j] = u ? sg.ua[j] : 0.f;
         ea.b[j] = u ? sg.ub[j] : 0.f;
+        ea.ce[j] = u ? sg.ce[j] : 0.0;
     }
+    ea.ce_self = sg.ce_self;
+    ea.mhigh = d.mtmp;
+    ea.part_err = d.part + 2L * d.npart;
     ea.bs = sg.bs;
     ea.nu = sg.nu;
     ea.fin = sg.fin;
@@ -313,15 +318,12 @@ template <int L> constexpr int y2d_nkz_c() {
     return p;
 }
 
-#define GK_B_POW2_1_2048(X) X(1) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
-#define GK_B_POW2_2_4096(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
-#define GK_B_POW2_2_1024(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
 
 int z_rows_host(int M) {
     switch (M) {
 #define GK_CASE(n) \
     case n: return z_rows_c<n>();
-        GK_B_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     }
     abort();
@@ -330,7 +332,7 @@ int epi_rows_host(int M) {
     switch (M) {
 #define GK_CASE(n) \
     case n: return epi_rows_c<n>();
-        GK_B_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     }
     abort();
@@ -339,15 +341,12 @@ int y2d_nkz_host(int PY) {
     switch (PY) {
 #define GK_CASE(n) \
     case n: return y2d_nkz_c<n>();
-        GK_B_POW2_2_4096(GK_CASE)
+        GK_LEN_Y(GK_CASE)
 #undef GK_CASE
     }
     abort();
 }
-int lanes_host(int L) {  // RegFFT<L>::G
-    const int E = L <= 512 ? (L < 8 ? L : 8) : L / 64;
-    return L / E;
-}
+int lanes_host(int L) { return reg_fft_G(L); }  // RegFFT<L>::G
 
 void* balloc(mmBatch* b, size_t bytes) {
     void* p = nullptr;
@@ -385,15 +384,24 @@ void bl_zfwd(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     const float2* twM = twiddle_f32(ls.L);
     const float2* twP = twiddle_f32(2 * ls.L);
     switch (ls.L) {
+#define GK_ZF(n, K) \
+    kb_zfwd<n, z_rows_c<n>(), K><<<ls.blocks, z_rows_c<n>() * RegFFT<n>::G, 0, b->s>>>(b->dsims, ls.idx, ls.start, \
+                                                                                     ls.cnt, sg, twM, twP)
 #define GK_CASE(n)                                                                                  \
     case n:                                                                                         \
-        kb_zfwd<n, z_rows_c<n>()><<<ls.blocks, z_rows_c<n>() * RegFFT<n>::G, 0, b->s>>>(b->dsims, ls.idx, \
-                                                                                      ls.start, ls.cnt, sg, twM, twP); \
+        switch (zfwd_nk(sg.nk)) {                                                                   \
+        case 0: GK_ZF(n, 0); break;                                                                 \
+        case 1: GK_ZF(n, 1); break;                                                                 \
+        case 2: GK_ZF(n, 2); break;                                                                 \
+        case 3: GK_ZF(n, 3); break;                                                                 \
+        default: GK_ZF(n, 5); break;                                                                \
+        }                                                                                           \
         break;
-        GK_B_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
+    #undef GK_ZF
     GK_LAUNCH_CHECK();
 }
 void bl_ypass(mmBatch* b, const LaunchSet& ls, const BStage& sg, bool fwd) {
@@ -404,7 +412,7 @@ void bl_ypass(mmBatch* b, const LaunchSet& ls, const BStage& sg, bool fwd) {
         if (fwd) kb_ypass<n, true><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw);  \
         else kb_ypass<n, false><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw);     \
         break;
-        GK_B_POW2_2_4096(GK_CASE)
+        GK_LEN_Y(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -415,7 +423,7 @@ void bl_xmul(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     switch (ls.L) {
 #define GK_CASE(n)                                                                            \
     case n: kb_xmul<n><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw); break;
-        GK_B_POW2_2_1024(GK_CASE)
+        GK_LEN_X(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -427,7 +435,7 @@ void bl_y2d(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
 #define GK_CASE(n)                                                                                           \
     case n: kb_y2d<n, y2d_nkz_c<n>()><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw); \
         break;
-        GK_B_POW2_2_4096(GK_CASE)
+        GK_LEN_Y(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -441,18 +449,21 @@ void bl_zinv_epi(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     case n: {                                                                                            \
         constexpr int TR = epi_rows_c<n>();                                                              \
         const int th = 3 * TR * RegFFT<n>::G;                                                            \
-        if (sg.fin) {                                                                                    \
-            if (sg.nu <= 1) kb_zinv_epi<n, TR, 1, true><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
-            else kb_zinv_epi<n, TR, 3, true><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);         \
+        if (sg.fin == 1) {                                                                               \
+            if (sg.nu <= 1) kb_zinv_epi<n, TR, 1, 1><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else kb_zinv_epi<n, TR, 3, 1><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);         \
+        } else if (sg.fin == 2) {                                                                        \
+            if (sg.nu <= 3) kb_zinv_epi<n, TR, 3, 2><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else kb_zinv_epi<n, TR, 6, 2><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);         \
         } else {                                                                                         \
-            if (sg.nu == 0) kb_zinv_epi<n, TR, 0, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
-            else if (sg.nu == 1) kb_zinv_epi<n, TR, 1, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
-            else if (sg.nu <= 3) kb_zinv_epi<n, TR, 3, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
-            else kb_zinv_epi<n, TR, 6, false><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);            \
+            if (sg.nu == 0) kb_zinv_epi<n, TR, 0, 0><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else if (sg.nu == 1) kb_zinv_epi<n, TR, 1, 0><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else if (sg.nu <= 3) kb_zinv_epi<n, TR, 3, 0><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP); \
+            else kb_zinv_epi<n, TR, 6, 0><<<ls.blocks, th, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twM, twP);            \
         }                                                                                                \
         break;                                                                                           \
     }
-        GK_B_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -481,14 +492,17 @@ void enqueue_btrial(mmBatch* b) {
                 sg.nk++;
             }
         const bool fin = !t.adaptive && i == t.S - 1;
+        const bool efin = t.fsal_final() && i == t.S - 1;
         for (int j = 0; j < i; j++)
-            if (t.a[i][j] != 0.f || (fin && t.b[j] != 0.f)) {
+            if (t.a[i][j] != 0.f || (fin && t.b[j] != 0.f) || (efin && t.b2[j] != t.b[j])) {
                 sg.uj[sg.nu] = j;
                 sg.ua[sg.nu] = t.a[i][j];
                 sg.ub[sg.nu] = fin ? t.b[j] : 0.f;
+                sg.ce[sg.nu] = efin ? (double)t.b2[j] - (double)t.b[j] : 0.0;
                 sg.nu++;
             }
-        sg.fin = fin;
+        sg.fin = fin ? 1 : efin ? 2 : 0;
+        sg.ce_self = efin ? (double)t.b2[i] - (double)t.b[i] : 0.0;
         sg.bs = fin ? t.b[i] : 0.f;
         sg.c = t.c[i];
         sg.cmax = t.cmax(i);
@@ -504,7 +518,7 @@ void enqueue_btrial(mmBatch* b) {
             GK_LAUNCH_CHECK();
         }
     }
-    if (t.adaptive) {
+    if (t.adaptive && !t.fsal_final()) {
         FinalArgs fa{};
         fa.S = t.S;
         for (int i = 0; i < 7; i++) {
@@ -588,7 +602,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
     b->xm = make_sets(b, d3, PX, [](const mmSim* s) {
         const int lpb = 256 / lanes_host(s->cp.P[0]);
         const int lpr = std::min(lpb, s->cp.P[1]);
-        return s->cp.Q * (s->cp.P[1] / lpr);
+        return s->cp.Q * div_up(s->cp.P[1], lpr);
     });
     b->y2 = make_sets(b, d2, PY, [](const mmSim* s) { return div_up(s->cp.Q, y2d_nkz_host(s->cp.P[1])); });
     b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });

@@ -155,11 +155,12 @@ __global__ void k_extract_kernel(const double2* __restrict__ W, int P0, int P1, 
 
 // =============================================================== plan
 static int pad_len(int n) {
-    // smallest supported FFT length >= 2n-1 (exact linear convolution, any P >= 2N-1)
+    // smallest register-FFT length 2^a or 3 * 2^a that is >= 2n (any P >= 2n - 1
+    // gives the same linear convolution; the reference pads to exactly 2n)
     if (n == 1) return 1;
-    int p = 1;
-    while (p < 2 * n - 1) p *= 2;
-    return p;
+    int p = 2;
+    while (p < 2 * n) p *= 2;
+    return (p / 4) * 3 >= 2 * n ? (p / 4) * 3 : p;
 }
 
 void conv_plan_init(ConvPlan& cp, const int N[3]) {
@@ -311,9 +312,6 @@ double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool 
 }
 
 // =============================================================== launchers
-#define GK_POW2_1_2048(X) X(1) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
-#define GK_POW2_2_4096(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
-#define GK_POW2_2_1024(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
 
 template <int L> constexpr int groups_per_block(int budget_lines_bytes) {
     // how many 3-line groups of length L fit in `budget` bytes of LDS (>= 1, <= 32, power of two)
@@ -333,20 +331,37 @@ template <int M> constexpr int z_rows() {
     return tr;
 }
 
-static void launch_zfwd(const ConvPlan& cp, const StageIn& si, hipStream_t s) {
+static void launch_zfwd(const ConvPlan& cp, const StageIn& in, hipStream_t s) {
     const int M = cp.M, Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
     const float2* twM = twiddle_f32(M);
     const float2* twP = twiddle_f32(cp.P[2]);
+    GK_ASSERT(in.nk >= 0 && in.nk <= 5, "stage build with %d earlier stages", in.nk);
+    // pad the stage list to the kernel's count with (m, 0): fmaf(0, m, z) == z
+    StageIn si = in;
+    const int NKp = zfwd_nk(in.nk);
+    for (int j = in.nk; j < 6; j++) {
+        si.k[j] = in.m;
+        si.a[j] = 0.f;
+    }
+    si.nk = NKp;
     switch (M) {
+#define GK_ZF(n, K) k_zfwd<n, TR, K><<<g, TR * RegFFT<n>::G, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz)
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
         constexpr int TR = z_rows<n>();                                                         \
         dim3 g(div_up(Ny, TR), Nx, 3);                                                          \
-        k_zfwd<n, TR><<<g, TR * RegFFT<n>::G, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz);         \
+        switch (NKp) {                                                                          \
+        case 0: GK_ZF(n, 0); break;                                                             \
+        case 1: GK_ZF(n, 1); break;                                                             \
+        case 2: GK_ZF(n, 2); break;                                                             \
+        case 3: GK_ZF(n, 3); break;                                                             \
+        default: GK_ZF(n, 5); break;                                                            \
+        }                                                                                       \
         break;                                                                                  \
     }
-        GK_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
+#undef GK_ZF
     default: abort();
     }
     GK_LAUNCH_CHECK();
@@ -364,7 +379,7 @@ static void launch_zinv(const ConvPlan& cp, float* h, const EngineState* st, hip
         k_zinv<n, TR><<<g, TR * RegFFT<n>::G, 0, s>>>(cp.B1, h, twM, twP, st, Nx, Ny, Nz);      \
         break;                                                                                  \
     }
-        GK_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -388,7 +403,7 @@ static int epi_rows_rt(int M) {
     switch (M) {
 #define GK_CASE(n) \
     case n: return epi_rows<n>();
-        GK_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -398,7 +413,9 @@ int conv_epi_blocks(const ConvPlan& cp) { return div_up(cp.N[1], epi_rows_rt(cp.
 
 static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineState* st, hipStream_t s) {
     const int M = cp.M, Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
-    GK_ASSERT(st != nullptr && ea.nu >= 0 && ea.nu <= 6 && (!ea.fin || ea.nu <= 3), "fused Z inverse: bad epilogue");
+    GK_ASSERT(st != nullptr && ea.nu >= 0 && ea.nu <= 6 && (ea.fin != 1 || ea.nu <= 3) &&
+                  (ea.fin != 2 || (ea.mhigh && ea.part_err)),
+              "fused Z inverse: bad epilogue");
     // pad the list of earlier stages to the kernel's bound with (m, 0, 0): fmaf(0, m, z) == z
     const int nu = ea.nu;
     EpiArgs e = ea;
@@ -406,6 +423,7 @@ static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineS
         e.k[j] = ea.m;
         e.a[j] = 0.f;
         e.b[j] = 0.f;
+        e.ce[j] = 0.0;
     }
     const float2* twM = twiddle_f32(M);
     const float2* twP = twiddle_f32(cp.P[2]);
@@ -414,18 +432,21 @@ static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineS
     case n: {                                                                                    \
         constexpr int TR = epi_rows<n>();                                                        \
         const dim3 g(div_up(Ny, TR), Nx), b(3 * TR * RegFFT<n>::G);                              \
-        if (e.fin) {                                                                             \
-            if (nu <= 1) k_zinv_epi<n, TR, 1, true><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else k_zinv_epi<n, TR, 3, true><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);         \
+        if (e.fin == 1) {                                                                        \
+            if (nu <= 1) k_zinv_epi<n, TR, 1, 1><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else k_zinv_epi<n, TR, 3, 1><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);         \
+        } else if (e.fin == 2) {                                                                 \
+            if (nu <= 3) k_zinv_epi<n, TR, 3, 2><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else k_zinv_epi<n, TR, 6, 2><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);         \
         } else {                                                                                 \
-            if (nu == 0) k_zinv_epi<n, TR, 0, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else if (nu == 1) k_zinv_epi<n, TR, 1, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else if (nu <= 3) k_zinv_epi<n, TR, 3, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else k_zinv_epi<n, TR, 6, false><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);            \
+            if (nu == 0) k_zinv_epi<n, TR, 0, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else if (nu == 1) k_zinv_epi<n, TR, 1, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else if (nu <= 3) k_zinv_epi<n, TR, 3, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
+            else k_zinv_epi<n, TR, 6, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);            \
         }                                                                                        \
         break;                                                                                   \
     }
-        GK_POW2_1_2048(GK_CASE)
+        GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -476,7 +497,7 @@ static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipS
         else k_ypass<n, false><<<g, kNT, 0, s>>>(cp.B2, cp.B1, tw, st, Nx, Ny, Q, nit);          \
         break;                                                                                  \
     }
-        GK_POW2_2_4096(GK_CASE)
+        GK_LEN_Y(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -492,11 +513,11 @@ static void launch_xmul(const ConvPlan& cp, const EngineState* st, hipStream_t s
         constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
         const int lpr = LPB < PY ? LPB : PY;                                                    \
         int g = 0;                                                                              \
-        const int nit = pipeline_split((long)Q * (PY / lpr), (long)kXBlocksPerCU * cu_count(), &g); \
+        const int nit = pipeline_split((long)Q * div_up(PY, lpr), (long)kXBlocksPerCU * cu_count(), &g); \
         k_xmul<n><<<g, kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, Nx, PY, Q, nit);                      \
         break;                                                                                  \
     }
-        GK_POW2_2_1024(GK_CASE)
+        GK_LEN_X(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
@@ -513,7 +534,7 @@ static void launch_y2d(const ConvPlan& cp, const EngineState* st, hipStream_t s)
         k_y2d<n, NKZ><<<div_up(Q, NKZ), kNT, 0, s>>>(cp.B1, cp.Kc, tw, st, Ny, Q);              \
         break;                                                                                  \
     }
-        GK_POW2_2_4096(GK_CASE)
+        GK_LEN_Y(GK_CASE)
 #undef GK_CASE
     default: abort();
     }

@@ -211,9 +211,15 @@ struct EpiArgs {
     float a[6], b[6];
     float bs;
     int nu;
-    int fin;
+    int fin;          // 0 torque; 1 fixed-step final; 2 adaptive FSAL final (see zinv_epi_tile)
     TorqueArgs t;
-    double* epart;  // non-null: per-block sums of the energy density of m_i (FP64)
+    double* epart;    // non-null: per-block sums of the energy density of m_i (FP64)
+    // fin == 2: b2_j - b_j of the k slots and of this stage (times dt on the
+    // device), where m_high goes, and the per-block maximum error
+    double ce[6];
+    double ce_self;
+    float* mhigh;
+    float* part_err;
 };
 
 struct FinalArgs {

@@ -15,6 +15,13 @@
 
 #include "gk_common.h"
 
+// Multiply-adds in the butterflies, twiddles and real-transform splits are
+// fused only where one expression holds both (fp contract "on"), never across
+// statements: the result of a transform is then the same in every kernel that
+// inlines it, whatever the surrounding code (the batched and single-simulation
+// passes give bit-identical fields).
+#define GK_FFT_FP _Pragma("clang fp contract(on)")
+
 namespace gk {
 
 template <typename T> struct V2;
@@ -31,10 +38,14 @@ template <typename T> __device__ __forceinline__ C2<T> mk(T a, T b) {
 template <typename T> __device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
 template <typename T> __device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
 template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+    GK_FFT_FP
     return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T> __device__ __forceinline__ C2<T> cconj(C2<T> a) { return mk<T>(a.x, -a.y); }
-template <typename T> __device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T> __device__ __forceinline__ C2<T> cscale(C2<T> a, T s) {
+    GK_FFT_FP
+    return mk<T>(a.x * s, a.y * s);
+}
 
 // multiply by -i (DIR<0) or +i (DIR>0)
 template <typename T, int DIR> __device__ __forceinline__ C2<T> rot90(C2<T> v) {
@@ -49,12 +60,14 @@ template <typename T, int DIR> __device__ __forceinline__ C2<T> twid(const C2<T>
 }
 
 template <typename T, int DIR> __device__ __forceinline__ void dft2(C2<T>* v) {
+    GK_FFT_FP
     C2<T> a = v[0], b = v[1];
     v[0] = cadd<T>(a, b);
     v[1] = csub<T>(a, b);
 }
 
 template <typename T, int DIR> __device__ __forceinline__ void dft4(C2<T>& v0, C2<T>& v1, C2<T>& v2, C2<T>& v3) {
+    GK_FFT_FP
     C2<T> t0 = cadd<T>(v0, v2), t1 = csub<T>(v0, v2);
     C2<T> t2 = cadd<T>(v1, v3), t3 = rot90<T, DIR>(csub<T>(v1, v3));
     v0 = cadd<T>(t0, t2);
@@ -65,6 +78,7 @@ template <typename T, int DIR> __device__ __forceinline__ void dft4(C2<T>& v0, C
 
 // multiply by exp(DIR * 2 pi i * k / 8), k = 1, 2, 3
 template <typename T, int DIR, int K> __device__ __forceinline__ C2<T> w8(C2<T> v) {
+    GK_FFT_FP
     const T h = (T)0.70710678118654752440;
     if (K == 2) return rot90<T, DIR>(v);
     if (DIR < 0) {
@@ -77,6 +91,7 @@ template <typename T, int DIR, int K> __device__ __forceinline__ C2<T> w8(C2<T> 
 }
 
 template <typename T, int DIR> __device__ __forceinline__ void dft8(C2<T>* v) {
+    GK_FFT_FP
     C2<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
     C2<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4<T, DIR>(e0, e1, e2, e3);
@@ -95,6 +110,7 @@ template <typename T, int DIR> __device__ __forceinline__ void dft8(C2<T>* v) {
 }
 
 template <typename T, int DIR> __device__ __forceinline__ void dft3(C2<T>* v) {
+    GK_FFT_FP
     const T s = (T)0.86602540378443864676;  // sin(2 pi / 3)
     C2<T> t1 = cadd<T>(v[1], v[2]);
     C2<T> t2 = mk<T>(v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y);
@@ -105,6 +121,7 @@ template <typename T, int DIR> __device__ __forceinline__ void dft3(C2<T>* v) {
 }
 
 template <typename T, int DIR> __device__ __forceinline__ void dft5(C2<T>* v) {
+    GK_FFT_FP
     const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
     const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
     C2<T> t1 = cadd<T>(v[1], v[4]), t2 = cadd<T>(v[2], v[3]);
@@ -121,6 +138,7 @@ template <typename T, int DIR> __device__ __forceinline__ void dft5(C2<T>* v) {
 }
 
 template <typename T, int DIR> __device__ __forceinline__ void dft7(C2<T>* v) {
+    GK_FFT_FP
     // cos / sin of 2 pi k / 7, k = 1, 2, 3
     const T c[4] = {(T)1, (T)0.62348980185873353053, (T)-0.22252093395631440429, (T)-0.90096886790241912624};
     const T sn[4] = {(T)0, (T)0.78183148246802980871, (T)0.97492791218182360702, (T)0.43388373911755812048};
@@ -239,9 +257,31 @@ __device__ __forceinline__ void fft_lds(C2<T>* lds, int ls, const C2<T>* tw, int
 // the same layout, so a whole transform needs only (passes - 1) exchanges
 // through LDS, and loading / storing the line from / to memory is coalesced.
 // `lds` is this line's exchange buffer (RegFFT<L>::LDS complex), `tw` the table of L twiddles.
+// Register-FFT lengths are 2^a and 3 * 2^a.  Elements per lane: 8 for 2^a up
+// to 512 (radix-8 passes), 12 for 3 * 2^a up to 768 (radix 4, 2 and one radix
+// 3), L/64 beyond (64 lanes per line); G = L/E lanes per line is a power of two.
+constexpr int reg_fft_E(int L) {
+    return L % 3 == 0 ? (L <= 12 ? L : (L / 12 <= 64 ? 12 : L / 64)) : (L <= 512 ? (L < 8 ? L : 8) : L / 64);
+}
+constexpr int reg_fft_G(int L) { return L / reg_fft_E(L); }
+constexpr bool reg_fft_length(int L) {
+    int m = L % 3 == 0 ? L / 3 : L;
+    while (m % 2 == 0) m /= 2;
+    return L >= 1 && m == 1;
+}
+// radix of the next register pass: the largest of 8, 4, 2 dividing both the
+// remaining length and the elements per lane, else 3
+constexpr int reg_radix(int rem, int E) {
+    return (rem % 8 == 0 && E % 8 == 0) ? 8
+         : (rem % 4 == 0 && E % 4 == 0) ? 4
+         : (rem % 2 == 0 && E % 2 == 0) ? 2
+         : (rem % 3 == 0 && E % 3 == 0) ? 3 : 0;
+}
+
 template <int L> struct RegFFT {
-    static constexpr int E = L <= 512 ? (L < 8 ? L : 8) : L / 64;  // elements per lane
-    static constexpr int G = L / E;                                // lanes per line
+    static_assert(reg_fft_length(L), "register FFT lengths are 2^a and 3 * 2^a");
+    static constexpr int E = reg_fft_E(L);  // elements per lane
+    static constexpr int G = L / E;         // lanes per line
     // exchange buffer per line: one float2 of padding per 8, so the stride-8
     // scatter of a radix-8 pass and neighbouring lines spread over all 64 banks
     static constexpr int LDS = L + L / 8;
@@ -253,14 +293,16 @@ __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
 // lane-strided layout (index g + G e) these are exactly the elements e < E/2;
 // the others are compile-time zeros / dead, and the compiler prunes the first
 // and last passes accordingly.
-template <int L> __device__ constexpr bool pad_half(int e) { return RegFFT<L>::E >= 2 && e >= RegFFT<L>::E / 2; }
+template <int L> __device__ constexpr bool pad_half(int e) {
+    return RegFFT<L>::E >= 2 && RegFFT<L>::E % 2 == 0 && e >= RegFFT<L>::E / 2;
+}
 
 template <typename T, int L, int DIR, int NS = 1>
 __device__ __forceinline__ void reg_fft(C2<T>* v, int g, C2<T>* lds, const C2<T>* tw) {
     if constexpr (NS < L) {
-        constexpr int R = next_radix(L / NS);
         constexpr int E = RegFFT<L>::E, G = RegFFT<L>::G;
-        static_assert(E % R == 0, "radix must divide the elements per lane");
+        constexpr int R = reg_radix(L / NS, E);
+        static_assert(R != 0 && E % R == 0, "radix must divide the elements per lane");
         constexpr int NB = E / R;  // butterflies per lane
         constexpr bool last = NS * R == L;
 #pragma unroll
@@ -322,6 +364,7 @@ __device__ __forceinline__ void batched_copy(int cnt, Ld ld, St st) {
 // E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2i,  W = exp(-2 pi i / P).
 template <typename T>
 __device__ __forceinline__ C2<T> r2c_split(C2<T> zk, C2<T> zmk, C2<T> wk /* exp(-2 pi i k/P) */) {
+    GK_FFT_FP
     C2<T> e = mk<T>((zk.x + zmk.x) * (T)0.5, (zk.y - zmk.y) * (T)0.5);
     // D = zk - conj(zmk);  O = D / 2i = (D.y/2, -D.x/2)
     T dx = zk.x - zmk.x, dy = zk.y + zmk.y;
@@ -334,6 +377,7 @@ __device__ __forceinline__ C2<T> r2c_split(C2<T> zk, C2<T> zmk, C2<T> wk /* exp(
 // then IFFT_M(Y) = x[2n] + i x[2n+1].
 template <typename T>
 __device__ __forceinline__ C2<T> c2r_merge(C2<T> xk, C2<T> xmk, C2<T> wk /* exp(-2 pi i k/P) */) {
+    GK_FFT_FP
     C2<T> s = mk<T>(xk.x + xmk.x, xk.y - xmk.y);
     C2<T> d = mk<T>(xk.x - xmk.x, xk.y + xmk.y);
     C2<T> t = cmul<T>(cconj<T>(wk), d);  // W^-k * d

@@ -255,13 +255,13 @@ void fft_rows_c2r(const float2* in, float* out, int P, long nrows, long n_inner,
 void fft3_c2c_f64(double2* d, const int n[3], int dir, hipStream_t s) {
     const long n12 = (long)n[1] * n[2];
     if (dir < 0) {
-        if (n[2] > 1) fft_lines<double, -1>(d, n[2], (long)n[0] * n[1], (long)n[0] * n[1], n[2], 0, 1, n[2], n[2], s);
-        if (n[1] > 1) fft_lines<double, -1>(d, n[1], (long)n[0] * n[2], n[2], 1, n12, n[2], n[1], n[1], s);
-        if (n[0] > 1) fft_lines<double, -1>(d, n[0], n12, n12, 1, 0, n12, n[0], n[0], s);
+        if (n[2] > 1) fft_lines_any<double, -1>(d, n[2], (long)n[0] * n[1], (long)n[0] * n[1], n[2], 0, 1, n[2], n[2], s);
+        if (n[1] > 1) fft_lines_any<double, -1>(d, n[1], (long)n[0] * n[2], n[2], 1, n12, n[2], n[1], n[1], s);
+        if (n[0] > 1) fft_lines_any<double, -1>(d, n[0], n12, n12, 1, 0, n12, n[0], n[0], s);
     } else {
-        if (n[2] > 1) fft_lines<double, +1>(d, n[2], (long)n[0] * n[1], (long)n[0] * n[1], n[2], 0, 1, n[2], n[2], s);
-        if (n[1] > 1) fft_lines<double, +1>(d, n[1], (long)n[0] * n[2], n[2], 1, n12, n[2], n[1], n[1], s);
-        if (n[0] > 1) fft_lines<double, +1>(d, n[0], n12, n12, 1, 0, n12, n[0], n[0], s);
+        if (n[2] > 1) fft_lines_any<double, +1>(d, n[2], (long)n[0] * n[1], (long)n[0] * n[1], n[2], 0, 1, n[2], n[2], s);
+        if (n[1] > 1) fft_lines_any<double, +1>(d, n[1], (long)n[0] * n[2], n[2], 1, n12, n[2], n[1], n[1], s);
+        if (n[0] > 1) fft_lines_any<double, +1>(d, n[0], n12, n12, 1, 0, n12, n[0], n[0], s);
     }
 }
 

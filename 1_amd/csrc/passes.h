@@ -8,6 +8,18 @@
 #include "engine.h"
 #include "fft_device.h"
 
+// Lengths the engine's register-FFT passes are built for (2^a and 3 * 2^a):
+// M = half the padded Z length, PY the padded Y length, PX the padded X length.
+#define GK_LEN_M(X)                                                                                              \
+    X(1) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) \
+        X(768) X(1024) X(1536) X(2048)
+#define GK_LEN_Y(X)                                                                                              \
+    X(2) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)    \
+        X(1024) X(1536) X(2048) X(3072) X(4096)
+#define GK_LEN_X(X)                                                                                              \
+    X(2) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)    \
+        X(1024)
+
 namespace gk {
 
 // =============================================================== passes
@@ -46,8 +58,12 @@ template <int M, int TR> constexpr int zfwd_lds() {
     return FFT_LDS > TILE_LDS ? FFT_LDS : TILE_LDS;
 }
 
-// body: component c, plane x, rows y0..y0+TR; lds holds zfwd_lds<M,TR>() complex
-template <int M, int TR>
+// body: component c, plane x, rows y0..y0+TR; lds holds zfwd_lds<M,TR>() complex.
+// NK earlier stages enter the stage build (a compile-time count, so every load
+// of a lane is issued before the first is used; unused slots hold m with a
+// zero coefficient).
+constexpr int zfwd_nk(int nk) { return nk <= 3 ? nk : 5; }
+template <int M, int TR, int NK>
 __device__ __forceinline__ void zfwd_tile(const StageIn& si, float2* __restrict__ B1, const float2* __restrict__ twM,
                                           const float2* __restrict__ twP, int Nx, int Ny, int Nz, int c, int x,
                                           int y0, float2* lds) {
@@ -62,7 +78,6 @@ __device__ __forceinline__ void zfwd_tile(const StageIn& si, float2* __restrict_
     float coef[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) coef[j] = dt * si.a[j];
-    const int nk = si.nk;
     float2 v[E];
     if ((Nz & 1) == 0) {
         // pairs (2n, 2n+1) as float2: rows start at an even float offset
@@ -72,7 +87,8 @@ __device__ __forceinline__ void zfwd_tile(const StageIn& si, float2* __restrict_
             float2 z = make_float2(0.f, 0.f);
             if (!pad_half<M>(e) && ok && 2 * n < Nz) {
                 z = *reinterpret_cast<const float2*>(si.m + rbase + 2 * n);
-                for (int j = 0; j < nk; j++) {
+#pragma unroll
+                for (int j = 0; j < NK; j++) {
                     const float2 kv = *reinterpret_cast<const float2*>(si.k[j] + rbase + 2 * n);
                     z.x = fmaf(coef[j], kv.x, z.x);
                     z.y = fmaf(coef[j], kv.y, z.y);
@@ -90,7 +106,8 @@ __device__ __forceinline__ void zfwd_tile(const StageIn& si, float2* __restrict_
                 const int i = 2 * n + h;
                 if (!pad_half<M>(e) && ok && i < Nz) {
                     float a = si.m[rbase + i];
-                    for (int j = 0; j < nk; j++) a = fmaf(coef[j], si.k[j][rbase + i], a);
+#pragma unroll
+                    for (int j = 0; j < NK; j++) a = fmaf(coef[j], si.k[j][rbase + i], a);
                     if (si.mout) si.mout[rbase + i] = a;
                     z2[h] = a;
                 }
@@ -122,13 +139,13 @@ __device__ __forceinline__ void zfwd_tile(const StageIn& si, float2* __restrict_
     }
 }
 
-template <int M, int TR>
+template <int M, int TR, int NK>
 __global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zfwd(StageIn si, float2* __restrict__ B1,
                                                               const float2* __restrict__ twM,
                                                               const float2* __restrict__ twP, int Nx, int Ny, int Nz) {
     __shared__ float2 lds[zfwd_lds<M, TR>()];
     if (si.st && !si.st->trial_active) return;
-    zfwd_tile<M, TR>(si, B1, twM, twP, Nx, Ny, Nz, blockIdx.z, blockIdx.y, blockIdx.x * TR, lds);
+    zfwd_tile<M, TR, NK>(si, B1, twM, twP, Nx, Ny, Nz, blockIdx.z, blockIdx.y, blockIdx.x * TR, lds);
 }
 
 // Z inverse: B1[c][x][kz][y0..y0+TR) -> rows of h.  The spectrum is read through
@@ -203,7 +220,12 @@ template <int M, int TR> constexpr int zinv_epi_lds() {
 // body: plane x, rows y0..y0+TR, all 3 components; the per-block max |k| goes to
 // ea.t.part[pidx]; lds / hext_s / red: zinv_epi_lds<M,TR>() complex, 3 and
 // 3 TR G floats
-template <int M, int TR, int NU, bool FIN>
+// FIN: 0 torque only; 1 last stage of a fixed-step tableau (m <- normalise(m +
+// dt sum b_j k_j)); 2 last stage of an adaptive FSAL tableau, whose stage input
+// is the high-order solution (a_Sj = b_j): it is stored to ea.mhigh and the
+// error dt sum (b2_j - b_j) k_j, this stage's k included, is reduced per block
+// into ea.part_err (rk.go:381-403), in place of a separate final pass.
+template <int M, int TR, int NU, int FIN>
 __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, const EpiArgs& ea,
                                               const float2* __restrict__ twM, const float2* __restrict__ twP,
                                               const EngineState* st, int Nx, int Ny, int Nz, int x, int y0, int pidx,
@@ -256,12 +278,17 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
         cb[j] = dt * ea.b[j];
     }
     const float cbs = dt * ea.bs;
+    double ce[NU > 0 ? NU : 1];
+#pragma unroll
+    for (int j = 0; j < NU; j++) ce[j] = FIN == 2 ? ea.ce[j] * (double)dt : 0.0;
+    const double ce_self = FIN == 2 ? ea.ce_self * (double)dt : 0.0;
     const long N = (long)Nx * Ny * Nz;
-    float lmax = 0.f;
+    float lmax = 0.f, emax = 0.f;
     double esum = 0.0;
     const bool want_e = ea.epart != nullptr;
-    // one cell from the accumulated stage input mi (and final sum hi) and the field h
-    auto finish = [&](const float mi[3], const float hi[3], const float h[3], float o[3]) {
+    // one cell from the accumulated stage input mi (and final sum hi, or the
+    // error of the earlier stages er) and the field h
+    auto finish = [&](const float mi[3], const float hi[3], const double er[3], const float h[3], float o[3]) {
         if (want_e) {
             // phi = -1/2 m.H_conv - m.B_ext - 1/2 k (m.u)^2 (field.go:64-71, gpu_local_contr3.cu:238-256)
             const double mx = mi[0], my = mi[1], mz = mi[2];
@@ -276,7 +303,11 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
         float k[3];
         llg_torque_cell(ea.t, hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
         lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
-        if constexpr (FIN) {
+        if constexpr (FIN == 2) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) emax = fmaxf(emax, (float)fabs(fma(ce_self, (double)k[q], er[q])));
+        }
+        if constexpr (FIN == 1) {
             float f[3];
 #pragma unroll
             for (int q = 0; q < 3; q++) f[q] = fmaf(cbs, k[q], hi[q]);
@@ -294,6 +325,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
         constexpr int IT = (TR * HS + NT - 1) / NT;
         const int nzh = Nz >> 1, P = nrows * nzh;
         float2 ai[IT][3], af[IT][3];
+        double ex[FIN == 2 ? IT : 1][3], ey[FIN == 2 ? IT : 1][3];
 #pragma unroll
         for (int it = 0; it < IT; it++) {
             const int p = threadIdx.x + it * NT;
@@ -302,6 +334,9 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
                 const long gi = rowbase + (long)r * Nz + 2 * n;
 #pragma unroll
                 for (int q = 0; q < 3; q++) ai[it][q] = af[it][q] = *reinterpret_cast<const float2*>(ea.m + q * N + gi);
+                if constexpr (FIN == 2)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) ex[it][q] = ey[it][q] = 0.0;
 #pragma unroll
                 for (int j = 0; j < NU; j++)
 #pragma unroll
@@ -309,9 +344,13 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
                         const float2 kv = *reinterpret_cast<const float2*>(ea.k[j] + q * N + gi);
                         ai[it][q].x = fmaf(ca[j], kv.x, ai[it][q].x);
                         ai[it][q].y = fmaf(ca[j], kv.y, ai[it][q].y);
-                        if constexpr (FIN) {
+                        if constexpr (FIN == 1) {
                             af[it][q].x = fmaf(cb[j], kv.x, af[it][q].x);
                             af[it][q].y = fmaf(cb[j], kv.y, af[it][q].y);
+                        }
+                        if constexpr (FIN == 2) {
+                            ex[it][q] = fma(ce[j], (double)kv.x, ex[it][q]);
+                            ey[it][q] = fma(ce[j], (double)kv.y, ey[it][q]);
                         }
                     }
             }
@@ -326,23 +365,29 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
 #pragma unroll
                 for (int q = 0; q < 3; q++) hv[q] = lds[(q * TR + r) * HS + n];
                 float mi[3], hi[3], h[3], o0[3], o1[3];
+                double er[3] = {0.0, 0.0, 0.0};
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     mi[q] = ai[it][q].x;
                     hi[q] = af[it][q].x;
                     h[q] = hv[q].x;
+                    if constexpr (FIN == 2) er[q] = ex[it][q];
                 }
-                finish(mi, hi, h, o0);
+                finish(mi, hi, er, h, o0);
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     mi[q] = ai[it][q].y;
                     hi[q] = af[it][q].y;
                     h[q] = hv[q].y;
+                    if constexpr (FIN == 2) er[q] = ey[it][q];
                 }
-                finish(mi, hi, h, o1);
+                finish(mi, hi, er, h, o1);
 #pragma unroll
-                for (int q = 0; q < 3; q++)
+                for (int q = 0; q < 3; q++) {
                     *reinterpret_cast<float2*>(ea.out + q * N + gi) = make_float2(o0[q], o1[q]);
+                    if constexpr (FIN == 2)
+                        *reinterpret_cast<float2*>(ea.mhigh + q * N + gi) = ai[it][q];
+                }
             }
         }
     } else {
@@ -352,6 +397,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
             const int r = p / Nz, z = p - r * Nz;
             const long gi = rowbase + (long)r * Nz + z;
             float mi[3], hi[3], h[3], o[3];
+            double er[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 mi[q] = hi[q] = ea.m[q * N + gi];
@@ -361,11 +407,15 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
                     const float kv = ea.k[j][q * N + gi];
                     mi[q] = fmaf(ca[j], kv, mi[q]);
                     hi[q] = fmaf(cb[j], kv, hi[q]);
+                    if constexpr (FIN == 2) er[q] = fma(ce[j], (double)kv, er[q]);
                 }
             }
-            finish(mi, hi, h, o);
+            finish(mi, hi, er, h, o);
 #pragma unroll
-            for (int q = 0; q < 3; q++) ea.out[q * N + gi] = o[q];
+            for (int q = 0; q < 3; q++) {
+                ea.out[q * N + gi] = o[q];
+                if constexpr (FIN == 2) ea.mhigh[q * N + gi] = mi[q];
+            }
         }
     }
     if (want_e) {
@@ -380,27 +430,31 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
             ea.epart[pidx] = e;
         }
     }
-    if (ea.t.part) {
-        red[threadIdx.x] = lmax;
+    // per-block maximum of v (>= 0) into out[pidx]
+    auto block_max = [&](float v, float* out) {
+        __syncthreads();
+        red[threadIdx.x] = v;
         __syncthreads();
         if constexpr (NT >= 64) {
             if (threadIdx.x < 64) {
                 float vm = 0.f;
                 for (int i = threadIdx.x; i < NT; i += 64) vm = fmaxf(vm, red[i]);
                 for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
-                if (threadIdx.x == 0) ea.t.part[pidx] = vm;
+                if (threadIdx.x == 0) out[pidx] = vm;
             }
         } else {
             if (threadIdx.x == 0) {
                 float vm = 0.f;
                 for (int i = 0; i < NT; i++) vm = fmaxf(vm, red[i]);
-                ea.t.part[pidx] = vm;
+                out[pidx] = vm;
             }
         }
-    }
+    };
+    if (ea.t.part) block_max(lmax, ea.t.part);
+    if constexpr (FIN == 2) block_max(emax, ea.part_err);
 }
 
-template <int M, int TR, int NU, bool FIN>
+template <int M, int TR, int NU, int FIN>
 __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void k_zinv_epi(const float2* __restrict__ B1, EpiArgs ea,
                                                                       const float2* __restrict__ twM,
                                                                       const float2* __restrict__ twP,
@@ -546,6 +600,7 @@ __device__ __forceinline__ void xmul_store(float2 (*v)[RegFFT<PX>::E], float2* _
 template <int PX>
 __device__ __forceinline__ void xmul_kernel(float2 (*v)[RegFFT<PX>::E], const float* __restrict__ Kc, int kz, int ky,
                                             int PY, int g) {
+    GK_FFT_FP
     using F = RegFFT<PX>;
     const int hx = PX / 2 + 1, hy = PY / 2 + 1;
     const bool fy = ky > PY / 2;
@@ -571,11 +626,12 @@ __device__ __forceinline__ void xmul_kernel(float2 (*v)[RegFFT<PX>::E], const fl
 }
 // column of item t for lane group li: items are (kz, run of lpr ky), with the
 // runs ky and PY - ky adjacent; ky < 0 when the lane group has no column
-__device__ __forceinline__ void xmul_column(long t, long items, int rows, int lpr, int li, int& kz, int& ky) {
+__device__ __forceinline__ void xmul_column(long t, long items, int rows, int lpr, int li, int PY, int& kz,
+                                            int& ky) {
     kz = (int)(t / rows);
     const int rr = (int)(t % rows);
     const int r = (rr & 1) ? rows - 1 - (rr >> 1) : (rr >> 1);
-    ky = (t < items && li < lpr) ? r * lpr + li : -1;
+    ky = (t < items && li < lpr && r * lpr + li < PY) ? r * lpr + li : -1;
 }
 
 template <int PX>
@@ -590,13 +646,13 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
     for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
     const int li = threadIdx.x / G, g = threadIdx.x % G;
     const int lpr = LPB < PY ? LPB : PY;  // columns per item
-    const int rows = PY / lpr;            // items per kz plane
+    const int rows = (PY + lpr - 1) / lpr;  // items per kz plane (the last may be partial)
     const long items = (long)Q * rows;
     const long t0 = (long)blockIdx.x * nit;
     float2 nxt[3][E];
     {
         int kz, ky;
-        xmul_column(t0, items, rows, lpr, li, kz, ky);
+        xmul_column(t0, items, rows, lpr, li, PY, kz, ky);
         xmul_load<PX>(nxt, B2, kz, ky, Nx, PY, Q, g);
     }
     __syncthreads();  // twiddle table
@@ -609,11 +665,11 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
             for (int e = 0; e < E; e++) v[c][e] = nxt[c][e];
         if (it + 1 < nit) {
             int kz1, ky1;
-            xmul_column(t + 1, items, rows, lpr, li, kz1, ky1);
+            xmul_column(t + 1, items, rows, lpr, li, PY, kz1, ky1);
             xmul_load<PX>(nxt, B2, kz1, ky1, Nx, PY, Q, g);
         }
         int kz, ky;
-        xmul_column(t, items, rows, lpr, li, kz, ky);
+        xmul_column(t, items, rows, lpr, li, PY, kz, ky);
 #pragma unroll
         for (int c = 0; c < 3; c++) reg_fft<float, PX, -1>(v[c], g, lds + li * F::LDS, twl);
         if (ky >= 0) xmul_kernel<PX>(v, Kc, kz, ky, PY, g);

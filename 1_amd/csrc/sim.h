@@ -15,6 +15,15 @@ struct Tableau {
     int S = 0, fsal = 0, adaptive = 0;
     float order = 0;
     float a[7][7] = {}, b[7] = {}, b2[7] = {}, c[7] = {};
+    // adaptive FSAL tableau whose last stage input is the high-order solution
+    // (a_{S-1,j} == b_j): its final combination and error run in that stage's
+    // epilogue
+    bool fsal_final() const {
+        if (!(adaptive && fsal)) return false;
+        for (int j = 0; j < S; j++)
+            if (a[S - 1][j] != b[j]) return false;
+        return true;
+    }
     // largest node among stages 0..i (the applied field's search time)
     float cmax(int i) const {
         float m = 0.f;

@@ -233,7 +233,8 @@ CtrlParams gk::ctrl_params(const mmSim* s) {
     c.part_k6 = s->part + s->npart;
     c.part_err = s->part + 2 * s->npart;
     c.n_k = s->nk_part;
-    c.n_err = kRedBlocks;
+    // per-block errors: from the last stage's epilogue (adaptive FSAL) or the final pass
+    c.n_err = s->tb.fsal_final() ? s->nk_part : kRedBlocks;
     c.part_e = s->wantEnergy ? s->epart : nullptr;
     c.cmax = s->tb.cmax(s->tb.S - 1);
     c.cell_volume = s->cell[0] * s->cell[1] * s->cell[2];
@@ -296,16 +297,24 @@ static void enqueue_trial(mmSim* s) {
         // from m and the k_j; the last stage of a fixed-step tableau also forms
         // and normalises the new m there (rk.go:381-390)
         const bool fin = !t.adaptive && i == t.S - 1;
+        // adaptive FSAL: the last stage's epilogue also forms m_high and the error (rk.go:381-403)
+        const bool efin = t.fsal_final() && i == t.S - 1;
         EpiArgs ea{};
         ea.m = s->m;
         for (int j = 0; j < i; j++)
-            if (t.a[i][j] != 0.f || (fin && t.b[j] != 0.f)) {
+            if (t.a[i][j] != 0.f || (fin && t.b[j] != 0.f) || (efin && t.b2[j] != t.b[j])) {
                 ea.k[ea.nu] = s->k[j];
                 ea.a[ea.nu] = t.a[i][j];
                 ea.b[ea.nu] = fin ? t.b[j] : 0.f;
+                ea.ce[ea.nu] = efin ? (double)t.b2[j] - (double)t.b[j] : 0.0;
                 ea.nu++;
             }
-        ea.fin = fin;
+        ea.fin = fin ? 1 : efin ? 2 : 0;
+        if (efin) {
+            ea.ce_self = (double)t.b2[i] - (double)t.b[i];
+            ea.mhigh = s->mtmp;
+            ea.part_err = s->part + 2 * s->npart;
+        }
         ea.bs = fin ? t.b[i] : 0.f;
         ea.out = fin ? s->m : s->k[i];
         // max|k| is needed for k0 (rk.go:353) and, with FSAL, for the last stage (next k0)
@@ -332,7 +341,7 @@ static void enqueue_trial(mmSim* s) {
     fa.part_err = s->part + 2 * s->npart;
     const float* kk[7];
     for (int i = 0; i < 7; i++) kk[i] = s->k[i] ? s->k[i] : s->k[0];
-    if (t.adaptive) {
+    if (t.adaptive && !t.fsal_final()) {
         pf.begin(6, s->s);
         launch_rk_final(s->m, kk, s->mtmp, s->st, fa, s->N, s->s);
         pf.end(s->s);

@@ -68,8 +68,25 @@ CASES = [
     ((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9)),   # film, 2-D path
     ((4, 16, 32), (4e-9, 4e-9, 4e-9)),                # cuboid, 3-D path
     ((3, 12, 20), (2e-9, 3e-9, 2.5e-9)),              # non-power-of-two data sizes
-    ((1, 24, 40), (2e-9, 2e-9, 2e-9)),
+    ((1, 24, 40), (2e-9, 2e-9, 2e-9)),                # padded 48 x 96: 3 * 2^a lengths
+    ((2, 48, 80), (3e-9, 3e-9, 3e-9)),                # padded 4 x 96 x 192
+    ((6, 40, 24), (3e-9, 2e-9, 2.5e-9)),              # padded 12 x 96 x 48
+    ((1, 96, 384), (3e-9, 3e-9, 3e-9)),               # padded 192 x 768
 ]
+
+
+def test_padded_lengths():
+    """Each axis is padded to the smallest 2^a or 3 * 2^a >= 2N (1 for N = 1):
+    the same linear convolution as the reference's 2N (kernel.go:79-90)."""
+    import importlib
+
+    mm = importlib.import_module("1_amd")
+    for size, P in (((1, 24, 40), (1, 48, 96)), ((3, 12, 20), (6, 24, 48)), ((4, 16, 32), (8, 32, 64)),
+                    ((6, 40, 24), (12, 96, 48)), ((7, 100, 129), (16, 256, 384))):
+        s = engine(mm, size, (3e-9, 3e-9, 3e-9), m0=random_m(size, 0))
+        st = s.state()
+        assert (int(st["px"]), int(st["py"]), int(st["pz"])) == P
+        s.free()
 
 
 @pytest.mark.parametrize("size,cell", CASES)
