@@ -159,14 +159,30 @@ __global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx,
     if (ky >= 0) xmul_store<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
 }
 
-template <int PY, int NKZ>
-__global__ __launch_bounds__(kNT) void kb_y2d(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+template <int PY>
+__global__ __launch_bounds__(kNT) void kb_y2r(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
                                               const float2* __restrict__ tw) {
-    __shared__ float2 lds[3 * NKZ * (PY + 1)];
+    using F = RegFFT<PY>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    constexpr bool TWL = PY <= 1024;
+    __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[TWL ? PY : 1];
     const BBlock bb = bblock(sims, idx, start, n);
     const BSim& d = *bb.d;
     if (bskip(d, sg)) return;
-    y2d_tile<PY, NKZ>(d.B1, d.Kc, tw, d.Ny, d.Q, bb.lb * NKZ, lds);
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const long kz = (long)bb.lb * LPB + li;
+    __syncthreads();  // twiddle table
+    if constexpr (E > 16) {
+        y2r_column_split<PY>(d.B1, d.Kc, kz, kz < d.Q, d.Ny, d.Q, g, lds + li * F::LDS, TWL ? twl : tw);
+    } else {
+        float2 v[3][E];
+        y2r_load<PY>(v, d.B1, kz, kz < d.Q, d.Ny, d.Q, g);
+        y2r_column<PY>(v, d.Kc, kz, kz < d.Q, d.Q, g, lds + li * F::LDS, TWL ? twl : tw);
+        if (kz < d.Q) y2r_store<PY>(v, d.B1, kz, d.Ny, d.Q, g);
+    }
 }
 
 template <int M, int TR, int NU, int FIN>
@@ -271,7 +287,6 @@ struct LaunchSet {
 
 int epi_rows_host(int M);
 int z_rows_host(int M);
-int y2d_nkz_host(int PY);
 
 }  // namespace
 
@@ -310,13 +325,6 @@ template <int M> constexpr int epi_rows_c() {
     while (tr > 1 && 3 * (M + 1) * (tr + 1) * 8 > 96 * 1024) tr /= 2;
     return tr;
 }
-template <int L> constexpr int y2d_nkz_c() {
-    int n = 49152 / (3 * (L + 1) * 8);
-    n = n < 1 ? 1 : (n > 32 ? 32 : n);
-    int p = 1;
-    while (p * 2 <= n) p *= 2;
-    return p;
-}
 
 
 int z_rows_host(int M) {
@@ -333,15 +341,6 @@ int epi_rows_host(int M) {
 #define GK_CASE(n) \
     case n: return epi_rows_c<n>();
         GK_LEN_M(GK_CASE)
-#undef GK_CASE
-    }
-    abort();
-}
-int y2d_nkz_host(int PY) {
-    switch (PY) {
-#define GK_CASE(n) \
-    case n: return y2d_nkz_c<n>();
-        GK_LEN_Y(GK_CASE)
 #undef GK_CASE
     }
     abort();
@@ -433,7 +432,7 @@ void bl_y2d(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     const float2* tw = twiddle_f32(ls.L);
     switch (ls.L) {
 #define GK_CASE(n)                                                                                           \
-    case n: kb_y2d<n, y2d_nkz_c<n>()><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw); \
+    case n: kb_y2r<n><<<ls.blocks, kNT, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, tw); \
         break;
         GK_LEN_Y(GK_CASE)
 #undef GK_CASE
@@ -604,7 +603,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         const int lpr = std::min(lpb, s->cp.P[1]);
         return s->cp.Q * div_up(s->cp.P[1], lpr);
     });
-    b->y2 = make_sets(b, d2, PY, [](const mmSim* s) { return div_up(s->cp.Q, y2d_nkz_host(s->cp.P[1])); });
+    b->y2 = make_sets(b, d2, PY, [](const mmSim* s) { return div_up(s->cp.Q, 256 / lanes_host(s->cp.P[1])); });
     b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });
     b->all256 = make_sets(b, all, [](const mmSim*) { return 0; },
                           [](const mmSim* s) { return std::max(1, std::min(1024, div_up(3L * s->N, 256 * 4))); });

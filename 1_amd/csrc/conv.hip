@@ -313,14 +313,6 @@ double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool 
 
 // =============================================================== launchers
 
-template <int L> constexpr int groups_per_block(int budget_lines_bytes) {
-    // how many 3-line groups of length L fit in `budget` bytes of LDS (>= 1, <= 32, power of two)
-    int n = budget_lines_bytes / (3 * (L + 1) * 8);
-    n = n < 1 ? 1 : (n > 32 ? 32 : n);
-    int p = 1;
-    while (p * 2 <= n) p *= 2;
-    return p;
-}
 
 // rows per Z block: enough consecutive y for 64-128 B transposed segments, at most 1024 threads
 template <int M> constexpr int z_rows() {
@@ -530,8 +522,10 @@ static void launch_y2d(const ConvPlan& cp, const EngineState* st, hipStream_t s)
     switch (PY) {
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \
-        constexpr int NKZ = groups_per_block<n>(49152);                                         \
-        k_y2d<n, NKZ><<<div_up(Q, NKZ), kNT, 0, s>>>(cp.B1, cp.Kc, tw, st, Ny, Q);              \
+        constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
+        int g = 0;                                                                              \
+        const int nit = pipeline_split(div_up(Q, LPB), (long)kYBlocksPerCU * cu_count(), &g);  \
+        k_y2r<n><<<g, kNT, 0, s>>>(cp.B1, cp.Kc, tw, st, Ny, Q, nit);                           \
         break;                                                                                  \
     }
         GK_LEN_Y(GK_CASE)

@@ -679,63 +679,138 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
     }
 }
 
-// 2-D: Y forward . kernel . Y inverse on NKZ consecutive kz rows, all 3 components, in place in B1
-template <int PY, int NKZ>
-__device__ __forceinline__ void y2d_tile(float2* __restrict__ B1, const float* __restrict__ Kc,
-                                         const float2* __restrict__ tw, int Ny, int Q, int kz0, float2* lds) {
-    constexpr int LS = PY + 1;
-    const int nkz = min(NKZ, Q - kz0);
-    const long cstride = (long)Q * Ny;
-    const int per = nkz * Ny;
-    batched_copy<float2, kNT>(
-        3 * per,
-        [&](int t) {
-            const int c = t / per, r = t - c * per;
-            return B1[c * cstride + (long)kz0 * Ny + r];
-        },
-        [&](int t, float2 v) {
-            const int c = t / per, r = t - c * per;
-            const int j = r / Ny, y = r - j * Ny;
-            lds[(c * NKZ + j) * LS + y] = v;
-        });
-    __syncthreads();
-    fft_lds<float, PY, -1, 3 * NKZ, kNT>(lds, LS, tw, Ny);
-    const int hy = PY / 2 + 1;
+// 2-D: Y forward . kernel . Y inverse of one kz column (3 components) in
+// registers, in place in B1[c][0][kz][y].  Lane group li of a block takes
+// column kz; the 4 stored kernel components (xx, yy, zz, yz) for ky <= PY/2
+// are read with the yz sign restored for ky > PY/2.  Components c0..c0+NC.
+template <int PY, int NC = 3>
+__device__ __forceinline__ void y2r_load(float2 (*v)[RegFFT<PY>::E], const float2* __restrict__ B1, long kz, bool ok,
+                                         int Ny, int Q, int g, int c0 = 0) {
+    using F = RegFFT<PY>;
+    const long cs = (long)Q * Ny;
+    const float2* p = B1 + (ok ? kz : 0) * Ny;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int i = g + F::G * e;
+            v[c][e] = (!pad_half<PY>(e) && ok && i < Ny) ? p[(c0 + c) * cs + i] : make_float2(0.f, 0.f);
+        }
+}
+template <int PY, int NC = 3>
+__device__ __forceinline__ void y2r_store(float2 (*v)[RegFFT<PY>::E], float2* __restrict__ B1, long kz, int Ny, int Q,
+                                          int g, int c0 = 0) {
+    using F = RegFFT<PY>;
+    const long cs = (long)Q * Ny;
+    float2* p = B1 + kz * Ny;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int e = 0; e < F::E; e++) {
+            const int i = g + F::G * e;
+            if (!pad_half<PY>(e) && i < Ny) p[(c0 + c) * cs + i] = v[c][e];
+        }
+}
+// kernel multiply; C0 = 0: all 3 components in v[0..2]; C0 = 1: only x in v[0];
+// C0 = 2: only y, z in v[0..1]
+template <int PY, int C0 = 0>
+__device__ __forceinline__ void y2r_kernel(float2 (*v)[RegFFT<PY>::E], const float* __restrict__ Kc, long kz, int Q,
+                                           int g) {
+    GK_FFT_FP
+    using F = RegFFT<PY>;
+    constexpr int hy = PY / 2 + 1;
     const long cs = (long)Q * hy;
-    for (int t = threadIdx.x; t < nkz * PY; t += kNT) {
-        const int j = t / PY, ky = t - j * PY;
-        const int kz = kz0 + j;
+#pragma unroll
+    for (int e = 0; e < F::E; e++) {
+        const int ky = g + F::G * e;
         const bool fy = ky > PY / 2;
-        const int kyp = fy ? PY - ky : ky;
-        const long b = (long)kz * hy + kyp;
-        const float kxx = Kc[b], kyy = Kc[cs + b], kzz = Kc[2 * cs + b];
-        const float kyz = fy ? -Kc[3 * cs + b] : Kc[3 * cs + b];
-        float2* px = &lds[(0 * NKZ + j) * LS + ky];
-        float2* py = &lds[(1 * NKZ + j) * LS + ky];
-        float2* pz = &lds[(2 * NKZ + j) * LS + ky];
-        const float2 a = *px, bb = *py, cc = *pz;
-        *px = make_float2(a.x * kxx, a.y * kxx);
-        *py = make_float2(bb.x * kyy + cc.x * kyz, bb.y * kyy + cc.y * kyz);
-        *pz = make_float2(bb.x * kyz + cc.x * kzz, bb.y * kyz + cc.y * kzz);
-    }
-    __syncthreads();
-    fft_lds<float, PY, +1, 3 * NKZ, kNT>(lds, LS, tw, PY);
-    for (int c = 0; c < 3; c++) {
-        float2* dst = B1 + c * cstride + (long)kz0 * Ny;
-        for (int t = threadIdx.x; t < nkz * Ny; t += kNT) {
-            const int j = t / Ny, y = t - j * Ny;
-            dst[t] = lds[(c * NKZ + j) * LS + y];
+        const long b = kz * hy + (fy ? PY - ky : ky);
+        if constexpr (C0 <= 1) {
+            const float kxx = Kc[b];
+            const float2 a = v[0][e];
+            v[0][e] = make_float2(a.x * kxx, a.y * kxx);
+        }
+        if constexpr (C0 != 1) {
+            constexpr int o = C0 == 0 ? 1 : 0;
+            const float kyy = Kc[cs + b], kzz = Kc[2 * cs + b];
+            const float kyz = fy ? -Kc[3 * cs + b] : Kc[3 * cs + b];
+            const float2 bb = v[o][e], cc = v[o + 1][e];
+            v[o][e] = make_float2(bb.x * kyy + cc.x * kyz, bb.y * kyy + cc.y * kyz);
+            v[o + 1][e] = make_float2(bb.x * kyz + cc.x * kzz, bb.y * kyz + cc.y * kzz);
         }
     }
 }
-
-template <int PY, int NKZ>
-__global__ __launch_bounds__(kNT) void k_y2d(float2* __restrict__ B1, const float* __restrict__ Kc,
-                                             const float2* __restrict__ tw, const EngineState* st, int Ny, int Q) {
-    __shared__ float2 lds[3 * NKZ * (PY + 1)];
-    if (st && !st->trial_active) return;
-    y2d_tile<PY, NKZ>(B1, Kc, tw, Ny, Q, blockIdx.x * NKZ, lds);
+// transforms of one loaded column (every lane group of the block calls it: the
+// exchanges synchronise the block)
+template <int PY, int NC = 3, int C0 = 0>
+__device__ __forceinline__ void y2r_column(float2 (*v)[RegFFT<PY>::E], const float* __restrict__ Kc, long kz, bool ok,
+                                           int Q, int g, float2* lds, const float2* tw) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) reg_fft<float, PY, -1>(v[c], g, lds, tw);
+    if (ok) y2r_kernel<PY, C0>(v, Kc, kz, Q, g);
+#pragma unroll
+    for (int c = 0; c < NC; c++) reg_fft<float, PY, +1>(v[c], g, lds, tw);
+}
+// long lines (more than 16 elements per lane): x alone, then the y-z pair,
+// to bound the registers of one column
+template <int PY>
+__device__ __forceinline__ void y2r_column_split(float2* __restrict__ B1, const float* __restrict__ Kc, long kz,
+                                                 bool ok, int Ny, int Q, int g, float2* lds, const float2* tw) {
+    constexpr int E = RegFFT<PY>::E;
+    {
+        float2 v[1][E];
+        y2r_load<PY, 1>(v, B1, kz, ok, Ny, Q, g, 0);
+        y2r_column<PY, 1, 1>(v, Kc, kz, ok, Q, g, lds, tw);
+        if (ok) y2r_store<PY, 1>(v, B1, kz, Ny, Q, g, 0);
+    }
+    {
+        float2 v[2][E];
+        y2r_load<PY, 2>(v, B1, kz, ok, Ny, Q, g, 1);
+        y2r_column<PY, 2, 2>(v, Kc, kz, ok, Q, g, lds, tw);
+        if (ok) y2r_store<PY, 2>(v, B1, kz, Ny, Q, g, 1);
+    }
 }
 
+// single simulation: persistent over kz columns, the next column loaded while
+// the current one is transformed (when its registers fit)
+template <int PY>
+__global__ __launch_bounds__(kNT) void k_y2r(float2* __restrict__ B1, const float* __restrict__ Kc,
+                                             const float2* __restrict__ tw, const EngineState* st, int Ny, int Q,
+                                             int nit) {
+    using F = RegFFT<PY>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    constexpr bool TWL = PY <= 1024;
+    __shared__ float2 lds[LPB * F::LDS];
+    __shared__ float2 twl[TWL ? PY : 1];
+    if (st && !st->trial_active) return;
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const long stride = (long)gridDim.x * LPB;
+    long kz = (long)blockIdx.x * LPB + li;
+    constexpr bool PIPE = E <= 8;
+    float2 nxt[PIPE ? 3 : 1][PIPE ? E : 1];
+    if constexpr (PIPE) y2r_load<PY>(nxt, B1, kz, kz < Q, Ny, Q, g);
+    __syncthreads();  // twiddle table
+    if constexpr (E > 16) {
+        for (int it = 0; it < nit; it++, kz += stride)
+            y2r_column_split<PY>(B1, Kc, kz, kz < Q, Ny, Q, g, lds + li * F::LDS, TWL ? twl : tw);
+    } else {
+        for (int it = 0; it < nit; it++, kz += stride) {
+            float2 v[3][E];
+            if constexpr (PIPE) {
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int e = 0; e < E; e++) v[c][e] = nxt[c][e];
+                if (it + 1 < nit) y2r_load<PY>(nxt, B1, kz + stride, kz + stride < Q, Ny, Q, g);
+            } else {
+                y2r_load<PY>(v, B1, kz, kz < Q, Ny, Q, g);
+            }
+            y2r_column<PY>(v, Kc, kz, kz < Q, Q, g, lds + li * F::LDS, TWL ? twl : tw);
+            if (kz < Q) y2r_store<PY>(v, B1, kz, Ny, Q, g);
+        }
+    }
+}
 
 }  // namespace gk
