@@ -30,7 +30,7 @@ namespace gk {
 
 // one simulation of a batch, as the batched kernels see it (device table)
 struct BSim {
-    int Nx, Ny, Nz, Q, PY;
+    int Nx, Ny, Nz, Q, PY, PX;
     long N;
     float* m;
     float* k[7];
@@ -157,6 +157,17 @@ __global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx,
 #pragma unroll
     for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, twl);
     if (ky >= 0) xmul_store<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
+}
+
+template <int PY, int PX>
+__global__ __launch_bounds__(kYxyThreads) void kb_yxy(const BSim* sims, const int* idx, const int* start, int n,
+                                                       BStage sg, const float2* __restrict__ twY,
+                                                       const float2* __restrict__ twX) {
+    __shared__ float2 lds[yxy_lds<PY, PX>()];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    yxy_plane<PY, PX>(d.B1, d.Kc, twY, twX, d.Nx, d.Ny, d.Q, bb.lb, lds);
 }
 
 template <int PY>
@@ -298,7 +309,7 @@ struct mmBatch {
     int* ddone = nullptr;
     std::vector<void*> owned;  // device allocations of the tables
     // per pass: one LaunchSet per length class
-    std::vector<LaunchSet> zf, yf, xm, yi, y2, zi, fin, all256;
+    std::vector<LaunchSet> zf, yf, xm, yi, y2, yx, zi, fin, all256;
     int useGraph = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -428,6 +439,33 @@ void bl_xmul(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     }
     GK_LAUNCH_CHECK();
 }
+// thin 3-D plane pass; the class key is PY * 64 + PX
+void bl_yxy(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    const int PY = ls.L / 64, PX = ls.L % 64;
+    const float2* twY = twiddle_f32(PY);
+    const float2* twX = twiddle_f32(PX);
+    switch (PY) {
+#define GK_YX(n, px)                                                                                              \
+    case px:                                                                                                      \
+        if constexpr (yxy_fits(px, n))                                                                            \
+            kb_yxy<n, px><<<ls.blocks, kYxyThreads, 0, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg, twY, twX); \
+        else                                                                                                      \
+            abort();                                                                                              \
+        break;
+#define GK_CASE(n)                                                                                                \
+    case n:                                                                                                       \
+        switch (PX) {                                                                                             \
+            GK_YX(n, 4) GK_YX(n, 6) GK_YX(n, 8) GK_YX(n, 12) GK_YX(n, 16)                                         \
+        default: abort();                                                                                         \
+        }                                                                                                         \
+        break;
+        GK_LEN_Y(GK_CASE)
+#undef GK_CASE
+#undef GK_YX
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
 void bl_y2d(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     const float2* tw = twiddle_f32(ls.L);
     switch (ls.L) {
@@ -511,6 +549,7 @@ void enqueue_btrial(mmBatch* b) {
         for (auto& ls : b->xm) bl_xmul(b, ls, sg);
         for (auto& ls : b->yi) bl_ypass(b, ls, sg, false);
         for (auto& ls : b->y2) bl_y2d(b, ls, sg);
+        for (auto& ls : b->yx) bl_yxy(b, ls, sg);
         for (auto& ls : b->zi) bl_zinv_epi(b, ls, sg);
         if (i == 0) {
             kb_ctrl_after_k0<<<n, 64, 0, b->s>>>(b->dsims, t.fsal);
@@ -565,6 +604,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         d.Nz = s->cp.N[2];
         d.Q = s->cp.Q;
         d.PY = s->cp.P[1];
+        d.PX = s->cp.P[0];
         d.N = s->N;
         d.m = s->m;
         for (int i = 0; i < 7; i++) d.k[i] = s->k[i];
@@ -585,10 +625,10 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
     GK_CHECK(hipMemcpy(b->dsims, h.data(), n * sizeof(BSim), hipMemcpyHostToDevice));
     b->ddone = (int*)balloc(b, n * sizeof(int));
     // launch sets per pass and length class
-    std::vector<int> all(n), d3, d2;
+    std::vector<int> all(n), d3, d2, dt;
     for (int j = 0; j < n; j++) {
         all[j] = j;
-        (sims[j]->cp.P[0] > 1 ? d3 : d2).push_back(j);
+        (sims[j]->cp.P[0] == 1 ? d2 : sims[j]->cp.yxy ? dt : d3).push_back(j);
     }
     auto M = [](const mmSim* s) { return s->cp.M; };
     auto PY = [](const mmSim* s) { return s->cp.P[1]; };
@@ -604,6 +644,8 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         return s->cp.Q * div_up(s->cp.P[1], lpr);
     });
     b->y2 = make_sets(b, d2, PY, [](const mmSim* s) { return div_up(s->cp.Q, 256 / lanes_host(s->cp.P[1])); });
+    b->yx = make_sets(b, dt, [](const mmSim* s) { return s->cp.P[1] * 64 + s->cp.P[0]; },
+                      [](const mmSim* s) { return s->cp.Q; });
     b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });
     b->all256 = make_sets(b, all, [](const mmSim*) { return 0; },
                           [](const mmSim* s) { return std::max(1, std::min(1024, div_up(3L * s->N, 256 * 4))); });

@@ -173,12 +173,13 @@ void conv_plan_init(ConvPlan& cp, const int N[3]) {
     cp.M = cp.P[2] / 2;
     cp.Q = cp.M + 1;
     cp.nK = cp.P[0] == 1 ? 4 : 6;
+    cp.yxy = cp.P[0] > 1 && yxy_fits(cp.P[0], cp.P[1]);
     GK_ASSERT(cp.M <= 2048 && cp.P[1] <= 4096 && cp.P[0] <= 1024, "grid too large for this build (padded %d %d %d)",
               cp.P[0], cp.P[1], cp.P[2]);
     const long hx = cp.P[0] / 2 + 1, hy = cp.P[1] / 2 + 1;
     const long kc = cp.P[0] == 1 ? (long)cp.nK * cp.Q * hy : (long)cp.nK * cp.Q * hy * hx;
     const long b1 = 3L * N[0] * cp.Q * N[1];
-    const long b2 = cp.P[0] == 1 ? 0 : 3L * cp.Q * cp.P[1] * N[0];
+    const long b2 = (cp.P[0] == 1 || cp.yxy) ? 0 : 3L * cp.Q * cp.P[1] * N[0];
     GK_CHECK(hipMalloc(&cp.Kc, kc * sizeof(float)));
     GK_CHECK(hipMalloc(&cp.B1, b1 * sizeof(float2)));
     cp.B2 = nullptr;  // layout [3][Q][Nx][Py]
@@ -196,7 +197,7 @@ size_t conv_plan_footprint(const int N[3], int* epi_blocks) {
     const long hx = P[0] / 2 + 1, hy = P[1] / 2 + 1;
     const long kc = P[0] == 1 ? (long)nK * Q * hy : (long)nK * Q * hy * hx;
     const long b1 = 3L * N[0] * Q * N[1];
-    const long b2 = P[0] == 1 ? 0 : 3L * Q * P[1] * N[0];
+    const long b2 = (P[0] == 1 || yxy_fits(P[0], P[1])) ? 0 : 3L * Q * P[1] * N[0];
     if (epi_blocks) *epi_blocks = div_up(N[1], epi_rows_rt(M)) * N[0];
     return kc * sizeof(float) + (b1 + b2) * sizeof(float2);
 }
@@ -535,6 +536,33 @@ static void launch_y2d(const ConvPlan& cp, const EngineState* st, hipStream_t s)
     GK_LAUNCH_CHECK();
 }
 
+static void launch_yxy(const ConvPlan& cp, const EngineState* st, hipStream_t s) {
+    const int PX = cp.P[0], PY = cp.P[1], Nx = cp.N[0], Ny = cp.N[1], Q = cp.Q;
+    const float2* twY = twiddle_f32(PY);
+    const float2* twX = twiddle_f32(PX);
+    switch (PY) {
+#define GK_YX(n, px)                                                                                       \
+    case px:                                                                                               \
+        if constexpr (yxy_fits(px, n))                                                                     \
+            k_yxy<n, px><<<Q, kYxyThreads, 0, s>>>(cp.B1, cp.Kc, twY, twX, st, Nx, Ny, Q);                 \
+        else                                                                                               \
+            abort();                                                                                       \
+        break;
+#define GK_CASE(n)                                                                                         \
+    case n:                                                                                                \
+        switch (PX) {                                                                                      \
+            GK_YX(n, 4) GK_YX(n, 6) GK_YX(n, 8) GK_YX(n, 12) GK_YX(n, 16)                                  \
+        default: abort();                                                                                  \
+        }                                                                                                  \
+        break;
+        GK_LEN_Y(GK_CASE)
+#undef GK_CASE
+#undef GK_YX
+    default: abort();
+    }
+    GK_LAUNCH_CHECK();
+}
+
 void Prof::begin(int cls, hipStream_t s) {
     if (!on) return;
     while ((int)pool.size() < used + 2) {
@@ -571,9 +599,13 @@ void conv_run_pass(const ConvPlan& cp, int cls, const StageIn& in, float* h, con
     const bool d3 = cp.P[0] > 1;
     switch (cls) {
     case 0: launch_zfwd(cp, in, s); break;
-    case 1: GK_ASSERT(d3, "no Y forward pass in 2-D"); launch_y3d(cp, in.st, true, s); break;
-    case 2: if (d3) launch_xmul(cp, in.st, s); else launch_y2d(cp, in.st, s); break;
-    case 3: GK_ASSERT(d3, "no Y inverse pass in 2-D"); launch_y3d(cp, in.st, false, s); break;
+    case 1: GK_ASSERT(d3 && !cp.yxy, "no separate Y forward pass"); launch_y3d(cp, in.st, true, s); break;
+    case 2:
+        if (cp.yxy) launch_yxy(cp, in.st, s);
+        else if (d3) launch_xmul(cp, in.st, s);
+        else launch_y2d(cp, in.st, s);
+        break;
+    case 3: GK_ASSERT(d3 && !cp.yxy, "no separate Y inverse pass"); launch_y3d(cp, in.st, false, s); break;
     case 4: if (epi) launch_zinv_epi(cp, *epi, in.st, s); else launch_zinv(cp, h, in.st, s); break;
     default: GK_ASSERT(false, "pass class %d", cls);
     }
@@ -588,6 +620,10 @@ void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, 
     if (cp.P[0] == 1) {
         p.begin(2, s);
         launch_y2d(cp, in.st, s);
+        p.end(s);
+    } else if (cp.yxy) {
+        p.begin(2, s);
+        launch_yxy(cp, in.st, s);
         p.end(s);
     } else {
         p.begin(1, s);

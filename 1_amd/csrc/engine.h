@@ -102,7 +102,8 @@ struct ConvPlan {
     int nK;     // 6 (3-D) or 4 (2-D)
     float* Kc;  // compressed kernel spectrum (see conv.hip)
     float2* B1; // [3][Nx][Q][Ny]
-    float2* B2; // 3-D: [3][Q][Py][Nx]
+    float2* B2; // 3-D: [3][Q][Py][Nx] (not used by the thin plane pass)
+    int yxy;    // thin 3-D: Y . X K X . Y inverse in one plane pass (no B2)
     size_t bytes;
 };
 

@@ -679,6 +679,127 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
     }
 }
 
+// Thin 3-D grids (padded X of 4 to 16): Y forward, X . K . X and Y inverse of
+// one kz plane in one pass.  The plane of Y spectra [3][Nx][PY] stays in LDS,
+// so per field evaluation the pass reads and writes B1 once, where the three
+// separate passes move B1 twice and the Y-padded B2 four times.  The X
+// transforms of length PX (Nx <= PX/2 nonzero inputs, Nx outputs kept) are
+// direct sums with the table exp(-2 pi i j / PX).
+constexpr int kYxyThreads = 256;
+constexpr int kYxyLdsBytes = 80 * 1024;  // two blocks per CU
+__host__ __device__ constexpr int yxy_lds_elems(int PX, int PY) {
+    return 3 * (PX / 2) * PY + (kYxyThreads / reg_fft_G(PY)) * (PY + PY / 8);
+}
+__host__ __device__ constexpr bool yxy_fits(int PX, int PY) {
+    return PX >= 4 && PX <= 16 && reg_fft_length(PY) && yxy_lds_elems(PX, PY) * 8 <= kYxyLdsBytes;
+}
+template <int PY, int PX> constexpr int yxy_lds() { return yxy_lds_elems(PX, PY); }
+template <int PY, int PX>
+__device__ __forceinline__ void yxy_plane(float2* __restrict__ B1, const float* __restrict__ Kc,
+                                          const float2* twY, const float2* __restrict__ twX, int Nx, int Ny, int Q,
+                                          int kz, float2* lds) {
+    using F = RegFFT<PY>;
+    constexpr int E = F::E, G = F::G, NT = kYxyThreads, LPB = NT / G;
+    constexpr int hx = PX / 2 + 1, hy = PY / 2 + 1;
+    float2* plane = lds;                          // [3 Nx][PY]
+    float2* exch = lds + 3 * (PX / 2) * PY;       // LPB exchange buffers
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const int nl = 3 * Nx, rounds = (nl + LPB - 1) / LPB;
+    float2 wx[PX];
+#pragma unroll
+    for (int j = 0; j < PX; j++) wx[j] = twX[j];
+    // Y forward of the 3 Nx lines (l = c Nx + x) of this kz
+    for (int r = 0; r < rounds; r++) {
+        const int l = r * LPB + li;
+        const bool ok = l < nl;
+        const float2* src = B1 + ((long)(ok ? l : 0) * Q + kz) * Ny;  // B1[c][x][kz][y], (c, x) = line l
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int i = g + G * e;
+            v[e] = (!pad_half<PY>(e) && ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
+        }
+        reg_fft<float, PY, -1>(v, g, exch + li * F::LDS, twY);
+        if (ok)
+#pragma unroll
+            for (int e = 0; e < E; e++) plane[(long)l * PY + g + G * e] = v[e];
+    }
+    __syncthreads();
+    // X . K . X for every ky of the plane
+    for (int ky = threadIdx.x; ky < PY; ky += NT) {
+        GK_FFT_FP
+        float2 a[3][PX];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int kx = 0; kx < PX; kx++) a[c][kx] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int x = 0; x < PX / 2; x++) {
+            if (x < Nx) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float2 u = plane[(long)(c * Nx + x) * PY + ky];
+#pragma unroll
+                    for (int kx = 0; kx < PX; kx++) a[c][kx] = cadd<float>(a[c][kx], cmul<float>(u, wx[(kx * x) % PX]));
+                }
+            }
+        }
+        const bool fy = ky > PY / 2;
+        const int kyp = fy ? PY - ky : ky;
+        const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
+#pragma unroll
+        for (int kx = 0; kx < PX; kx++) {
+            const bool fx = kx > PX / 2;
+            const float* kk = kb + (fx ? PX - kx : kx) * 6;
+            const float kxx = kk[0], kyy = kk[1], kzz = kk[2];
+            const float kyz = fy ? -kk[3] : kk[3], kxz = fx ? -kk[4] : kk[4], kxy = (fx != fy) ? -kk[5] : kk[5];
+            const float2 p = a[0][kx], q = a[1][kx], w = a[2][kx];
+            a[0][kx] = make_float2(p.x * kxx + q.x * kxy + w.x * kxz, p.y * kxx + q.y * kxy + w.y * kxz);
+            a[1][kx] = make_float2(p.x * kxy + q.x * kyy + w.x * kyz, p.y * kxy + q.y * kyy + w.y * kyz);
+            a[2][kx] = make_float2(p.x * kxz + q.x * kyz + w.x * kzz, p.y * kxz + q.y * kyz + w.y * kzz);
+        }
+#pragma unroll
+        for (int x = 0; x < PX / 2; x++) {
+            if (x < Nx) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    float2 t = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int kx = 0; kx < PX; kx++) t = cadd<float>(t, cmul<float>(a[c][kx], cconj<float>(wx[(kx * x) % PX])));
+                    plane[(long)(c * Nx + x) * PY + ky] = t;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // Y inverse, first Ny kept
+    for (int r = 0; r < rounds; r++) {
+        const int l = r * LPB + li;
+        const bool ok = l < nl;
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = ok ? plane[(long)l * PY + g + G * e] : make_float2(0.f, 0.f);
+        reg_fft<float, PY, +1>(v, g, exch + li * F::LDS, twY);
+        if (ok) {
+            float2* dst = B1 + ((long)l * Q + kz) * Ny;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int i = g + G * e;
+                if (!pad_half<PY>(e) && i < Ny) dst[i] = v[e];
+            }
+        }
+    }
+}
+
+template <int PY, int PX>
+__global__ __launch_bounds__(kYxyThreads) void k_yxy(float2* __restrict__ B1, const float* __restrict__ Kc,
+                                                      const float2* __restrict__ twY, const float2* __restrict__ twX,
+                                                      const EngineState* st, int Nx, int Ny, int Q) {
+    __shared__ float2 lds[yxy_lds<PY, PX>()];
+    if (st && !st->trial_active) return;
+    yxy_plane<PY, PX>(B1, Kc, twY, twX, Nx, Ny, Q, blockIdx.x, lds);
+}
+
 // 2-D: Y forward . kernel . Y inverse of one kz column (3 components) in
 // registers, in place in B1[c][0][kz][y].  Lane group li of a block takes
 // column kz; the 4 stored kernel components (xx, yy, zz, yz) for ky <= PY/2
