@@ -163,11 +163,17 @@ template <int PY, int PX>
 __global__ __launch_bounds__(kYxyThreads) void kb_yxy(const BSim* sims, const int* idx, const int* start, int n,
                                                        BStage sg, const float2* __restrict__ twY,
                                                        const float2* __restrict__ twX) {
+    using Y = Yxy<PY, PX>;
     __shared__ float2 lds[yxy_lds<PY, PX>()];
     const BBlock bb = bblock(sims, idx, start, n);
     const BSim& d = *bb.d;
     if (bskip(d, sg)) return;
-    yxy_plane<PY, PX>(d.B1, d.Kc, twY, twX, d.Nx, d.Ny, d.Q, bb.lb, lds);
+    float2 wx[PX];
+#pragma unroll
+    for (int j = 0; j < PX; j++) wx[j] = twX[j];
+    float2 v[Y::RMAX][Y::E];
+    yxy_load<PY, PX>(v, d.B1, d.Nx, d.Ny, d.Q, bb.lb);
+    yxy_plane<PY, PX>(v, d.B1, d.Kc, twY, wx, d.Nx, d.Ny, d.Q, bb.lb, lds);
 }
 
 template <int PY>
@@ -314,6 +320,8 @@ struct mmBatch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long long trials = 0;
+    int* hdone = nullptr;       // pinned host copy of the done flags (mm_batch_enqueue)
+    hipEvent_t ev = nullptr;    // end of the last enqueued work
 };
 
 namespace {
@@ -657,12 +665,16 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         if (cp.P[1] > 1) (void)twiddle_f32(cp.P[1]);
         if (cp.P[0] > 1) (void)twiddle_f32(cp.P[0]);
     }
-    GK_CHECK(hipDeviceSynchronize());
+    // tables above were copied synchronously; other batches' streams keep running
+    GK_CHECK(hipStreamSynchronize(b->s));
     return b;
 }
 
 void mm_batch_free(mmBatch* b) {
     if (!b) return;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    if (b->hdone) (void)hipHostFree(b->hdone);
+    if (b->ev) (void)hipEventDestroy(b->ev);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void* p : b->owned) (void)hipFree(p);
@@ -690,7 +702,7 @@ void mm_batch_start(mmBatch* b, const double* seconds, const long long* steps) {
 }
 
 // enqueue n batched trials and wait; returns the number of simulations not done
-int mm_batch_trials(mmBatch* b, int n) {
+void mm_batch_enqueue(mmBatch* b, int n) {
     if (b->useGraph) {
         if (!b->exec) {
             GK_CHECK(hipStreamBeginCapture(b->s, hipStreamCaptureModeRelaxed));
@@ -706,12 +718,37 @@ int mm_batch_trials(mmBatch* b, int n) {
     const int ns = (int)b->sims.size();
     kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone);
     GK_LAUNCH_CHECK();
-    std::vector<int> done(ns);
-    GK_CHECK(hipMemcpyAsync(done.data(), b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));
-    GK_CHECK(hipStreamSynchronize(b->s));
+    if (!b->hdone) {
+        GK_CHECK(hipHostMalloc(&b->hdone, ns * sizeof(int), hipHostMallocDefault));
+        GK_CHECK(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
+    }
+    GK_CHECK(hipMemcpyAsync(b->hdone, b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));
+    GK_CHECK(hipEventRecord(b->ev, b->s));
+}
+
+int mm_batch_ready(mmBatch* b) {
+    if (!b->ev) return 1;
+    const hipError_t e = hipEventQuery(b->ev);
+    if (e == hipErrorNotReady) return 0;
+    GK_CHECK(e);
+    return 1;
+}
+
+int mm_batch_flags(mmBatch* b, int* done) {
+    GK_ASSERT(b->ev != nullptr, "mm_batch_flags before mm_batch_enqueue");
+    GK_CHECK(hipEventSynchronize(b->ev));
+    const int ns = (int)b->sims.size();
     int active = 0;
-    for (int d : done) active += !d;
+    for (int i = 0; i < ns; i++) {
+        if (done) done[i] = b->hdone[i];
+        active += !b->hdone[i];
+    }
     return active;
+}
+
+int mm_batch_trials(mmBatch* b, int n) {
+    mm_batch_enqueue(b, n);
+    return mm_batch_flags(b, nullptr);
 }
 
 // done flags of every simulation (1 = reached its stop time / step target)

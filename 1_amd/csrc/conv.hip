@@ -129,9 +129,13 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* a, double 
 }
 
 // spectrum -> compressed kernel (and optionally the reference's full layout)
+// Compressed layouts: 2-D [comp][kz][ky]; 3-D [kz][ky][kx][comp] (the 6 values
+// of one frequency together, for the X . K . X pass); thin 3-D plane-pass
+// plans [kz][comp][kx][ky] (consecutive ky together, one thread per ky).
 __global__ void k_extract_kernel(const double2* __restrict__ W, int P0, int P1, int P2, int Q, double norm,
-                                 float* __restrict__ Kc, int comp, int nK, float* __restrict__ full_comp,
-                                 unsigned long long* maxre, unsigned long long* maxim) {
+                                 float* __restrict__ Kc, int comp, int nK, int plane_layout,
+                                 float* __restrict__ full_comp, unsigned long long* maxre,
+                                 unsigned long long* maxim) {
     const long total = (long)P0 * P1 * Q;
     const int hx = P0 / 2 + 1, hy = P1 / 2 + 1;
     double lre = 0, lim = 0;
@@ -146,6 +150,7 @@ __global__ void k_extract_kernel(const double2* __restrict__ W, int P0, int P1, 
         if (full_comp) full_comp[t] = (float)re;
         if (kx < hx && ky < hy) {
             if (P0 == 1) Kc[((long)comp * Q + kz) * hy + ky] = (float)re;
+            else if (plane_layout) Kc[(((long)kz * nK + comp) * hx + kx) * hy + ky] = (float)re;
             else Kc[(((long)kz * hy + ky) * hx + kx) * nK + comp] = (float)re;
         }
     }
@@ -294,7 +299,7 @@ double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool 
     for (int k = 0; k < cp.nK; k++) {
         k_real_to_complex<<<grid1d(NP), kBlock, 0, s>>>(K + k * NP, W, NP);
         fft3_c2c_f64(W, cp.P, -1, s);
-        k_extract_kernel<<<grid1d(fulls), kBlock, 0, s>>>(W, P0, P1, P2, cp.Q, norm, cp.Kc, k, cp.nK,
+        k_extract_kernel<<<grid1d(fulls), kBlock, 0, s>>>(W, P0, P1, P2, cp.Q, norm, cp.Kc, k, cp.nK, cp.yxy,
                                                             full_dev ? full_dev + k * fulls : nullptr, mx, mx + 1);
         GK_LAUNCH_CHECK();
     }

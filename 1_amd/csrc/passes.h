@@ -694,110 +694,165 @@ __host__ __device__ constexpr bool yxy_fits(int PX, int PY) {
     return PX >= 4 && PX <= 16 && reg_fft_length(PY) && yxy_lds_elems(PX, PY) * 8 <= kYxyLdsBytes;
 }
 template <int PY, int PX> constexpr int yxy_lds() { return yxy_lds_elems(PX, PY); }
-template <int PY, int PX>
-__device__ __forceinline__ void yxy_plane(float2* __restrict__ B1, const float* __restrict__ Kc,
-                                          const float2* twY, const float2* __restrict__ twX, int Nx, int Ny, int Q,
-                                          int kz, float2* lds) {
+template <int PY, int PX> struct Yxy {
     using F = RegFFT<PY>;
-    constexpr int E = F::E, G = F::G, NT = kYxyThreads, LPB = NT / G;
-    constexpr int hx = PX / 2 + 1, hy = PY / 2 + 1;
-    float2* plane = lds;                          // [3 Nx][PY]
-    float2* exch = lds + 3 * (PX / 2) * PY;       // LPB exchange buffers
-    const int li = threadIdx.x / G, g = threadIdx.x % G;
-    const int nl = 3 * Nx, rounds = (nl + LPB - 1) / LPB;
-    float2 wx[PX];
+    static constexpr int E = F::E, G = F::G, LPB = kYxyThreads / G;
+    static constexpr int RMAX = (3 * (PX / 2) + LPB - 1) / LPB;  // rounds of lines per lane group, at most
+};
+
+// the 3 Nx lines (l = c Nx + x) of plane kz into the lane group's registers
+template <int PY, int PX>
+__device__ __forceinline__ void yxy_load(float2 (*v)[RegFFT<PY>::E], const float2* __restrict__ B1, int Nx, int Ny,
+                                         int Q, int kz) {
+    using Y = Yxy<PY, PX>;
+    const int li = threadIdx.x / Y::G, g = threadIdx.x % Y::G;
 #pragma unroll
-    for (int j = 0; j < PX; j++) wx[j] = twX[j];
-    // Y forward of the 3 Nx lines (l = c Nx + x) of this kz
-    for (int r = 0; r < rounds; r++) {
-        const int l = r * LPB + li;
-        const bool ok = l < nl;
+    for (int r = 0; r < Y::RMAX; r++) {
+        const int l = r * Y::LPB + li;
+        const bool ok = l < 3 * Nx;
         const float2* src = B1 + ((long)(ok ? l : 0) * Q + kz) * Ny;  // B1[c][x][kz][y], (c, x) = line l
-        float2 v[E];
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int i = g + G * e;
-            v[e] = (!pad_half<PY>(e) && ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
+        for (int e = 0; e < Y::E; e++) {
+            const int i = g + Y::G * e;
+            v[r][e] = (!pad_half<PY>(e) && ok && i < Ny) ? src[i] : make_float2(0.f, 0.f);
         }
-        reg_fft<float, PY, -1>(v, g, exch + li * F::LDS, twY);
-        if (ok)
+    }
+}
+
+// the plane pass on loaded lines (called by every thread of the block)
+template <int PY, int PX>
+__device__ __forceinline__ void yxy_plane(float2 (*v)[RegFFT<PY>::E], float2* __restrict__ B1,
+                                          const float* __restrict__ Kc, const float2* twY, const float2* wx, int Nx,
+                                          int Ny, int Q, int kz, float2* lds) {
+    using Y = Yxy<PY, PX>;
+    using F = RegFFT<PY>;
+    constexpr int E = Y::E, G = Y::G, NT = kYxyThreads, LPB = Y::LPB, RMAX = Y::RMAX;
+    constexpr int hx = PX / 2 + 1, hy = PY / 2 + 1;
+    float2* plane = lds;                     // [3 Nx][PY]
+    float2* exch = lds + 3 * (PX / 2) * PY;  // LPB exchange buffers
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const int nl = 3 * Nx;
+    // Y forward
 #pragma unroll
-            for (int e = 0; e < E; e++) plane[(long)l * PY + g + G * e] = v[e];
+    for (int r = 0; r < RMAX; r++) {
+        const int l = r * LPB + li;
+        if (r * LPB < nl) {  // uniform: some lane group has a line in this round
+            reg_fft<float, PY, -1>(v[r], g, exch + li * F::LDS, twY);
+            if (l < nl)
+#pragma unroll
+                for (int e = 0; e < E; e++) plane[(long)l * PY + g + G * e] = v[r][e];
+        }
     }
     __syncthreads();
     // X . K . X for every ky of the plane
     for (int ky = threadIdx.x; ky < PY; ky += NT) {
         GK_FFT_FP
+        constexpr bool BF = PX == 4 || PX == 8;  // radix-4 / radix-8 butterflies, else direct sums
         float2 a[3][PX];
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int kx = 0; kx < PX; kx++) a[c][kx] = make_float2(0.f, 0.f);
+        if constexpr (BF) {
 #pragma unroll
-        for (int x = 0; x < PX / 2; x++) {
-            if (x < Nx) {
+            for (int x = 0; x < PX / 2; x++)
+                if (x < Nx)
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const float2 u = plane[(long)(c * Nx + x) * PY + ky];
+                    for (int c = 0; c < 3; c++) a[c][x] = plane[(long)(c * Nx + x) * PY + ky];
 #pragma unroll
-                    for (int kx = 0; kx < PX; kx++) a[c][kx] = cadd<float>(a[c][kx], cmul<float>(u, wx[(kx * x) % PX]));
+            for (int c = 0; c < 3; c++) dft<float, PX, -1>(a[c]);
+        } else {
+#pragma unroll
+            for (int x = 0; x < PX / 2; x++) {
+                if (x < Nx) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const float2 u = plane[(long)(c * Nx + x) * PY + ky];
+#pragma unroll
+                        for (int kx = 0; kx < PX; kx++)
+                            a[c][kx] = cadd<float>(a[c][kx], cmul<float>(u, wx[(kx * x) % PX]));
+                    }
                 }
             }
         }
+        // kernel layout of plane-pass plans: [kz][component][kx <= PX/2][ky <= PY/2], so that
+        // consecutive ky (threads) read consecutive floats
         const bool fy = ky > PY / 2;
         const int kyp = fy ? PY - ky : ky;
-        const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
+        const float* kb = Kc + (long)kz * 6 * hx * hy + kyp;
 #pragma unroll
         for (int kx = 0; kx < PX; kx++) {
             const bool fx = kx > PX / 2;
-            const float* kk = kb + (fx ? PX - kx : kx) * 6;
-            const float kxx = kk[0], kyy = kk[1], kzz = kk[2];
-            const float kyz = fy ? -kk[3] : kk[3], kxz = fx ? -kk[4] : kk[4], kxy = (fx != fy) ? -kk[5] : kk[5];
+            const float* kk = kb + (fx ? PX - kx : kx) * hy;
+            constexpr int cs = hx * hy;
+            const float kxx = kk[0], kyy = kk[cs], kzz = kk[2 * cs];
+            const float kyz = fy ? -kk[3 * cs] : kk[3 * cs], kxz = fx ? -kk[4 * cs] : kk[4 * cs];
+            const float kxy = (fx != fy) ? -kk[5 * cs] : kk[5 * cs];
             const float2 p = a[0][kx], q = a[1][kx], w = a[2][kx];
             a[0][kx] = make_float2(p.x * kxx + q.x * kxy + w.x * kxz, p.y * kxx + q.y * kxy + w.y * kxz);
             a[1][kx] = make_float2(p.x * kxy + q.x * kyy + w.x * kyz, p.y * kxy + q.y * kyy + w.y * kyz);
             a[2][kx] = make_float2(p.x * kxz + q.x * kyz + w.x * kzz, p.y * kxz + q.y * kyz + w.y * kzz);
         }
+        if constexpr (BF) {
 #pragma unroll
-        for (int x = 0; x < PX / 2; x++) {
-            if (x < Nx) {
+            for (int c = 0; c < 3; c++) dft<float, PX, +1>(a[c]);
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    float2 t = make_float2(0.f, 0.f);
+            for (int x = 0; x < PX / 2; x++)
+                if (x < Nx)
 #pragma unroll
-                    for (int kx = 0; kx < PX; kx++) t = cadd<float>(t, cmul<float>(a[c][kx], cconj<float>(wx[(kx * x) % PX])));
-                    plane[(long)(c * Nx + x) * PY + ky] = t;
+                    for (int c = 0; c < 3; c++) plane[(long)(c * Nx + x) * PY + ky] = a[c][x];
+        } else {
+#pragma unroll
+            for (int x = 0; x < PX / 2; x++) {
+                if (x < Nx) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        float2 t = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int kx = 0; kx < PX; kx++)
+                            t = cadd<float>(t, cmul<float>(a[c][kx], cconj<float>(wx[(kx * x) % PX])));
+                        plane[(long)(c * Nx + x) * PY + ky] = t;
+                    }
                 }
             }
         }
     }
     __syncthreads();
     // Y inverse, first Ny kept
-    for (int r = 0; r < rounds; r++) {
+#pragma unroll
+    for (int r = 0; r < RMAX; r++) {
+        if (r * LPB >= nl) break;  // uniform
         const int l = r * LPB + li;
         const bool ok = l < nl;
-        float2 v[E];
+        float2 w[E];
 #pragma unroll
-        for (int e = 0; e < E; e++) v[e] = ok ? plane[(long)l * PY + g + G * e] : make_float2(0.f, 0.f);
-        reg_fft<float, PY, +1>(v, g, exch + li * F::LDS, twY);
+        for (int e = 0; e < E; e++) w[e] = ok ? plane[(long)l * PY + g + G * e] : make_float2(0.f, 0.f);
+        reg_fft<float, PY, +1>(w, g, exch + li * F::LDS, twY);
         if (ok) {
             float2* dst = B1 + ((long)l * Q + kz) * Ny;
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int i = g + G * e;
-                if (!pad_half<PY>(e) && i < Ny) dst[i] = v[e];
+                if (!pad_half<PY>(e) && i < Ny) dst[i] = w[e];
             }
         }
     }
 }
 
+// single simulation: one block per kz plane
 template <int PY, int PX>
 __global__ __launch_bounds__(kYxyThreads) void k_yxy(float2* __restrict__ B1, const float* __restrict__ Kc,
                                                       const float2* __restrict__ twY, const float2* __restrict__ twX,
                                                       const EngineState* st, int Nx, int Ny, int Q) {
+    using Y = Yxy<PY, PX>;
     __shared__ float2 lds[yxy_lds<PY, PX>()];
     if (st && !st->trial_active) return;
-    yxy_plane<PY, PX>(B1, Kc, twY, twX, Nx, Ny, Q, blockIdx.x, lds);
+    float2 wx[PX];
+#pragma unroll
+    for (int j = 0; j < PX; j++) wx[j] = twX[j];
+    float2 v[Y::RMAX][Y::E];
+    yxy_load<PY, PX>(v, B1, Nx, Ny, Q, blockIdx.x);
+    yxy_plane<PY, PX>(v, B1, Kc, twY, wx, Nx, Ny, Q, blockIdx.x, lds);
 }
 
 // 2-D: Y forward . kernel . Y inverse of one kz column (3 components) in

@@ -600,7 +600,10 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     GK_CHECK(hipMemset(s->epart, 0, (size_t)s->npart * sizeof(double)));
     s->wantEnergy = p->wantEnergy != 0;
     if (mode == MM_MODE_ABI) abi_init(s, s->cell);
-    GK_CHECK(hipDeviceSynchronize());
+    // set-up work ran on the library stream and this simulation's stream (the
+    // others, e.g. running batches, are not waited for)
+    GK_CHECK(hipStreamSynchronize(stream()));
+    GK_CHECK(hipStreamSynchronize(s->s));
     return s;
 }
 

@@ -94,6 +94,9 @@ ENGINE = {
     "mm_batch_size": (I, [P]),
     "mm_batch_start": (None, [P, P, P]),
     "mm_batch_trials": (I, [P, I]),
+    "mm_batch_enqueue": (None, [P, I]),
+    "mm_batch_ready": (I, [P]),
+    "mm_batch_flags": (I, [P, P]),
     "mm_batch_done": (None, [P, P]),
     "mm_batch_run": (C.c_longlong, [P, P, I]),
 }

@@ -1,12 +1,13 @@
 """Continuous-batching request loop for LLG simulations on one GPU.
 
 Requests (a simulation set-up plus a simulated duration) arrive over time.
-The server keeps the running simulations in one batch (``Batch``, i.e.
+The server keeps the running simulations in batches (``Batch``, i.e.
 ``mm_batch_*``: one launch per pass and padded length per RK stage, captured
-into a HIP graph) and advances it `poll` batched trials at a time.  Between
+into a HIP graph), one per size group on its own stream, and advances each
+`poll` batched trials per enqueue without waiting for the others.  Between
 chunks it retires the simulations that reached their stop time, admits the
 requests that have arrived, in arrival order, while their device footprint
-(``mm_sim_footprint``, exact) fits the HBM budget, and rebuilds the batch when
+(``mm_sim_footprint``, exact) fits the HBM budget, and rebuilds a batch when
 its membership changed.  A new member starts with ``mm_sim_begin_run`` (Run()'s
 set-up) and is then stepped only by the batch, so requests that arrive while
 others are on the GPU overlap with them.
@@ -84,16 +85,38 @@ def footprint(cfg):
     return int(s._lib.mm_sim_footprint(C.byref(p)))
 
 
+class _Group:
+    """Simulations advanced by one batch on its own stream."""
+
+    def __init__(self):
+        self.members = []   # requests whose simulations the batch references
+        self.batch = None
+        self.dirty = False  # rebuild before the next enqueue
+        self.busy = False   # an enqueue is in flight
+
+
 class Server:
     """Continuous batching over a request list whose arrivals are in seconds
-    from the start of `serve`."""
+    from the start of `serve`.
 
-    def __init__(self, hbm_bytes=HBM_BYTES, poll=16, graph=True, keep_results=False):
+    Requests are split by cell count into groups (`split_cells` boundaries),
+    each advanced by its own batch on its own stream, `poll` trials per
+    enqueue, without waiting for the others: a batched trial costs about the
+    work of its largest members, so light simulations that need many short
+    steps are not held to the cadence of heavy ones, and their launches
+    overlap the heavy batch on the device."""
+
+    def __init__(self, hbm_bytes=HBM_BYTES, poll=16, graph=True, keep_results=False, split_cells=(16384, 131072)):
         self.hbm_bytes = hbm_bytes
         self.poll = poll
         self.graph = graph
         self.keep_results = keep_results
+        self.split_cells = tuple(split_cells)
         self.results = {}
+
+    def _group_of(self, r):
+        n = int(np.prod(r.cfg["size"]))
+        return sum(n > b for b in self.split_cells)
 
     def serve(self, requests, clock=time.perf_counter, max_seconds=None):
         """Run every request to completion; returns per-request (arrival, start,
@@ -103,19 +126,17 @@ class Server:
             r.footprint = footprint(r.cfg)
             if r.footprint > self.hbm_bytes:
                 raise InputErr("request %d needs %d bytes, more than the budget" % (r.rid, r.footprint))
+        groups = [_Group() for _ in range(len(self.split_cells) + 1)]
         t0 = clock()
-        active, batch = [], None
-        dirty = False
         used = 0
         rebuilds = trials = 0
         max_active = max_used = 0
         qi = 0
-        while qi < len(pending) or active:
+        while qi < len(pending) or any(g.members for g in groups):
             now = clock() - t0
             if max_seconds is not None and now > max_seconds:
                 raise RuntimeError("serving did not finish within %g s" % max_seconds)
             # admit arrived requests in order while they fit
-            joined = []
             while qi < len(pending) and pending[qi].arrival <= now and used + pending[qi].footprint <= self.hbm_bytes:
                 r = pending[qi]
                 qi += 1
@@ -124,44 +145,63 @@ class Server:
                 r.sim._lib.mm_sim_begin_run(r.sim._ptr, r.duration)
                 r.start = clock() - t0
                 used += r.footprint
-                joined.append(r)
-            if joined:
-                active.extend(joined)
-                dirty = True
-                max_active = max(max_active, len(active))
-                max_used = max(max_used, used)
-            if dirty:
-                if batch is not None:
-                    batch.free()
-                batch = Batch([r.sim for r in active], graph=self.graph) if active else None
-                rebuilds += batch is not None
-                dirty = False
-            if not active:
-                # idle until the next arrival
-                wait = pending[qi].arrival - (clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-                continue
-            batch.trials(self.poll)
-            trials += self.poll
-            done = batch.done()
-            if done.any():
-                t = clock() - t0
-                keep = []
-                for r, d in zip(active, done):
-                    if d:
-                        r.finish = t
-                        if self.keep_results:
-                            self.results[r.rid] = r.sim.m
-                        r.sim.free()
-                        r.sim = None
-                        used -= r.footprint
-                    else:
-                        keep.append(r)
-                active = keep
-                dirty = True
-        if batch is not None:
-            batch.free()
+                g = groups[self._group_of(r)]
+                g.members.append(r)
+                g.dirty = True
+            max_active = max(max_active, sum(len(g.members) for g in groups))
+            max_used = max(max_used, used)
+            if qi < len(pending) and pending[qi].arrival <= now:
+                # an arrived request waits for memory that finished members still hold
+                for g in groups:
+                    if any(r.finish is not None for r in g.members):
+                        g.dirty = True
+            launched = False
+            for g in groups:
+                if g.busy:
+                    if not g.batch.ready():
+                        continue
+                    done = g.batch.flags()
+                    g.busy = False
+                    t = clock() - t0
+                    for r, d in zip(g.members, done):
+                        if d and r.finish is None:
+                            r.finish = t
+                            if self.keep_results:
+                                self.results[r.rid] = r.sim.m
+                    # finished members are skipped by the batched kernels; the batch is
+                    # rebuilt without them once they are half of it (or at an admission)
+                    nfin = sum(r.finish is not None for r in g.members)
+                    if 2 * nfin >= len(g.members):
+                        g.dirty = True
+                if g.dirty:
+                    if g.batch is not None:
+                        g.batch.free()
+                        g.batch = None
+                    keep = []
+                    for r in g.members:
+                        if r.finish is not None:
+                            r.sim.free()
+                            r.sim = None
+                            used -= r.footprint
+                        else:
+                            keep.append(r)
+                    g.members = keep
+                    if keep:
+                        g.batch = Batch([r.sim for r in keep], graph=self.graph)
+                        rebuilds += 1
+                    g.dirty = False
+                if g.batch is not None:
+                    g.batch.enqueue(self.poll)
+                    g.busy = True
+                    trials += self.poll
+                    launched = True
+            if not launched:
+                if any(g.busy for g in groups):
+                    time.sleep(0.0002)
+                elif qi < len(pending):
+                    wait = pending[qi].arrival - (clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
         t_end = clock() - t0
         return dict(elapsed=t_end, rebuilds=rebuilds, batched_trials=trials, max_active=max_active,
                     max_bytes=max_used)

@@ -469,6 +469,20 @@ class Batch:
         self._lib.mm_batch_done(self._ptr, out.ctypes.data)
         return out.astype(bool)
 
+    def enqueue(self, n):
+        """Enqueue n batched trials (and the done flags' copy) without waiting."""
+        self._lib.mm_batch_enqueue(self._ptr, int(n))
+
+    def ready(self):
+        """Whether the work of the last enqueue has finished."""
+        return bool(self._lib.mm_batch_ready(self._ptr))
+
+    def flags(self):
+        """Done flags after the last enqueue (waits for it if needed)."""
+        out = np.zeros(len(self.sims), np.int32)
+        self._lib.mm_batch_flags(self._ptr, out.ctypes.data)
+        return out.astype(bool)
+
     def run(self, seconds, poll=16):
         sec = self._seconds(seconds)
         return int(self._lib.mm_batch_run(self._ptr, sec.ctypes.data, int(poll)))

@@ -154,6 +154,14 @@ int mm_batch_size(mmBatch* b);
 void mm_batch_start(mmBatch* b, const double* seconds, const long long* steps);
 /* enqueue n batched trials and wait; returns the simulations not yet done */
 int mm_batch_trials(mmBatch* b, int n);
+/* The same without waiting: enqueue n batched trials and the copy of the done
+ * flags on the batch's own stream.  mm_batch_ready is 1 once that work has
+ * finished; mm_batch_flags then fills done[n_sims] (may be null) and returns
+ * the simulations not yet done (it waits if the work is still running).
+ * Several batches on their own streams can run concurrently this way. */
+void mm_batch_enqueue(mmBatch* b, int n);
+int mm_batch_ready(mmBatch* b);
+int mm_batch_flags(mmBatch* b, int* done);
 /* done[j] = 1 once simulation j reached its stop time / step target */
 void mm_batch_done(mmBatch* b, int* done);
 /* mm_batch_start(seconds) then trials until all are done (polling every
