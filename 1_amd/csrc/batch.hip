@@ -33,6 +33,7 @@ struct BSim {
     int Nx, Ny, Nz, Q, PY, PX;
     long N;
     float* m;
+    float* mi;  // stage input stored by the Z forward pass (BStage::use_mi)
     float* k[7];
     float* mtmp;
     float2* B1;
@@ -58,6 +59,7 @@ struct BStage {
     float ua[6], ub[6];
     float bs;
     int fin;             // epilogue mode (zinv_epi_tile's FIN)
+    int use_mi;          // the Z forward pass stores m_i, the epilogue reads it (nu = 0)
     double ce[6], ce_self;  // fin == 2: b2_j - b_j of the slots and of this stage
     float c, cmax;
     int partslot;  // -1: none, 0: |k0|, 1: |k_last|
@@ -102,6 +104,7 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, c
     si.m = d.m;
     si.st = d.st;
     si.nk = sg.nk;
+    si.mout = sg.use_mi ? d.mi : nullptr;
     for (int j = 0; j < 6; j++) {
         si.k[j] = j < sg.nk ? d.k[sg.kj[j]] : d.m;
         si.a[j] = sg.a[j];
@@ -216,7 +219,7 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim*
     const int nyb = (d.Ny + TR - 1) / TR;
     const int yb = bb.lb % nyb, x = bb.lb / nyb;
     EpiArgs ea{};
-    ea.m = d.m;
+    ea.m = sg.use_mi ? d.mi : d.m;
     ea.out = FIN == 1 ? d.m : d.k[sg.i];
     for (int j = 0; j < 6; j++) {
         const bool u = j < sg.nu;
@@ -548,6 +551,8 @@ void enqueue_btrial(mmBatch* b) {
             }
         sg.fin = fin ? 1 : efin ? 2 : 0;
         sg.ce_self = efin ? (double)t.b2[i] - (double)t.b[i] : 0.0;
+        sg.use_mi = (!fin && !efin && sg.nu >= 2) ? 1 : 0;
+        if (sg.use_mi) sg.nu = 0;
         sg.bs = fin ? t.b[i] : 0.f;
         sg.c = t.c[i];
         sg.cmax = t.cmax(i);
@@ -615,6 +620,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         d.PX = s->cp.P[0];
         d.N = s->N;
         d.m = s->m;
+        d.mi = s->mi;
         for (int i = 0; i < 7; i++) d.k[i] = s->k[i];
         d.mtmp = s->mtmp;
         d.B1 = s->cp.B1;

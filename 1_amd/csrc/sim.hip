@@ -310,6 +310,13 @@ static void enqueue_trial(mmSim* s) {
                 ea.nu++;
             }
         ea.fin = fin ? 1 : efin ? 2 : 0;
+        if (!fin && !efin && ea.nu >= 2) {
+            // the Z forward pass stores m_i, which the epilogue reads instead of
+            // rebuilding it from m and nu earlier stages (same fmaf chain, same bits)
+            si.mout = s->mi;
+            ea.m = s->mi;
+            ea.nu = 0;
+        }
         if (efin) {
             ea.ce_self = (double)t.b2[i] - (double)t.b[i];
             ea.mhigh = s->mtmp;

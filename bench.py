@@ -143,13 +143,61 @@ def dp_trial_bytes(size):
     return 6 * b_conv + 12 * N * 58
 
 
+def run_groups(mm, sims, groups, poll, graph, clock, t1, done_at):
+    """Advance simulations (already started) in size groups, each in its own
+    batch on its own stream, `poll` trials per asynchronous enqueue, until all
+    are done; a group's batch is rebuilt from its running members when half
+    have finished.  Fills done_at (seconds since t1); returns (batched trials,
+    rebuilds)."""
+    state = []
+    for members in groups:
+        if len(members):
+            state.append({"members": np.asarray(members), "batch": None, "busy": False, "dirty": True,
+                          "alive": True})
+    trials = rebuilds = 0
+    while state:
+        launched = False
+        for g in list(state):
+            if g["busy"]:
+                if not g["batch"].ready():
+                    continue
+                d = g["batch"].flags()
+                g["busy"] = False
+                fin = g["members"][d]
+                done_at[fin[np.isnan(done_at[fin])]] = clock() - t1
+                if 2 * int(d.sum()) >= len(d):
+                    g["dirty"] = True
+            if g["dirty"]:
+                if g["batch"] is not None:
+                    g["batch"].free()
+                    g["batch"] = None
+                run = g["members"][np.isnan(done_at[g["members"]])]
+                g["dirty"] = False
+                if not len(run):
+                    g["alive"] = False
+                    continue
+                g["members"] = run
+                g["batch"] = mm.Batch([sims[j] for j in run], graph=graph)
+                rebuilds += 1
+            g["batch"].enqueue(poll)
+            g["busy"] = True
+            trials += poll
+            launched = True
+        state = [g for g in state if g["alive"]]
+        if not launched and state:
+            time.sleep(0.0002)
+    return trials, rebuilds
+
+
 def bench_sweep(args, mm):
-    """Batched sweep: all members in one batch (mm_batch_*) against members run
-    one at a time on the same GPU.  Setup (kernel spectra) is excluded from
-    both; latency = time from the start of the batch to each member's done flag
-    (members all submitted at t = 0).  When at most half of a batch is still
-    running it is rebuilt from the running members (the finished ones would
-    otherwise keep their length classes' launches alive)."""
+    """Batched sweep against members run one at a time on the same GPU.  All
+    members start together (t = 0); after a first stretch of trials with every
+    member in one batch (which gives the full-batch trial time), they are
+    advanced in size groups (cells <= 20480, <= 81920, above), each group a
+    batch on its own stream enqueued asynchronously, so that the many short
+    trials of small members are not held to the cadence of large ones.  Setup
+    (kernel spectra) is excluded from both sides; latency = time from the start
+    to each member's done flag."""
     cfgs = mm.workloads.sweep(args.sweep_sims, seed=4, duration=args.sweep_ns * 1e-9)
     n = len(cfgs)
     t0 = time.perf_counter()
@@ -166,24 +214,18 @@ def bench_sweep(args, mm):
     b.trials(1)  # captures the batched trial
     # full-batch trial time: the next `poll` trials, every member running
     ta = time.perf_counter()
-    active = b.trials(args.poll)
+    b.trials(args.poll)
     t_full = (time.perf_counter() - ta) / args.poll
     trials += 1 + args.poll
-    while True:
-        now = time.perf_counter() - t1
-        d = b.done()
-        fin = members[d]
-        done_at[fin[np.isnan(done_at[fin])]] = now
-        if active == 0:
-            break
-        run = members[~d]
-        if len(run) <= len(members) // 2 and len(run) < len(members):
-            b.free()
-            members = run
-            b = mm.Batch([sims[j] for j in members], graph=not args.no_graph)
-            rebuilds += 1
-        active = b.trials(args.poll)
-        trials += args.poll
+    d = b.done()
+    done_at[d] = time.perf_counter() - t1
+    b.free()
+    cells = np.array([np.prod(c["size"]) for c in cfgs])
+    bounds = (20480, 81920)
+    grp = np.searchsorted(np.asarray(bounds), cells, side="left")
+    groups = [np.nonzero((grp == k) & ~d)[0] for k in range(len(bounds) + 1)]
+    tr, rebuilds = run_groups(mm, sims, groups, args.poll, not args.no_graph, time.perf_counter, t1, done_at)
+    trials += tr
     t_batch = time.perf_counter() - t1
     b.free()
     full_bytes = sum(dp_trial_bytes(c["size"]) for c in cfgs)
@@ -220,6 +262,7 @@ def bench_sweep(args, mm):
         "batch_seconds": round(t_batch, 4),
         "batched_trials": trials,
         "batch_rebuilds": rebuilds,
+        "groups_cells_upto": [20480, 81920, None],
         "full_batch_trial": {"ms": round(t_full * 1e3, 4), "model_bytes": int(full_bytes),
                              "hbm_frac": round(full_bytes / t_full / (HBM_PEAK_GBS * 1e9), 4)},
         "accepted_steps_total": int(steps.sum()),
