@@ -359,6 +359,12 @@ extern "C" {
 
 mmOdt* mm_odt_open(const char* path, const char* title, int ncols, const char* const* names,
                    const char* const* units) {
+    return mm_odt_open_desc(path, title, ncols, names, units, 0, nullptr, nullptr);
+}
+
+mmOdt* mm_odt_open_desc(const char* path, const char* title, int ncols, const char* const* names,
+                        const char* const* units, int ndesc, const char* const* desc_keys,
+                        const char* const* desc_values) {
     FILE* f = std::fopen(path, "w");
     if (!f) {
         fail(std::string("cannot create ") + path);
@@ -367,7 +373,13 @@ mmOdt* mm_odt_open(const char* path, const char* title, int ncols, const char* c
     mmOdt* t = new mmOdt;
     t->f = f;
     t->ncols = ncols;
-    std::fprintf(f, "# ODT 1.0\n# Table Start\n# Title:  %s\n# Units:", title ? title : "");
+    std::fprintf(f, "# ODT 1.0\n");
+    if (ndesc > 0) {
+        std::fprintf(f, "# Begin: Header\n");
+        for (int i = 0; i < ndesc; i++) std::fprintf(f, "# Desc: %s :  %s\n", desc_keys[i], desc_values[i]);
+        std::fprintf(f, "# End: Header\n");
+    }
+    std::fprintf(f, "# Table Start\n# Title:  %s\n# Units:", title ? title : "");
     for (int i = 0; i < ncols; i++) std::fprintf(f, "{%s} \t", units[i]);
     std::fprintf(f, "\n# Columns:");
     for (int i = 0; i < ncols; i++) std::fprintf(f, "%s \t", names[i]);
@@ -378,6 +390,7 @@ mmOdt* mm_odt_open(const char* path, const char* title, int ncols, const char* c
 int mm_odt_row(mmOdt* t, const double* values) {
     for (int i = 0; i < t->ncols; i++) std::fprintf(t->f, "%-20.10g \t", values[i]);
     std::fprintf(t->f, "\n");
+    std::fflush(t->f);
     return std::ferror(t->f) ? fail("write error") : 0;
 }
 

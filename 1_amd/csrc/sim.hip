@@ -653,6 +653,18 @@ long mm_sim_run(mmSim* s, double seconds) {
     return fused_run(s, -1, stop, true);
 }
 
+long mm_sim_run_until(mmSim* s, double stop) {
+    if (s->mode == MM_MODE_ABI) {
+        long n = 0;
+        while (s->htime < stop) {
+            abi_step(s);
+            n++;
+        }
+        return n;
+    }
+    return fused_run(s, -1, stop, true);
+}
+
 void mm_sim_steps(mmSim* s, int n) {
     if (s->mode == MM_MODE_ABI) {
         for (int i = 0; i < n; i++) abi_step(s);

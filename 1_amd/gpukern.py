@@ -87,6 +87,7 @@ ENGINE = {
     "mm_sim_footprint": (C.c_ulonglong, [P]),
     "mm_sim_begin_run": (None, [P, C.c_double]),
     "mm_sim_relax": (C.c_long, [P, C.c_long]),
+    "mm_sim_run_until": (C.c_long, [P, C.c_double]),
     "mm_sim_set_field": (None, [P, I, P, P]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
@@ -148,6 +149,7 @@ IO = {
     "mm_tensor_read": (I, [CP, P, P, P]),
     "mm_tensor_write": (I, [CP, I, P, P]),
     "mm_odt_open": (P, [CP, CP, I, PP, PP]),
+    "mm_odt_open_desc": (P, [CP, CP, I, PP, PP, I, PP, PP]),
     "mm_odt_row": (I, [P, P]),
     "mm_odt_close": (I, [P]),
 }

@@ -84,13 +84,16 @@ def write_tensor(path, data):
 
 
 class OdtTable:
-    """ODT 1.0 table: OdtTable(path, title, [(name, unit), ...]); row(values); close()."""
+    """ODT 1.0 table: OdtTable(path, title, [(name, unit), ...], desc={key: value});
+    row(values); close()."""
 
-    def __init__(self, path, title, columns):
+    def __init__(self, path, title, columns, desc=None):
         self._lib = gpukern.load()
         self.ncols = len(columns)
-        self._t = self._lib.mm_odt_open(str(path).encode(), title.encode(), self.ncols,
-                                        _strs([c[0] for c in columns]), _strs([c[1] for c in columns]))
+        desc = desc or {}
+        self._t = self._lib.mm_odt_open_desc(str(path).encode(), title.encode(), self.ncols,
+                                             _strs([c[0] for c in columns]), _strs([c[1] for c in columns]),
+                                             len(desc), _strs(list(desc)), _strs(list(desc.values())))
         if not self._t:
             raise IOErr(self._lib.mm_io_error().decode())
 

@@ -24,6 +24,82 @@ class InputErr(ValueError):
     """Invalid command argument (the reference panics with InputErr)."""
 
 
+# columns of the data table, simoutput.go:133-147
+TABULATE = ("time", "m", "b", "j", "id", "e", "maxdm/dt", "minmaxmz", "corepos")
+
+
+def _component(c):
+    c = int(c)
+    if c not in (0, 1, 2):
+        raise InputErr("component must be 0 (x), 1 (y) or 2 (z)")
+    return c
+
+
+def _parabola_peak(f0, f1, f2):
+    # interpolate_maxpos (corepos.go:36-40) with the neighbours at -1 and +1, in float32
+    f0, f1, f2 = np.float32(f0), np.float32(f1), np.float32(f2)
+    b = (f2 - f1) / np.float32(2)
+    a = ((f2 - f0) - (f0 - f1)) / np.float32(2)
+    return -b / (np.float32(2) * a)
+
+
+class _Output:
+    """A saved quantity with its period (Periodic, simoutput.go:64-87)."""
+
+    def __init__(self, kind, version=1, binary=True):
+        self.kind, self.version, self.binary = kind, version, binary
+        self.period = np.float32(0)
+        self.since = np.float32(0)
+
+    def need_save(self, t32):
+        return t32 == 0 or np.float32(t32 - self.since) >= self.period
+
+    def due_internal(self, unit32):
+        return float(np.float32(self.since + self.period)) / unit32
+
+    def save(self, sim):
+        import os
+
+        from . import ovf
+
+        t32 = sim._time_f32()
+        if self.kind == "table":
+            path = sim._table_row()
+        else:
+            st = sim.state()
+            path = os.path.join(sim._outdir, "%s%08d.omf" % (self.kind, sim._autosave_idx))
+            if self.kind == "m":
+                data, mult, unit = sim.m, sim._msat, "A/m"
+            else:   # torque: dm/dt of the current m in units gamma*Msat
+                data, mult, unit = sim.torque(sim.m), 1.0, ""
+            ovf.write_ovf(path, data, sim._cell, version=self.version, binary=self.binary, multiplier=mult,
+                          valueunit=unit, desc=sim._ovf_desc(), total_time=st["time"] * st["unit_time"],
+                          stage_time=float(self.period))
+        self.since = t32
+        return path
+
+
+def _resolve_output(what, format):
+    # resolve (simoutput.go:93-131)
+    what, format = str(what), str(format)
+    if what == "m":
+        fmts = {"binary": (1, True), "binary4": (1, True), "omf": (1, True), "binary_2": (2, True),
+                "binary4_2": (2, True), "omf_2": (2, True), "text": (1, False), "ascii": (1, False)}
+        if format not in fmts:
+            raise InputErr("unknown format %s. options are: binary (=binary4,omf,binary4_2, omf_2), text (=ascii)"
+                           % format)
+        return _Output("m", *fmts[format])
+    if what == "table":
+        return _Output("table")
+    if what == "torque":
+        if format not in ("binary", "text"):
+            raise InputErr("unknown format %s for torque. options are: binary, text" % format)
+        return _Output("torque", 1, format == "binary")
+    if what in ("phi", "energydensity"):
+        raise InputErr("saving %s is not supported by this engine" % what)
+    raise InputErr("unknown output quantity %s. options are: m, table, torque" % what)
+
+
 class Sim:
     def __init__(self, mode="fused"):
         if mode not in MODES:
@@ -44,7 +120,17 @@ class Sim:
         self._mindm = self._maxdm = self._mindt = self._maxdt = 0.0
         self._exchinconv = True
         self._demag_accuracy = 8
+        self._want_energy = False
         self._m0 = None
+        # output (sim.go:179-209): schedule, table columns, output id
+        self._outdir = "."
+        self._schedule = []
+        self._tab = {k: k in ("time", "m", "b", "id") for k in TABULATE}
+        self._table = None
+        self._table_path = None
+        self._autosave_idx = -1
+        self._m_fresh = False
+        self._user_desc = {}
         self._ptr = None
         self._lib = gpukern.load()
         self._keep = []
@@ -159,8 +245,22 @@ class Sim:
         self._maxdt = float(v)
 
     def exchinconv(self, flag):
+        """ExchInConv (simfield.go): the engine folds exchange into the convolution."""
         self._invalidate()
-        self._exchinconv = bool(flag)
+        if not flag:
+            raise InputErr("exchinconv false is not supported: exchange is part of the convolution kernel")
+        self._exchinconv = True
+
+    def exchtype(self, neighbours):
+        """ExchType (simfield.go:41-44): only the 6-neighbour Laplacian."""
+        self._invalidate()
+        if int(neighbours) != 6:
+            raise InputErr("exchange type %d is not supported (only 6)" % neighbours)
+
+    def demagaccuracy(self, accuracy):
+        """DemagAccuracy (simfield.go:280-287)."""
+        self._invalidate()
+        self._demag_accuracy = int(accuracy)
 
     # ------------------------------------------------------------ magnetisation (simconfig.go)
     def uniform(self, mz, my, mx):
@@ -169,6 +269,55 @@ class Sim:
         m = np.empty((3, X, Y, Z), np.float32)
         m[0], m[1], m[2] = mx, my, mz
         self._m0 = m
+
+    def _mlocal(self):
+        # initMLocal: a zero magnetisation of the grid size until one is set
+        self._invalidate()
+        if self._m0 is None:
+            self._m0 = np.zeros((3,) + tuple(self._need_size()), np.float32)
+        return self._m0
+
+    def setmcell(self, z, y, x, mz, my, mx):
+        """SetMCell (simconfig.go:36-43): cell of user indices (x, y, z) = internal (Z, Y, X)."""
+        m = self._mlocal()
+        m[0, x, y, z], m[1, x, y, z], m[2, x, y, z] = mx, my, mz
+
+    def vortex(self, circulation, polarization):
+        """Vortex (simconfig.go:74-95): in-plane circulation around the grid centre,
+        core along the thickness with the given polarization."""
+        m = self._mlocal()
+        X, Y, Z = m.shape[1:]
+        cy, cz = Y // 2, Z // 2
+        yy, zz = np.meshgrid(np.arange(Y) - cy, np.arange(Z) - cz, indexing="ij")
+        m[0] = 0.0
+        m[1] = zz * circulation
+        m[2] = -yy * circulation
+        m[:, :, cy, cz] = 0.0
+        m[0, :, cy, cz] = polarization
+
+    def setmrange(self, z1, y1, x1, z2, y2, x2, mz, my, mx):
+        """SetMRange (simconfig.go:20-32): cells of user indices x1..x2, y1..y2,
+        z1..z2 (inclusive; internal Z, Y, X) set to (mx, my, mz)."""
+        m = self._mlocal()
+        sl = (slice(x1, x2 + 1), slice(y1, y2 + 1), slice(z1, z2 + 1))
+        m[(0,) + sl], m[(1,) + sl], m[(2,) + sl] = mx, my, mz
+
+    def setrandom(self, seed=None):
+        """SetRandom (simconfig.go:314-321): every component uniform on [-0.5, 0.5)
+        (numpy's generator, not Go's math/rand); `seed` is a seed or a Generator."""
+        m = self._mlocal()
+        m[...] = (np.random.default_rng(seed).random(m.shape) - 0.5).astype(np.float32)
+
+    def addnoise(self, amplitude, seed=None):
+        """AddNoise (simconfig.go:303-311): each component += amplitude * U[-1, 1)."""
+        m = self._mlocal()
+        m += (2 * amplitude * (np.random.default_rng(seed).random(m.shape) - 0.5)).astype(np.float32)
+
+    def run_until(self, stop):
+        """Run()'s step loop up to the absolute internal time `stop`, without
+        Run()'s normalisation at the start (runs interrupted at output times)."""
+        self._init()
+        return int(self._lib.mm_sim_run_until(self._ptr, float(stop)))
 
     def setm_array(self, m):
         """Initial magnetisation in the internal layout (3, X, Y, Z)."""
@@ -201,65 +350,250 @@ class Sim:
         w = (t - t0) / (t1 - t0)
         return np.asarray(b0, float) + w * (np.asarray(b1, float) - np.asarray(b0, float))
 
-    def save(self, what, format="binary", outdir="."):
-        """Save(what, format) (simoutput.go:56-61, 90-117): "m" as
-        outdir/m%08d.omf in OVF 1.0 binary ("binary", "binary4", "omf"),
-        OVF 2.0 binary ("binary_2", "binary4_2", "omf_2") or OVF 1.0 text
-        ("text", "ascii"); "table" appends time, <m> and B (user x, y, z) to
-        outdir/datatable.odt."""
-        from . import ovf
-
+    def outputdir(self, path):
+        """Directory of the saved files (simoutput.go:30-40); created if absent."""
         import os
 
+        os.makedirs(path, exist_ok=True)
+        self._outdir = str(path)
+
+    def autosave(self, what, format, interval):
+        """Autosave(what, format, interval) (simoutput.go:46-52): save every
+        `interval` seconds of simulated time, checked before each step of
+        Run/Step and after each step of Relax."""
+        out = _resolve_output(what, format)
+        out.period = np.float32(interval)
+        self._schedule.append(out)
+
+    def tabulate(self, what, want):
+        """Tabulate(what, bool) (simoutput.go:210-215): columns of the table
+        (time, m, b, j, id, e, maxdm/dt, minmaxmz, corepos)."""
+        if self._table is not None:
+            raise InputErr("tabulate must be called before the datatable is opened (Before run, desc, autosave, ...).")
+        key = str(what).lower()
+        if key not in TABULATE:
+            raise InputErr("Illegal argument: %s options are: %s" % (what, list(TABULATE)))
+        self._tab[key] = bool(want)
+
+    def desc(self, key, value):
+        """Desc(key, value) (simmetadata.go:19-30): a header entry of the table
+        (which it opens) and of every OVF file saved afterwards."""
+        self._user_desc[str(key)] = str(value)
+        if self._table is None:
+            self._open_table()
+
+    def save(self, what, format="binary", outdir=None):
+        """Save(what, format) (simoutput.go:56-61, 90-117): "m" as
+        m%08d.omf in OVF 1.0 binary ("binary", "binary4", "omf"), OVF 2.0
+        binary ("binary_2", "binary4_2", "omf_2") or OVF 1.0 text ("text",
+        "ascii"); "torque" as torque%08d.omf; "table" appends a row to
+        datatable.odt.  The number in the file name is the output id, which
+        advances once per state of m that is saved (simrun.go:117-133).
+        Returns the path written."""
         self._init()
-        if what == "m":
-            fmts = {"binary": (1, True), "binary4": (1, True), "omf": (1, True), "binary_2": (2, True),
-                    "binary4_2": (2, True), "omf_2": (2, True), "text": (1, False), "ascii": (1, False)}
-            if format not in fmts:
-                raise InputErr("unknown format %s. options are: %s" % (format, ", ".join(fmts)))
-            version, binary = fmts[format]
-            idx = getattr(self, "_autosave_idx", 0)
-            path = os.path.join(outdir, "m%08d.omf" % idx)
-            self._autosave_idx = idx + 1
-            st = self.state()
-            t = st["time"] * st["unit_time"]
-            desc = {"time": "%g" % t, "msat": "%g" % self._msat, "aexch": "%g" % self._aexch,
-                    "alpha": "%g" % self._alpha, "torque": "%g" % st["max_torque"]}
-            ovf.write_ovf(path, self.m, self._cell, version=version,
-                          binary=binary, multiplier=self._msat, desc=desc, total_time=t)
-            return path
-        if what == "table":
-            path = os.path.join(outdir, "datatable.odt")
-            st = self.state()
-            t = st["time"] * st["unit_time"]
-            m = self.average_m()
-            b = self.field_at(t)
-            if getattr(self, "_table", None) is None:
-                self._table = ovf.OdtTable(path, "mumax data", [("Time", "s"), ("Mx/Ms", ""), ("My/Ms", ""),
-                                                               ("Mz/Ms", ""), ("Bx", "T"), ("By", "T"),
-                                                               ("Bz", "T")])
-            self._table.row([t, m[2], m[1], m[0], b[2], b[1], b[0]])
-            return path
-        raise InputErr("unknown output quantity %s. options are: m, table" % what)
+        if outdir is not None:
+            self.outputdir(outdir)
+        out = _resolve_output(what, format)
+        self._assure_output()
+        return out.save(self)
 
     def close_table(self):
-        if getattr(self, "_table", None) is not None:
+        if self._table is not None:
             self._table.close()
             self._table = None
 
+    def _assure_output(self):
+        # assureOutputUpToDate (simrun.go:117-133): a new output id per saved state
+        if not self._m_fresh:
+            self._autosave_idx += 1
+            self._m_fresh = True
+
+    def _time_f32(self):
+        # float32(s.time) * s.UnitTime(): the output schedule works in float32 seconds
+        st = self.state()
+        return np.float32(np.float32(st["time"]) * np.float32(st["unit_time"]))
+
+    def _save_scheduled(self):
+        if not self._schedule:
+            return
+        t = self._time_f32()
+        for out in self._schedule:
+            if out.need_save(t):
+                self._assure_output()
+                out.save(self)
+
+    def _ovf_desc(self):
+        st = self.state()
+        b = self.field_at(st["time"] * st["unit_time"])
+        d = {"msat": self._msat, "aexch": self._aexch, "alpha": self._alpha,
+             "time": st["time"] * st["unit_time"], "Bx": b[2], "By": b[1], "Bz": b[0],
+             "torque": st["max_torque"], "id": self._autosave_idx, "iteration": int(st["steps"])}
+        d = {k: "%.10g" % v for k, v in d.items()}
+        d.update(self._user_desc)
+        return d
+
+    def _open_table(self):
+        import os
+
+        from . import ovf
+
+        cols = []
+        if self._tab["time"]:
+            cols += [("Time", "s")]
+        if self._tab["m"]:
+            cols += [("Mx/Ms", ""), ("My/Ms", ""), ("Mz/Ms", "")]
+        if self._tab["b"]:
+            cols += [("Bx", "T"), ("By", "T"), ("Bz", "T")]
+        if self._tab["j"]:
+            cols += [("jx", "A/m2"), ("jy", "A/m2"), ("jz", "A/m2")]
+        if self._tab["e"]:
+            cols += [("Energy", "J")]
+        if self._tab["maxdm/dt"]:
+            cols += [("max_dm/dt", "gammaMs")]
+        if self._tab["minmaxmz"]:
+            cols += [("min_Mz/Ms", ""), ("max_Mz/Ms", "")]
+        if self._tab["corepos"]:
+            cols += [("core_x", "m"), ("core_y", "m")]
+        if self._tab["id"]:
+            cols += [("id", "")]
+        self._table_path = os.path.join(self._outdir, "datatable.odt")
+        self._table = ovf.OdtTable(self._table_path, "mumax data", cols, desc=self._user_desc)
+
+    def _table_row(self):
+        # Table.Save (simoutput.go:217-272)
+        if self._table is None:
+            self._open_table()
+        st = self.state()
+        t = st["time"] * st["unit_time"]
+        row = []
+        if self._tab["time"]:
+            row += [t]
+        if self._tab["m"]:
+            m = self.average_m()
+            row += [m[2], m[1], m[0]]
+        if self._tab["b"]:
+            b = self.field_at(t)
+            row += [b[2], b[1], b[0]]
+        if self._tab["j"]:
+            row += [0.0, 0.0, 0.0]
+        if self._tab["e"]:
+            row += [self.energy_si()]
+        if self._tab["maxdm/dt"]:
+            row += [st["max_torque"]]
+        if self._tab["minmaxmz"]:
+            mz = self.m[0]
+            row += [float(mz.min()), float(mz.max())]
+        if self._tab["corepos"]:
+            row += list(self.getcorepos())
+        if self._tab["id"]:
+            row += [self._autosave_idx]
+        self._table.row(row)
+        return self._table_path
+
+    # ------------------------------------------------------------ getters (simget.go)
+    def getm(self, component):
+        """Average of a user component (0 = x, 1 = y, 2 = z) of m (simget.go:25-33)."""
+        return float(np.float32(self.average_m()[2 - _component(component)]))
+
+    def getmpos(self, component, z, y, x):
+        """m component at cell (user indices x = z arg, ..., simget.go:36-38)."""
+        return float(self.m[2 - _component(component), x, y, z])
+
+    def getmaxm(self, component):
+        return float(self.m[2 - _component(component)].max())
+
+    def getminm(self, component):
+        return float(self.m[2 - _component(component)].min())
+
+    def getmaxtorque(self):
+        """Maximum torque of the last step in units gamma*Msat (simget.go:64-67)."""
+        return float(np.float32(self.state()["max_torque"]))
+
+    def getcorepos(self):
+        """Vortex core position in metres, centre = (0, 0) (corepos.go:11-34):
+        the cell of largest |m_z| in the first layer away from the edges,
+        refined by a parabola through its neighbours."""
+        mz = np.abs(self.m[0, 0]).astype(np.float32)   # [Y][Z]
+        Y, Z = mz.shape
+        if Y < 3 or Z < 3:
+            raise InputErr("the core position needs at least 3 x 3 cells")
+        inner = mz[1:-1, 1:-1]
+        iy, iz = np.unravel_index(int(np.argmax(inner)), inner.shape)
+        iy, iz = iy + 1, iz + 1
+        f0 = mz[iy, iz]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            px = np.float32(iz) + _parabola_peak(f0, mz[iy, iz - 1], mz[iy, iz + 1]) - np.float32(Z) / 2
+            py = np.float32(iy) + _parabola_peak(f0, mz[iy - 1, iz], mz[iy + 1, iz]) - np.float32(Y) / 2
+        return float(np.float32(px * np.float32(self._cell[2]))), float(np.float32(py * np.float32(self._cell[1])))
+
+    def gete(self):
+        """Total energy in SI units (simget.go:74-77)."""
+        return float(np.float32(self.energy_si()))
+
+    def gettime(self):
+        st = self.state()
+        return st["time"] * st["unit_time"]
+
     # ------------------------------------------------------------ run (simrun.go)
     def run(self, seconds):
+        """Run(seconds) (simrun.go:19-57): normalise m, then step until the
+        time has advanced by `seconds`, saving scheduled output before each
+        step.  Returns the number of steps taken."""
         self._init()
-        return int(self._lib.mm_sim_run(self._ptr, float(seconds)))
+        if not self._schedule:
+            n = int(self._lib.mm_sim_run(self._ptr, float(seconds)))
+        else:
+            st = self.state()
+            stop = st["time"] + float(seconds) / st["unit_time"]
+            self._lib.mm_sim_run(self._ptr, 0.0)      # Run()'s normalisation
+            self._m_fresh = False
+            n = self._run_scheduled(stop)
+        self._m_fresh = False
+        self._assure_output()
+        return n
+
+    def _run_scheduled(self, stop):
+        # The steps between two outputs run on the device in one stretch: it
+        # ends at the first step boundary at or after a time just below the
+        # next output's due time; from there single steps find the boundary
+        # where the reference's float32 check first passes.
+        n = 0
+        while True:
+            st = self.state()
+            t = st["time"]
+            if not t < stop:
+                return n
+            self._save_scheduled()
+            unit = float(np.float32(st["unit_time"]))
+            due = min(out.due_internal(unit) for out in self._schedule)
+            target = min(stop, due * (1.0 - 1e-5))
+            if target > t:
+                n += self.run_until(target)
+            else:
+                self._lib.mm_sim_steps(self._ptr, 1)
+                n += 1
+            self._m_fresh = False
 
     def relax(self, max_steps=0):
         """Relax() (relax.go:14-25): steps with energy control until a step
-        raises the total energy (or max_steps > 0 steps were taken); returns
-        the number of steps taken."""
+        raises the total energy (or max_steps > 0 steps were taken); scheduled
+        output is saved after each step.  Returns the number of steps taken."""
         self._init()
-        return int(self._lib.mm_sim_relax(self._ptr, int(max_steps)))
+        self._m_fresh = False
+        if not self._schedule:
+            return int(self._lib.mm_sim_relax(self._ptr, int(max_steps)))
+        n, prev = 0, np.inf
+        while True:
+            self._lib.mm_sim_relax(self._ptr, 1)
+            n += 1
+            self._m_fresh = False
+            self._save_scheduled()
+            e = self.state()["energy"]
+            if e > prev or (max_steps > 0 and n >= max_steps):
+                return n
+            prev = e
 
-    def hysteresis(self, bz0, by0, bx0, bz1, by1, bx1, steps, save=True, outdir="."):
+    def hysteresis(self, bz0, by0, bx0, bz1, by1, bx1, steps, save=True, outdir=None):
         """Hysteresis() (simhysteresis.go:16-27): field scan in `steps`
         intervals, relaxing and saving m (omf) and a table row at each point."""
         for i in range(steps + 1):
@@ -271,8 +605,15 @@ class Sim:
                 self.save("table", "ascii", outdir)
 
     def steps(self, n):
+        """Steps(n) (simrun.go:88-92): n times Step()."""
         self._init()
-        self._lib.mm_sim_steps(self._ptr, int(n))
+        if not self._schedule:
+            self._lib.mm_sim_steps(self._ptr, int(n))
+        else:
+            for _ in range(int(n)):
+                self._save_scheduled()
+                self._lib.mm_sim_steps(self._ptr, 1)
+        self._m_fresh = False
 
     def step(self):
         self.steps(1)
@@ -397,6 +738,7 @@ class Sim:
         p.exchInConv = int(self._exchinconv)
         p.wantDemag = 1
         p.demagAccuracy = self._demag_accuracy
+        p.wantEnergy = int(self._want_energy)
         p.solver = SOLVERS[self._solver]
         p.maxError, p.dt = self._maxerror, self._dt
         p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt

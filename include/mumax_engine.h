@@ -65,6 +65,12 @@ void mm_sim_free(mmSim* s);
 
 /* Run(duration): steps until time >= stop (simrun.go:19-57); returns accepted steps */
 long mm_sim_run(mmSim* s, double seconds);
+/* The step loop of Run() (simrun.go:29-49) up to an absolute stop time in
+ * internal units (state[0] and state[8] give the time and the unit), without
+ * Run()'s normalisation at the start: a Run() interrupted at output times is
+ * mm_sim_run(s, 0) (the normalisation) followed by mm_sim_run_until for each
+ * stretch between outputs.  Returns the number of accepted steps. */
+long mm_sim_run_until(mmSim* s, double stop);
 /* Steps(n) (simrun.go:88-92) */
 void mm_sim_steps(mmSim* s, int n);
 /* Enqueue n steps without waiting (fused mode; for timing); mm_sim_sync waits. */

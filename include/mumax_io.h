@@ -47,10 +47,16 @@ int mm_ovf_write(const char* path, const float* m, const mmOvfInfo* info, const 
 int mm_tensor_read(const char* path, int* rank, int* size, float* data);
 int mm_tensor_write(const char* path, int rank, const int* size, const float* data);
 
-/* ODT 1.0 table: header with title, units and columns, one row per call */
+/* ODT 1.0 table: header with title, units and columns, one row per call
+ * (each row is flushed to the file: mumax/omf/odt.go:38-139) */
 typedef struct mmOdt mmOdt;
 mmOdt* mm_odt_open(const char* path, const char* title, int ncols, const char* const* names,
                    const char* const* units);
+/* The same with ndesc "# Desc: key :  value" lines in a "# Begin: Header" block
+ * before the table (Desc(), simmetadata.go:19-30, odt.go:60-68, 117-121). */
+mmOdt* mm_odt_open_desc(const char* path, const char* title, int ncols, const char* const* names,
+                        const char* const* units, int ndesc, const char* const* desc_keys,
+                        const char* const* desc_values);
 int mm_odt_row(mmOdt* t, const double* values);
 int mm_odt_close(mmOdt* t);
 
