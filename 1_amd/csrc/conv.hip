@@ -27,6 +27,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "engine.h"
@@ -101,6 +104,148 @@ __global__ void k_facekernel6(double* __restrict__ K, int P0, int P1, int P2, do
                 face_integral_f64(B, R, cell, s, acc);
                 for (int d = s; d < 3; d++) kk[c_kern_idx[s][d]] = B[d];
             }
+        }
+        for (int k = 0; k < 6; k++) K[k * NP + idx] = kk[k];
+    }
+}
+
+// ---------------------------------------------------------------- Newell tensor
+// The demagnetising tensor N between two equal rectangular cells, averaged over
+// the target cell (Newell, Williams & Dunlop, J. Geophys. Res. 98, 9551 (1993)):
+// N_xx = -(1/(4 pi V)) d2x d2y d2z f(X, Y, Z) and N_xy the same of g, where d2
+// is the second difference [1, -2, 1] with the cell size as step.  The kernel
+// is K = -N (field per unit magnetisation, the sign of FaceKernel6).
+// Cancellation in the 27-term sums grows like (|R|/d)^6; from |R| >= 12 d_max
+// on the same cell average is taken by 3-point Gauss-Legendre quadrature in
+// each coordinate of both cells (error ~1e-8 relative there).
+__device__ double newell_f(double x, double y, double z) {
+    x = fabs(x), y = fabs(y), z = fabs(z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double R = sqrt(x2 + y2 + z2);
+    if (R == 0.0) return 0.0;
+    double t = (2.0 * x2 - y2 - z2) * R / 6.0;
+    const double rxz = sqrt(x2 + z2), rxy = sqrt(x2 + y2);
+    if (y > 0.0 && rxz > 0.0) t += 0.5 * y * (z2 - x2) * asinh(y / rxz);
+    if (z > 0.0 && rxy > 0.0) t += 0.5 * z * (y2 - x2) * asinh(z / rxy);
+    if (x > 0.0 && y > 0.0 && z > 0.0) t -= x * y * z * atan(y * z / (x * R));
+    return t;
+}
+
+__device__ double newell_g(double x, double y, double z) {
+    const double sgn = ((x < 0.0) != (y < 0.0)) ? -1.0 : 1.0;
+    x = fabs(x), y = fabs(y), z = fabs(z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double R = sqrt(x2 + y2 + z2);
+    if (R == 0.0) return 0.0;
+    const double rxy = sqrt(x2 + y2), ryz = sqrt(y2 + z2), rxz = sqrt(x2 + z2);
+    double t = -x * y * R / 3.0;
+    if (z > 0.0 && rxy > 0.0) t += x * y * z * asinh(z / rxy);
+    if (x > 0.0 && ryz > 0.0) t += y / 6.0 * (3.0 * z2 - y2) * asinh(x / ryz);
+    if (y > 0.0 && rxz > 0.0) t += x / 6.0 * (3.0 * z2 - x2) * asinh(y / rxz);
+    if (z > 0.0) t -= z2 * z / 6.0 * atan(x * y / (z * R));
+    if (y > 0.0) t -= z * y2 / 2.0 * atan(x * z / (y * R));
+    if (x > 0.0) t -= z * x2 / 2.0 * atan(y * z / (x * R));
+    return sgn * t;
+}
+
+// sum over the 27 neighbours with weights (-1, 2, -1)^3 of fn(X+i dx, Y+j dy, Z+k dz)
+template <bool G>
+__device__ double newell_sum(double X, double Y, double Z, double dx, double dy, double dz) {
+    const double w[3] = {-1.0, 2.0, -1.0};
+    double s = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double sk = 0.0;
+            for (int k = 0; k < 3; k++) {
+                const double a = X + (i - 1) * dx, b = Y + (j - 1) * dy, c = Z + (k - 1) * dz;
+                sk += w[k] * (G ? newell_g(a, b, c) : newell_f(a, b, c));
+            }
+            s += w[i] * w[j] * sk;
+        }
+    return s / (4.0 * 3.14159265358979323846 * dx * dy * dz);
+}
+
+// K = -N at displacement R (order xx yy zz yz xz xy)
+__device__ void newell_kernel(double K[6], const double R[3], const double d[3]) {
+    const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    const double dmax = fmax(d[0], fmax(d[1], d[2]));
+    if (r < 12.0 * dmax) {
+        K[0] = -newell_sum<false>(R[0], R[1], R[2], d[0], d[1], d[2]);
+        K[1] = -newell_sum<false>(R[1], R[2], R[0], d[1], d[2], d[0]);
+        K[2] = -newell_sum<false>(R[2], R[0], R[1], d[2], d[0], d[1]);
+        K[3] = -newell_sum<true>(R[1], R[2], R[0], d[1], d[2], d[0]);
+        K[4] = -newell_sum<true>(R[0], R[2], R[1], d[0], d[2], d[1]);
+        K[5] = -newell_sum<true>(R[0], R[1], R[2], d[0], d[1], d[2]);
+        return;
+    }
+    // V * E[D(R + a - b)], a and b uniform in the cell, D the point-dipole field
+    const double gx[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
+    const double gw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    double u[3][9], wu[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            for (int a = 0; a < 3; a++) u[a][3 * i + j] = 0.5 * (gx[i] - gx[j]) * d[a];
+            wu[3 * i + j] = gw[i] * gw[j];
+        }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 9; i++)
+        for (int j = 0; j < 9; j++) {
+            const double x = R[0] + u[0][i], y = R[1] + u[1][j];
+            const double wij = wu[i] * wu[j];
+            for (int k = 0; k < 9; k++) {
+                const double z = R[2] + u[2][k];
+                const double r2 = x * x + y * y + z * z;
+                const double ir2 = 1.0 / r2;
+                const double ir3 = ir2 / sqrt(r2);
+                const double w = wij * wu[k] * ir3;
+                const double w3 = 3.0 * w * ir2;
+                acc[0] += w3 * x * x - w;
+                acc[1] += w3 * y * y - w;
+                acc[2] += w3 * z * z - w;
+                acc[3] += w3 * y * z;
+                acc[4] += w3 * x * z;
+                acc[5] += w3 * x * y;
+            }
+        }
+    const double s = d[0] * d[1] * d[2] / (4.0 * 3.14159265358979323846);
+    for (int c = 0; c < 6; c++) K[c] = s * acc[c];
+}
+
+// one octant (displacements >= 0) of the Newell kernel: O[6][H0][H1][H2]
+__global__ void k_newell_octant(double* __restrict__ O, int H0, int H1, int H2, double c0, double c1, double c2) {
+    const long NH = (long)H0 * H1 * H2;
+    const double d[3] = {c0, c1, c2};
+    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < NH; idx += (long)gridDim.x * blockDim.x) {
+        const int z = (int)(idx % H2);
+        const long r = idx / H2;
+        const int y = (int)(r % H1), x = (int)(r / H1);
+        const double R[3] = {x * c0, y * c1, z * c2};
+        double K[6];
+        newell_kernel(K, R, d);
+        for (int k = 0; k < 6; k++) O[k * NH + idx] = K[k];
+    }
+}
+
+// the wrapped grid from the octant: diagonal even, yz odd in y and z, xz odd
+// in x and z, xy odd in x and y; FaceKernel6's displacement range (unwrap)
+__global__ void k_newell_expand(const double* __restrict__ O, int H0, int H1, int H2, double* __restrict__ K,
+                                int P0, int P1, int P2) {
+    const long NP = (long)P0 * P1 * P2, NH = (long)H0 * H1 * H2;
+    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < NP; idx += (long)gridDim.x * blockDim.x) {
+        const int zw = (int)(idx % P2);
+        const long r = idx / P2;
+        const int yw = (int)(r % P1), xw = (int)(r / P1);
+        int x, y, z;
+        double kk[6] = {0, 0, 0, 0, 0, 0};
+        if (unwrap(xw, P0, x) && unwrap(yw, P1, y) && unwrap(zw, P2, z)) {
+            const long o = ((long)abs(x) * H1 + abs(y)) * H2 + abs(z);
+            const double sx = x < 0 ? -1.0 : 1.0, sy = y < 0 ? -1.0 : 1.0, sz = z < 0 ? -1.0 : 1.0;
+            kk[0] = O[0 * NH + o];
+            kk[1] = O[1 * NH + o];
+            kk[2] = O[2 * NH + o];
+            kk[3] = sy * sz * O[3 * NH + o];
+            kk[4] = sx * sz * O[4 * NH + o];
+            kk[5] = sx * sy * O[5 * NH + o];
         }
         for (int k = 0; k < 6; k++) K[k * NP + idx] = kk[k];
     }
@@ -185,7 +330,8 @@ void conv_plan_init(ConvPlan& cp, const int N[3]) {
     const long kc = cp.P[0] == 1 ? (long)cp.nK * cp.Q * hy : (long)cp.nK * cp.Q * hy * hx;
     const long b1 = 3L * N[0] * cp.Q * N[1];
     const long b2 = (cp.P[0] == 1 || cp.yxy) ? 0 : 3L * cp.Q * cp.P[1] * N[0];
-    GK_CHECK(hipMalloc(&cp.Kc, kc * sizeof(float)));
+    cp.Kc = nullptr;  // conv_acquire_kernel
+    cp.kc_elems = kc;
     GK_CHECK(hipMalloc(&cp.B1, b1 * sizeof(float2)));
     cp.B2 = nullptr;  // layout [3][Q][Nx][Py]
     if (b2) GK_CHECK(hipMalloc(&cp.B2, b2 * sizeof(float2)));
@@ -207,8 +353,78 @@ size_t conv_plan_footprint(const int N[3], int* epi_blocks) {
     return kc * sizeof(float) + (b1 + b2) * sizeof(float2);
 }
 
+// ---------------------------------------------------------------- shared kernel spectra
+// One spectrum per (device, padded size, cell, kernel, accuracy, exchange):
+// simulations of one shape -- a parameter sweep, a served batch -- build it
+// once and share it (the reference recomputes or reads a disk cache per
+// simulation, kernelpipe.go:20-65, wisdom.go:30-133).
+namespace {
+struct KernelKey {
+    int dev, P0, P1, P2, type, accuracy, exch;
+    uint64_t c0, c1, c2;
+    auto tie() const { return std::tie(dev, P0, P1, P2, type, accuracy, exch, c0, c1, c2); }
+    bool operator<(const KernelKey& o) const { return tie() < o.tie(); }
+};
+struct KernelEntry {
+    float* Kc;
+    int refs;
+    double imag;
+};
+std::mutex g_kmu;
+std::map<KernelKey, KernelEntry> g_kernels;
+
+uint64_t bits(double v) {
+    uint64_t u;
+    memcpy(&u, &v, 8);
+    return u;
+}
+}  // namespace
+
+double conv_acquire_kernel(ConvPlan& cp, const KernelSpec& ks) {
+    KernelKey key{};
+    GK_CHECK(hipGetDevice(&key.dev));
+    key.P0 = cp.P[0], key.P1 = cp.P[1], key.P2 = cp.P[2];
+    key.type = ks.type;
+    key.accuracy = ks.type == MM_KERNEL_FACE6 ? ks.accuracy : 0;
+    key.exch = ks.exch ? 1 : 0;
+    key.c0 = bits(ks.cell[0]), key.c1 = bits(ks.cell[1]), key.c2 = bits(ks.cell[2]);
+    std::lock_guard<std::mutex> lock(g_kmu);
+    auto it = g_kernels.find(key);
+    if (it != g_kernels.end()) {
+        it->second.refs++;
+        cp.Kc = it->second.Kc;
+        return it->second.imag;
+    }
+    GK_CHECK(hipMalloc(&cp.Kc, cp.kc_elems * sizeof(float)));
+    const double imag = conv_build_kernel(cp, ks, nullptr, nullptr);
+    g_kernels[key] = KernelEntry{cp.Kc, 1, imag};
+    return imag;
+}
+
+static void release_kernel(float* Kc) {
+    std::lock_guard<std::mutex> lock(g_kmu);
+    for (auto it = g_kernels.begin(); it != g_kernels.end(); ++it)
+        if (it->second.Kc == Kc) {
+            if (--it->second.refs == 0) {
+                GK_CHECK(hipFree(Kc));
+                g_kernels.erase(it);
+            }
+            return;
+        }
+    GK_CHECK(hipFree(Kc));
+}
+
+int conv_kernel_cache_size() {
+    int dev = 0;
+    GK_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_kmu);
+    int n = 0;
+    for (auto& kv : g_kernels) n += kv.first.dev == dev;
+    return n;
+}
+
 void conv_plan_free(ConvPlan& cp) {
-    if (cp.Kc) GK_CHECK(hipFree(cp.Kc));
+    if (cp.Kc) release_kernel(cp.Kc);
     if (cp.B1) GK_CHECK(hipFree(cp.B1));
     if (cp.B2) GK_CHECK(hipFree(cp.B2));
     cp.Kc = nullptr;
@@ -221,15 +437,32 @@ static int wrapi(int number, int max) {
     return number;
 }
 
-// real-space FaceKernel6 (+ Exch6) over the padded grid P, 6 x P doubles on the device
-static double* realspace_kernel_f64(const int P[3], const double cell[3], int accuracy, bool exch, hipStream_t s) {
+// real-space demag kernel (FaceKernel6 or Newell) + Exch6 over the padded grid
+// P, 6 x P doubles on the device
+static double* realspace_kernel_f64(const int P[3], const KernelSpec& ks, hipStream_t s) {
     const int P0 = P[0], P1 = P[1], P2 = P[2];
+    const double* cell = ks.cell;
     const long NP = (long)P0 * P1 * P2;
     double* K = nullptr;
     GK_CHECK(hipMalloc(&K, 6 * NP * sizeof(double)));
-    k_facekernel6<<<grid1d(NP), kBlock, 0, s>>>(K, P0, P1, P2, cell[0], cell[1], cell[2], accuracy);
-    GK_LAUNCH_CHECK();
-    if (exch) {
+    if (ks.type == MM_KERNEL_NEWELL) {
+        // displacements 0 .. P/2 per axis cover unwrap's range [-(P-1)/2, P/2-1] in magnitude
+        const int H0 = P0 / 2 + 1, H1 = P1 / 2 + 1, H2 = P2 / 2 + 1;
+        const long NH = (long)H0 * H1 * H2;
+        double* O = nullptr;
+        GK_CHECK(hipMalloc(&O, 6 * NH * sizeof(double)));
+        k_newell_octant<<<grid1d(NH, 64), 64, 0, s>>>(O, H0, H1, H2, cell[0], cell[1], cell[2]);
+        GK_LAUNCH_CHECK();
+        k_newell_expand<<<grid1d(NP), kBlock, 0, s>>>(O, H0, H1, H2, K, P0, P1, P2);
+        GK_LAUNCH_CHECK();
+        GK_CHECK(hipStreamSynchronize(s));
+        GK_CHECK(hipFree(O));
+    } else {
+        GK_ASSERT(ks.type == MM_KERNEL_FACE6, "unknown demag kernel type %d", ks.type);
+        k_facekernel6<<<grid1d(NP), kBlock, 0, s>>>(K, P0, P1, P2, cell[0], cell[1], cell[2], ks.accuracy);
+        GK_LAUNCH_CHECK();
+    }
+    if (ks.exch) {
         // Exch6NgbrKernel written in the reference's order: with a padded size of 1,
         // wrap(+-1,1) = 0 overwrites the centre (exchange.go:58-65)
         std::vector<std::pair<long, double>> pts;
@@ -267,10 +500,19 @@ __global__ void k_to_float(const double* __restrict__ in, float* __restrict__ ou
         out[i] = (float)in[i];
 }
 
-void conv_realspace_kernel(const int P[3], const double cell[3], int accuracy, bool exch, float* out_host) {
+void conv_realspace_kernel_f64(const int P[3], const KernelSpec& ks, double* out_host) {
     hipStream_t s = stream();
     const long NP = (long)P[0] * P[1] * P[2];
-    double* K = realspace_kernel_f64(P, cell, accuracy, exch, s);
+    double* K = realspace_kernel_f64(P, ks, s);
+    GK_CHECK(hipMemcpyAsync(out_host, K, 6 * NP * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync();
+    GK_CHECK(hipFree(K));
+}
+
+void conv_realspace_kernel(const int P[3], const KernelSpec& ks, float* out_host) {
+    hipStream_t s = stream();
+    const long NP = (long)P[0] * P[1] * P[2];
+    double* K = realspace_kernel_f64(P, ks, s);
     float* f = nullptr;
     GK_CHECK(hipMalloc(&f, 6 * NP * sizeof(float)));
     k_to_float<<<grid1d(6 * NP), kBlock, 0, s>>>(K, f, 6 * NP);
@@ -281,12 +523,11 @@ void conv_realspace_kernel(const int P[3], const double cell[3], int accuracy, b
     GK_CHECK(hipFree(f));
 }
 
-double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool exch, float* full_host,
-                         float* full_dev) {
+double conv_build_kernel(ConvPlan& cp, const KernelSpec& ks, float* full_host, float* full_dev) {
     hipStream_t s = stream();
     const int P0 = cp.P[0], P1 = cp.P[1], P2 = cp.P[2];
     const long NP = (long)P0 * P1 * P2;
-    double* K = realspace_kernel_f64(cp.P, cell, accuracy, exch, s);
+    double* K = realspace_kernel_f64(cp.P, ks, s);
     double2* W = nullptr;
     unsigned long long* mx = nullptr;
     GK_CHECK(hipMalloc(&W, NP * sizeof(double2)));

@@ -8,6 +8,7 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/mumax_engine.h"
 #include "gk_common.h"
 
 namespace gk {
@@ -95,32 +96,51 @@ struct StageIn {
 };
 
 // ---------------------------------------------------------------- convolution plan
+// What the convolution kernel is made of: cell size (internal units), the demag
+// kernel (MM_KERNEL_FACE6 with `accuracy` charges per face edge, or
+// MM_KERNEL_NEWELL) and whether the 6-neighbour exchange is folded in.
+struct KernelSpec {
+    double cell[3];
+    int accuracy;
+    int exch;
+    int type;
+};
+
 struct ConvPlan {
     int N[3];   // data size (internal X,Y,Z)
     int P[3];   // padded FFT size
     int M, Q;   // Pz/2, Pz/2+1
     int nK;     // 6 (3-D) or 4 (2-D)
-    float* Kc;  // compressed kernel spectrum (see conv.hip)
+    float* Kc;  // compressed kernel spectrum (see conv.hip), shared between plans of one spec
+    long kc_elems;
     float2* B1; // [3][Nx][Q][Ny]
     float2* B2; // 3-D: [3][Q][Py][Nx] (not used by the thin plane pass)
     int yxy;    // thin 3-D: Y . X K X . Y inverse in one plane pass (no B2)
     size_t bytes;
 };
 
-// Choose padded sizes, allocate buffers.  Kc is filled by conv_build_kernel.
+// Choose padded sizes, allocate buffers.  Kc is set by conv_acquire_kernel.
 void conv_plan_init(ConvPlan& cp, const int N[3]);
+// Frees the buffers and releases the plan's kernel.
 void conv_plan_free(ConvPlan& cp);
+// Points cp.Kc at the kernel spectrum of spec `ks` for cp's padded size on the
+// current device, building it (conv_build_kernel) if no live plan holds it:
+// plans of the same padded size and spec share one spectrum, freed with the
+// last of them.  Returns the spectrum's max|Im|/max|Re|.
+double conv_acquire_kernel(ConvPlan& cp, const KernelSpec& ks);
+// number of distinct kernel spectra held on the current device
+int conv_kernel_cache_size();
 // bytes conv_plan_init allocates for data size N, and the blocks of the fused Z inverse pass
 size_t conv_plan_footprint(const int N[3], int* epi_blocks);
-// FaceKernel6 (accuracy x accuracy charges per face) in FP64 on the device,
-// optionally plus the 6-neighbour exchange kernel, FFT in FP64, stored
-// compressed in cp.Kc.  If full != null, also writes nK x [Px][Py][Q] floats
+// FaceKernel6 (accuracy x accuracy charges per face) or the Newell tensor in
+// FP64 on the device, optionally plus the 6-neighbour exchange kernel, FFT in
+// FP64, stored compressed in cp.Kc (cp.kc_elems floats, allocated by the caller).  If full != null, also writes nK x [Px][Py][Q] floats
 // in the reference's layout (conv.go:110-167).  Returns max|Im|/max|Re| of the
 // spectrum (the reference warns above 1e-4, conv.go:153).
-double conv_build_kernel(ConvPlan& cp, const double cell[3], int accuracy, bool exch, float* full_host,
-                         float* full_dev);
-// real-space demag (+exchange) kernel over the padded grid P, 6 x P floats to the host
-void conv_realspace_kernel(const int P[3], const double cell[3], int accuracy, bool exch, float* out_host);
+double conv_build_kernel(ConvPlan& cp, const KernelSpec& ks, float* full_host, float* full_dev);
+// real-space demag (+exchange) kernel over the padded grid P, 6 x P floats / doubles to the host
+void conv_realspace_kernel(const int P[3], const KernelSpec& ks, float* out_host);
+void conv_realspace_kernel_f64(const int P[3], const KernelSpec& ks, double* out_host);
 // Optional per-kernel-class event timing (bench/profiling).  Classes:
 // 0 Z fwd, 1 Y fwd, 2 X.K.X (3-D) or Y.K.Y (2-D), 3 Y inv, 4 Z inv (+ torque epilogue),
 // 5 separate torque, 6 separate RK final (adaptive tableaux), 7 control/copy.

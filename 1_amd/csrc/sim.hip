@@ -489,6 +489,16 @@ static void abi_step(mmSim* s) {
     s->hsteps++;
 }
 
+// the convolution kernel of this simulation for a cell size in internal units
+static KernelSpec kernel_spec(const mmSim* s, const double cell[3]) {
+    KernelSpec ks{};
+    for (int i = 0; i < 3; i++) ks.cell[i] = cell[i];
+    ks.accuracy = s->p.demagAccuracy > 0 ? s->p.demagAccuracy : 8;
+    ks.exch = s->p.exchType == 6 && s->p.exchInConv;
+    ks.type = s->p.demagKernel;
+    return ks;
+}
+
 static void abi_init(mmSim* s, const double cellInt[3]) {
     // initConv (init.go:234-274) + NewConv/loadKernel6 (conv.go:35-167) through the drop-in API
     int size[3] = {s->p.size[0], s->p.size[1], s->p.size[2]};
@@ -501,8 +511,7 @@ static void abi_init(mmSim* s, const double cellInt[3]) {
     const int nK = s->Pabi[0] == 1 ? 4 : 6;
     const long NP = (long)s->Pabi[0] * s->Pabi[1] * s->Pabi[2];
     std::vector<float> Kr((size_t)6 * NP);
-    conv_realspace_kernel(s->Pabi, cellInt, s->p.demagAccuracy > 0 ? s->p.demagAccuracy : 8,
-                          s->p.exchInConv && s->p.exchType == 6, Kr.data());
+    conv_realspace_kernel(s->Pabi, kernel_spec(s, cellInt), Kr.data());
     gpuFFT3dPlan* kplan = new_gpuFFT3dPlan_padded(s->Pabi, s->Pabi);
     float* devIn = new_gpu_array((int)NP);
     float* devOut = new_gpu_array((int)phys);
@@ -545,6 +554,8 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     GK_ASSERT(!p->periodic[0] && !p->periodic[1] && !p->periodic[2], "periodic boundaries are not supported");
     GK_ASSERT(p->msat > 0 && p->aexch > 0, "msat and aexch must be set (init.go:87-97)");
     GK_ASSERT(p->wantDemag, "the engine needs demag on");
+    GK_ASSERT(p->demagKernel == MM_KERNEL_FACE6 || p->demagKernel == MM_KERNEL_NEWELL, "unknown demag kernel %d",
+              p->demagKernel);
     GK_ASSERT(p->exchType == 0 || (p->exchType == 6 && p->exchInConv),
               "exchange must be 0 or 6-neighbour folded into the convolution");
     s->N = p->size[0] * p->size[1] * p->size[2];
@@ -594,8 +605,7 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     // convolution (init.go:234-274)
     conv_plan_init(s->cp, p->size);
     s->bytes += s->cp.bytes;
-    s->kernelImagRatio = conv_build_kernel(s->cp, s->cell, p->demagAccuracy > 0 ? p->demagAccuracy : 8,
-                                           p->exchType == 6 && p->exchInConv, nullptr, nullptr);
+    s->kernelImagRatio = conv_acquire_kernel(s->cp, kernel_spec(s, s->cell));
     if (s->kernelImagRatio > 1e-4)
         fprintf(stderr, "Warning: FFT Kernel max imaginary part ratio = %g\n", s->kernelImagRatio);
     s->conv = true;
@@ -811,11 +821,27 @@ long mm_sim_kernel_spectrum(mmSim* s, float* out) {
     if (!out) return total;
     float* dev = nullptr;
     GK_CHECK(hipMalloc(&dev, total * sizeof(float)));
-    ConvPlan tmp = s->cp;
-    conv_build_kernel(tmp, s->cell, s->p.demagAccuracy > 0 ? s->p.demagAccuracy : 8,
-                      s->p.exchType == 6 && s->p.exchInConv, out, dev);
+    ConvPlan tmp = s->cp;  // a private spectrum: the engine's may be shared
+    GK_CHECK(hipMalloc(&tmp.Kc, tmp.kc_elems * sizeof(float)));
+    conv_build_kernel(tmp, kernel_spec(s, s->cell), out, dev);
+    GK_CHECK(hipFree(tmp.Kc));
     GK_CHECK(hipFree(dev));
     return total;
+}
+
+void mm_demag_kernel(const int P[3], const double cellsize[3], int kernel, int accuracy, double* out) {
+    GK_ASSERT(P[0] > 0 && P[1] > 0 && P[2] > 0, "bad padded size %d %d %d", P[0], P[1], P[2]);
+    GK_ASSERT(kernel == MM_KERNEL_FACE6 || kernel == MM_KERNEL_NEWELL, "unknown demag kernel %d", kernel);
+    KernelSpec ks{};
+    for (int i = 0; i < 3; i++) ks.cell[i] = cellsize[i];
+    ks.accuracy = accuracy > 0 ? accuracy : 8;
+    ks.exch = 0;
+    ks.type = kernel;
+    conv_realspace_kernel_f64(P, ks, out);
+}
+
+int mm_kernel_cache_size(void) {
+    return conv_kernel_cache_size();
 }
 
 unsigned long long mm_sim_device_bytes(mmSim* s) { return s->bytes; }

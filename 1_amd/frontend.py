@@ -150,7 +150,7 @@ COMMANDS = {
     # geometry (simgeom.go)
     "gridsize": ("gridsize", "iii"), "cellsize": ("cellsize", "fff"), "partsize": ("partsize", "fff"),
     "demagaccuracy": ("demagaccuracy", "i"), "exchinconv": ("exchinconv", "b"), "exchtype": ("exchtype", "i"),
-    "demag": ("_demag", "b"), "energy": ("_energy", "b"),
+    "demag": ("_demag", "b"), "energy": ("_energy", "b"), "kerneltype": ("kerneltype", "s"),
     # excitation (simfield.go)
     "applystatic": ("applystatic", "sfff"), "applypointwise": ("applypointwise", "sdfff"),
     # solver (simsolver.go)
@@ -183,7 +183,7 @@ UNSUPPORTED = (
     "setalpharange", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortexinarray", "vortex_in_array", "setvortexellips",
     "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrf", "applyrframp", "applypulse",
-    "applyrotating", "applyrotatingburst", "applysawtooth", "kerneltype", "edgecorrection", "saveh",
+    "applyrotating", "applyrotatingburst", "applysawtooth", "edgecorrection", "saveh",
     "savebenchmark", "exchangetype",
 )
 

@@ -88,6 +88,8 @@ ENGINE = {
     "mm_sim_begin_run": (None, [P, C.c_double]),
     "mm_sim_relax": (C.c_long, [P, C.c_long]),
     "mm_sim_run_until": (C.c_long, [P, C.c_double]),
+    "mm_demag_kernel": (None, [P, P, I, I, P]),
+    "mm_kernel_cache_size": (I, []),
     "mm_sim_set_field": (None, [P, I, P, P]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
@@ -131,6 +133,7 @@ class Params(C.Structure):
         ("fieldT", C.POINTER(C.c_double)),
         ("fieldB", C.POINTER(C.c_double)),
         ("wantEnergy", C.c_int),
+        ("demagKernel", C.c_int),
     ]
 
 

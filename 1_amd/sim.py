@@ -121,6 +121,7 @@ class Sim:
         self._exchinconv = True
         self._demag_accuracy = 8
         self._want_energy = False
+        self._kernel = 0
         self._m0 = None
         # output (sim.go:179-209): schedule, table columns, output id
         self._outdir = "."
@@ -256,6 +257,19 @@ class Sim:
         self._invalidate()
         if int(neighbours) != 6:
             raise InputErr("exchange type %d is not supported (only 6)" % neighbours)
+
+    def kerneltype(self, name):
+        """KernelType (simfield.go): the demag kernel.  "facekernel6" (also the
+        reference's generator names "mumaxkern-go", "mumaxkern") is the
+        reference's FaceKernel6; "newell" the Newell tensor (cell-averaged)."""
+        self._invalidate()
+        key = str(name).lower()
+        if key in ("facekernel6", "mumaxkern-go", "mumaxkern"):
+            self._kernel = 0
+        elif key == "newell":
+            self._kernel = 1
+        else:
+            raise InputErr("unknown kernel type %s. options are: facekernel6, newell" % name)
 
     def demagaccuracy(self, accuracy):
         """DemagAccuracy (simfield.go:280-287)."""
@@ -739,6 +753,7 @@ class Sim:
         p.wantDemag = 1
         p.demagAccuracy = self._demag_accuracy
         p.wantEnergy = int(self._want_energy)
+        p.demagKernel = self._kernel
         p.solver = SOLVERS[self._solver]
         p.maxError, p.dt = self._maxerror, self._dt
         p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt

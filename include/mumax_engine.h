@@ -31,6 +31,9 @@ extern "C" {
 
 enum { MM_MODE_FUSED = 0, MM_MODE_ABI = 1 };
 enum { MM_ANIS_NONE = 0, MM_ANIS_UNIAXIAL = 1 };
+/* demag kernel: the reference's FaceKernel6 (demag.go:26-84) or the Newell
+ * tensor (cell-averaged field of a uniformly magnetised cell, FP64) */
+enum { MM_KERNEL_FACE6 = 0, MM_KERNEL_NEWELL = 1 };
 /* solver codes: rk.go tableaux (solver.go:16-53) */
 enum { MM_RK2 = 2, MM_RK12 = 12, MM_RK3 = 3, MM_RK23 = 23, MM_RK4 = 4, MM_RKCK = 45, MM_RKDP = 54 };
 
@@ -55,6 +58,7 @@ typedef struct {
     const double* fieldT;   /* nField times (s) */
     const double* fieldB;   /* nField*3 tesla, internal order */
     int wantEnergy;         /* energy at the FSAL stage (rk.go:323-337) */
+    int demagKernel;        /* MM_KERNEL_* (0: FaceKernel6, as the reference) */
 } mmParams;
 
 typedef struct mmSim mmSim;
@@ -104,8 +108,19 @@ void mm_sim_average_m(mmSim* s, double* avg);
  * xx yy zz yz).  out may be NULL to query; returns the number of floats. */
 long mm_sim_kernel_spectrum(mmSim* s, float* out);
 
-/* bytes of device memory held by the simulation */
+/* Real-space demag kernel in FP64 over a padded grid P (internal X,Y,Z), cell
+ * in metres: 6 x [P0][P1][P2] doubles (xx yy zz yz xz xy) at wrapped
+ * displacements, zero where FaceKernel6's loop leaves them (demag.go:35-53);
+ * kernel = MM_KERNEL_*, accuracy = face-charge points per edge (FaceKernel6).
+ * The values are in the internal unit system (the cell is scaled by the
+ * exchange length of msat/aexch, which a dimensionless kernel does not see). */
+void mm_demag_kernel(const int P[3], const double cellsize[3], int kernel, int accuracy, double* out);
+
+/* bytes of device memory held by the simulation (a kernel spectrum shared with
+ * other simulations of the same padded size, cell and kernel is counted by each) */
 unsigned long long mm_sim_device_bytes(mmSim* s);
+/* distinct kernel spectra held on the current device by live simulations */
+int mm_kernel_cache_size(void);
 
 /* Fixed-step fused mode only.  Enqueue n steps bracketed by HIP events on the
  * engine's stream; returns the elapsed ms.  The _profile variant also brackets
