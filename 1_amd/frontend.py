@@ -20,6 +20,9 @@ Modes (main.go, main_master.go, main_slave.go, main_python.go, daemon.go):
         process's command stream and getter replies go to its stdin
   python -m 1_amd.frontend --daemon [dirs]  run every *.in / *.py whose
         .out directory does not exist yet, each in its own process
+  python -m 1_amd.frontend --daemon --batch [dirs]  the same, but the .in
+        scripts found together that are set-up commands, one run and saves
+        run as one continuously batched workload (1_amd/serve.py)
 
 Errors end the process like the reference's refsh with CrashOnError: an
 unknown command exits with status 251 (-5), a wrong argument count with 255
@@ -322,6 +325,116 @@ def out_dir(infile):
     return os.path.splitext(infile)[0] + ".out"
 
 
+# commands that need the engine or the step loop: a script is served in a batch
+# only if none of them comes before its one `run`
+_STEPPING = {"run", "step", "steps", "relax", "hysteresis", "autosave", "save", "savem", "include", "getm",
+             "getmpos", "getmaxm", "getminm", "getmaxtorque", "getcorepos", "gete", "gettime"}
+
+
+def read_script(stream):
+    lines = []
+    line, eof = read_nonempty_line(stream)
+    while not eof:
+        if line:
+            lines.append(line)
+        line, eof = read_nonempty_line(stream)
+    return lines
+
+
+def batch_plan(lines):
+    """(set-up lines, run duration in s, save lines) when the script is set-up
+    commands, one `run` and then only `save` commands -- the shape the batched
+    server can run without stopping the batch -- else None."""
+    names = [ln[0].lower() for ln in lines]
+    if names.count("run") != 1:
+        return None
+    i = names.index("run")
+    if any(n in _STEPPING for n in names[:i]) or any(n != "save" for n in names[i + 1:]):
+        return None
+    if len(lines[i]) != 2:
+        return None
+    try:
+        duration = parse_arg(lines[i][1], "d")
+    except ShellExit:
+        return None
+    return lines[:i], duration, lines[i + 1:]
+
+
+def _finish_out(outdir):
+    running = os.path.join(outdir, "running")
+    if os.path.exists(running):
+        os.replace(running, os.path.join(outdir, "finished"))
+
+
+def _start_out(outdir):
+    os.makedirs(outdir, exist_ok=True)
+    with open(os.path.join(outdir, "running"), "w") as f:
+        f.write("This simulation was started on:\n %s\nThis file is renamed to \"finished\" when the simulation "
+                "is ready.\n" % time.ctime())
+
+
+def serve_scripts(infiles, mode="fused", silent=True, poll=16):
+    """Runs .in scripts of the batchable shape (batch_plan) together through the
+    continuous-batching server (serve.Server): all set-ups first, every `run`
+    advanced in the batches of its size group, each script's saves as soon as
+    its own run has finished.  Every script writes what it writes when run
+    alone (a simulation computes the same numbers alone and in any mix).
+    Returns {infile: exit status}; scripts of another shape are not run and
+    are missing from the result."""
+    from . import serve
+
+    reqs, status = [], {}
+    for k, f in enumerate(infiles):
+        with open(f) as fh:
+            plan = batch_plan(read_script(fh))
+        if plan is None:
+            continue
+        setup, duration, saves = plan
+        outdir = out_dir(f)
+        _start_out(outdir)
+        shell = Shell(outdir, reply=io_null(), log=None if silent else sys.stderr, mode=mode)
+        try:
+            for ln in setup:
+                shell.call(ln[0], ln[1:])
+            shell.sim.params()
+            if shell.sim._m0 is None:
+                raise InputErr("Initial magnetization was not set")
+        except (ShellExit, InputErr) as e:
+            sys.stderr.write("%s: %s\n" % (f, e))
+            status[f] = e.status if isinstance(e, ShellExit) else EXIT_INPUT
+            shell.close()
+            continue
+
+        def done(_request, shell=shell, saves=saves, f=f):
+            try:
+                shell.ran = True
+                for ln in saves:
+                    shell.call(ln[0], ln[1:])
+                status[f] = 0
+            except (ShellExit, InputErr) as e:
+                sys.stderr.write("%s: %s\n" % (f, e))
+                status[f] = e.status if isinstance(e, ShellExit) else EXIT_INPUT
+            shell.sim.close_table()
+            if status[f] == 0:
+                _finish_out(out_dir(f))
+
+        cfg = dict(size=shell.sim._size, duration=duration)
+        reqs.append(serve.Request(k, 0.0, cfg, make=lambda shell=shell: shell.sim, on_finish=done))
+    if reqs:
+        serve.Server(poll=poll).serve(reqs)
+    return status
+
+
+class io_null:
+    """A reply stream that drops what it is given (batched scripts have no getters)."""
+
+    def write(self, s):
+        return len(s)
+
+    def flush(self):
+        pass
+
+
 def run_script(infile, stream=None, reply=None, mode="fused", silent=False):
     """The slave's work for one input (main_slave.go:14-96): output directory
     infile.out with a "running" file renamed to "finished" at the end.
@@ -395,6 +508,12 @@ def run_python(infile, mode="fused", silent=False):
 def find_input(dirs, rand=False):
     """An input file with a known extension whose .out directory does not exist
     (daemon.go:110-160)."""
+    found = find_inputs(dirs, rand, first=True)
+    return found[0] if found else None
+
+
+def find_inputs(dirs, rand=False, first=False):
+    out = []
     for d in dirs:
         try:
             names = os.listdir(d)
@@ -408,15 +527,33 @@ def find_input(dirs, rand=False):
         for n in names:
             base, ext = os.path.splitext(n)
             if ext in KNOWN_EXTENSIONS and base + ".out" not in have:
-                return os.path.join(d, n)
-    return None
+                out.append(os.path.join(d, n))
+                if first:
+                    return out
+    return out
 
 
-def daemon(dirs, watch=60, walltime=0, rand=False, argv_extra=(), sleep=time.sleep):
+def _run_child(f, argv_extra):
+    print("[daemon] Starting simulation: %s" % f, file=sys.stderr)
+    cmd = [sys.executable, "-m", __package__ + ".frontend", *argv_extra, f]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ)
+    env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+    rc = subprocess.call(cmd, env=env)
+    print("[daemon] Finished simulation %s (status %d)" % (f, rc), file=sys.stderr)
+    return rc
+
+
+def daemon(dirs, watch=60, walltime=0, rand=False, argv_extra=(), sleep=time.sleep, batch=False, mode="fused"):
     """DaemonMain (daemon.go:29-100): claim an input by creating its .out
     directory, run it in a child process, repeat; with watch == 0 exit when no
     input is left, else look again every `watch` seconds; walltime > 0 hours
-    ends the daemon (a running simulation is not aborted)."""
+    ends the daemon (a running simulation is not aborted).
+
+    With batch, every input found in one look is claimed at once; the .in
+    scripts of the batchable shape (batch_plan) run together in this process
+    through the continuous-batching server, the others in child processes as
+    above."""
     start = time.time()
     dirs = list(dirs) or ["."]
     sleeping = False
@@ -424,7 +561,28 @@ def daemon(dirs, watch=60, walltime=0, rand=False, argv_extra=(), sleep=time.sle
         if walltime > 0 and time.time() - start > walltime * 3600:
             print("[daemon] Reached maximum walltime: exiting", file=sys.stderr)
             return 0
-        f = find_input(dirs, rand)
+        if batch:
+            claimed = []
+            for f in find_inputs(dirs, rand):
+                try:
+                    os.mkdir(out_dir(f))
+                    claimed.append(f)
+                except OSError as e:
+                    sys.stderr.write("%s\n" % e)
+            if claimed:
+                sleeping = False
+                ins = [f for f in claimed if f.endswith(".in")]
+                print("[daemon] Serving %d input(s) in batches" % len(ins), file=sys.stderr)
+                served = serve_scripts(ins, mode=mode)
+                for f, rc in sorted(served.items()):
+                    print("[daemon] Finished simulation %s (status %d)" % (f, rc), file=sys.stderr)
+                for f in claimed:
+                    if f not in served:
+                        _run_child(f, argv_extra)
+                continue
+            f = None
+        else:
+            f = find_input(dirs, rand)
         if f is None:
             if watch == 0:
                 print("[daemon] No new input files and -watch=0: exiting", file=sys.stderr)
@@ -440,13 +598,7 @@ def daemon(dirs, watch=60, walltime=0, rand=False, argv_extra=(), sleep=time.sle
         except OSError as e:
             sys.stderr.write("%s\n" % e)
             continue
-        print("[daemon] Starting simulation: %s" % f, file=sys.stderr)
-        cmd = [sys.executable, "-m", __package__ + ".frontend", *argv_extra, f]
-        root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-        env = dict(os.environ)
-        env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
-        rc = subprocess.call(cmd, env=env)
-        print("[daemon] Finished simulation %s (status %d)" % (f, rc), file=sys.stderr)
+        _run_child(f, argv_extra)
 
 
 def main(argv=None):
@@ -460,11 +612,13 @@ def main(argv=None):
     ap.add_argument("--watch", type=int, default=60)
     ap.add_argument("--walltime", type=int, default=0)
     ap.add_argument("--random", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="with --daemon: serve the inputs found together in batches (set-up, one run, saves)")
     ap.add_argument("--mode", choices=sorted(_sim.MODES), default="fused")
     a = ap.parse_args(argv)
     if a.daemon:
         extra = ["--mode", a.mode] + (["--silent"] if a.silent else [])
-        return daemon(a.files, a.watch, a.walltime, a.random, extra)
+        return daemon(a.files, a.watch, a.walltime, a.random, extra, batch=a.batch, mode=a.mode)
     if not a.files:
         sys.stderr.write("no input files\n")
         return 255

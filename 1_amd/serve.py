@@ -28,9 +28,14 @@ HBM_BYTES = 288 * 10 ** 9
 
 
 class Request:
-    __slots__ = ("rid", "arrival", "cfg", "duration", "sim", "start", "finish", "footprint")
+    """A simulation to run for cfg["duration"] seconds from `arrival` on.  The
+    simulation is built from the workload dict cfg, or by `make()` (an
+    un-initialised ``Sim``; cfg then needs only "size" and "duration").
+    `on_finish(request)` runs when it has finished, before it is freed."""
 
-    def __init__(self, rid, arrival, cfg):
+    __slots__ = ("rid", "arrival", "cfg", "duration", "sim", "start", "finish", "footprint", "make", "on_finish")
+
+    def __init__(self, rid, arrival, cfg, make=None, on_finish=None):
         self.rid = rid
         self.arrival = arrival
         self.cfg = cfg
@@ -39,6 +44,11 @@ class Request:
         self.start = None
         self.finish = None
         self.footprint = None
+        self.make = make
+        self.on_finish = on_finish
+
+    def build(self):
+        return self.make() if self.make is not None else wl.make_sim(self.cfg)
 
 
 def request_stream(n, rate=200.0, seed=5, min_cells=(16, 16, 1), max_cells=(512, 512, 4), dmin=0.1e-9,
@@ -78,9 +88,11 @@ def request_stream(n, rate=200.0, seed=5, min_cells=(16, 16, 1), max_cells=(512,
     return out
 
 
-def footprint(cfg):
-    """Device bytes the simulation of cfg allocates (mm_sim_footprint)."""
-    s = wl.make_sim(cfg)
+def footprint(cfg_or_request):
+    """Device bytes the simulation of a workload dict or a Request allocates
+    (mm_sim_footprint)."""
+    r = cfg_or_request
+    s = r.build() if isinstance(r, Request) else wl.make_sim(r)
     p = s.params()
     return int(s._lib.mm_sim_footprint(C.byref(p)))
 
@@ -123,7 +135,7 @@ class Server:
         finish) in seconds relative to the start, and loop statistics."""
         pending = sorted(requests, key=lambda r: r.arrival)
         for r in pending:
-            r.footprint = footprint(r.cfg)
+            r.footprint = footprint(r)
             if r.footprint > self.hbm_bytes:
                 raise InputErr("request %d needs %d bytes, more than the budget" % (r.rid, r.footprint))
         groups = [_Group() for _ in range(len(self.split_cells) + 1)]
@@ -140,7 +152,7 @@ class Server:
             while qi < len(pending) and pending[qi].arrival <= now and used + pending[qi].footprint <= self.hbm_bytes:
                 r = pending[qi]
                 qi += 1
-                r.sim = wl.make_sim(r.cfg)
+                r.sim = r.build()
                 r.sim._init()
                 r.sim._lib.mm_sim_begin_run(r.sim._ptr, r.duration)
                 r.start = clock() - t0
@@ -168,6 +180,8 @@ class Server:
                             r.finish = t
                             if self.keep_results:
                                 self.results[r.rid] = r.sim.m
+                            if r.on_finish is not None:
+                                r.on_finish(r)
                     # finished members are skipped by the batched kernels; the batch is
                     # rebuilt without them once they are half of it (or at an admission)
                     nfin = sum(r.finish is not None for r in g.members)
@@ -216,7 +230,7 @@ def serve_serial(requests, clock=time.perf_counter, keep_results=False):
         now = clock() - t0
         if r.arrival > now:
             time.sleep(r.arrival - now)
-        s = wl.make_sim(r.cfg)
+        s = r.build()
         s._init()
         r.start = clock() - t0
         s.run(r.duration)

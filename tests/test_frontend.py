@@ -214,6 +214,21 @@ def test_python_mode_setters_only(fe, tmp_path):
     assert fe.run_python(str(bad), silent=True) == fe.EXIT_PARSE
 
 
+def test_batch_plan(fe):
+    """Scripts of set-up commands, one run and then only saves are batchable."""
+    def plan(text):
+        return fe.batch_plan(fe.read_script(io.StringIO(text)))
+
+    p = plan("msat 8e5\ngridsize 8 8 1\nuniform 1 0 0\nrun 1e-9\nsave m binary\nsave table ascii\n")
+    assert p is not None and p[1] == 1e-9 and [ln[0] for ln in p[0]] == ["msat", "gridsize", "uniform"]
+    assert [ln[1] for ln in p[2]] == ["m", "table"]
+    assert plan("msat 8e5\nrun 1e-9\n")[2] == []
+    for bad in ("msat 8e5\n", "run 1e-9\nrun 1e-9\n", "autosave m binary 1e-12\nrun 1e-9\n",
+                "run 1e-9\ngetm 0\n", "save m binary\nrun 1e-9\n", "run 1e-9\nsteps 3\n", "run x\n",
+                "include a.in\nrun 1e-9\n"):
+        assert plan(bad) is None, bad
+
+
 def test_daemon_runs_new_inputs_once(tmp_path):
     (tmp_path / "a.in").write_text("msat 800e3\n")
     (tmp_path / "b.in").write_text("msat 800e3\n")
@@ -374,3 +389,40 @@ def test_relax_with_autosave_same_steps(simmod, tmp_path):
     assert len(t) >= 2 and all(np.diff(t) >= 2e-12 * (1 - 1e-6))
     print("relax: %d steps, %d table rows" % (na, len(t)))
     a.free(), b.free()
+
+
+@pytest.mark.gpu
+def test_daemon_batch_serves_scripts_like_serial(fe, tmp_path):
+    """The daemon's batch mode runs the batchable scripts together through the
+    continuous-batching server and writes, for each, what the script writes
+    when run alone; a script of another shape runs in a child process."""
+    def script(nx, ny, t, extra=""):
+        return ("msat 800e3\naexch 1.3e-11\nalpha 0.1\ngridsize %d %d 1\ncellsize 3.90625e-9 3.90625e-9 3e-9\n"
+                "solvertype rkdp\nmaxerror 1e-5\nuniform 1 0.25 0.1\nseed %d\naddnoise 0.02\n"
+                "applystatic field -24.6e-3 4.3e-3 0\n%srun %g\nsave m binary\nsave table ascii\n"
+                % (nx, ny, nx, extra, t))
+    texts = {"a": script(64, 32, 40e-12), "b": script(128, 48, 25e-12), "c": script(40, 24, 60e-12),
+             "d": script(32, 32, 10e-12, extra="tabulate e true\n")}
+    serial, batched = tmp_path / "serial", tmp_path / "batched"
+    serial.mkdir(), batched.mkdir()
+    for k, txt in texts.items():
+        (serial / (k + ".in")).write_text(txt)
+        (batched / (k + ".in")).write_text(txt)
+    (batched / "e.in").write_text(script(32, 16, 5e-12) + "getm 0\n")   # not batchable
+    for k in texts:
+        assert fe.run_script(str(serial / (k + ".in")), silent=True) == 0
+    r = subprocess.run([sys.executable, "-m", "1_amd.frontend", "--daemon", "--batch", "--watch", "0", "--silent",
+                        str(batched)], capture_output=True, text=True, env=_env(), timeout=300, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    assert "Serving 5 input(s) in batches" in r.stderr
+    ovf = import_module("1_amd.ovf")
+    for k in texts:
+        a, _ = ovf.read_ovf(serial / (k + ".out") / "m00000000.omf")
+        b, _ = ovf.read_ovf(batched / (k + ".out") / "m00000000.omf")
+        assert np.array_equal(a, b), k
+        ta = open(serial / (k + ".out") / "datatable.odt").read()
+        tb = open(batched / (k + ".out") / "datatable.odt").read()
+        assert ta == tb, k
+        assert (batched / (k + ".out") / "finished").exists()
+    assert (batched / "e.out" / "finished").exists()
+    assert "Starting simulation: %s" % (batched / "e.in") in r.stderr
