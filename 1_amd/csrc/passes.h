@@ -686,12 +686,27 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
 // transforms of length PX (Nx <= PX/2 nonzero inputs, Nx outputs kept) are
 // direct sums with the table exp(-2 pi i j / PX).
 constexpr int kYxyThreads = 256;
-constexpr int kYxyLdsBytes = 80 * 1024;  // two blocks per CU
-__host__ __device__ constexpr int yxy_lds_elems(int PX, int PY) {
-    return 3 * (PX / 2) * PY + (kYxyThreads / reg_fft_G(PY)) * (PY + PY / 8);
-}
+constexpr int kYxyLdsBytes = 80 * 1024;
+// Which plans take the plane pass: a plane of 3 (PX/2) lines plus separate
+// FFT exchange buffers within 80 KB (the sizes its registers were sized for).
 __host__ __device__ constexpr bool yxy_fits(int PX, int PY) {
-    return PX >= 4 && PX <= 16 && reg_fft_length(PY) && yxy_lds_elems(PX, PY) * 8 <= kYxyLdsBytes;
+    return PX >= 4 && PX <= 16 && reg_fft_length(PY) &&
+           (3 * (PX / 2) * PY + (kYxyThreads / reg_fft_G(PY)) * (PY + PY / 8)) * 8 <= kYxyLdsBytes;
+}
+// LDS of the pass: the plane's rows l = r LPB + li at l PY; the FFT exchange
+// buffers overlay the rows of the last round of lines (r = RMAX - 1), which are
+// written only after every lane group's last forward FFT and read back before
+// any inverse FFT (yxy_plane), so the two share memory: ~50 KB instead of
+// ~66 KB for PY = 512, PX = 8, three blocks per CU instead of two.
+__host__ __device__ constexpr int yxy_lds_elems(int PX, int PY) {
+    return (3 * (PX / 2) * PY) >
+                   ((3 * (PX / 2) + kYxyThreads / reg_fft_G(PY) - 1) / (kYxyThreads / reg_fft_G(PY)) - 1) *
+                           (kYxyThreads / reg_fft_G(PY)) * PY +
+                       (kYxyThreads / reg_fft_G(PY)) * (PY + PY / 8)
+               ? 3 * (PX / 2) * PY
+               : ((3 * (PX / 2) + kYxyThreads / reg_fft_G(PY) - 1) / (kYxyThreads / reg_fft_G(PY)) - 1) *
+                         (kYxyThreads / reg_fft_G(PY)) * PY +
+                     (kYxyThreads / reg_fft_G(PY)) * (PY + PY / 8);
 }
 template <int PY, int PX> constexpr int yxy_lds() { return yxy_lds_elems(PX, PY); }
 template <int PY, int PX> struct Yxy {
@@ -728,8 +743,9 @@ __device__ __forceinline__ void yxy_plane(float2 (*v)[RegFFT<PY>::E], float2* __
     using F = RegFFT<PY>;
     constexpr int E = Y::E, G = Y::G, NT = kYxyThreads, LPB = Y::LPB, RMAX = Y::RMAX;
     constexpr int hx = PX / 2 + 1, hy = PY / 2 + 1;
-    float2* plane = lds;                     // [3 Nx][PY]
-    float2* exch = lds + 3 * (PX / 2) * PY;  // LPB exchange buffers
+    float2* plane = lds;                                  // [3 Nx][PY]
+    float2* exch = lds + (long)(RMAX - 1) * LPB * PY;     // LPB exchange buffers over the last round's rows
+    static_assert((RMAX - 1) * LPB * PY + LPB * F::LDS <= yxy_lds<PY, PX>(), "LDS layout");
     const int li = threadIdx.x / G, g = threadIdx.x % G;
     const int nl = 3 * Nx;
     // Y forward
@@ -738,6 +754,8 @@ __device__ __forceinline__ void yxy_plane(float2 (*v)[RegFFT<PY>::E], float2* __
         const int l = r * LPB + li;
         if (r * LPB < nl) {  // uniform: some lane group has a line in this round
             reg_fft<float, PY, -1>(v[r], g, exch + li * F::LDS, twY);
+            // the last round's rows hold the exchange buffers until every lane group is through
+            if (r == RMAX - 1) __syncthreads();
             if (l < nl)
 #pragma unroll
                 for (int e = 0; e < E; e++) plane[(long)l * PY + g + G * e] = v[r][e];
@@ -818,15 +836,17 @@ __device__ __forceinline__ void yxy_plane(float2 (*v)[RegFFT<PY>::E], float2* __
         }
     }
     __syncthreads();
-    // Y inverse, first Ny kept
+    // Y inverse, first Ny kept; the last round first, whose rows the exchange
+    // buffers take over once every lane group has read them
 #pragma unroll
-    for (int r = 0; r < RMAX; r++) {
-        if (r * LPB >= nl) break;  // uniform
+    for (int r = RMAX - 1; r >= 0; r--) {
+        if (r * LPB >= nl) continue;  // uniform
         const int l = r * LPB + li;
         const bool ok = l < nl;
         float2 w[E];
 #pragma unroll
         for (int e = 0; e < E; e++) w[e] = ok ? plane[(long)l * PY + g + G * e] : make_float2(0.f, 0.f);
+        if (r == RMAX - 1) __syncthreads();
         reg_fft<float, PY, +1>(w, g, exch + li * F::LDS, twY);
         if (ok) {
             float2* dst = B1 + ((long)l * Q + kz) * Ny;

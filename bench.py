@@ -233,6 +233,7 @@ def bench_sweep(args, mm):
     cells = np.array([np.prod(c["size"]) for c in cfgs])
     for s in sims:
         s.free()
+    print("bench.py sweep: %d simulations batched in %.2f s" % (n, t_batch), file=sys.stderr, flush=True)
     # serial: an evenly spaced sample of members, each alone
     k = min(n, args.serial_sample)
     pick = np.unique(np.linspace(0, n - 1, k).astype(int))
@@ -290,13 +291,15 @@ def bench_serve(args, mm):
     n = len(reqs)
     srv = sv.Server(poll=args.poll, graph=not args.no_graph)
     stats = srv.serve(reqs, max_seconds=args.serve_max_seconds)
+    print("bench.py serve: %d requests served in batches in %.2f s" % (len(reqs), stats["elapsed"]), file=sys.stderr,
+          flush=True)
     lat = sv.latency_stats(reqs)
     rate = n / stats["elapsed"]
     work = np.array([np.prod(r.cfg["size"]) * r.duration for r in reqs])
     k = n if args.serve_serial <= 0 else min(n, args.serve_serial)
     pick = np.unique(np.linspace(0, n - 1, k).astype(int))
     service = {}
-    for j in pick:
+    for i, j in enumerate(pick):
         r = reqs[j]
         s = mm.workloads.make_sim(r.cfg)
         ta = time.perf_counter()
@@ -305,6 +308,8 @@ def bench_serve(args, mm):
         s.sync()
         service[j] = time.perf_counter() - ta
         s.free()
+        if (i + 1) % 32 == 0:
+            print("bench.py serve: %d of %d requests run alone" % (i + 1, len(pick)), file=sys.stderr, flush=True)
     svc = np.array([service[j] if j in service else service[min(service, key=lambda i: abs(np.log(work[i] / work[j])))]
                     for j in range(n)])
     t_free, fin = 0.0, np.zeros(n)
