@@ -344,7 +344,7 @@ template <int M> constexpr int epi_rows_c() {
     int p = 1;
     while (p * 2 <= tr) p *= 2;
     tr = p;
-    while (tr > 1 && 3 * (M + 1) * (tr + 1) * 8 > 96 * 1024) tr /= 2;
+    while (tr > 1 && zinv_epi_lds_elems(M, tr) * 8 > 80 * 1024) tr /= 2;
     return tr;
 }
 

@@ -625,8 +625,8 @@ static void launch_zinv(const ConvPlan& cp, float* h, const EngineState* st, hip
     GK_LAUNCH_CHECK();
 }
 
-// rows per block of the fused Z inverse: 3 x TR lane groups <= 1024 threads, the
-// 3 transposed tiles of (M+1) x (TR+1) complex within 96 KB of LDS
+// rows per block of the fused Z inverse: 3 x TR lane groups <= 1024 threads, its
+// LDS (transposed tiles, FFT exchange buffers) within 80 KB (two blocks per CU)
 template <int M> constexpr int epi_rows() {
     constexpr int G = RegFFT<M>::G;
     int tr = 1024 / (3 * G);
@@ -634,7 +634,7 @@ template <int M> constexpr int epi_rows() {
     int p = 1;
     while (p * 2 <= tr) p *= 2;
     tr = p;
-    while (tr > 1 && 3 * (M + 1) * (tr + 1) * 8 > 96 * 1024) tr /= 2;
+    while (tr > 1 && zinv_epi_lds_elems(M, tr) * 8 > 80 * 1024) tr /= 2;
     return tr;
 }
 

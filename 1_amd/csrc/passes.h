@@ -211,10 +211,14 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void k_zinv(const float2* __rest
 // h itself never leaves the chip.  NU bounds the number of earlier stages read
 // (the launcher points unused slots at m with zero coefficients, which leaves
 // every fmaf chain unchanged), so all loads are unconditional.
-template <int M, int TR> constexpr int zinv_epi_lds() {
-    constexpr int Q = M + 1, TS = TR + 1, HS = M / 2 > 0 ? M / 2 : 1;
-    constexpr int TILE = 3 * Q * TS, FFTB = 3 * TR * RegFFT<M>::LDS, HB = 3 * TR * HS;
+__host__ __device__ constexpr int zinv_epi_lds_elems(int M, int TR) {
+    const int Q = M + 1, TS = TR + 1, HS = M / 2 > 0 ? M / 2 : 1;
+    const int TILE = 3 * Q * TS, FFTB = 3 * TR * (M + M / 8), HB = 3 * TR * HS;  // RegFFT<M>::LDS = M + M/8
     return TILE > FFTB ? (TILE > HB ? TILE : HB) : (FFTB > HB ? FFTB : HB);
+}
+template <int M, int TR> constexpr int zinv_epi_lds() {
+    static_assert(RegFFT<M>::LDS == M + M / 8, "exchange buffer size");
+    return zinv_epi_lds_elems(M, TR);
 }
 
 // body: plane x, rows y0..y0+TR, all 3 components; the per-block max |k| goes to
