@@ -284,9 +284,20 @@ __global__ __launch_bounds__(64) void kb_ctrl_end(const BSim* sims) {
     const BSim& d = sims[blockIdx.x];
     ctrl_end_body(d.st, d.cp);
 }
-__global__ void kb_done(const BSim* sims, int n, int* done) {
+// done flags; with rem, each running simulation's remaining trials at its
+// current dt, (stop - time) / dt, or its remaining steps under a step target
+__global__ void kb_done(const BSim* sims, int n, int* done, float* rem) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) done[j] = sims[j].st->done;
+    if (j >= n) return;
+    const EngineState* st = sims[j].st;
+    done[j] = st->done;
+    if (rem) {
+        double r = 0.0;
+        if (!st->done)
+            r = st->stop < INFINITY ? (st->stop - st->time) / fmax((double)st->dt, 1e-30)
+                                    : (double)(st->steps_target - st->steps);
+        rem[j] = (float)fmin(r, 1e30);
+    }
 }
 
 }  // namespace gk
@@ -316,6 +327,7 @@ struct mmBatch {
     hipStream_t s = nullptr;
     BSim* dsims = nullptr;
     int* ddone = nullptr;
+    float* drem = nullptr;      // remaining trials per simulation (kb_done)
     std::vector<void*> owned;  // device allocations of the tables
     // per pass: one LaunchSet per length class
     std::vector<LaunchSet> zf, yf, xm, yi, y2, yx, zi, fin, all256;
@@ -324,6 +336,7 @@ struct mmBatch {
     hipGraphExec_t exec = nullptr;
     long long trials = 0;
     int* hdone = nullptr;       // pinned host copy of the done flags (mm_batch_enqueue)
+    float* hrem = nullptr;      // pinned host copy of drem
     hipEvent_t ev = nullptr;    // end of the last enqueued work
 };
 
@@ -638,6 +651,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
     b->dsims = (BSim*)balloc(b, n * sizeof(BSim));
     GK_CHECK(hipMemcpy(b->dsims, h.data(), n * sizeof(BSim), hipMemcpyHostToDevice));
     b->ddone = (int*)balloc(b, n * sizeof(int));
+    b->drem = (float*)balloc(b, n * sizeof(float));
     // launch sets per pass and length class
     std::vector<int> all(n), d3, d2, dt;
     for (int j = 0; j < n; j++) {
@@ -680,6 +694,7 @@ void mm_batch_free(mmBatch* b) {
     if (!b) return;
     if (b->s) (void)hipStreamSynchronize(b->s);
     if (b->hdone) (void)hipHostFree(b->hdone);
+    if (b->hrem) (void)hipHostFree(b->hrem);
     if (b->ev) (void)hipEventDestroy(b->ev);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -722,13 +737,15 @@ void mm_batch_enqueue(mmBatch* b, int n) {
     }
     b->trials += n;
     const int ns = (int)b->sims.size();
-    kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone);
+    kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone, b->drem);
     GK_LAUNCH_CHECK();
     if (!b->hdone) {
         GK_CHECK(hipHostMalloc(&b->hdone, ns * sizeof(int), hipHostMallocDefault));
+        GK_CHECK(hipHostMalloc(&b->hrem, ns * sizeof(float), hipHostMallocDefault));
         GK_CHECK(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
     }
     GK_CHECK(hipMemcpyAsync(b->hdone, b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));
+    GK_CHECK(hipMemcpyAsync(b->hrem, b->drem, ns * sizeof(float), hipMemcpyDeviceToHost, b->s));
     GK_CHECK(hipEventRecord(b->ev, b->s));
 }
 
@@ -752,6 +769,14 @@ int mm_batch_flags(mmBatch* b, int* done) {
     return active;
 }
 
+int mm_batch_remaining(mmBatch* b, float* rem) {
+    GK_ASSERT(b->ev != nullptr, "mm_batch_remaining before mm_batch_enqueue");
+    GK_CHECK(hipEventSynchronize(b->ev));
+    const int ns = (int)b->sims.size();
+    for (int i = 0; i < ns; i++) rem[i] = b->hrem[i];
+    return ns;
+}
+
 int mm_batch_trials(mmBatch* b, int n) {
     mm_batch_enqueue(b, n);
     return mm_batch_flags(b, nullptr);
@@ -760,7 +785,7 @@ int mm_batch_trials(mmBatch* b, int n) {
 // done flags of every simulation (1 = reached its stop time / step target)
 void mm_batch_done(mmBatch* b, int* done) {
     const int ns = (int)b->sims.size();
-    kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone);
+    kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone, nullptr);
     GK_LAUNCH_CHECK();
     GK_CHECK(hipMemcpyAsync(done, b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));
     GK_CHECK(hipStreamSynchronize(b->s));

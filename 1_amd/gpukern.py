@@ -100,6 +100,7 @@ ENGINE = {
     "mm_batch_enqueue": (None, [P, I]),
     "mm_batch_ready": (I, [P]),
     "mm_batch_flags": (I, [P, P]),
+    "mm_batch_remaining": (I, [P, P]),
     "mm_batch_done": (None, [P, P]),
     "mm_batch_run": (C.c_longlong, [P, P, I]),
 }

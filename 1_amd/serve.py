@@ -100,11 +100,24 @@ def footprint(cfg_or_request):
 class _Group:
     """Simulations advanced by one batch on its own stream."""
 
-    def __init__(self):
+    def __init__(self, lane=False):
         self.members = []   # requests whose simulations the batch references
         self.batch = None
         self.dirty = False  # rebuild before the next enqueue
         self.busy = False   # an enqueue is in flight
+        self.lane = lane    # holds the long-running members of a size class
+        self.polls = 0      # enqueues since the last rebuild
+
+
+def long_runners(rem, factor, floor):
+    """Indices of the members whose remaining trials exceed `factor` times the
+    median of the running members' and `floor` (at most half of them)."""
+    run = np.nonzero(rem > 0)[0]
+    if len(run) < 4:
+        return []
+    med = float(np.median(rem[run]))
+    pick = [int(i) for i in run if rem[i] > factor * med and rem[i] > floor]
+    return pick if len(pick) <= len(run) // 2 else []
 
 
 class Server:
@@ -116,14 +129,25 @@ class Server:
     enqueue, without waiting for the others: a batched trial costs about the
     work of its largest members, so light simulations that need many short
     steps are not held to the cadence of heavy ones, and their launches
-    overlap the heavy batch on the device."""
+    overlap the heavy batch on the device.
 
-    def __init__(self, hbm_bytes=HBM_BYTES, poll=16, graph=True, keep_results=False, split_cells=(16384, 131072)):
+    Every size class also has a lane for its long runners: after `lane_after`
+    enqueues of a group, members whose remaining trials at their current dt
+    (``mm_batch_remaining``) exceed `lane_factor` times the group's median
+    move to the class's lane, a batch of their own whose trials are cheap, so
+    the few simulations that need many more steps than the rest no longer set
+    the pace of everything else (lane_factor=None: no lanes)."""
+
+    def __init__(self, hbm_bytes=HBM_BYTES, poll=16, graph=True, keep_results=False, split_cells=(16384, 131072),
+                 lane_factor=3.0, lane_floor=1024, lane_after=4):
         self.hbm_bytes = hbm_bytes
         self.poll = poll
         self.graph = graph
         self.keep_results = keep_results
         self.split_cells = tuple(split_cells)
+        self.lane_factor = lane_factor
+        self.lane_floor = lane_floor
+        self.lane_after = lane_after
         self.results = {}
 
     def _group_of(self, r):
@@ -138,7 +162,9 @@ class Server:
             r.footprint = footprint(r)
             if r.footprint > self.hbm_bytes:
                 raise InputErr("request %d needs %d bytes, more than the budget" % (r.rid, r.footprint))
-        groups = [_Group() for _ in range(len(self.split_cells) + 1)]
+        nc = len(self.split_cells) + 1
+        groups = [_Group() for _ in range(nc)] + [_Group(lane=True) for _ in range(nc)]
+        moved = 0
         t0 = clock()
         used = 0
         rebuilds = trials = 0
@@ -187,6 +213,17 @@ class Server:
                     nfin = sum(r.finish is not None for r in g.members)
                     if 2 * nfin >= len(g.members):
                         g.dirty = True
+                    if self.lane_factor and not g.lane and g.polls >= self.lane_after:
+                        # the batch holds the first len(done) members (admissions append)
+                        rem = g.batch.remaining()
+                        rem[[r.finish is not None for r in g.members[:len(done)]]] = 0
+                        pick = set(long_runners(rem, self.lane_factor, self.lane_floor))
+                        if pick:
+                            lane = groups[nc + groups.index(g)]
+                            lane.members += [g.members[i] for i in sorted(pick)]
+                            g.members = [r for i, r in enumerate(g.members) if i not in pick]
+                            g.dirty = lane.dirty = True
+                            moved += len(pick)
                 if g.dirty:
                     if g.batch is not None:
                         g.batch.free()
@@ -204,9 +241,11 @@ class Server:
                         g.batch = Batch([r.sim for r in keep], graph=self.graph)
                         rebuilds += 1
                     g.dirty = False
+                    g.polls = 0
                 if g.batch is not None:
                     g.batch.enqueue(self.poll)
                     g.busy = True
+                    g.polls += 1
                     trials += self.poll
                     launched = True
             if not launched:
@@ -218,7 +257,7 @@ class Server:
                         time.sleep(min(wait, 0.01))
         t_end = clock() - t0
         return dict(elapsed=t_end, rebuilds=rebuilds, batched_trials=trials, max_active=max_active,
-                    max_bytes=max_used)
+                    max_bytes=max_used, moved_to_lanes=moved)
 
 
 def serve_serial(requests, clock=time.perf_counter, keep_results=False):

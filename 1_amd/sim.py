@@ -840,6 +840,13 @@ class Batch:
         self._lib.mm_batch_flags(self._ptr, out.ctypes.data)
         return out.astype(bool)
 
+    def remaining(self):
+        """Each member's remaining trials at its current dt after the last
+        enqueue (0 once done)."""
+        out = np.zeros(len(self.sims), np.float32)
+        self._lib.mm_batch_remaining(self._ptr, out.ctypes.data)
+        return out
+
     def run(self, seconds, poll=16):
         sec = self._seconds(seconds)
         return int(self._lib.mm_batch_run(self._ptr, sec.ctypes.data, int(poll)))

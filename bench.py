@@ -143,50 +143,75 @@ def dp_trial_bytes(size):
     return 6 * b_conv + 12 * N * 58
 
 
-def run_groups(mm, sims, groups, poll, graph, clock, t1, done_at):
+def run_groups(mm, sims, groups, poll, graph, clock, t1, done_at, lane_factor=3.0, lane_floor=1024, lane_after=4):
     """Advance simulations (already started) in size groups, each in its own
     batch on its own stream, `poll` trials per asynchronous enqueue, until all
     are done; a group's batch is rebuilt from its running members when half
-    have finished.  Fills done_at (seconds since t1); returns (batched trials,
-    rebuilds)."""
+    have finished.  After `lane_after` enqueues of a group, its members whose
+    remaining trials (``mm_batch_remaining``) exceed `lane_factor` times the
+    group's median move to that group's lane, a batch of their own (as
+    serve.Server does).  Fills done_at (seconds since t1); returns (batched
+    trials, rebuilds, members moved to lanes)."""
+    sv = importlib.import_module("1_amd.serve")
+
+    def group(members, lane=None):
+        # members: the group's simulations; bm: those its current batch holds
+        return {"members": np.asarray(members, dtype=int), "bm": None, "batch": None, "busy": False, "dirty": True,
+                "polls": 0, "lane": lane}
+
     state = []
     for members in groups:
-        if len(members):
-            state.append({"members": np.asarray(members), "batch": None, "busy": False, "dirty": True,
-                          "alive": True})
-    trials = rebuilds = 0
-    while state:
+        lane = group([]) if lane_factor else None
+        state.append(group(members, lane))
+        if lane is not None:
+            state.append(lane)
+    trials = rebuilds = moved = 0
+    while any(len(g["members"]) or g["busy"] for g in state):
         launched = False
-        for g in list(state):
+        for g in state:
             if g["busy"]:
                 if not g["batch"].ready():
                     continue
                 d = g["batch"].flags()
                 g["busy"] = False
-                fin = g["members"][d]
+                fin = g["bm"][d]
                 done_at[fin[np.isnan(done_at[fin])]] = clock() - t1
                 if 2 * int(d.sum()) >= len(d):
                     g["dirty"] = True
+                if g["lane"] is not None and g["polls"] >= lane_after:
+                    rem = g["batch"].remaining()
+                    rem[d] = 0
+                    pick = sv.long_runners(rem, lane_factor, lane_floor)
+                    if pick:
+                        lane = g["lane"]
+                        go = g["bm"][pick]
+                        lane["members"] = np.concatenate([lane["members"], go])
+                        g["members"] = g["members"][~np.isin(g["members"], go)]
+                        g["dirty"] = lane["dirty"] = True
+                        moved += len(pick)
             if g["dirty"]:
                 if g["batch"] is not None:
                     g["batch"].free()
                     g["batch"] = None
                 run = g["members"][np.isnan(done_at[g["members"]])]
+                g["members"] = g["bm"] = run
                 g["dirty"] = False
-                if not len(run):
-                    g["alive"] = False
-                    continue
-                g["members"] = run
-                g["batch"] = mm.Batch([sims[j] for j in run], graph=graph)
-                rebuilds += 1
-            g["batch"].enqueue(poll)
-            g["busy"] = True
-            trials += poll
-            launched = True
-        state = [g for g in state if g["alive"]]
-        if not launched and state:
+                g["polls"] = 0
+                if len(run):
+                    g["batch"] = mm.Batch([sims[j] for j in run], graph=graph)
+                    rebuilds += 1
+            if g["batch"] is not None and len(g["members"]):
+                g["batch"].enqueue(poll)
+                g["busy"] = True
+                g["polls"] += 1
+                trials += poll
+                launched = True
+        if not launched and any(len(g["members"]) or g["busy"] for g in state):
             time.sleep(0.0002)
-    return trials, rebuilds
+    for g in state:
+        if g["batch"] is not None:
+            g["batch"].free()
+    return trials, rebuilds, moved
 
 
 def bench_sweep(args, mm):
@@ -224,7 +249,8 @@ def bench_sweep(args, mm):
     bounds = (20480, 81920)
     grp = np.searchsorted(np.asarray(bounds), cells, side="left")
     groups = [np.nonzero((grp == k) & ~d)[0] for k in range(len(bounds) + 1)]
-    tr, rebuilds = run_groups(mm, sims, groups, args.poll, not args.no_graph, time.perf_counter, t1, done_at)
+    tr, rebuilds, moved = run_groups(mm, sims, groups, args.poll, not args.no_graph, time.perf_counter, t1, done_at,
+                                     lane_factor=None if args.no_lanes else args.lane_factor)
     trials += tr
     t_batch = time.perf_counter() - t1
     b.free()
@@ -264,6 +290,7 @@ def bench_sweep(args, mm):
         "batched_trials": trials,
         "batch_rebuilds": rebuilds,
         "groups_cells_upto": [20480, 81920, None],
+        "moved_to_lanes": moved,
         "full_batch_trial": {"ms": round(t_full * 1e3, 4), "model_bytes": int(full_bytes),
                              "hbm_frac": round(full_bytes / t_full / (HBM_PEAK_GBS * 1e9), 4)},
         "accepted_steps_total": int(steps.sum()),
@@ -289,7 +316,7 @@ def bench_serve(args, mm):
     sv = importlib.import_module("1_amd.serve")
     reqs = sv.request_stream(args.serve_requests, rate=args.serve_rate, seed=5)
     n = len(reqs)
-    srv = sv.Server(poll=args.poll, graph=not args.no_graph)
+    srv = sv.Server(poll=args.poll, graph=not args.no_graph, lane_factor=None if args.no_lanes else args.lane_factor)
     stats = srv.serve(reqs, max_seconds=args.serve_max_seconds)
     print("bench.py serve: %d requests served in batches in %.2f s" % (len(reqs), stats["elapsed"]), file=sys.stderr,
           flush=True)
@@ -334,6 +361,7 @@ def bench_serve(args, mm):
         "elapsed_s": round(stats["elapsed"], 3),
         "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4), "mean": round(lat["mean"], 4)},
         "batch_rebuilds": stats["rebuilds"],
+        "moved_to_lanes": stats["moved_to_lanes"],
         "max_concurrent": stats["max_active"],
         "max_device_bytes": stats["max_bytes"],
         "serial": {"measured": int(len(pick)), "of": n,
@@ -368,6 +396,10 @@ def main():
     ap.add_argument("--sweep-ns", type=float, default=0.5)
     ap.add_argument("--poll", type=int, default=16, help="batched trials between done-flag polls")
     ap.add_argument("--no-graph", action="store_true", help="batched trials as eager launches (no HIP graph)")
+    ap.add_argument("--no-lanes", action="store_true",
+                    help="keep long-running members in their size group (no lanes of their own)")
+    ap.add_argument("--lane-factor", type=float, default=3.0,
+                    help="members with more than this times their group's median remaining trials move to a lane")
     ap.add_argument("--serial-sample", type=int, default=32, help="sweep members timed alone for the serial rate")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch a streaming kernel of known traffic (for HBM counter scaling)")

@@ -183,6 +183,12 @@ int mm_batch_trials(mmBatch* b, int n);
 void mm_batch_enqueue(mmBatch* b, int n);
 int mm_batch_ready(mmBatch* b);
 int mm_batch_flags(mmBatch* b, int* done);
+/* After mm_batch_enqueue: each simulation's remaining trials at the dt it will
+ * try next, (stop - time) / dt (0 once done; remaining steps under a step
+ * target), as of the end of that work, into rem[n_sims]; waits like
+ * mm_batch_flags.  For schedulers that give long-running members their own
+ * batch.  Returns n_sims. */
+int mm_batch_remaining(mmBatch* b, float* rem);
 /* done[j] = 1 once simulation j reached its stop time / step target */
 void mm_batch_done(mmBatch* b, int* done);
 /* mm_batch_start(seconds) then trials until all are done (polling every

@@ -39,3 +39,26 @@ def test_admission_stays_within_the_hbm_budget(mm, sv):
     stats = srv.serve(reqs, max_seconds=300)
     assert stats["max_bytes"] <= budget
     assert all(r.finish is not None for r in reqs)
+
+
+def test_lanes_move_long_runners_and_keep_results(mm, sv):
+    """Members moved to their size class's lane (a batch of their own) mid-run
+    still compute what they compute alone."""
+    reqs = small_stream(sv, 12, seed=23)
+    srv = sv.Server(poll=2, keep_results=True, lane_factor=1.2, lane_floor=0, lane_after=1)
+    stats = srv.serve(reqs, max_seconds=300)
+    print("moved to lanes:", stats["moved_to_lanes"])
+    assert stats["moved_to_lanes"] > 0
+    for r in reqs:
+        s = mm.workloads.make_sim(r.cfg)
+        s.run(r.duration)
+        assert np.array_equal(s.m, srv.results[r.rid]), r.rid
+        s.free()
+
+
+def test_long_runners_rule(sv):
+    rem = np.array([0, 10, 12, 11, 100, 9, 0, 50], np.float32)
+    assert sv.long_runners(rem, 3.0, 0) == [4, 7]
+    assert sv.long_runners(rem, 3.0, 60) == [4]
+    assert sv.long_runners(np.array([5, 100, 0], np.float32), 3.0, 0) == []      # fewer than 4 running
+    assert sv.long_runners(np.array([1, 1, 9, 9, 9, 9], np.float32), 1.5, 0) == []  # more than half
