@@ -174,6 +174,7 @@ void launch_set_relax(EngineState* st, int on, hipStream_t s);
 // ---------------------------------------------------------------- RK kernels
 struct TorqueArgs {
     float alpha, dtg;       // damping, 1/(1+alpha^2)
+    const float* alphaMap;  // per-cell damping multiplier (null: none); dtg keeps the global alpha (torque.go:49-51)
     int anisType;
     float kAnis;            // 2K1/(mu0 Ms^2) (init.go:103-108)
     float u[3];
@@ -202,8 +203,12 @@ __device__ __forceinline__ void stage_field(const TorqueArgs& a, const EngineSta
 
 // LL torque of one cell: h = H + B_ext + uniaxial anisotropy of m
 // (gpu_local_contr3.cu), k = dtg (m x h - alpha m x (m x h)) (gpu_torque.cu:20-50).
-__device__ __forceinline__ void llg_torque_cell(const TorqueArgs& a, const float hext[3], float Mx, float My,
-                                                float Mz, float Hx, float Hy, float Hz, float k[3]) {
+// damping of cell i: alpha * alphaMap[i] (gpu_torque.cu's alpha_map)
+__device__ __forceinline__ float cell_alpha(const TorqueArgs& a, long i) {
+    return a.alphaMap ? a.alpha * a.alphaMap[i] : a.alpha;
+}
+__device__ __forceinline__ void llg_torque_cell(const TorqueArgs& a, float alpha, const float hext[3], float Mx,
+                                                float My, float Mz, float Hx, float Hy, float Hz, float k[3]) {
     Hx += hext[0];
     Hy += hext[1];
     Hz += hext[2];
@@ -215,9 +220,9 @@ __device__ __forceinline__ void llg_torque_cell(const TorqueArgs& a, const float
     }
     const float cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx;
     const float dx = My * cz - Mz * cy, dy = Mz * cx - Mx * cz, dz = Mx * cy - My * cx;
-    k[0] = a.dtg * (cx - a.alpha * dx);
-    k[1] = a.dtg * (cy - a.alpha * dy);
-    k[2] = a.dtg * (cz - a.alpha * dz);
+    k[0] = a.dtg * (cx - alpha * dx);
+    k[1] = a.dtg * (cy - alpha * dy);
+    k[2] = a.dtg * (cz - alpha * dz);
 }
 
 // Epilogue of the fused Z inverse pass (one RK stage):

@@ -292,7 +292,8 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
     const bool want_e = ea.epart != nullptr;
     // one cell from the accumulated stage input mi (and final sum hi, or the
     // error of the earlier stages er) and the field h
-    auto finish = [&](const float mi[3], const float hi[3], const double er[3], const float h[3], float o[3]) {
+    auto finish = [&](long ci, const float mi[3], const float hi[3], const double er[3], const float h[3],
+                      float o[3]) {
         if (want_e) {
             // phi = -1/2 m.H_conv - m.B_ext - 1/2 k (m.u)^2 (field.go:64-71, gpu_local_contr3.cu:238-256)
             const double mx = mi[0], my = mi[1], mz = mi[2];
@@ -305,7 +306,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
             esum += phi;
         }
         float k[3];
-        llg_torque_cell(ea.t, hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
+        llg_torque_cell(ea.t, cell_alpha(ea.t, ci), hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
         lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
         if constexpr (FIN == 2) {
 #pragma unroll
@@ -377,7 +378,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
                     h[q] = hv[q].x;
                     if constexpr (FIN == 2) er[q] = ex[it][q];
                 }
-                finish(mi, hi, er, h, o0);
+                finish(gi, mi, hi, er, h, o0);
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     mi[q] = ai[it][q].y;
@@ -385,7 +386,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
                     h[q] = hv[q].y;
                     if constexpr (FIN == 2) er[q] = ey[it][q];
                 }
-                finish(mi, hi, er, h, o1);
+                finish(gi + 1, mi, hi, er, h, o1);
 #pragma unroll
                 for (int q = 0; q < 3; q++) {
                     *reinterpret_cast<float2*>(ea.out + q * N + gi) = make_float2(o0[q], o1[q]);
@@ -414,7 +415,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
                     if constexpr (FIN == 2) er[q] = fma(ce[j], (double)kv, er[q]);
                 }
             }
-            finish(mi, hi, er, h, o);
+            finish(gi, mi, hi, er, h, o);
 #pragma unroll
             for (int q = 0; q < 3; q++) {
                 ea.out[q * N + gi] = o[q];

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, flo
     float lmax = 0.f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         float k[3];
-        llg_torque_cell(a, hext, m[i], m[N + i], m[2 * N + i], hk[i], hk[N + i], hk[2 * N + i], k);
+        llg_torque_cell(a, cell_alpha(a, i), hext, m[i], m[N + i], m[2 * N + i], hk[i], hk[N + i], hk[2 * N + i], k);
         hk[i] = k[0];
         hk[N + i] = k[1];
         hk[2 * N + i] = k[2];

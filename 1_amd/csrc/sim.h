@@ -73,6 +73,7 @@ struct mmSim {
     std::vector<void*> retired;      // replaced field tables, freed with the simulation
     double* fT = nullptr;
     float* fB = nullptr;
+    float* alphaMap = nullptr;       // device copy of mmParams.alphaMap (N floats)
     FieldTable field{};
     ConvPlan cp{};
     bool conv = false;

@@ -257,6 +257,7 @@ TorqueArgs gk::torque_args(const mmSim* s, float c, float* part) {
     TorqueArgs a{};
     a.alpha = (float)s->p.alpha;
     a.dtg = (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha));
+    a.alphaMap = s->alphaMap;
     a.anisType = s->p.anisType;
     a.kAnis = s->kAnis;
     for (int i = 0; i < 3; i++) a.u[i] = s->u[i];
@@ -411,7 +412,7 @@ static void abi_calc_heff(mmSim* s, float* m, float* h) {
 static void abi_torque(mmSim* s, float* m, float* h) {
     // torque.go:33-53 with dt = 1
     const float alpha = (float)s->p.alpha;
-    gpu_deltaM(m, h, alpha, nullptr, 1.f / (1.f + alpha * alpha), s->N);
+    gpu_deltaM(m, h, alpha, s->alphaMap, 1.f / (1.f + alpha * alpha), s->N);
 }
 
 static void abi_step(mmSim* s) {
@@ -602,6 +603,11 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     GK_CHECK(hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
     s->hdt = h.dt;
     upload_field(s);
+    if (p->alphaMap) {
+        s->alphaMap = (float*)dmalloc(s, (size_t)s->N * sizeof(float));
+        GK_CHECK(hipMemcpy(s->alphaMap, p->alphaMap, (size_t)s->N * sizeof(float), hipMemcpyHostToDevice));
+    }
+    s->p.alphaMap = nullptr;  // the caller's array is not kept
     // convolution (init.go:234-274)
     conv_plan_init(s->cp, p->size);
     s->bytes += s->cp.bytes;
@@ -629,7 +635,7 @@ void mm_sim_free(mmSim* s) {
     GK_CHECK(hipStreamSynchronize(s->s));
     for (int i = 0; i < 7; i++)
         if (s->k[i]) GK_CHECK(hipFree(s->k[i]));
-    for (float* p : {s->m, s->mi, s->mtmp, s->fB})
+    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap})
         if (p) GK_CHECK(hipFree(p));
     for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT, (void*)s->part,
                     (void*)s->epart})
@@ -778,7 +784,7 @@ void mm_sim_heff(mmSim* s, const float* m, float* h) {
 void mm_sim_torque(mmSim* s, const float* m, float* k) {
     eval_heff(s, m, s->mtmp);
     const float alpha = (float)s->p.alpha;
-    gpu_deltaM(s->mi, s->mtmp, alpha, nullptr, (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha)), s->N);
+    gpu_deltaM(s->mi, s->mtmp, alpha, s->alphaMap, (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha)), s->N);
     GK_CHECK(hipMemcpy(k, s->mtmp, s->N3 * sizeof(float), hipMemcpyDeviceToHost));
 }
 
@@ -878,6 +884,7 @@ unsigned long long mm_sim_footprint(const mmParams* p) {
     b += sizeof(EngineState) + (size_t)(1 << 16) * (sizeof(double) + sizeof(int)) + 4 * sizeof(double);
     b += 3 * (size_t)npart * sizeof(float) + (size_t)npart * sizeof(double);  // part, epart
     if (p->nField >= 2) b += (size_t)p->nField * (sizeof(double) + 3 * sizeof(float));
+    if (p->alphaMap) b += (size_t)(N3 / 3) * sizeof(float);
     return b;
 }
 

@@ -154,6 +154,7 @@ COMMANDS = {
     "gridsize": ("gridsize", "iii"), "cellsize": ("cellsize", "fff"), "partsize": ("partsize", "fff"),
     "demagaccuracy": ("demagaccuracy", "i"), "exchinconv": ("exchinconv", "b"), "exchtype": ("exchtype", "i"),
     "demag": ("_demag", "b"), "energy": ("_energy", "b"), "kerneltype": ("kerneltype", "s"),
+    "setalpha": ("setalpha", "iiif"), "setalpharange": ("setalpharange", "iiiiiif"),
     # excitation (simfield.go)
     "applystatic": ("applystatic", "sfff"), "applypointwise": ("applypointwise", "sdfff"),
     # solver (simsolver.go)
@@ -182,8 +183,7 @@ COMMANDS = {
 # implement; they stop the script with an InputErr naming the command.
 UNSUPPORTED = (
     "k2", "aniscubic", "spinpolarization", "xi", "temperature", "currentdensity", "currentmask", "fieldmask",
-    "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "setalpha",
-    "setalpharange", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
+    "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortexinarray", "vortex_in_array", "setvortexellips",
     "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrf", "applyrframp", "applypulse",
     "applyrotating", "applyrotatingburst", "applysawtooth", "edgecorrection", "saveh",

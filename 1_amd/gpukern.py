@@ -135,6 +135,7 @@ class Params(C.Structure):
         ("fieldB", C.POINTER(C.c_double)),
         ("wantEnergy", C.c_int),
         ("demagKernel", C.c_int),
+        ("alphaMap", C.POINTER(C.c_float)),
     ]
 
 

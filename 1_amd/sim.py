@@ -122,6 +122,7 @@ class Sim:
         self._demag_accuracy = 8
         self._want_energy = False
         self._kernel = 0
+        self._alpha_map = None
         self._m0 = None
         # output (sim.go:179-209): schedule, table columns, output id
         self._outdir = "."
@@ -257,6 +258,23 @@ class Sim:
         self._invalidate()
         if int(neighbours) != 6:
             raise InputErr("exchange type %d is not supported (only 6)" % neighbours)
+
+    def _alpha_mask(self):
+        # initAlphaMask (simgeom.go:226-240): a multiplier of 1 in every cell
+        self._invalidate()
+        if self._alpha_map is None:
+            self._alpha_map = np.ones(tuple(self._need_size()), np.float32)
+        return self._alpha_map
+
+    def setalpha(self, z, y, x, alpha):
+        """SetAlpha (simgeom.go:246-250): damping multiplier of the cell of user
+        indices (x, y, z) = the arguments' (z, y, x) order of the reference."""
+        self._alpha_mask()[x, y, z] = alpha
+
+    def setalpharange(self, x1, y1, z1, x2, y2, z2, alpha):
+        """SetAlphaRange (simgeom.go:253-264): multiplier over user x1..x2,
+        y1..y2, z1..z2 (inclusive)."""
+        self._alpha_mask()[z1:z2 + 1, y1:y2 + 1, x1:x2 + 1] = alpha
 
     def kerneltype(self, name):
         """KernelType (simfield.go): the demag kernel.  "facekernel6" (also the
@@ -754,6 +772,10 @@ class Sim:
         p.demagAccuracy = self._demag_accuracy
         p.wantEnergy = int(self._want_energy)
         p.demagKernel = self._kernel
+        if self._alpha_map is not None:
+            amap = np.ascontiguousarray(self._alpha_map, np.float32)
+            self._keep_alpha = amap
+            p.alphaMap = amap.ctypes.data_as(C.POINTER(C.c_float))
         p.solver = SOLVERS[self._solver]
         p.maxError, p.dt = self._maxerror, self._dt
         p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt

@@ -59,6 +59,10 @@ typedef struct {
     const double* fieldB;   /* nField*3 tesla, internal order */
     int wantEnergy;         /* energy at the FSAL stage (rk.go:323-337) */
     int demagKernel;        /* MM_KERNEL_* (0: FaceKernel6, as the reference) */
+    const float* alphaMap;  /* N host floats: per-cell damping multiplier (SetAlpha /
+                               SetAlphaRange, simgeom.go:226-264); null: uniform alpha.
+                               The torque's 1/(1+alpha^2) keeps the global alpha
+                               (torque.go:49-51). Copied at mm_sim_new. */
 } mmParams;
 
 typedef struct mmSim mmSim;

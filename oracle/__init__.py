@@ -104,6 +104,7 @@ class Oracle:
         L.or_sim_torque.argtypes = [P, P, P]
         L.or_sim_set_field.argtypes = [P, C.c_int, P, P]
         L.or_sim_set_want_energy.argtypes = [P, C.c_int]
+        L.or_sim_set_alpha_map.argtypes = [P, P]
         L.or_sim_trial_log.argtypes = [P, P, P, C.c_long]
         L.or_sim_trial_log.restype = C.c_long
         L.or_sim_energy.argtypes = [P]
@@ -338,6 +339,11 @@ class Sim:
         acc = np.zeros(max(n, 1), np.int32)
         self.orc.lib.or_sim_trial_log(self.ptr, dt.ctypes.data, acc.ctypes.data, n)
         return dt[:n], acc[:n]
+
+    def set_alpha_map(self, amap):
+        """per-cell damping multiplier (internal [X][Y][Z] order), None: none"""
+        self._amap = None if amap is None else np.ascontiguousarray(amap, np.float32).reshape(-1)
+        self.orc.lib.or_sim_set_alpha_map(self.ptr, None if self._amap is None else self._amap.ctypes.data)
 
     def set_field(self, t, b):
         """replace the applied field table (t: n times in s, b: n x 3 tesla)"""

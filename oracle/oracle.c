@@ -556,8 +556,15 @@ void or_local_fields_phi(const or_real* m, or_real* h, or_real* phi, int N, cons
 }
 
 void or_deltaM(const or_real* m, or_real* h, or_real alpha, or_real dtgilb, int N) {
+    or_deltaM_map(m, h, alpha, NULL, dtgilb, N);
+}
+
+/* gpu_torque.cu with alpha_map: the damping of cell i is alpha * amap[i];
+ * dtgilb stays that of the global alpha (torque.go:49-51) */
+void or_deltaM_map(const or_real* m, or_real* h, or_real alpha_mul, const or_real* amap, or_real dtgilb, int N) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < N; i++) {
+        const or_real alpha = amap ? alpha_mul * amap[i] : alpha_mul;
         or_real Mx = m[i], My = m[N + i], Mz = m[2 * N + i];
         or_real Hx = h[i], Hy = h[N + i], Hz = h[2 * N + i];
         or_real cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx; /* m x H */
@@ -674,6 +681,7 @@ struct or_sim {
     double* log_dt; /* per trial: dt used and accepted flag (the GPU engine's trial log) */
     int* log_acc;
     long log_n, log_cap;
+    or_real* amap; /* per-cell damping multiplier (SetAlpha, simgeom.go:246-264), NULL: none */
 };
 
 static void log_trial(or_sim* s, double dt, int acc) {
@@ -839,6 +847,7 @@ void or_sim_free(or_sim* s) {
     for (int i = 0; i < MAX_STAGES; i++) free(s->k[i]);
     free(s->fieldT); free(s->fieldB);
     free(s->log_dt); free(s->log_acc);
+    free(s->amap);
     free(s);
 }
 
@@ -868,7 +877,7 @@ static void calc_heff(or_sim* s, const or_real* m, or_real* h, int energy) {
 
 static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53,98-102 */
     double a = s->p.alpha;
-    or_deltaM(m, h, (or_real)a, (or_real)(1.0 / (1.0 + a * a)), s->N);
+    or_deltaM_map(m, h, (or_real)a, s->amap, (or_real)(1.0 / (1.0 + a * a)), s->N);
 }
 
 static void madd(or_real* a, double c, const or_real* b, size_t n) {
@@ -1035,6 +1044,15 @@ long or_sim_trial_log(const or_sim* s, double* dt, int* acc, long cap) {
 
 /* Sim.wantEnergy (relax.go:16): energy of each step, steps may not raise it */
 void or_sim_set_want_energy(or_sim* s, int on) { s->p.wantEnergy = on; }
+
+void or_sim_set_alpha_map(or_sim* s, const float* amap) {
+    free(s->amap);
+    s->amap = NULL;
+    if (!amap) return;
+    s->amap = (or_real*)malloc((size_t)s->N * sizeof(or_real));
+    if (!s->amap) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    for (int i = 0; i < s->N; i++) s->amap[i] = (or_real)amap[i];
+}
 
 double or_sim_energy(or_sim* s) {
     calc_heff(s, s->m, s->m1, 1);

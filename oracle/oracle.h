@@ -84,6 +84,7 @@ void or_local_fields(const or_real* m, or_real* h, int N, const or_real hext[3],
 void or_local_fields_phi(const or_real* m, or_real* h, or_real* phi, int N, const or_real hext[3],
                          int anisType, or_real k, const or_real axis[3]);
 void or_deltaM(const or_real* m, or_real* h, or_real alpha, or_real dtgilb, int N);
+void or_deltaM_map(const or_real* m, or_real* h, or_real alpha_mul, const or_real* amap, or_real dtgilb, int N);
 void or_normalize(or_real* m, int N);
 void or_scale_dot(or_real* r, const or_real* a, const or_real* b, or_real s, int N);
 double or_reduce(int op, const or_real* x, int n);
@@ -138,6 +139,8 @@ void or_set_tableau_f32(int on);
 long or_sim_trial_log(const or_sim* s, double* dt, int* acc, long cap);
 /* energy-controlled stepping on/off (relax.go:16, rk.go:424-426) */
 void or_sim_set_want_energy(or_sim* s, int on);
+/* per-cell damping multiplier, N floats (NULL: none); copied */
+void or_sim_set_alpha_map(or_sim* s, const float* amap);
 /* total energy (internal units * cell volume), energy.go:38-52 */
 double or_sim_energy(or_sim* s);
 

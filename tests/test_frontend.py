@@ -138,6 +138,11 @@ def test_initial_state_commands(fe, tmp_path):
     cy, cz = 2, 2
     assert np.all(m[0, :, cy, cz] == -1) and np.all(m[1:, :, cy, cz] == 0)
     assert m[1, 0, 0, 4] == 2 and m[2, 0, 0, 4] == 2     # (x - cx) * c, -(y - cy) * c
+    sh.call("setalpha", ["4", "1", "0", "2.5"])                   # cell user (x=4, y=1, z=0)
+    sh.call("setalpharange", ["0", "0", "1", "1", "3", "1", "0.5"])  # x 0..1, y 0..3, z = 1
+    am = sh.sim._alpha_map
+    assert am.shape == (2, 4, 5) and am[0, 1, 4] == 2.5 and np.all(am[1, :, :2] == 0.5)
+    assert (am == 1).sum() == am.size - 1 - 8
     sh.call("seed", ["7"])
     sh.call("setrandom", [])
     a = m.copy()

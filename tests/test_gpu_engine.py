@@ -496,3 +496,56 @@ def test_hysteresis_replayed_in_oracle(mm, orc, tmp_path):
     rows = [ln for ln in open(tmp_path / "datatable.odt").read().splitlines() if not ln.startswith("#")]
     assert len(rows) == 3
     assert float(rows[-1].split()[4]) == pytest.approx(20e-3, abs=1e-9)
+
+
+# ------------------------------------------------------------------ per-cell damping (SetAlpha)
+def _alpha_map(size, seed):
+    return np.random.default_rng(seed).uniform(0.3, 3.0, size).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", ["fused", "abi"])
+def test_alpha_map_torque_and_rk4_vs_oracle(mm, orc, mode):
+    """SetAlpha's damping multiplier (simgeom.go:246-264, torque.go:49-51):
+    the torque and 20 RK4 steps (the fused path applies it in the Z inverse
+    epilogue) against the oracle with the same map."""
+    size, cell = (2, 16, 32), (4e-9, 4e-9, 4e-9)
+    m0 = smooth_m(size, 5)
+    amap = _alpha_map(size, 6)
+    s = engine(mm, size, cell, mode=mode, solver="rk4", dt=2e-14, m0=m0, alpha=0.1, msat=1e6, aexch=1.5e-11)
+    s._alpha_map = amap.copy()
+    o = oracle(orc, size, cell, m0, solver="rk4", dt=2e-14, alpha=0.1, msat=1e6, aexch=1.5e-11)
+    o.set_alpha_map(amap)
+    k = s.torque(m0)
+    kref = o.torque(m0).reshape(k.shape)
+    assert np.abs(k - kref).max() < 5e-6 * np.abs(kref).max()
+    s.steps(20)
+    for _ in range(20):
+        o.step()
+    d = np.abs(s.m - o.m.reshape(s.m.shape)).max()
+    print(mode, "max |dm| after 20 RK4 steps with an alpha map:", d)
+    assert d < 1e-5
+    # the map changes the dynamics
+    p = engine(mm, size, cell, mode=mode, solver="rk4", dt=2e-14, m0=m0, alpha=0.1, msat=1e6, aexch=1.5e-11)
+    p.steps(20)
+    assert np.abs(p.m - s.m).max() > 1e-4
+
+
+def test_alpha_map_of_ones_is_no_map_and_batched_equals_alone(mm):
+    size, cell = (1, 32, 64), (3e-9, 3.90625e-9, 3.90625e-9)
+    m0 = random_m(size, 7)
+    a = engine(mm, size, cell, m0=m0, alpha=0.2)
+    b = engine(mm, size, cell, m0=m0, alpha=0.2)
+    b._alpha_map = np.ones(size, np.float32)
+    a.run(5e-12), b.run(5e-12)
+    assert np.array_equal(a.m, b.m)
+    # a member with a map computes the same in a batch as alone
+    alone = engine(mm, size, cell, m0=m0, alpha=0.2)
+    alone._alpha_map = _alpha_map(size, 8)
+    alone.run(8e-12)
+    sims = [engine(mm, size, cell, m0=m0, alpha=0.2), engine(mm, (2, 16, 24), (3e-9, 3e-9, 3e-9),
+                                                             m0=random_m((2, 16, 24), 9), alpha=0.1)]
+    sims[0]._alpha_map = _alpha_map(size, 8)
+    bt = mm.Batch(sims)
+    bt.run([8e-12, 5e-12])
+    assert np.array_equal(sims[0].m, alone.m)
+    bt.free()

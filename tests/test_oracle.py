@@ -242,6 +242,30 @@ def test_torque_closed_form(orc):
     assert np.allclose(np.linalg.norm(k, axis=0), np.linalg.norm(mxh, axis=0) / np.sqrt(1 + a * a))
 
 
+def test_sim_torque_with_alpha_map(orc):
+    """The simulation's torque with a per-cell damping multiplier: damping
+    alpha * map[i], prefactor 1/(1+alpha^2) of the global alpha (torque.go:49-51,
+    gpu_torque.cu's alpha_map); a map of ones is no map."""
+    size, cell = (2, 6, 8), (3e-9, 3e-9, 3e-9)
+    rng = np.random.default_rng(11)
+    m = orc.normalize(rng.standard_normal((3,) + size).reshape(3, -1)).reshape((3,) + size).astype(np.float32)
+    amap = rng.uniform(0.2, 3.0, size).astype(np.float32)
+    a = 0.3
+    s = orc.sim(m, size=size, cellsize=cell, msat=8e5, aexch=1.3e-11, alpha=a, solver="rk4", dt=1e-14)
+    k0 = s.torque(m).reshape(3, -1)
+    h = s.heff(m).reshape(3, -1)
+    s.set_alpha_map(np.ones(size, np.float32))
+    assert np.array_equal(s.torque(m).reshape(3, -1), k0)
+    s.set_alpha_map(amap)
+    k = s.torque(m).reshape(3, -1)
+    m2 = m.reshape(3, -1).astype(np.float64)
+    mxh = np.cross(m2.T, h.T).T
+    ref = (mxh - a * amap.reshape(-1).astype(np.float64) * np.cross(m2.T, mxh.T).T) / (1 + a * a)
+    assert np.allclose(k, ref, rtol=1e-12, atol=1e-14)
+    s.set_alpha_map(None)
+    assert np.array_equal(s.torque(m).reshape(3, -1), k0)
+
+
 def test_normalize(orc):
     m = np.random.default_rng(8).standard_normal((3, 40)) * 3
     assert np.allclose(np.linalg.norm(orc.normalize(m), axis=0), 1.0)
