@@ -53,7 +53,38 @@ struct FieldTable {
     const double* t;  // device
     const float* b;   // device, 3n
     float b0[3];      // static value (n == 1)
+    int fn;           // MM_FIELD_* function of time instead of the table (0: table)
+    double fp[4];     // its parameters
+    float fb[3];      // its amplitude (tesla, internal X, Y, Z)
 };
+
+// Applied-field functions of time, t in seconds (simfield.go:155-260): pulse
+// (fp: duration, rise time; raised-cosine edges), RF (freq), RF ramp (freq,
+// ramp time), sawtooth (freq), rotating (freq, phases of X, Y, Z).  The scale
+// is computed in double and rounded to float32, as the reference does.
+__host__ __device__ inline void field_function(int fn, const double fp[4], const float fb[3], double t,
+                                               float out[3]) {
+    const double pi = 3.14159265358979323846;
+    float s[3] = {0.f, 0.f, 0.f};
+    if (fn == MM_FIELD_PULSE) {
+        const double d = fp[0], r = fp[1];
+        double sc = 0.0;
+        if (t > 0 && t < r) sc = 0.5 - 0.5 * cos(t * pi / r);
+        else if (t >= r && t < d + r) sc = 1.0;
+        else if (t >= d + r && t < d + 2.0 * r) sc = 0.5 + 0.5 * cos((t - d - r) * pi / r);
+        s[0] = s[1] = s[2] = (float)sc;
+    } else if (fn == MM_FIELD_RF) {
+        s[0] = s[1] = s[2] = (float)sin(fp[0] * 2 * pi * t);
+    } else if (fn == MM_FIELD_RF_RAMP) {
+        const double fac = t < fp[1] ? t / fp[1] : 1.0;
+        s[0] = s[1] = s[2] = (float)(sin(fp[0] * 2 * pi * t) * fac);
+    } else if (fn == MM_FIELD_SAWTOOTH) {
+        s[0] = s[1] = s[2] = (float)(asin(sin(fp[0] * 2 * pi * t)) / (pi / 2));
+    } else if (fn == MM_FIELD_ROTATING) {
+        for (int c = 0; c < 3; c++) s[c] = (float)sin(fp[0] * 2 * pi * t + fp[1 + c]);
+    }
+    for (int c = 0; c < 3; c++) out[c] = s[c] * fb[c];
+}
 
 // The reference's pointwise field keeps its search index and only moves it
 // forward (simfield.go:126-133): the interval used at time t is the one ending
@@ -79,6 +110,10 @@ __host__ __device__ inline void applied_field_at(int n, const double* T, const f
 }
 
 __device__ __forceinline__ void applied_field(const FieldTable& f, double t_si, double t_search_si, float out[3]) {
+    if (f.fn) {
+        field_function(f.fn, f.fp, f.fb, t_si, out);
+        return;
+    }
     applied_field_at(f.n, f.t, f.b, f.b0, t_si, t_search_si, out);
 }
 

@@ -212,6 +212,10 @@ void* gk::dmalloc(mmSim* s, size_t bytes) {
 // applied field at internal time t; t_search (internal) is the latest time the
 // field was evaluated at, t included (the forward-only search of simfield.go:126-133)
 static void applied_field_host(const mmSim* s, double t, double t_search, float out[3]) {
+    if (s->field.fn) {
+        field_function(s->field.fn, s->field.fp, s->field.fb, t * s->unitTime, out);
+        return;
+    }
     const float* b = s->fieldB.data();
     applied_field_at(s->p.nField, s->fieldT.data(), b, b, t * s->unitTime, t_search * s->unitTime, out);
 }
@@ -903,6 +907,18 @@ void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b) {
     s->fieldB.resize(3 * n);
     for (int i = 0; i < 3 * n; i++) s->fieldB[i] = (float)b[i];
     upload_field(s);
+}
+
+void mm_sim_set_field_function(mmSim* s, int kind, const double* b, const double* params) {
+    GK_ASSERT(kind >= MM_FIELD_PULSE && kind <= MM_FIELD_ROTATING, "unknown field function %d", kind);
+    GK_CHECK(hipStreamSynchronize(s->s));
+    s->p.nField = 0;
+    s->fieldT.clear();
+    s->fieldB.clear();
+    upload_field(s);  // retires a table a batch may still read
+    s->field.fn = kind;
+    for (int i = 0; i < 4; i++) s->field.fp[i] = params[i];
+    for (int c = 0; c < 3; c++) s->field.fb[c] = (float)b[c];
 }
 
 long mm_sim_relax(mmSim* s, long max_steps) {

@@ -157,6 +157,8 @@ COMMANDS = {
     "setalpha": ("setalpha", "iiif"), "setalpharange": ("setalpharange", "iiiiiif"),
     # excitation (simfield.go)
     "applystatic": ("applystatic", "sfff"), "applypointwise": ("applypointwise", "sdfff"),
+    "applypulse": ("applypulse", "sfffdd"), "applyrf": ("applyrf", "sfffd"), "applyrframp": ("applyrframp", "sfffdd"),
+    "applysawtooth": ("applysawtooth", "sfffd"), "applyrotating": ("applyrotating", "sfffdddd"),
     # solver (simsolver.go)
     "solvertype": ("solvertype", "s"), "maxerror": ("maxerror", "f"), "dt": ("dt", "f"),
     "mindm": ("mindm", "f"), "maxdm": ("maxdm", "f"), "mindt": ("mindt", "f"), "maxdt": ("maxdt", "f"),
@@ -185,8 +187,7 @@ UNSUPPORTED = (
     "k2", "aniscubic", "spinpolarization", "xi", "temperature", "currentdensity", "currentmask", "fieldmask",
     "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortexinarray", "vortex_in_array", "setvortexellips",
-    "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrf", "applyrframp", "applypulse",
-    "applyrotating", "applyrotatingburst", "applysawtooth", "edgecorrection", "saveh",
+    "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrotatingburst", "edgecorrection", "saveh",
     "savebenchmark", "exchangetype",
 )
 

@@ -91,6 +91,7 @@ ENGINE = {
     "mm_demag_kernel": (None, [P, P, I, I, P]),
     "mm_kernel_cache_size": (I, []),
     "mm_sim_set_field": (None, [P, I, P, P]),
+    "mm_sim_set_field_function": (None, [P, I, P, P]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
     "mm_batch_set_graph": (None, [P, I]),

@@ -28,6 +28,31 @@ class InputErr(ValueError):
 TABULATE = ("time", "m", "b", "j", "id", "e", "maxdm/dt", "minmaxmz", "corepos")
 
 
+def field_function(kind, params, b, t):
+    """Value (tesla, internal X, Y, Z) of a field function of time at t seconds,
+    as the engine evaluates it (simfield.go:155-260; scale rounded to float32)."""
+    f, pi = params[0], np.pi
+    if kind == 1:
+        d, r = params[0], params[1]
+        sc = 0.0
+        if 0 < t < r:
+            sc = 0.5 - 0.5 * np.cos(t * pi / r)
+        elif r <= t < d + r:
+            sc = 1.0
+        elif d + r <= t < d + 2.0 * r:
+            sc = 0.5 + 0.5 * np.cos((t - d - r) * pi / r)
+        s = [sc] * 3
+    elif kind == 2:
+        s = [np.sin(f * 2 * pi * t)] * 3
+    elif kind == 3:
+        s = [np.sin(f * 2 * pi * t) * (t / params[1] if t < params[1] else 1.0)] * 3
+    elif kind == 4:
+        s = [np.arcsin(np.sin(f * 2 * pi * t)) / (pi / 2)] * 3
+    else:
+        s = [np.sin(f * 2 * pi * t + params[1 + c]) for c in range(3)]
+    return np.array([np.float32(s[c]) * np.float32(b[c]) for c in range(3)], np.float64)
+
+
 def _component(c):
     c = int(c)
     if c not in (0, 1, 2):
@@ -114,6 +139,7 @@ class Sim:
         self._axis = None
         self._field = []  # list of (t, [bX, bY, bZ]); static: single entry with t=None
         self._static = None
+        self._fn = None   # (kind, (bX, bY, bZ), params): a field function of time
         self._solver = "rk12"       # DEFAULT_SOLVERTYPE (sim.go:24)
         self._maxerror = 1e-4       # DEFAULT_MAXERROR (sim.go:25)
         self._dt = 0.0
@@ -190,7 +216,37 @@ class Sim:
             raise InputErr("only 'field' can be applied")
         self._static = (hx, hy, hz)
         self._field = []
+        self._fn = None
         self._push_field()
+
+    # functions of time (simfield.go:155-260); amplitude in the reference's (hz, hy, hx) order
+    def _apply_fn(self, what, kind, hz, hy, hx, params):
+        if what.lower() not in ("field", "b"):
+            raise InputErr("only 'field' can be applied")
+        self._static = None
+        self._field = []
+        self._fn = (kind, (float(hx), float(hy), float(hz)), tuple(float(p) for p in params) + (0.0,) * (4 - len(params)))
+        self._push_field()
+
+    def applypulse(self, what, hz, hy, hx, duration, risetime):
+        """ApplyPulse: raised-cosine rise and fall of `risetime`, `duration` at full amplitude."""
+        self._apply_fn(what, 1, hz, hy, hx, [duration, risetime])
+
+    def applyrf(self, what, hz, hy, hx, freq):
+        """ApplyRf: B sin(2 pi f t)."""
+        self._apply_fn(what, 2, hz, hy, hx, [freq])
+
+    def applyrframp(self, what, hz, hy, hx, freq, ramptime):
+        """ApplyRfRamp: B sin(2 pi f t) min(t / ramptime, 1)."""
+        self._apply_fn(what, 3, hz, hy, hx, [freq, ramptime])
+
+    def applysawtooth(self, what, hz, hy, hx, freq):
+        """ApplySawtooth: B asin(sin(2 pi f t)) / (pi/2)."""
+        self._apply_fn(what, 4, hz, hy, hx, [freq])
+
+    def applyrotating(self, what, hz, hy, hx, freq, phasex, phasey, phasez):
+        """ApplyRotating: component c is B_c sin(2 pi f t + phase_c)."""
+        self._apply_fn(what, 5, hz, hy, hx, [freq, phasex, phasey, phasez])
 
     def applypointwise(self, what, t, hz, hy, hx):
         if what.lower() not in ("field", "b"):
@@ -198,6 +254,7 @@ class Sim:
         if self._static is not None:
             self._field = []
         self._static = None
+        self._fn = None
         self._field.append((float(t), (hx, hy, hz)))
         self._push_field()
 
@@ -213,6 +270,11 @@ class Sim:
     def _push_field(self):
         # a pointwise field needs two points before it can be evaluated
         # (simfield.go:123-125); the engine keeps the previous field until then
+        if self._ptr and self._fn is not None:
+            kind, b, prm = self._fn
+            b, prm = np.ascontiguousarray(b, np.float64), np.ascontiguousarray(prm, np.float64)
+            self._lib.mm_sim_set_field_function(self._ptr, kind, b.ctypes.data, prm.ctypes.data)
+            return
         if self._ptr and (self._static is not None or len(self._field) != 1):
             t, b = self._field_table()
             n = 0 if t is None else len(t)
@@ -369,7 +431,9 @@ class Sim:
 
     # ------------------------------------------------------------ output (simoutput.go)
     def field_at(self, t):
-        """Applied field in tesla (internal X, Y, Z) at time t (simfield.go:87-153)."""
+        """Applied field in tesla (internal X, Y, Z) at time t (simfield.go:87-260)."""
+        if self._fn is not None:
+            return field_function(self._fn[0], self._fn[2], self._fn[1], t)
         if self._static is not None:
             return np.array(self._static, float)
         if not self._field:
@@ -799,6 +863,8 @@ class Sim:
         self._ptr = self._lib.mm_sim_new(C.byref(p), m0.ctypes.data, MODES[self.mode])
         if not self._ptr:
             raise RuntimeError("mm_sim_new failed")
+        if self._fn is not None:
+            self._push_field()
 
 
 class Batch:

@@ -34,6 +34,8 @@ enum { MM_ANIS_NONE = 0, MM_ANIS_UNIAXIAL = 1 };
 /* demag kernel: the reference's FaceKernel6 (demag.go:26-84) or the Newell
  * tensor (cell-averaged field of a uniformly magnetised cell, FP64) */
 enum { MM_KERNEL_FACE6 = 0, MM_KERNEL_NEWELL = 1 };
+/* applied-field functions of time (simfield.go:155-260) */
+enum { MM_FIELD_PULSE = 1, MM_FIELD_RF = 2, MM_FIELD_RF_RAMP = 3, MM_FIELD_SAWTOOTH = 4, MM_FIELD_ROTATING = 5 };
 /* solver codes: rk.go tableaux (solver.go:16-53) */
 enum { MM_RK2 = 2, MM_RK12 = 12, MM_RK3 = 3, MM_RK23 = 23, MM_RK4 = 4, MM_RKCK = 45, MM_RKDP = 54 };
 
@@ -152,6 +154,12 @@ void mm_sim_begin_run(mmSim* s, double seconds);
  * none).  Takes effect at the next trial; a batch built before keeps the
  * table it copied (the old table stays allocated until mm_sim_free). */
 void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b);
+/* Replace the applied field by a function of time (MM_FIELD_*), amplitude b[3]
+ * in tesla (internal X, Y, Z), params[4]: PULSE (duration s, rise time s),
+ * RF (frequency Hz), RF_RAMP (frequency, ramp time), SAWTOOTH (frequency),
+ * ROTATING (frequency, phase X, Y, Z rad) -- ApplyPulse, ApplyRf, ApplyRfRamp,
+ * ApplySawtooth, ApplyRotating (simfield.go:155-260). */
+void mm_sim_set_field_function(mmSim* s, int kind, const double* b, const double* params);
 /* Relax() (relax.go:14-25): wantEnergy on (trials that raise the energy are
  * rejected, rk.go:424-426), steps until a step raises the energy, or until
  * max_steps steps when max_steps > 0 (the reference has no bound); returns

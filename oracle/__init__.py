@@ -105,6 +105,8 @@ class Oracle:
         L.or_sim_set_field.argtypes = [P, C.c_int, P, P]
         L.or_sim_set_want_energy.argtypes = [P, C.c_int]
         L.or_sim_set_alpha_map.argtypes = [P, P]
+        L.or_sim_set_field_function.argtypes = [P, C.c_int, P, P]
+        L.or_sim_field_at.argtypes = [P, C.c_double, P]
         L.or_sim_trial_log.argtypes = [P, P, P, C.c_long]
         L.or_sim_trial_log.restype = C.c_long
         L.or_sim_energy.argtypes = [P]
@@ -339,6 +341,19 @@ class Sim:
         acc = np.zeros(max(n, 1), np.int32)
         self.orc.lib.or_sim_trial_log(self.ptr, dt.ctypes.data, acc.ctypes.data, n)
         return dt[:n], acc[:n]
+
+    def set_field_function(self, kind, b, params):
+        """applied field as a function of time: kind 1 pulse, 2 RF, 3 RF ramp,
+        4 sawtooth, 5 rotating; b internal (X, Y, Z) tesla; params[4]"""
+        b = np.ascontiguousarray(b, np.float64)
+        p = np.zeros(4)
+        p[: len(params)] = params
+        self.orc.lib.or_sim_set_field_function(self.ptr, int(kind), b.ctypes.data, p.ctypes.data)
+
+    def field_at(self, t):
+        out = np.zeros(3)
+        self.orc.lib.or_sim_field_at(self.ptr, float(t), out.ctypes.data)
+        return out
 
     def set_alpha_map(self, amap):
         """per-cell damping multiplier (internal [X][Y][Z] order), None: none"""

@@ -682,6 +682,8 @@ struct or_sim {
     int* log_acc;
     long log_n, log_cap;
     or_real* amap; /* per-cell damping multiplier (SetAlpha, simgeom.go:246-264), NULL: none */
+    int fn;        /* applied-field function of time (1 pulse, 2 RF, 3 RF ramp, 4 sawtooth, 5 rotating), 0: table */
+    double fp[4], fb[3];
 };
 
 static void log_trial(or_sim* s, double dt, int acc) {
@@ -785,8 +787,33 @@ static void set_tableau(or_sim* s) {
 }
 
 /* applied field in tesla at SI time t (simfield.go:87-153) */
+/* pulsedField, rfField, rfRamp, sawtooth, rotatingField .GetAppliedField
+ * (simfield.go:155-260): scale in double, rounded to float32, times the
+ * float32 amplitude */
+static void field_function(const or_sim* s, double t, double B[3]) {
+    const double pi = 3.14159265358979323846, f = s->fp[0];
+    double sc[3] = {0, 0, 0};
+    switch (s->fn) {
+    case 1: {
+        const double d = s->fp[0], r = s->fp[1];
+        double v = 0;
+        if (t > 0 && t < r) v = 0.5 - 0.5 * cos(t * pi / r);
+        else if (t >= r && t < d + r) v = 1;
+        else if (t >= d + r && t < d + 2 * r) v = 0.5 + 0.5 * cos((t - d - r) * pi / r);
+        sc[0] = sc[1] = sc[2] = v;
+        break;
+    }
+    case 2: sc[0] = sc[1] = sc[2] = sin(f * 2 * pi * t); break;
+    case 3: sc[0] = sc[1] = sc[2] = sin(f * 2 * pi * t) * (t < s->fp[1] ? t / s->fp[1] : 1.0); break;
+    case 4: sc[0] = sc[1] = sc[2] = asin(sin(f * 2 * pi * t)) / (pi / 2); break;
+    case 5: for (int c = 0; c < 3; c++) sc[c] = sin(f * 2 * pi * t + s->fp[1 + c]); break;
+    }
+    for (int c = 0; c < 3; c++) B[c] = (double)((float)sc[c] * (float)s->fb[c]);
+}
+
 static void applied_field(or_sim* s, double t, double B[3]) {
     B[0] = B[1] = B[2] = 0;
+    if (s->fn) { field_function(s, t, B); return; }
     if (s->p.nField == 0) return;
     if (s->p.nField == 1) { for (int i = 0; i < 3; i++) B[i] = s->fieldB[i]; return; }
     for (; s->fieldLastIdx < s->p.nField; s->fieldLastIdx++)
@@ -1044,6 +1071,14 @@ long or_sim_trial_log(const or_sim* s, double* dt, int* acc, long cap) {
 
 /* Sim.wantEnergy (relax.go:16): energy of each step, steps may not raise it */
 void or_sim_set_want_energy(or_sim* s, int on) { s->p.wantEnergy = on; }
+
+void or_sim_set_field_function(or_sim* s, int kind, const double* b, const double* params) {
+    s->fn = kind;
+    for (int i = 0; i < 4; i++) s->fp[i] = params[i];
+    for (int c = 0; c < 3; c++) s->fb[c] = b[c];
+}
+
+void or_sim_field_at(or_sim* s, double t_si, double* out) { applied_field(s, t_si, out); }
 
 void or_sim_set_alpha_map(or_sim* s, const float* amap) {
     free(s->amap);

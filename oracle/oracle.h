@@ -139,6 +139,11 @@ void or_set_tableau_f32(int on);
 long or_sim_trial_log(const or_sim* s, double* dt, int* acc, long cap);
 /* energy-controlled stepping on/off (relax.go:16, rk.go:424-426) */
 void or_sim_set_want_energy(or_sim* s, int on);
+/* applied field as a function of time (1 pulse, 2 RF, 3 RF ramp, 4 sawtooth,
+ * 5 rotating; params as mm_sim_set_field_function), replacing the table */
+void or_sim_set_field_function(or_sim* s, int kind, const double* b, const double* params);
+/* the applied field (tesla, internal order) at t seconds */
+void or_sim_field_at(or_sim* s, double t_si, double* out);
 /* per-cell damping multiplier, N floats (NULL: none); copied */
 void or_sim_set_alpha_map(or_sim* s, const float* amap);
 /* total energy (internal units * cell volume), energy.go:38-52 */

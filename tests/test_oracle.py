@@ -266,6 +266,32 @@ def test_sim_torque_with_alpha_map(orc):
     assert np.array_equal(s.torque(m).reshape(3, -1), k0)
 
 
+def test_field_functions_special_values(orc):
+    """ApplyPulse / ApplyRf / ApplyRfRamp / ApplySawtooth / ApplyRotating
+    (simfield.go:155-260) at points where each formula has a known value."""
+    size, cell = (1, 4, 4), (3e-9, 3e-9, 3e-9)
+    m = np.zeros((3,) + size, np.float32)
+    m[2] = 1
+    s = orc.sim(m, size=size, cellsize=cell, msat=8e5, aexch=1.3e-11, alpha=0.1, solver="rk4", dt=1e-14)
+    b = np.array([0.25, -0.5, 0.125])            # exact in float32
+    f = 2e9
+    s.set_field_function(2, b, [f])              # RF: sin(2 pi f t)
+    assert np.allclose(s.field_at(1 / (4 * f)), b) and np.allclose(s.field_at(3 / (4 * f)), -b)
+    assert np.abs(s.field_at(1 / f)).max() < 1e-7
+    s.set_field_function(4, b, [f])              # sawtooth: a triangle wave of amplitude 1
+    assert np.allclose(s.field_at(1 / (4 * f)), b) and np.allclose(s.field_at(1 / (8 * f)), 0.5 * b, rtol=1e-6)
+    assert np.allclose(s.field_at(5 / (8 * f)), -0.5 * b, rtol=1e-6)
+    s.set_field_function(3, b, [f, 1e-9])        # RF ramp: linear envelope up to 1 ns
+    assert np.allclose(s.field_at(1 / (4 * f)), b * (1 / (4 * f)) / 1e-9, rtol=1e-6)
+    assert np.allclose(s.field_at(2e-9 + 1 / (4 * f)), b)
+    s.set_field_function(1, b, [2e-9, 1e-9])     # pulse: 1 ns cosine rise, 2 ns flat, 1 ns fall
+    for t, v in ((-1e-9, 0.0), (0.5e-9, 0.5), (1e-9, 1.0), (2.9e-9, 1.0), (3.5e-9, 0.5), (4.2e-9, 0.0)):
+        assert np.allclose(s.field_at(t), v * b, atol=1e-7), t
+    s.set_field_function(5, b, [f, 0.0, np.pi / 2, np.pi])   # rotating, per-component phase
+    assert np.allclose(s.field_at(0.0), [0.0, b[1], 0.0], atol=1e-7)
+    assert np.allclose(s.field_at(1 / (4 * f)), [b[0], 0.0, -b[2]], atol=1e-7)
+
+
 def test_normalize(orc):
     m = np.random.default_rng(8).standard_normal((3, 40)) * 3
     assert np.allclose(np.linalg.norm(orc.normalize(m), axis=0), 1.0)
