@@ -374,6 +374,16 @@ def _start_out(outdir):
                 "is ready.\n" % time.ctime())
 
 
+class _NullStream:
+    """A reply stream that drops what it is given (batched scripts have no getters)."""
+
+    def write(self, s):
+        return len(s)
+
+    def flush(self):
+        pass
+
+
 def serve_scripts(infiles, mode="fused", silent=True, poll=16):
     """Runs .in scripts of the batchable shape (batch_plan) together through the
     continuous-batching server (serve.Server): all set-ups first, every `run`
@@ -393,7 +403,7 @@ def serve_scripts(infiles, mode="fused", silent=True, poll=16):
         setup, duration, saves = plan
         outdir = out_dir(f)
         _start_out(outdir)
-        shell = Shell(outdir, reply=io_null(), log=None if silent else sys.stderr, mode=mode)
+        shell = Shell(outdir, reply=_NullStream(), log=None if silent else sys.stderr, mode=mode)
         try:
             for ln in setup:
                 shell.call(ln[0], ln[1:])
@@ -425,15 +435,6 @@ def serve_scripts(infiles, mode="fused", silent=True, poll=16):
         serve.Server(poll=poll).serve(reqs)
     return status
 
-
-class io_null:
-    """A reply stream that drops what it is given (batched scripts have no getters)."""
-
-    def write(self, s):
-        return len(s)
-
-    def flush(self):
-        pass
 
 
 def run_script(infile, stream=None, reply=None, mode="fused", silent=False):
