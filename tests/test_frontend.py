@@ -152,6 +152,26 @@ def test_initial_state_commands(fe, tmp_path):
     sh.close()
 
 
+def test_field_function_commands(fe, simmod, tmp_path):
+    """applyrf / applypulse / applyrframp / applysawtooth / applyrotating take the
+    reference's argument order (what, hz, hy, hx, ...) and replace the field;
+    applystatic replaces them again."""
+    sh = fe.Shell(str(tmp_path / "o.out"), reply=io.StringIO())
+    sh.call("applyrf", ["field", "0.01", "0.02", "0.03", "1e9"])
+    kind, b, prm = sh.sim._fn
+    assert kind == 2 and b == tuple(float(np.float32(x)) for x in (0.03, 0.02, 0.01)) and prm[0] == 1e9
+    assert sh.sim.field_at(0.25e-9) == pytest.approx(np.array(b), rel=1e-6)     # sin(pi/2) = 1
+    sh.call("applypulse", ["field", "0", "0", "0.1", "2e-9", "1e-9"])
+    assert sh.sim._fn[0] == 1 and sh.sim.field_at(2e-9)[0] == pytest.approx(float(np.float32(0.1)))
+    sh.call("applyrotating", ["field", "0.1", "0.1", "0.1", "1e9", "0", "1.5707963267948966", "0"])
+    assert sh.sim.field_at(0.0) == pytest.approx([0.0, float(np.float32(0.1)), 0.0], abs=1e-7)
+    sh.call("applystatic", ["field", "0", "0", "0.5"])
+    assert sh.sim._fn is None and sh.sim.field_at(1.0)[0] == 0.5
+    with pytest.raises(fe.InputErr):
+        sh.call("applyrf", ["current", "0", "0", "1", "1e9"])
+    sh.close()
+
+
 def test_output_schedule_rule(simmod):
     """Periodic.NeedSave (simoutput.go:80-82): due at t == 0 and when float32
     time - last save >= period."""
