@@ -172,7 +172,8 @@ COMMANDS = {
     "hysteresis": ("_hysteresis", "ffffffi"),
     # output (simoutput.go, simdebug.go, simmetadata.go)
     "save": ("save", "ss"), "autosave": ("autosave", "ssf"), "tabulate": ("tabulate", "sb"),
-    "savem": ("_savem", "ss"), "desc": ("desc", "ss"), "subsampleoutput": ("_subsample", "i"),
+    "savem": ("_savem", "ss"), "saveh": ("_saveh", "ss"), "desc": ("desc", "ss"),
+    "subsampleoutput": ("_subsample", "i"),
     # getters (simget.go)
     "getm": ("_getm", "i"), "getmpos": ("_getmpos", "iiii"), "getmaxm": ("_getmaxm", "i"),
     "getminm": ("_getminm", "i"), "getmaxtorque": ("_getmaxtorque", ""), "getcorepos": ("_getcorepos", ""),
@@ -187,7 +188,7 @@ UNSUPPORTED = (
     "k2", "aniscubic", "spinpolarization", "xi", "temperature", "currentdensity", "currentmask", "fieldmask",
     "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortexinarray", "vortex_in_array", "setvortexellips",
-    "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrotatingburst", "edgecorrection", "saveh",
+    "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrotatingburst", "edgecorrection",
     "savebenchmark", "exchangetype",
 )
 
@@ -290,16 +291,26 @@ class Shell:
     def _hysteresis(self, bz0, by0, bx0, bz1, by1, bx1, steps):
         self.sim.hysteresis(bz0, by0, bx0, bz1, by1, bx1, steps)
 
-    def _savem(self, fname, format):
-        # SaveM (simdebug.go:26-30): m in units of Msat to a named file
+    def _save_named(self, fname, format, data):
+        # saveOmf (simoutput.go:414-426): OVF 1.0, multiplier 1, unit "Msat", to
+        # the file name as given (relative to the working directory)
         from . import ovf
 
         kinds = {"binary": True, "text": False}
         if format not in kinds:
             raise InputErr("unknown format %s. options are: binary, text" % format)
+        ovf.write_ovf(fname, data, self.sim._cell, version=1, binary=kinds[format], multiplier=1.0,
+                      valueunit="Msat", desc=self.sim._ovf_desc())
+
+    def _savem(self, fname, format):
+        # SaveM (simdebug.go:26-30): m in units of Msat
         self.sim._init()
-        ovf.write_ovf(os.path.join(self.outdir, fname), self.sim.m, self.sim._cell, version=1,
-                      binary=kinds[format], multiplier=1.0, valueunit="Msat", desc=self.sim._ovf_desc())
+        self._save_named(fname, format, self.sim.m)
+
+    def _saveh(self, fname, format):
+        # SaveH (simdebug.go:18-23): the effective field of the current m, in units of Msat
+        self.sim._init()
+        self._save_named(fname, format, self.sim.heff(self.sim.m))
 
     def _demag(self, on):
         if not on:

@@ -451,3 +451,21 @@ def test_daemon_batch_serves_scripts_like_serial(fe, tmp_path):
         assert (batched / (k + ".out") / "finished").exists()
     assert (batched / "e.out" / "finished").exists()
     assert "Starting simulation: %s" % (batched / "e.in") in r.stderr
+
+
+@pytest.mark.gpu
+def test_savem_and_saveh(fe, simmod, tmp_path):
+    """SaveM / SaveH (simdebug.go:18-30): m and the effective field of the
+    current m, OVF 1.0, to the file names as given."""
+    inp = tmp_path / "f.in"
+    mf, hf = tmp_path / "mfinal.omf", tmp_path / "hfinal.omf"
+    inp.write_text(FILM + "run 5e-12\nsavem %s binary\nsaveh %s binary\n" % (mf, hf))
+    assert fe.run_script(str(inp), silent=True) == 0
+    ovf = import_module("1_amd.ovf")
+    s = _film_sim(simmod)
+    s.run(5e-12)
+    m, meta = ovf.read_ovf(mf)
+    assert np.array_equal(m, s.m) and meta["valueunit"] == "Msat" and meta["multiplier"] == 1.0
+    h, _ = ovf.read_ovf(hf)
+    assert np.array_equal(h, s.heff(s.m))
+    s.free()
