@@ -166,6 +166,9 @@ COMMANDS = {
     # initial magnetisation (simconfig.go)
     "uniform": ("uniform", "fff"), "setmcell": ("setmcell", "iiifff"), "setmrange": ("setmrange", "iiiiiifff"),
     "vortex": ("vortex", "ii"), "setrandom": ("_setrandom", ""), "addnoise": ("_addnoise", "f"),
+    "sbw": ("sbw", ""), "snw": ("snw", ""), "abw": ("abw", ""), "anw": ("anw", ""),
+    "setvortexellips": ("setvortexellips", "ffffii"), "setvortexrectangle": ("setvortexrectangle", "ffffii"),
+    "vortexinarray": ("vortexinarray", "ffffffii"),
     "randomseed": ("_seed", "l"), "seed": ("_seed", "l"), "loadm": ("loadm", "s"),
     # run (simrun.go, relax.go, simhysteresis.go)
     "run": ("run", "d"), "step": ("step", ""), "steps": ("steps", "i"), "relax": ("relax", ""),
@@ -187,8 +190,7 @@ COMMANDS = {
 UNSUPPORTED = (
     "k2", "aniscubic", "spinpolarization", "xi", "temperature", "currentdensity", "currentmask", "fieldmask",
     "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
-    "antidotarrayellips", "squareholearray", "vortexinarray", "vortex_in_array", "setvortexellips",
-    "setvortexrectangle", "sbw", "snw", "abw", "anw", "setm", "applyrotatingburst", "edgecorrection",
+    "antidotarrayellips", "squareholearray", "vortex_in_array", "setm", "applyrotatingburst", "edgecorrection",
     "savebenchmark", "exchangetype",
 )
 

@@ -396,6 +396,92 @@ class Sim:
         sl = (slice(x1, x2 + 1), slice(y1, y2 + 1), slice(z1, z2 + 1))
         m[(0,) + sl], m[(1,) + sl], m[(2,) + sl] = mx, my, mz
 
+    def _walls(self, phi_of_j):
+        # SBW/SNW/ABW/ANW (simconfig.go:212-289): theta over the Z (user x) index
+        m = self._mlocal()
+        X, Y, Z = m.shape[1:]
+        k = np.arange(Z)
+        theta = np.pi * ((k - Z // 2) + 0.5) / Z
+        phi = np.broadcast_to(phi_of_j(np.arange(Y), Y), (Y,))
+        th, ph = np.meshgrid(theta, phi, indexing="xy")   # [Y][Z]
+        m[0] = np.sin(th).astype(np.float32)
+        m[1] = (np.cos(th) * np.cos(ph)).astype(np.float32)
+        m[2] = (np.cos(th) * np.sin(ph)).astype(np.float32)
+
+    def sbw(self):
+        """SBW: starting state of a symmetric Bloch wall."""
+        self._walls(lambda j, Y: np.zeros(Y))
+
+    def snw(self):
+        """SNW: starting state of a symmetric Neel wall."""
+        self._walls(lambda j, Y: np.full(Y, np.pi / 2.0))
+
+    def abw(self):
+        """ABW: starting state of an asymmetric Bloch wall (phi over the user y index)."""
+        self._walls(lambda j, Y: np.pi * ((j - Y // 2) + 0.5) / Y)
+
+    def anw(self):
+        """ANW (simconfig.go:272-289, phi = pi/4 as there)."""
+        self._walls(lambda j, Y: np.full(Y, np.pi / 4.0))
+
+    def _vortex_block(self, cx, cy, sx, sy, circulation, polarization, ellipse):
+        # SetVortexEllips / SetVortexRectangle (simconfig.go:159-209), float32 arithmetic
+        m = self._mlocal()
+        f = np.float32
+        c1, c2 = f(self._cell[1]), f(self._cell[2])
+        cxi, cyi = int(f(cx) / c2), int(f(cy) / c1)
+        rx, ry = int(f(sx) / c2 / f(2)), int(f(sy) / c1 / f(2))
+        X, Y, Z = m.shape[1:]
+        for j in range(-ry, ry + 1):
+            for k in range(-rx, rx + 1):
+                if ellipse:
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        inside = f(j * j) / f(ry * ry) + f(k * k) / f(rx * rx) <= f(1)
+                    if not inside:
+                        continue
+                y, z = j + cyi, k + cxi
+                if not (0 <= y < Y and 0 <= z < Z):
+                    raise InputErr("the vortex extends beyond the grid")
+                m[0, :, y, z] = 0.0
+                m[1, :, y, z] = k * circulation
+                m[2, :, y, z] = -j * circulation
+        if not (0 <= cyi < Y and 0 <= cxi < Z):
+            raise InputErr("the vortex centre is outside the grid")
+        m[:, :, cyi, cxi] = 0.0
+        m[0, :, cyi, cxi] = polarization
+
+    def setvortexellips(self, cx, cy, sx, sy, circulation, polarization):
+        """SetVortexEllips: a vortex inside the ellipse of centre (cx, cy) and
+        axes (sx, sy) metres (user x, y)."""
+        self._vortex_block(cx, cy, sx, sy, int(circulation), int(polarization), True)
+
+    def setvortexrectangle(self, cx, cy, sx, sy, circulation, polarization):
+        """SetVortexRectangle: the same over the rectangle sx x sy."""
+        self._vortex_block(cx, cy, sx, sy, int(circulation), int(polarization), False)
+
+    def vortexinarray(self, vi, vj, usx, usy, sepx, sepy, circulation, polarization):
+        """VortexInArray (simconfig.go:127-157): the vortex of unit (vi, vj) of
+        an array of unit size (usx, usy) and separation (sepx, sepy), metres."""
+        m = self._mlocal()
+        f = np.float32
+        c1, c2 = f(self._cell[1]), f(self._cell[2])
+        bx, sxc = f(usx) / c2, f(sepx) / c2
+        by, syc = f(usy) / c1, f(sepy) / c1
+        cx = f(vi) * bx + bx / f(2) - f(0.5)
+        cy = f(vj) * by + by / f(2) - f(0.5)
+        X, Y, Z = m.shape[1:]
+        j0, j1 = int(f(vj) * by + syc / f(2)), int(f(vj + 1) * by - syc / f(2))
+        k0, k1 = int(f(vi) * bx + sxc / f(2)), int(f(vi + 1) * bx - sxc / f(2))
+        if j0 < 0 or k0 < 0 or j1 > Y or k1 > Z or not (0 <= int(cy) < Y and 0 <= int(cx) < Z):
+            raise InputErr("the vortex unit extends beyond the grid")
+        for j in range(j0, j1):
+            for k in range(k0, k1):
+                m[0, :, j, k] = 0.0
+                m[1, :, j, k] = (f(k) - cx) * f(circulation)
+                m[2, :, j, k] = -(f(j) - cy) * f(circulation)
+        m[:, :, int(cy), int(cx)] = 0.0
+        m[0, :, int(cy), int(cx)] = polarization
+
     def setrandom(self, seed=None):
         """SetRandom (simconfig.go:314-321): every component uniform on [-0.5, 0.5)
         (numpy's generator, not Go's math/rand); `seed` is a seed or a Generator."""

@@ -152,6 +152,40 @@ def test_initial_state_commands(fe, tmp_path):
     sh.close()
 
 
+def test_wall_and_vortex_array_states(fe, tmp_path):
+    """SBW/SNW/ABW/ANW, SetVortexEllips/Rectangle, VortexInArray
+    (simconfig.go:127-289) in user index order."""
+    sh = fe.Shell(str(tmp_path / "o.out"), reply=io.StringIO())
+    sh.call("gridsize", ["16", "8", "1"])
+    sh.call("cellsize", ["2e-9", "2e-9", "2e-9"])
+    sh.call("sbw", [])
+    m = sh.sim._m0
+    assert np.allclose(np.linalg.norm(m, axis=0), 1, atol=1e-6)
+    assert np.allclose(m[0, 0, 3, :], np.sin(np.pi * (np.arange(16) - 8 + 0.5) / 16), atol=1e-7)
+    assert np.abs(m[2]).max() == 0 and m[0, 0, 0, 0] < -0.99 and m[0, 0, 0, 15] > 0.99
+    sh.call("snw", [])
+    assert np.abs(sh.sim._m0[1]).max() < 1e-7
+    sh.call("abw", [])
+    m = sh.sim._m0
+    assert np.allclose(m[2, 0, :, 8] / np.cos(np.pi * 0.5 / 16), np.sin(np.pi * (np.arange(8) - 4 + 0.5) / 8), atol=1e-6)
+    sh.call("uniform", ["1", "0", "0"])
+    sh.call("setvortexellips", ["16e-9", "8e-9", "12e-9", "8e-9", "1", "-1"])   # centre cell (8, 4), radii 3, 2
+    m = sh.sim._m0
+    assert tuple(m[:, 0, 4, 8]) == (-1, 0, 0)
+    assert tuple(m[:, 0, 4, 11]) == (0, 3, 0) and tuple(m[:, 0, 6, 8]) == (0, 0, -2)
+    assert tuple(m[:, 0, 6, 11]) == (0, 0, 1)              # outside the ellipse: untouched (+x)
+    sh.call("setvortexrectangle", ["16e-9", "8e-9", "12e-9", "8e-9", "-1", "1"])
+    assert tuple(sh.sim._m0[:, 0, 6, 11]) == (0, -3, 2)
+    sh.call("uniform", ["0", "0", "1"])
+    sh.call("vortexinarray", ["1", "0", "16e-9", "16e-9", "4e-9", "4e-9", "1", "1"])   # unit 8 x 8 cells, k 9..14
+    m = sh.sim._m0
+    assert tuple(m[:, 0, 3, 11]) == (1, 0, 0) and tuple(m[:, 0, 2, 9]) == (0, np.float32(9 - 11.5), np.float32(-(2 - 3.5)))
+    assert tuple(m[:, 0, 2, 15]) == (1, 0, 0) and tuple(m[:, 0, 0, 9]) == (1, 0, 0)   # outside the unit
+    with pytest.raises(fe.InputErr):
+        sh.call("setvortexellips", ["2e-9", "8e-9", "12e-9", "8e-9", "1", "1"])
+    sh.close()
+
+
 def test_field_function_commands(fe, simmod, tmp_path):
     """applyrf / applypulse / applyrframp / applysawtooth / applyrotating take the
     reference's argument order (what, hz, hy, hx, ...) and replace the field;
