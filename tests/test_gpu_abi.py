@@ -410,3 +410,43 @@ def test_gaussian_noise_statistics(lib):
     lib.gpu_gaussian_noise(d.p, F32(0.0), F32(1.0), n)
     y = d.get()
     assert not np.array_equal(x, y)
+
+
+def test_cubic_anisotropy_is_minus_gradient_of_its_energy(lib):
+    """anisType 2 (gpu_local_contr3.cu's cubic branch): the field equals minus
+    the gradient, by central differences, of
+    E = k0/2 (a0^2 a1^2 + a1^2 a2^2 + a0^2 a2^2) + k1/2 a0^2 a1^2 a2^2 with
+    a_i = m . u_i; with k1 = 0 the energy density phi is E (-m . H / 4 = E for a
+    quartic)."""
+    rng = np.random.default_rng(21)
+    N = 300
+    m = rng.standard_normal((3, N)).astype(F32)
+    m /= np.linalg.norm(m, axis=0)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    u = np.ascontiguousarray(q.T, F32)            # rows u_0, u_1, u_2, orthonormal
+    k0, k1 = 0.7, 0.4
+    hext = np.zeros(3, F32)
+
+    def energy(mm):
+        a = u.astype(np.float64) @ mm
+        return 0.5 * k0 * (a[0] ** 2 * a[1] ** 2 + a[1] ** 2 * a[2] ** 2 + a[0] ** 2 * a[2] ** 2) + \
+            0.5 * k1 * a[0] ** 2 * a[1] ** 2 * a[2] ** 2
+
+    m64, eps = m.astype(np.float64), 1e-6
+    grad = np.zeros((3, N))
+    for c in range(3):
+        d = np.zeros((3, 1))
+        d[c] = eps
+        grad[c] = (energy(m64 + d) - energy(m64 - d)) / (2 * eps)
+    dm, dh = Dev(lib, m), Dev(lib, np.zeros_like(m))
+    kk = np.array([k0, k1], F32)
+    lib.gpu_add_local_fields(dm.p, dh.p, N, hext.ctypes.data, None, 2, kk.ctypes.data, u.reshape(-1).ctypes.data)
+    h = dh.get().reshape(3, N)
+    assert np.abs(h + grad).max() < 1e-5
+    kk0 = np.array([k0, 0.0], F32)
+    dm, dh, dp = Dev(lib, m), Dev(lib, np.zeros_like(m)), Dev(lib, np.zeros(N, F32))
+    lib.gpu_add_local_fields_H_and_phi(dm.p, dh.p, dp.p, N, hext.ctypes.data, None, 2, kk0.ctypes.data,
+                                       u.reshape(-1).ctypes.data)
+    a = u.astype(np.float64) @ m64
+    e4 = 0.5 * k0 * (a[0] ** 2 * a[1] ** 2 + a[1] ** 2 * a[2] ** 2 + a[0] ** 2 * a[2] ** 2)
+    assert np.abs(dp.get() - e4).max() < 1e-6
