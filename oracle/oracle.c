@@ -6,7 +6,7 @@
  *
  * Each function cites the reference file:line it restates (paths relative to
  * the reference root).  How each part is pinned (tests/test_oracle_*.py):
- *   or_units             closed form: lex(Py)=5.686 nm, t_u=5.654 ps (SURVEY a14 values)
+ *   or_units             closed form: lex(Py)=5.686 nm, t_u=5.654 ps (Ms 8e5 A/m, A 1.3e-11 J/m)
  *   or_facekernel6       self term: trace -> -1, cube -> -1/3 each; thin film Nxx -> -1;
  *                        far field -> point dipole; symmetry K(-r)=+-K(r); float32 twin
  *                        bit-compared with or_facekernel6_f32 (the literal transcription)
@@ -210,7 +210,7 @@ void or_facekernel6_f32(const int P[3], const float cell[3], int accuracy, const
 void or_add_exch6_kernel(const int P[3], const or_real cell[3], or_real* K) {
     /* exchange.go:44-69 builds a fresh 6NGBR kernel and init.go:256-267 adds it
      * element-wise.  Writes happen in the reference's order, so a padded size of 1
-     * maps wrap(+-1,1)=0 onto the centre and the last write wins. */
+     * maps wrap(+-1,1)=0 onto the centre and the last write wins (exchange.go:58-65). */
     size_t NP = (size_t)P[0] * P[1] * P[2];
     or_real* E = (or_real*)xcalloc(NP, sizeof(or_real));
     or_real hx = cell[X] * cell[X], hy = cell[Y] * cell[Y], hz = cell[Z] * cell[Z];

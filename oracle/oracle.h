@@ -9,7 +9,7 @@
  * same source compiled with -DOR_REAL_FLOAT gives liboracle_f32.so, an FP32
  * twin used to measure the FP32-vs-FP64 floor of the algorithm itself.
  *
- * Layout conventions follow the reference:
+ * Layout conventions follow the reference (internal X = thickness, Z fastest; planar components):
  *   - internal axes (X,Y,Z) = user (z,y,x); X is the film thickness;
  *   - vector fields are planar: [3][X][Y][Z], Z fastest;
  *   - kernels are 6 real arrays (xx,yy,zz,yz,xz,xy) over the padded grid,
@@ -53,7 +53,7 @@ void or_facekernel6(const int P[3], const or_real cell[3], int accuracy, const i
 /* float32 transcription of faceIntegral for bit-level comparison (tensor.Vector is [3]float32). */
 void or_facekernel6_f32(const int P[3], const float cell[3], int accuracy, const int periodic[3], float* K);
 /* Exch6NgbrKernel (exchange.go:44-69), ADDED into K (init.go:256-267), including the
- * wrap(+-1,1)=0 overwrite of the centre when a padded size is 1. */
+ * wrap(+-1,1)=0 overwrite of the centre when a padded size is 1 (exchange.go:58-65). */
 void or_add_exch6_kernel(const int P[3], const or_real cell[3], or_real* K);
 
 /* ---------------- FFT ---------------- */

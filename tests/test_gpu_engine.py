@@ -4,7 +4,7 @@ the FP64 oracle.
 Tolerances: the engine computes in FP32 (like the reference); fields and
 torques are compared relative to their maximum (FP32 FFT round-off of a few
 1e-7 x log2 P), trajectories by max |dm| with the oracle replaying the GPU's
-accepted-dt schedule (SURVEY: parity at the same dt sequence), bar 1e-5.
+accepted-dt schedule (trajectories are compared at the same accepted-dt sequence), bar 1e-5.
 """
 import numpy as np
 import pytest

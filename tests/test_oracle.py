@@ -328,7 +328,7 @@ def _macrospin(orc, solver, dt, alpha=0.1, B=0.1, t_end_internal=50.0, maxerror=
     t = st["time"]
     H = B / u["field"]
     dtg = 1 / (1 + alpha ** 2)
-    # internal-coordinate LL: mz = tanh(a dtg H t + atanh(mz0)), phi = phi0 - dtg H t
+    # internal-coordinate LL (the reference's closed form, dm/dt = -m x h sense): mz = tanh(a dtg H t + atanh(mz0)), phi = phi0 - dtg H t
     mz = np.tanh(alpha * dtg * H * t + np.arctanh(np.cos(th0)))
     phi = -dtg * H * t
     r = np.sqrt(1 - mz ** 2)
