@@ -284,13 +284,14 @@ __global__ __launch_bounds__(64) void kb_ctrl_end(const BSim* sims) {
     const BSim& d = sims[blockIdx.x];
     ctrl_end_body(d.st, d.cp);
 }
-// done flags; with rem, each running simulation's remaining trials at its
+// status flags (0 running, 1 done, 2 stopped on a NaN dt or a non-finite torque or
+// error, i.e. failed); with rem, each running simulation's remaining trials at its
 // current dt, (stop - time) / dt, or its remaining steps under a step target
 __global__ void kb_done(const BSim* sims, int n, int* done, float* rem) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const EngineState* st = sims[j].st;
-    done[j] = st->done;
+    done[j] = st->done ? (st->nan_dt || st->nonfinite ? 2 : 1) : 0;
     if (rem) {
         double r = 0.0;
         if (!st->done)
@@ -782,7 +783,7 @@ int mm_batch_trials(mmBatch* b, int n) {
     return mm_batch_flags(b, nullptr);
 }
 
-// done flags of every simulation (1 = reached its stop time / step target)
+// status of every simulation (0 running, 1 reached its stop time / step target, 2 failed)
 void mm_batch_done(mmBatch* b, int* done) {
     const int ns = (int)b->sims.size();
     kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone, nullptr);

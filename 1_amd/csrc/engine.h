@@ -42,7 +42,12 @@ struct EngineState {
     int relax;          // Relax(): done once an accepted step raised the energy (relax.go:14-25)
     double relax_prev;  // energy after the previous step of the relaxation
     double field_tmax;  // latest stage time (internal) the applied field was evaluated at
+    int nonfinite;      // max |torque| or the error estimate was NaN or infinite: the run stopped
 };
+
+// |x| for a per-block maximum that must not lose a NaN: fmaxf returns the other
+// operand of a NaN, so a NaN maps to +inf, which every later fmaxf keeps.
+__device__ __forceinline__ float abs_keep_nan(float x) { return x == x ? fabsf(x) : INFINITY; }
 
 constexpr int kMaxStepTrials = 10;  // rk.go:277
 
@@ -306,6 +311,7 @@ struct CtrlParams {
     float minDm, maxDm, minDt, maxDt;  // minDt/maxDt in internal units
     double* log_dt;
     int* log_accept;
+    float* log_err;         // per trial: the error estimate (adaptive) or 0
     int log_cap;
     const float* part_k0;   // per-block max |k0| (stage 0 torque)
     const float* part_k6;   // per-block max |k_last| (FSAL tableau)

@@ -307,10 +307,10 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
         }
         float k[3];
         llg_torque_cell(ea.t, cell_alpha(ea.t, ci), hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
-        lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
+        lmax = fmaxf(lmax, fmaxf(abs_keep_nan(k[0]), fmaxf(abs_keep_nan(k[1]), abs_keep_nan(k[2]))));
         if constexpr (FIN == 2) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) emax = fmaxf(emax, (float)fabs(fma(ce_self, (double)k[q], er[q])));
+            for (int q = 0; q < 3; q++) emax = fmaxf(emax, abs_keep_nan((float)fma(ce_self, (double)k[q], er[q])));
         }
         if constexpr (FIN == 1) {
             float f[3];

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, flo
         hk[i] = k[0];
         hk[N + i] = k[1];
         hk[2 * N + i] = k[2];
-        lmax = fmaxf(lmax, fmaxf(fabsf(k[0]), fmaxf(fabsf(k[1]), fabsf(k[2]))));
+        lmax = fmaxf(lmax, fmaxf(abs_keep_nan(k[0]), fmaxf(abs_keep_nan(k[1]), abs_keep_nan(k[2]))));
     }
     if (a.part) {
         const float bm = block_max_nonneg(lmax);

@@ -61,7 +61,7 @@ __device__ __forceinline__ void rk_final_body(float* __restrict__ m, const KPtrs
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 mtmp[c * N + i] = hi[c];
-                lmax = fmaxf(lmax, (float)fabs(er[c]));
+                lmax = fmaxf(lmax, abs_keep_nan((float)er[c]));
             }
         } else {
             const float s = rsqrtf(hi[0] * hi[0] + hi[1] * hi[1] + hi[2] * hi[2]);
@@ -123,6 +123,7 @@ __device__ __forceinline__ void ctrl_after_k0_body(EngineState* st, int fsal_cop
     if (threadIdx.x != 0) return;
     if (!p.adaptive) st->dt = p.inputDt;
     st->max_torque = maxTorque;
+    if (!isfinite(maxTorque)) st->nonfinite = 1;
     float dt = st->dt;
     const float dm = dt * maxTorque;
     if (dm < p.minDm) dt = p.minDm / maxTorque;
@@ -148,6 +149,7 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     if (p.part_e) st->energy = e_red * p.cell_volume;
     if (p.fold_k0) {
         st->max_torque = k0_red;
+        if (!isfinite(k0_red)) st->nonfinite = 1;
         if (!p.adaptive) st->dt = p.inputDt;
     }
     const float dtu = st->dt_used;
@@ -160,6 +162,7 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     if (p.adaptive) {
         const float err = err_red;
         st->err = err;
+        if (!isfinite(err)) st->nonfinite = 1;
         float factor = (float)pow((double)(p.maxError / err), 1.0 / (double)p.errorOrder);
         if (factor > 2.f) factor = 2.f;
         if (factor < 0.01f) factor = 0.01f;
@@ -175,10 +178,11 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     }
     if (bad) st->rejects_total++;
     st->accept = !bad;
-    st->step_end = !bad || st->trials >= kMaxStepTrials;
+    st->step_end = !bad || st->trials >= kMaxStepTrials || st->nonfinite;
     if (st->log_n < p.log_cap) {
         p.log_dt[st->log_n] = (double)dtu;
         p.log_accept[st->log_n] = !bad;
+        p.log_err[st->log_n] = p.adaptive ? st->err : 0.f;
     }
     st->log_n++;
     if (st->step_end) {
@@ -188,7 +192,7 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
             if (st->energy > st->relax_prev) st->done = 1;
             st->relax_prev = st->energy;
         }
-        if (st->time >= st->stop || st->steps >= st->steps_target || st->nan_dt) st->done = 1;
+        if (st->time >= st->stop || st->steps >= st->steps_target || st->nan_dt || st->nonfinite) st->done = 1;
     }
 }
 

@@ -61,6 +61,7 @@ struct mmSim {
     EngineState* st = nullptr;
     double* log_dt = nullptr;
     int* log_acc = nullptr;
+    float* log_err = nullptr;  // per trial: the error estimate (adaptive) or 0
     int log_cap = 1 << 16;
     double* dsum = nullptr;
     float* part = nullptr;  // 3 x npart: |k0|, |k_last|, error
@@ -99,6 +100,7 @@ struct mmSim {
     bool hfsal = false;
     std::vector<double> hlog_dt;
     std::vector<int> hlog_acc;
+    std::vector<float> hlog_err;
 };
 
 

@@ -253,6 +253,7 @@ CtrlParams gk::ctrl_params(const mmSim* s) {
     c.maxDt = (float)(s->p.maxDt / s->unitTime);
     c.log_dt = s->log_dt;
     c.log_accept = s->log_acc;
+    c.log_err = s->log_err;
     c.log_cap = s->log_cap;
     return c;
 }
@@ -383,6 +384,10 @@ static long fused_run(mmSim* s, long long nsteps, double stop, bool wait) {
             fprintf(stderr, "libgpukern: dt = NaN (rk.go:448-452)\n");
             abort();
         }
+        if (h.nonfinite) {
+            fprintf(stderr, "libgpukern: the torque or the error estimate is NaN or infinite at step %lld\n", h.steps);
+            abort();
+        }
         if (h.done) break;
         if (batch < 64) batch *= 2;
     }
@@ -489,6 +494,7 @@ static void abi_step(mmSim* s) {
         good = !bad;
         s->hlog_dt.push_back(dtu);
         s->hlog_acc.push_back(!bad);
+        s->hlog_err.push_back(t.adaptive ? s->herr : 0.f);
     }
     gpu_normalize_uniform(m, s->N);
     s->hsteps++;
@@ -596,6 +602,7 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->st = (EngineState*)dmalloc(s, sizeof(EngineState));
     s->log_dt = (double*)dmalloc(s, s->log_cap * sizeof(double));
     s->log_acc = (int*)dmalloc(s, s->log_cap * sizeof(int));
+    s->log_err = (float*)dmalloc(s, s->log_cap * sizeof(float));
     s->dsum = (double*)dmalloc(s, 4 * sizeof(double));
     EngineState h{};
     h.dt = (float)(p->dt / s->unitTime);  // init.go:280-284
@@ -641,8 +648,8 @@ void mm_sim_free(mmSim* s) {
         if (s->k[i]) GK_CHECK(hipFree(s->k[i]));
     for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap})
         if (p) GK_CHECK(hipFree(p));
-    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->dsum, (void*)s->fT, (void*)s->part,
-                    (void*)s->epart})
+    for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->log_err, (void*)s->dsum, (void*)s->fT,
+                    (void*)s->part, (void*)s->epart})
         if (p) GK_CHECK(hipFree(p));
     for (void* p : s->retired) GK_CHECK(hipFree(p));
     conv_plan_free(s->cp);
@@ -758,6 +765,20 @@ int mm_sim_trial_log(mmSim* s, double* dt, int* accept, int max) {
     if (n <= 0) return 0;
     GK_CHECK(hipMemcpy(dt, s->log_dt, n * sizeof(double), hipMemcpyDeviceToHost));
     GK_CHECK(hipMemcpy(accept, s->log_acc, n * sizeof(int), hipMemcpyDeviceToHost));
+    return n;
+}
+
+int mm_sim_trial_errors(mmSim* s, float* err, int max) {
+    if (s->mode == MM_MODE_ABI) {
+        int n = (int)std::min<size_t>(max, s->hlog_err.size());
+        size_t off = s->hlog_err.size() - n;
+        for (int i = 0; i < n; i++) err[i] = s->hlog_err[off + i];
+        return n;
+    }
+    EngineState h = read_state(s);
+    int n = std::min(std::min(h.log_n, s->log_cap), max);
+    if (n <= 0) return 0;
+    GK_CHECK(hipMemcpy(err, s->log_err, n * sizeof(float), hipMemcpyDeviceToHost));
     return n;
 }
 
@@ -885,7 +906,7 @@ unsigned long long mm_sim_footprint(const mmParams* p) {
     size_t b = conv_plan_footprint(p->size, &epi);
     const int npart = std::max(kRedBlocks, epi);
     b += N3 * sizeof(float) * (size_t)(1 + t.S + 2);  // m, k_0..k_{S-1}, m_i, m_tmp
-    b += sizeof(EngineState) + (size_t)(1 << 16) * (sizeof(double) + sizeof(int)) + 4 * sizeof(double);
+    b += sizeof(EngineState) + (size_t)(1 << 16) * (sizeof(double) + sizeof(int) + sizeof(float)) + 4 * sizeof(double);
     b += 3 * (size_t)npart * sizeof(float) + (size_t)npart * sizeof(double);  // part, epart
     if (p->nField >= 2) b += (size_t)p->nField * (sizeof(double) + 3 * sizeof(float));
     if (p->alphaMap) b += (size_t)(N3 / 3) * sizeof(float);

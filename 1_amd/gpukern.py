@@ -75,6 +75,7 @@ ENGINE = {
     "mm_sim_device_m": (P, [P]),
     "mm_sim_state": (None, [P, P]),
     "mm_sim_trial_log": (I, [P, P, P, I]),
+    "mm_sim_trial_errors": (I, [P, P, I]),
     "mm_sim_heff": (None, [P, P, P]),
     "mm_sim_torque": (None, [P, P, P]),
     "mm_sim_energy": (C.c_double, [P]),

@@ -31,9 +31,13 @@ class Request:
     """A simulation to run for cfg["duration"] seconds from `arrival` on.  The
     simulation is built from the workload dict cfg, or by `make()` (an
     un-initialised ``Sim``; cfg then needs only "size" and "duration").
-    `on_finish(request)` runs when it has finished, before it is freed."""
+    `on_finish(request)` runs when it has finished, before it is freed.
+    `status` is "ok" once it reached its stop time, "failed" if it stopped on
+    a NaN dt or a non-finite torque or error estimate (the reference stops a
+    run there, rk.go:448-452, simrun.go:48-50)."""
 
-    __slots__ = ("rid", "arrival", "cfg", "duration", "sim", "start", "finish", "footprint", "make", "on_finish")
+    __slots__ = ("rid", "arrival", "cfg", "duration", "sim", "start", "finish", "footprint", "make", "on_finish",
+                 "status")
 
     def __init__(self, rid, arrival, cfg, make=None, on_finish=None):
         self.rid = rid
@@ -46,6 +50,7 @@ class Request:
         self.footprint = None
         self.make = make
         self.on_finish = on_finish
+        self.status = None
 
     def build(self):
         return self.make() if self.make is not None else wl.make_sim(self.cfg)
@@ -199,11 +204,13 @@ class Server:
                     if not g.batch.ready():
                         continue
                     done = g.batch.flags()
+                    status = g.batch.status()
                     g.busy = False
                     t = clock() - t0
-                    for r, d in zip(g.members, done):
+                    for r, d, stat in zip(g.members, done, status):
                         if d and r.finish is None:
                             r.finish = t
+                            r.status = "failed" if stat == 2 else "ok"
                             if self.keep_results:
                                 self.results[r.rid] = r.sim.m
                             if r.on_finish is not None:
@@ -274,6 +281,7 @@ def serve_serial(requests, clock=time.perf_counter, keep_results=False):
         r.start = clock() - t0
         s.run(r.duration)
         r.finish = clock() - t0
+        r.status = "ok"
         if keep_results:
             results[r.rid] = s.m
         s.free()

@@ -836,6 +836,13 @@ class Sim:
         n = self._lib.mm_sim_trial_log(self._ptr, dt.ctypes.data, acc.ctypes.data, max_entries)
         return dt[:n], acc[:n]
 
+    def trial_errors(self, max_entries=1 << 16):
+        """The error estimate of each logged trial (0 for fixed-step solvers)."""
+        self._init()
+        err = np.zeros(max_entries, np.float32)
+        n = self._lib.mm_sim_trial_errors(self._ptr, err.ctypes.data, max_entries)
+        return err[:n]
+
     def heff(self, m):
         self._init()
         m = np.ascontiguousarray(m, np.float32)
@@ -976,6 +983,7 @@ class Batch:
         if not self._ptr:
             raise RuntimeError("mm_batch_new failed")
         self._lib.mm_batch_set_graph(self._ptr, int(bool(graph)))
+        self._status = np.zeros(len(self.sims), np.int32)
 
     def __len__(self):
         return len(self.sims)
@@ -993,11 +1001,14 @@ class Batch:
 
     def trials(self, n):
         """Enqueue n batched trials and wait; returns the simulations not yet done."""
-        return int(self._lib.mm_batch_trials(self._ptr, int(n)))
+        self._lib.mm_batch_enqueue(self._ptr, int(n))
+        return int((~self.flags()).sum())
 
     def done(self):
+        """Done flags now (a failed member is done too; see status())."""
         out = np.zeros(len(self.sims), np.int32)
         self._lib.mm_batch_done(self._ptr, out.ctypes.data)
+        self._status = out
         return out.astype(bool)
 
     def enqueue(self, n):
@@ -1009,10 +1020,21 @@ class Batch:
         return bool(self._lib.mm_batch_ready(self._ptr))
 
     def flags(self):
-        """Done flags after the last enqueue (waits for it if needed)."""
+        """Done flags after the last enqueue (waits for it if needed); a
+        failed member is done too (see status())."""
         out = np.zeros(len(self.sims), np.int32)
         self._lib.mm_batch_flags(self._ptr, out.ctypes.data)
+        self._status = out
         return out.astype(bool)
+
+    def status(self):
+        """Per-member status as of the last flags() / done(): 0 running,
+        1 done, 2 failed (stopped on a NaN dt, rk.go:448-452, or a NaN or
+        infinite torque or error estimate)."""
+        return self._status.copy()
+
+    def failed(self):
+        return self._status == 2
 
     def remaining(self):
         """Each member's remaining trials at its current dt after the last
@@ -1023,7 +1045,9 @@ class Batch:
 
     def run(self, seconds, poll=16):
         sec = self._seconds(seconds)
-        return int(self._lib.mm_batch_run(self._ptr, sec.ctypes.data, int(poll)))
+        n = int(self._lib.mm_batch_run(self._ptr, sec.ctypes.data, int(poll)))
+        self.done()
+        return n
 
     def free(self):
         if self._ptr:

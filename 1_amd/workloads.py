@@ -153,6 +153,38 @@ def sweep_member(seed, j, duration=0.5e-9, sizes=SWEEP_SIZES, thick=SWEEP_THICK,
     )
 
 
+def padded_length(n):
+    """The engine's padded length of an axis of n cells: the smallest 2^a or
+    3 * 2^a >= 2n (1 for n = 1), the same linear convolution as the reference's
+    2n (kernel.go:79-90)."""
+    if n == 1:
+        return 1
+    best, a = None, 1
+    while a < 4 * n:
+        for c in (a, 3 * a):
+            if c >= 2 * n and (best is None or c < best):
+                best = c
+        a *= 2
+    return best
+
+
+def sweep_length_cover(cfgs):
+    """Indices of members of a sweep that together take every padded Z length
+    and every (padded X, padded Y) pair the sweep holds -- the length classes
+    of the batched passes (greedy, smallest members first)."""
+    keys = []
+    for c in cfgs:
+        P = [padded_length(n) for n in c["size"]]
+        keys.append({("M", P[2] // 2), ("XY", P[0], P[1])})
+    need = set().union(*keys)
+    pick = []
+    while need:
+        j = max(range(len(cfgs)), key=lambda j: (len(keys[j] & need), -int(np.prod(cfgs[j]["size"]))))
+        pick.append(j)
+        need -= keys[j]
+    return sorted(pick)
+
+
 def sweep(n=512, seed=4, duration=0.5e-9, **kw):
     """The batched sweep: n independent simulations (see sweep_member)."""
     return [sweep_member(seed, j, duration, **kw) for j in range(n)]

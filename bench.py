@@ -92,7 +92,7 @@ def cpu_baseline(cfg, seconds):
         if el >= seconds or (el / n) * (n + 1) > 2.0 * seconds:
             break
     N = int(np.prod(cfg["size"]))
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    cores = used_cores()
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -109,7 +109,33 @@ def cpu_baseline(cfg, seconds):
         "kind": "port",
         "sample": "%d RK4 step(s) of the same 256x256x16 workload in %.1f s (FP64 oracle, OpenMP, kernel setup "
                   "%.1f s excluded); cpu: %s" % (n, el, setup, model),
-    }
+    }, s, n
+
+
+def used_cores():
+    """Threads the OpenMP oracle runs on: OMP_NUM_THREADS if set, bounded by
+    the CPUs this process may run on (its affinity mask)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(omp, avail) if omp > 0 else avail
+
+
+def parity_vs_oracle(mm, cfg, o, n):
+    """The GPU engine run from the same m0 for the same n RK4 steps as the
+    oracle sim `o` of the CPU leg: max |dm| and the relative energy error of
+    the final state (FP64 device sum against the FP64 oracle)."""
+    s = mm.workloads.make_sim(cfg)
+    s.steps(n)
+    m = s.m
+    e_gpu = s.energy()
+    s.free()
+    e_o = o.energy()
+    return {"vs": "FP64 CPU oracle, same m0 and dt, %d RK4 step(s)" % n,
+            "max_abs_dm": float(np.abs(m - o.m.reshape(m.shape)).max()),
+            "rel_energy_err": float(abs(e_gpu - e_o) / abs(e_o))}
 
 
 def calibrate_copy(lib, n=32 << 20, reps=3):
@@ -486,7 +512,8 @@ def main():
     if args.calibrate:
         out["calibration"] = calibrate_copy(lib)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+        out["cpu_baseline"], o, n = cpu_baseline(cfg, args.cpu_seconds)
+        out["parity"] = parity_vs_oracle(mm, cfg, o, n)
     print(json.dumps(out), flush=True)
 
 

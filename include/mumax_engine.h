@@ -99,6 +99,9 @@ void mm_sim_state(mmSim* s, double* st);
 /* per-trial schedule (dt internal, accepted flag) of the last `max` trials since
  * creation, oldest first; returns the number written */
 int mm_sim_trial_log(mmSim* s, double* dt, int* accept, int max);
+/* the error estimate of the same trials (0 for fixed-step tableaux): the max-abs
+ * of m_low - m_high from which each trial's next dt was chosen (rk.go:403-420) */
+int mm_sim_trial_errors(mmSim* s, float* err, int max);
 
 /* one evaluation of the effective field / torque of m (host 3N floats) */
 void mm_sim_heff(mmSim* s, const float* m, float* h);
@@ -189,8 +192,11 @@ void mm_batch_start(mmBatch* b, const double* seconds, const long long* steps);
 int mm_batch_trials(mmBatch* b, int n);
 /* The same without waiting: enqueue n batched trials and the copy of the done
  * flags on the batch's own stream.  mm_batch_ready is 1 once that work has
- * finished; mm_batch_flags then fills done[n_sims] (may be null) and returns
- * the simulations not yet done (it waits if the work is still running).
+ * finished; mm_batch_flags then fills done[n_sims] (may be null) with each
+ * simulation's status -- 0 running, 1 done (stop time or step target reached),
+ * 2 failed (stopped because dt became NaN, rk.go:448-452, or its torque or error
+ * estimate was NaN or infinite) -- and returns the simulations still running
+ * (it waits if the work is still running).
  * Several batches on their own streams can run concurrently this way. */
 void mm_batch_enqueue(mmBatch* b, int n);
 int mm_batch_ready(mmBatch* b);
@@ -201,7 +207,7 @@ int mm_batch_flags(mmBatch* b, int* done);
  * mm_batch_flags.  For schedulers that give long-running members their own
  * batch.  Returns n_sims. */
 int mm_batch_remaining(mmBatch* b, float* rem);
-/* done[j] = 1 once simulation j reached its stop time / step target */
+/* done[j]: status of simulation j as in mm_batch_flags (0 running, 1 done, 2 failed) */
 void mm_batch_done(mmBatch* b, int* done);
 /* mm_batch_start(seconds) then trials until all are done (polling every
  * `poll` trials); returns the batched trials enqueued */

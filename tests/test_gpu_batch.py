@@ -81,3 +81,83 @@ def test_batch_rk4_fixed_step_equals_alone(mm):
     for s, m in zip(sims, ref):
         assert np.array_equal(s.m, m)
         assert s.state()["steps"] == 30
+
+
+def test_sweep_length_classes_batched_equal_alone_and_oracle(mm, orc):
+    """The 512-simulation sweep (seed 4) at its own sizes: one member per
+    length class of the batched passes (every padded Z length, every padded
+    (X, Y) pair) plus the 256x256x4 member (padded 8 x 512 x 512, the largest
+    plane-pass class), 20 Dormand-Prince steps each: batched == alone bit for
+    bit (trial log and m), and the largest member's trials replayed in the
+    FP64 oracle within 1e-5."""
+    cfgs = mm.workloads.sweep(512, seed=4)
+    pick = mm.workloads.sweep_length_cover(cfgs)
+    big = [j for j, c in enumerate(cfgs) if c["size"] == (4, 256, 256)]
+    assert big
+    pick = sorted(set(pick) | {big[0]})
+    sel = [cfgs[j] for j in pick]
+    nsteps = 20
+    ref = []
+    for c in sel:
+        s = mm.workloads.make_sim(c)
+        s.steps(nsteps)
+        st = s.state()
+        assert (int(st["px"]), int(st["py"]), int(st["pz"])) == tuple(mm.workloads.padded_length(n) for n in c["size"])
+        ref.append((s.m, s.trial_log()))
+        s.free()
+    sims = [mm.workloads.make_sim(c) for c in sel]
+    b = mm.Batch(sims)
+    b.start(steps=nsteps)
+    while b.trials(8):
+        pass
+    assert (b.status() == 1).all()
+    for j, (s, (m, (dt, acc))) in enumerate(zip(sims, ref)):
+        dt2, acc2 = s.trial_log()
+        assert np.array_equal(dt2, dt) and np.array_equal(acc2, acc), pick[j]
+        assert np.array_equal(s.m, m), (pick[j], float(np.abs(s.m - m).max()))
+        assert s.state()["steps"] == nsteps
+    b.free()
+    # the largest member in the FP64 oracle, replaying its trials
+    k = pick.index(big[0])
+    c = sel[k]
+    dts, acc = ref[k][1]
+    o = orc.sim(c["m0"], **mm.workloads.oracle_kwargs(c))
+    i = 0
+    while i < len(dts):
+        j = i
+        while not acc[j] and j - i < 9:
+            j += 1
+        o.step_replay(dts[i:j + 1], acc[i:j + 1])
+        i = j + 1
+    d = float(np.abs(ref[k][0] - o.m.reshape(ref[k][0].shape)).max())
+    print("sweep cover: %d members %s; 256x256x4 member %d after %d steps (%d trials): max|dm| vs FP64 %.3g"
+          % (len(pick), pick, big[0], nsteps, len(dts), d))
+    assert d < 1e-5
+    for s in sims:
+        s.free()
+
+
+def test_failed_member_is_reported_and_batch_mates_unaffected(mm):
+    """A member whose m0 holds a NaN cell stops as failed (the reference stops
+    a run whose dt turns NaN, rk.go:448-452); its batch-mates compute what they
+    compute alone, bit for bit."""
+    cfgs = [mm.workloads.sweep_member(12, j, duration=0.004e-9, **SMALL) for j in range(4)]
+    bad = dict(cfgs[1])
+    m0 = bad["m0"].copy()
+    m0[0, 0, 0, 3] = np.nan
+    bad["m0"] = m0
+    cfgs[1] = bad
+    ref = {j: alone(mm, c, c["duration"]) for j, c in enumerate(cfgs) if j != 1}
+    sims = [mm.workloads.make_sim(c) for c in cfgs]
+    b = mm.Batch(sims)
+    b.run([c["duration"] for c in cfgs])
+    st = b.status()
+    assert st[1] == 2 and list(np.delete(st, 1)) == [1, 1, 1], st
+    assert b.failed().tolist() == [False, True, False, False]
+    assert sims[1].state()["steps"] <= 2
+    for j, (m, (dt, acc), _) in ref.items():
+        assert np.array_equal(sims[j].trial_log()[0], dt)
+        assert np.array_equal(sims[j].m, m), j
+    b.free()
+    for s in sims:
+        s.free()

@@ -72,7 +72,14 @@ CASES = [
     ((2, 48, 80), (3e-9, 3e-9, 3e-9)),                # padded 4 x 96 x 192
     ((6, 40, 24), (3e-9, 2e-9, 2.5e-9)),              # padded 12 x 96 x 48
     ((1, 96, 384), (3e-9, 3e-9, 3e-9)),               # padded 192 x 768
+    # full 3-D path (padded X >= 24: Z forward, Y forward, X.K.X with the
+    # compressed kernel's sign restoration, Y inverse, fused Z inverse)
+    ((12, 32, 32), (4e-9, 4e-9, 4e-9)),               # padded 24 x 64 x 64 (X = 3 * 2^3)
+    ((16, 64, 64), (4e-9, 4e-9, 4e-9)),               # padded 32 x 128 x 128
+    ((10, 20, 48), (2.5e-9, 3e-9, 2e-9)),             # padded 24 x 48 x 96, anisotropic cells
+    ((24, 12, 16), (3e-9, 3e-9, 3e-9)),               # padded 48 x 24 x 32 (thick)
 ]
+CASES_3D = CASES[7:]
 
 
 def test_padded_lengths():
@@ -122,7 +129,7 @@ def test_heff_and_torque_vs_oracle(mm, orc, size, cell, mode):
     assert np.abs(k - kref).max() < 5e-6 * np.abs(href).max()
 
 
-@pytest.mark.parametrize("size,cell", CASES[:2])
+@pytest.mark.parametrize("size,cell", CASES[:2] + CASES_3D)
 def test_rk4_steps_vs_oracle(mm, orc, size, cell):
     m0 = smooth_m(size, 3)
     dt = 2e-14
@@ -138,6 +145,54 @@ def test_rk4_steps_vs_oracle(mm, orc, size, cell):
     st = s.state()
     assert st["steps"] == 20
     assert st["time"] == pytest.approx(o.state()["time"], rel=1e-6)
+
+
+def pass_counts(mm, s, steps=1):
+    """Launch counts per pass class over `steps` profiled steps of `s`:
+    [Z forward, Y forward, X.K.X / plane / 2-D, Y inverse, Z inverse, ...]."""
+    lib = mm.load()
+    ms = np.zeros(8)
+    n = np.zeros(8, np.int32)
+    lib.mm_sim_profile_steps(s._ptr, steps, ms.ctypes.data, n.ctypes.data)
+    return n
+
+
+@pytest.mark.parametrize("size,cell", CASES_3D)
+def test_full_3d_path_runs_separate_passes(mm, size, cell):
+    """The CASES_3D grids take the separate Y forward, X.K.X and Y inverse
+    passes (not the thin-plane or 2-D pass), so the parity tests above pin them."""
+    s = engine(mm, size, cell, solver="rk4", dt=1e-14, m0=random_m(size, 0))
+    n = pass_counts(mm, s)
+    assert n[1] == n[2] == n[3] == 4 and n[0] == n[4] == 4, n
+    s.free()
+
+
+def test_cuboid_256x256x16_vs_oracle(mm, orc):
+    """The benchmark's workload at full size (256x256x16 cells of 4 nm, padded
+    32 x 512 x 512; k_zfwd / k_ypass<512> / k_xmul<32> / k_zinv_epi): H_eff and
+    the torque of the seeded random m0 (5e-6 relative to max |H|), then 20
+    fixed-dt RK4 steps (1e-14 s) against the FP64 oracle (max |dm| <= 1e-5)."""
+    cfg = mm.workloads.cuboid_rk4()
+    s = mm.workloads.make_sim(cfg)
+    o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+    m0 = cfg["m0"]
+    h = s.heff(m0)
+    href = o.heff(m0).reshape(h.shape)
+    eh = float(np.abs(h - href).max() / np.abs(href).max())
+    k = s.torque(m0)
+    kref = o.torque(m0).reshape(k.shape)
+    ek = float(np.abs(k - kref).max() / np.abs(href).max())
+    s.steps(20)
+    for _ in range(20):
+        o.step()
+    dm = float(np.abs(s.m - o.m.reshape(s.m.shape)).max())
+    print("cuboid 256x256x16: H_eff rel %.3g, torque rel %.3g, 20 RK4 steps max|dm| %.3g" % (eh, ek, dm))
+    assert eh < 5e-6 and ek < 5e-6
+    assert dm < 1e-5
+    assert s.state()["time"] == pytest.approx(o.state()["time"], rel=1e-6)
+    n = pass_counts(mm, s)
+    assert n[1] == n[3] == 4, n
+    s.free()
 
 
 def replay_log(o, dts, acc, start=0, stop=None):
@@ -180,6 +235,81 @@ def test_dp_run_replayed_in_oracle(mm, orc, mode):
     ref = o.m.reshape(got.shape)
     assert np.abs(got - ref).max() < 1e-5
     assert s.state()["steps"] == o.state()["steps"]
+
+
+def _f32_next_dt(dt, err, max_error, errororder):
+    """The step-size update rk.go:404-420 in the reference's float32 host
+    arithmetic: factor = float32((maxError/err)^(1/errororder)) clamped to
+    [0.01, 2], dt' = dt * factor."""
+    q = np.float32(max_error) / np.float32(err)
+    f = np.float32(np.power(np.float64(q), 1.0 / errororder))
+    f = np.float32(min(max(f, np.float32(0.01)), np.float32(2.0)))
+    return np.float32(np.float32(dt) * f)
+
+
+CTRL_CASES = [("film_128x32x1", None), ("cuboid_4x32x32", (4, 32, 32))]
+
+
+def _ctrl_cfg(mm, which):
+    if which[1] is None:
+        return mm.workloads.film_sp4_like(seed=1)
+    cfg = mm.workloads.cuboid_rk4(size=which[1])
+    cfg.update(solver="rkdp", maxerror=1e-5, dt=0.0)
+    return cfg
+
+
+@pytest.mark.parametrize("mode", ["fused", "abi"])
+@pytest.mark.parametrize("which", CTRL_CASES, ids=[c[0] for c in CTRL_CASES])
+def test_device_controller_follows_the_reference_update(mm, which, mode):
+    """Every logged trial of the engine's own Dormand-Prince run (no replay):
+    accepted exactly when err <= 2 maxError (rk.go:421-428), and the next
+    trial's dt is the float32 update of rk.go:404-420 from that trial's error
+    estimate (growth clamp 2, shrink clamp 0.01, errororder 4), to 1 ulp."""
+    cfg = _ctrl_cfg(mm, which)
+    s = mm.workloads.make_sim(cfg, mode)
+    s.steps(200)
+    dts, acc = s.trial_log()
+    errs = s.trial_errors()
+    assert len(dts) == len(errs) >= 200
+    assert dts[0] == np.float32(1e-4)
+    n_clamped = 0
+    for i in range(len(dts) - 1):
+        assert bool(acc[i]) == (errs[i] <= np.float32(2e-5)), i
+        want = _f32_next_dt(dts[i], errs[i], 1e-5, 4)
+        got = np.float32(dts[i + 1])
+        assert abs(int(got.view(np.int32)) - int(want.view(np.int32))) <= 1, (i, got, want, errs[i])
+        n_clamped += bool(np.float32(1e-5) / errs[i] >= 16)
+    assert n_clamped >= 5  # the doubling phase ran through the growth clamp
+
+
+@pytest.mark.parametrize("which", CTRL_CASES, ids=[c[0] for c in CTRL_CASES])
+def test_device_controller_own_run_vs_oracle_own_run(mm, orc, which):
+    """The engine and the FP64 oracle each lead their own Dormand-Prince run
+    from the same m0 over the first 200 trials (no schedule replayed): the
+    same accept/reject decisions, and dt within twice the deviation of the
+    FP32 twin leading its own run (or 1e-5 relative): dt is chosen from an
+    error estimate of ~1e-5, in which float32 fields leave a relative noise of
+    ~1e-4 that the controller carries into the next dt."""
+    from oracle import Oracle
+
+    cfg = _ctrl_cfg(mm, which)
+    s = mm.workloads.make_sim(cfg, "fused")
+    s.steps(200)
+    dg, ag = s.trial_log()
+    runs = []
+    for O in (orc, Oracle("f32")):
+        o = O.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+        while len(o.trial_log()[0]) < 200:
+            o.step()
+        runs.append(o.trial_log())
+    (d64, a64), (d32, a32) = runs
+    n = 200
+    e_gpu = float(np.abs(dg[:n] / d64[:n] - 1).max())
+    e_32 = float(np.abs(d32[:n] / d64[:n] - 1).max())
+    print("%s: first %d trials, rejects GPU %d, FP64 %d, FP32 twin %d; max rel. dt deviation from FP64: GPU %.3g, "
+          "FP32 twin %.3g" % (which[0], n, n - ag[:n].sum(), n - a64[:n].sum(), n - a32[:n].sum(), e_gpu, e_32))
+    assert np.array_equal(ag[:n], a64[:n])
+    assert e_gpu <= max(1e-5, 2 * e_32)
 
 
 def test_film_1ns_dp_inside_fp32_envelope_of_fp64_oracle(mm, orc):

@@ -364,6 +364,75 @@ def test_forced_reject_path(orc):
     assert err < 1e-5
 
 
+# ------------------------------------------------------------------ step-size controller (rk.go:404-420)
+def _ring(orc, solver, n=64, maxerror=1e-5, dt=0.0):
+    """n uncoupled cells (no demag, no exchange) precessing undamped about
+    internal X at one tilt, their azimuths spread evenly over the circle: the
+    max-abs of the error estimate over all cells no longer depends on the
+    precession phase, so err is a function of dt alone, err ~ C dt^(p+1) for an
+    embedded estimate of local order p+1."""
+    size = (1, 1, n)
+    ph = 2 * np.pi * np.arange(n) / n
+    m0 = np.zeros((3,) + size, np.float32)
+    m0[0, 0, 0] = 0.6
+    m0[1, 0, 0] = 0.8 * np.cos(ph)
+    m0[2, 0, 0] = 0.8 * np.sin(ph)
+    u = orc.units(8e5, 1.3e-11)
+    return orc.sim(m0, size=size, cellsize=(3e-9,) * 3, msat=8e5, aexch=1.3e-11, alpha=0.0, solver=solver,
+                   maxerror=maxerror, dt=dt * u["time"], want_demag=False, exch_type=0,
+                   field=(np.zeros(1), np.array([[0.1, 0.0, 0.0]])))
+
+
+@pytest.mark.parametrize("solver", ["rkdp", "rkck", "rk23", "rk12"])
+def test_controller_growth_is_clamped_at_two(orc, solver):
+    """From the default first dt (1e-4 internal, sim.go:30) the error is far
+    below maxError, so the first trials double dt exactly: the factor
+    (maxError/err)^(1/errororder) is clamped to 2 (rk.go:409-411)."""
+    s = _ring(orc, solver)
+    for _ in range(9):
+        s.step()
+    dt, acc = s.trial_log()
+    assert dt[0] == 1e-4
+    assert np.all(acc[:9] == 1)
+    assert np.array_equal(dt[1:9], dt[:8] * 2.0)
+
+
+def test_controller_shrink_is_clamped_at_one_hundredth(orc):
+    """A first dt far beyond the error bar is rejected and the retry uses
+    exactly 0.01 of it: the factor's lower clamp (rk.go:412-414) and the reject
+    test err > 2 maxError (rk.go:421-428)."""
+    s = _ring(orc, "rkdp", dt=5000.0)
+    s.step()
+    dt, acc = s.trial_log()
+    assert acc[0] == 0 and dt[1] == dt[0] * 0.01
+
+
+@pytest.mark.parametrize("solver,order,lo,hi", [("rkdp", 4, -0.30, -0.22), ("rkck", 4, -0.33, -0.22),
+                                                ("rk23", 2, -0.56, -0.44)])
+def test_controller_log_dt_contracts_by_one_minus_local_order_over_errororder(orc, solver, order, lo, hi):
+    """With err = C dt^(order+1) the update l' = l + (log maxError - log err) /
+    errororder of l = log dt moves a deviation from the fixed point by the
+    factor 1 - (order+1)/errororder: -1/4 for Dormand-Prince and Cash-Karp
+    (errororder 4, rk.go:196-212, 170-195) and -1/2 for Bogacki-Shampine
+    (errororder 2, rk.go:160-169).  An errororder of 5 would give 0 (the
+    first correction lands on the fixed point) and one of 3 -2/3.  Checked as
+    the ratio of successive log-dt steps once the doubling phase is over,
+    while the steps stand well above rounding; the controller then sits at
+    err = maxError (no safety factor)."""
+    s = _ring(orc, solver)
+    for _ in range(40):
+        s.step()
+    dt, acc = s.trial_log()
+    d = np.diff(np.log(dt))
+    k = int(np.argmax(np.abs(d - np.log(2.0)) > 1e-12))  # first step that is not a doubling
+    big = d[k:][np.abs(d[k:]) > 2e-3]
+    n = int(np.argmax(np.abs(d[k:]) <= 2e-3))
+    ratios = d[k + 1:k + n] / d[k:k + n - 1]
+    assert len(ratios) >= 3, (big, ratios)
+    assert np.all((ratios > lo) & (ratios < hi)), ratios
+    assert abs(s.state()["error"] / 1e-5 - 1) < 2e-3
+
+
 # ------------------------------------------------------------------ energy (energy.go, field.go:64-87)
 def _energy_sim(orc, m0, **kw):
     cell = kw.pop("cellsize", (2e-9, 2e-9, 2e-9))
