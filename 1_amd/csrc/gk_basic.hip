@@ -41,11 +41,16 @@ void sync() {
     GK_CHECK(hipGetLastError());
     GK_CHECK(hipStreamSynchronize(stream()));
 }
+static thread_local bool g_tables_frozen = false;
+void tables_frozen(bool on) { g_tables_frozen = on; }
+bool tables_frozen() { return g_tables_frozen; }
+
 template <typename C, typename R>
 static C* make_table(std::map<int, C*>& cache, int L) {
     std::lock_guard<std::mutex> lk(g_tw_mutex);
     auto it = cache.find(L);
     if (it != cache.end()) return it->second;
+    GK_ASSERT(!g_tables_frozen, "twiddle table of length %d is missing in a call that may not allocate", L);
     std::vector<C> t(L);
     for (int k = 0; k < L; k++) {
         long double a = -2.0L * 3.141592653589793238462643383279L * (long double)k / (long double)L;

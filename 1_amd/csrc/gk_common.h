@@ -56,4 +56,10 @@ inline dim3 grid1d(long n, int block = kBlock) {
 const float2* twiddle_f32(int L);
 const double2* twiddle_f64(int L);
 
+// While set on the calling thread, a table lookup that would have to build
+// (allocate) a missing table aborts with a message instead: the no-allocation
+// entry points run with it set.
+void tables_frozen(bool on);
+bool tables_frozen();
+
 }  // namespace gk

@@ -24,14 +24,30 @@ void fft_rows_r2c(const float* in, float2* out, int P, long nrows, long n_inner,
 void fft_rows_c2r(const float2* in, float* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
                   long out_inner, long out_outer, int n_real, hipStream_t s);
 
+// Device memory for the temporaries of the any-length transforms below: when
+// `scr` is given, their packed rows and Bluestein buffers are carved from it
+// (stream-ordered reuse, nothing allocated or freed); else they are allocated
+// per call.  `cap` counts float2 elements.
+struct FFTScratch {
+    float2* base = nullptr;
+    size_t cap = 0;
+};
+
 // The same for any length (gk_fft_any.hip): power-of-two lengths go to the
 // kernels above, lengths with factors 2, 3, 5, 7 to LDS Stockham passes with a
 // runtime radix list, others through Bluestein's chirp-z convolution.
 template <typename T, int DIR>
 void fft_lines_any(C2<T>* data, int L, long nlines, long n_inner, long inner, long outer, long estride, int nvalid,
-                   int nkeep, hipStream_t s);
+                   int nkeep, hipStream_t s, const FFTScratch* scr = nullptr);
 void fft_rows_r2c_any(const float* in, float2* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
-                      long out_inner, long out_outer, int n_real, hipStream_t s);
+                      long out_inner, long out_outer, int n_real, hipStream_t s, const FFTScratch* scr = nullptr);
+// float2 elements of scratch the FP32 calls above need for these sizes (0: none)
+size_t fft_lines_any_scratch(int L, long nlines);
+size_t fft_rows_r2c_any_scratch(int P, long nrows);
+// Build (once, cached) every twiddle / chirp table the FP32 transforms of this
+// length and direction read, so that a later call finds them all.
+void fft_lines_any_tables(int L, int dir);
+void fft_rows_any_tables(int P, int dir);
 void fft_rows_c2r_any(const float2* in, float* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
                       long out_inner, long out_outer, int n_real, hipStream_t s);
 

@@ -12,6 +12,7 @@
 // No scratch zeroing and no separate pad/unpad launches are needed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -271,6 +272,29 @@ void fft3_c2c_f64(double2* d, const int n[3], int dir, hipStream_t s) {
 // drop-in plan
 using namespace gk;
 
+// Scratch of gpuFFT3dPlan_forward_noalloc: the packed rows and Bluestein buffers
+// of the any-length transforms, 4 Mi float2 (32 MiB) per device, held by every
+// device the library runs on until the process ends.  Host code never passes
+// `scratch` itself to a kernel: it takes the device address that
+// hipGetSymbolAddress gives on each device (scratch_device_ptr).
+constexpr size_t kFFTScratchElems = size_t(4) << 20;
+static __device__ __align__(256) float2 scratch[kFFTScratchElems];
+
+static float2* scratch_device_ptr() {
+    static std::mutex mu;
+    static std::map<int, float2*> ptr;  // device -> address of its copy of `scratch`
+    int dev = 0;
+    GK_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptr.find(dev);
+    if (it != ptr.end()) return it->second;
+    void* a = nullptr;
+    GK_CHECK(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    GK_ASSERT(a != nullptr, "no device address for the FFT scratch array");
+    ptr[dev] = static_cast<float2*>(a);
+    return ptr[dev];
+}
+
 struct gpuFFT3dPlan {
     int size[3];
     int padded[3];
@@ -291,6 +315,17 @@ gpuFFT3dPlan* new_gpuFFT3dPlan_padded(int* size, int* paddedSize) {
         p->padded[i] = paddedSize[i];
     }
     p->pzc = paddedSize[2] / 2 + 1;
+    // the twiddle and chirp tables of both directions are built here, once, so
+    // that no transform call has to allocate them (without a device they are
+    // built by the first transform instead)
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {
+        for (int dir = -1; dir <= 1; dir += 2) {
+            fft_rows_any_tables(paddedSize[2], dir);
+            if (paddedSize[1] > 1) fft_lines_any_tables(paddedSize[1], dir);
+            if (paddedSize[0] > 1) fft_lines_any_tables(paddedSize[0], dir);
+        }
+    }
     return p;
 }
 
@@ -309,6 +344,36 @@ void gpuFFT3dPlan_forward(gpuFFT3dPlan* p, float* input, float* output) {
     if (P1 > 1) fft_lines_any<float, -1>(out, P1, (long)N0 * Q, Q, 1, (long)P1 * Q, Q, N1, P1, s);
     // X: every (ky,kz) column; planes x>=N0 are zero padding and not read
     if (P0 > 1) fft_lines_any<float, -1>(out, P0, (long)P1 * Q, (long)P1 * Q, 1, 0, (long)P1 * Q, N0, P0, s);
+    sync();
+}
+
+// float2 elements of scratch the forward transform of plan p needs: its three
+// passes run one after the other on one stream, so they share it
+static size_t forward_scratch_elems(const gpuFFT3dPlan* p) {
+    const long N0 = p->size[0], N1 = p->size[1];
+    const int P0 = p->padded[0], P1 = p->padded[1], P2 = p->padded[2], Q = p->pzc;
+    size_t need = fft_rows_r2c_any_scratch(P2, N0 * N1);
+    if (P1 > 1) need = std::max(need, fft_lines_any_scratch(P1, N0 * Q));
+    if (P0 > 1) need = std::max(need, fft_lines_any_scratch(P0, (long)P1 * Q));
+    return need;
+}
+
+void gpuFFT3dPlan_forward_noalloc(gpuFFT3dPlan* p, float* input, float* output) {
+    const size_t need = forward_scratch_elems(p);
+    GK_ASSERT(need <= kFFTScratchElems,
+              "gpuFFT3dPlan_forward_noalloc: padded size %d x %d x %d needs %zu scratch elements (8 bytes each); "
+              "the maximum is %zu (use gpuFFT3dPlan_forward)",
+              p->padded[0], p->padded[1], p->padded[2], need, kFFTScratchElems);
+    const int N0 = p->size[0], N1 = p->size[1], N2 = p->size[2];
+    const int P0 = p->padded[0], P1 = p->padded[1], P2 = p->padded[2], Q = p->pzc;
+    const FFTScratch sc{need ? scratch_device_ptr() : nullptr, kFFTScratchElems};
+    float2* out = (float2*)output;
+    hipStream_t s = stream();
+    tables_frozen(true);
+    fft_rows_r2c_any(input, out, P2, (long)N0 * N1, N1, N2, (long)N1 * N2, Q, (long)P1 * Q, N2, s, &sc);
+    if (P1 > 1) fft_lines_any<float, -1>(out, P1, (long)N0 * Q, Q, 1, (long)P1 * Q, Q, N1, P1, s, &sc);
+    if (P0 > 1) fft_lines_any<float, -1>(out, P0, (long)P1 * Q, (long)P1 * Q, 1, 0, (long)P1 * Q, N0, P0, s, &sc);
+    tables_frozen(false);
     sync();
 }
 

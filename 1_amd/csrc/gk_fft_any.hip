@@ -134,6 +134,7 @@ static const BlueTables& blue_tables(int L, int dir) {
     std::lock_guard<std::mutex> lock(mu);
     auto it = cache.find({L, dir});
     if (it != cache.end()) return it->second;
+    GK_ASSERT(!tables_frozen(), "chirp table of length %d is missing in a call that may not allocate", L);
     BlueTables t;
     t.L = L;
     t.Lp = 1;
@@ -197,9 +198,32 @@ static int grid_for(long n) {
     return (int)(g < 1 ? 1 : g);
 }
 
+static int blue_length(int L) {
+    int Lp = 1;
+    while (Lp < 2 * L - 1) Lp *= 2;
+    return Lp;
+}
+
+size_t fft_lines_any_scratch(int L, long nlines) {
+    if (nlines <= 0 || fft_length_supported(L)) return 0;
+    RadixList rl;
+    if (factor_radices(L, rl)) return 0;
+    return (size_t)nlines * blue_length(L);
+}
+
+void fft_lines_any_tables(int L, int dir) {
+    RadixList rl;
+    if (fft_length_supported(L) || factor_radices(L, rl)) {
+        twiddle_f32(L);
+        return;
+    }
+    blue_tables(L, dir);
+    twiddle_f32(blue_length(L));
+}
+
 template <typename T, int DIR>
 void fft_lines_any(C2<T>* data, int L, long nlines, long n_inner, long inner, long outer, long estride, int nvalid,
-                   int nkeep, hipStream_t s) {
+                   int nkeep, hipStream_t s, const FFTScratch* sc) {
     if (nlines <= 0) return;
     if (fft_length_supported(L)) {
         fft_lines<T, DIR>(data, L, nlines, n_inner, inner, outer, estride, nvalid, nkeep, s);
@@ -221,7 +245,13 @@ void fft_lines_any(C2<T>* data, int L, long nlines, long n_inner, long inner, lo
         const BlueTables& bt = blue_tables(L, DIR);
         GK_ASSERT(bt.Lp <= 4096 * 2 && fft_length_supported(bt.Lp), "FFT length %d is not supported", L);
         float2* scr = nullptr;
-        GK_CHECK(hipMalloc(&scr, nlines * bt.Lp * sizeof(float2)));
+        if (sc) {
+            GK_ASSERT((size_t)nlines * bt.Lp <= sc->cap, "FFT scratch of %zu elements, %ld needed", sc->cap,
+                      nlines * (long)bt.Lp);
+            scr = sc->base;
+        } else {
+            GK_CHECK(hipMalloc(&scr, nlines * bt.Lp * sizeof(float2)));
+        }
         k_blue_pre<<<grid_for(nlines * bt.Lp), 256, 0, s>>>(data, scr, bt.w, L, bt.Lp, nlines, n_inner, inner, outer,
                                                           estride, nvalid);
         GK_LAUNCH_CHECK();
@@ -232,17 +262,23 @@ void fft_lines_any(C2<T>* data, int L, long nlines, long n_inner, long inner, lo
         k_blue_post<<<grid_for(nlines * nkeep), 256, 0, s>>>(data, scr, bt.w, bt.Lp, nlines, n_inner, inner, outer,
                                                            estride, nkeep);
         GK_LAUNCH_CHECK();
-        GK_CHECK(hipStreamSynchronize(s));
-        GK_CHECK(hipFree(scr));
+        if (!sc) {
+            GK_CHECK(hipStreamSynchronize(s));
+            GK_CHECK(hipFree(scr));
+        }
     } else {
         GK_ASSERT(false, "FP64 FFT length %d is not supported", L);
     }
 }
 
-template void fft_lines_any<float, -1>(float2*, int, long, long, long, long, long, int, int, hipStream_t);
-template void fft_lines_any<float, +1>(float2*, int, long, long, long, long, long, int, int, hipStream_t);
-template void fft_lines_any<double, -1>(double2*, int, long, long, long, long, long, int, int, hipStream_t);
-template void fft_lines_any<double, +1>(double2*, int, long, long, long, long, long, int, int, hipStream_t);
+template void fft_lines_any<float, -1>(float2*, int, long, long, long, long, long, int, int, hipStream_t,
+                                       const FFTScratch*);
+template void fft_lines_any<float, +1>(float2*, int, long, long, long, long, long, int, int, hipStream_t,
+                                       const FFTScratch*);
+template void fft_lines_any<double, -1>(double2*, int, long, long, long, long, long, int, int, hipStream_t,
+                                        const FFTScratch*);
+template void fft_lines_any<double, +1>(double2*, int, long, long, long, long, long, int, int, hipStream_t,
+                                        const FFTScratch*);
 
 // ---------------------------------------------------------------- real rows of any length
 // row r: input at (r / n_inner) in_outer + (r % n_inner) in_inner (n_real floats),
@@ -310,8 +346,27 @@ __global__ void k_rows_unpack(const float2* __restrict__ Y, float* __restrict__ 
     }
 }
 
+size_t fft_rows_r2c_any_scratch(int P, long nrows) {
+    if (nrows <= 0 || (P % 2 == 0 && fft_length_supported(P / 2))) return 0;
+    const int H = P % 2 == 0 ? P / 2 : P;
+    return (size_t)nrows * H + fft_lines_any_scratch(H, nrows);
+}
+
+void fft_rows_any_tables(int P, int dir) {
+    if (P % 2 == 0) {
+        twiddle_f32(P);
+        if (fft_length_supported(P / 2)) {
+            twiddle_f32(P / 2);
+            return;
+        }
+        fft_lines_any_tables(P / 2, dir);
+    } else {
+        fft_lines_any_tables(P, dir);
+    }
+}
+
 void fft_rows_r2c_any(const float* in, float2* out, int P, long nrows, long n_inner, long in_inner, long in_outer,
-                      long out_inner, long out_outer, int n_real, hipStream_t s) {
+                      long out_inner, long out_outer, int n_real, hipStream_t s, const FFTScratch* sc) {
     if (nrows <= 0) return;
     if (P % 2 == 0 && fft_length_supported(P / 2)) {
         fft_rows_r2c(in, out, P, nrows, n_inner, in_inner, in_outer, out_inner, out_outer, n_real, s);
@@ -320,10 +375,19 @@ void fft_rows_r2c_any(const float* in, float2* out, int P, long nrows, long n_in
     const bool even = P % 2 == 0;
     const int H = even ? P / 2 : P;
     float2* z = nullptr;
-    GK_CHECK(hipMalloc(&z, nrows * H * sizeof(float2)));
+    FFTScratch rest;  // what the packed rows leave for the line transform
+    if (sc) {
+        GK_ASSERT(fft_rows_r2c_any_scratch(P, nrows) <= sc->cap, "FFT scratch of %zu elements, %zu needed", sc->cap,
+                  fft_rows_r2c_any_scratch(P, nrows));
+        z = sc->base;
+        rest.base = sc->base + (size_t)nrows * H;
+        rest.cap = sc->cap - (size_t)nrows * H;
+    } else {
+        GK_CHECK(hipMalloc(&z, nrows * H * sizeof(float2)));
+    }
     k_rows_pack<<<grid_for(nrows * H), 256, 0, s>>>(in, z, H, nrows, n_inner, in_inner, in_outer, n_real, even);
     GK_LAUNCH_CHECK();
-    fft_lines_any<float, -1>(z, H, nrows, nrows, H, 0, 1, H, H, s);
+    fft_lines_any<float, -1>(z, H, nrows, nrows, H, 0, 1, H, H, s, sc ? &rest : nullptr);
     if (even) {
         k_rows_split<<<grid_for(nrows * (H + 1)), 256, 0, s>>>(z, out, twiddle_f32(P), H, nrows, n_inner, out_inner,
                                                               out_outer);
@@ -332,8 +396,10 @@ void fft_rows_r2c_any(const float* in, float2* out, int P, long nrows, long n_in
                                                                  out_outer);
     }
     GK_LAUNCH_CHECK();
-    GK_CHECK(hipStreamSynchronize(s));
-    GK_CHECK(hipFree(z));
+    if (!sc) {
+        GK_CHECK(hipStreamSynchronize(s));
+        GK_CHECK(hipFree(z));
+    }
 }
 
 void fft_rows_c2r_any(const float2* in, float* out, int P, long nrows, long n_inner, long in_inner, long in_outer,

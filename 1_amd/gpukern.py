@@ -58,6 +58,7 @@ GPUKERN = {
     "new_gpuFFT3dPlan_padded": (P, [P, P]),
     "delete_gpuFFT3dPlan": (None, [P]),
     "gpuFFT3dPlan_forward": (None, [P, P, P]),
+    "gpuFFT3dPlan_forward_noalloc": (None, [P, P, P]),
     "gpuFFT3dPlan_inverse": (None, [P, P, P]),
     "gpuFFT3dPlan_normalization": (I, [P]),
     "gpu_gaussian_noise": (None, [P, F, F, I]),

@@ -116,6 +116,16 @@ void gpuFFT3dPlan_forward(gpuFFT3dPlan* plan, float* input, float* output); /* g
 /* unnormalised inverse; `input` is used as scratch and overwritten, as in the reference */
 void gpuFFT3dPlan_inverse(gpuFFT3dPlan* plan, float* input, float* output); /* gpu_fft0.h:75 */
 int gpuFFT3dPlan_normalization(gpuFFT3dPlan* plan);                        /* gpu_fft0.cu:112 */
+/* No counterpart in the reference.  gpuFFT3dPlan_forward for callers that
+ * cannot allocate device memory (HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same transform and the same bits, but
+ * the temporaries of non-power-of-two lengths (packed rows, Bluestein buffers)
+ * live in a fixed 32 MiB device array per GPU instead of per-call allocations,
+ * and the plan's tables come from new_gpuFFT3dPlan_padded.  Plans whose
+ * temporaries exceed 4 Mi complex elements are rejected (abort with a message).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (other
+ * host threads) race. */
+void gpuFFT3dPlan_forward_noalloc(gpuFFT3dPlan* plan, float* input, float* output);
 
 /* ---- thermal noise (gpu_temperature.h:25) ---- */
 void gpu_gaussian_noise(float* devData, float mean, float stddev, int n); /* gpu_temperature.h:25 */

@@ -55,3 +55,20 @@ def test_python_binding_tables_match_headers():
     declared = declared_functions()
     bound = set(gk.GPUKERN) | set(gk.ENGINE) | set(gk.IO)
     assert declared == bound, (declared - bound, bound - declared)
+
+
+def test_noalloc_forward_rejects_plans_beyond_its_scratch():
+    """gpuFFT3dPlan_forward_noalloc checks its fixed scratch (4 Mi complex
+    elements) before any device call: Z rows of half-length 3001 (prime) over
+    2000 rows need 2000 x 3001 packed elements plus a Bluestein buffer, so it
+    aborts with a message (the library's error behaviour).  Runs in a child
+    process on the CPU; input and output are never touched."""
+    import subprocess
+    import sys
+
+    code = ("import ctypes as C; lib = C.CDLL(%r); lib.new_gpuFFT3dPlan_padded.restype = C.c_void_p; "
+            "p = lib.new_gpuFFT3dPlan_padded((C.c_int*3)(1, 2000, 3001), (C.c_int*3)(1, 2000, 6002)); "
+            "lib.gpuFFT3dPlan_forward_noalloc(C.c_void_p(p), None, None)" % LIB)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "gpuFFT3dPlan_forward_noalloc" in r.stderr and "the maximum is 4194304" in r.stderr, r.stderr
