@@ -645,6 +645,17 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         d.npart = s->npart;
         d.nfin = fin_blocks(s);
         d.t = torque_args(s, 0.f, nullptr);
+        if (s->field.n >= 2) {
+            // the batch keeps its own copy of the pointwise field table, so the
+            // simulation can replace its table in place (mm_sim_set_field)
+            const int nf = s->field.n;
+            double* ft = (double*)balloc(b, nf * sizeof(double));
+            float* fbv = (float*)balloc(b, 3 * nf * sizeof(float));
+            GK_CHECK(hipMemcpy(ft, s->field.t, nf * sizeof(double), hipMemcpyDeviceToDevice));
+            GK_CHECK(hipMemcpy(fbv, s->field.b, 3 * nf * sizeof(float), hipMemcpyDeviceToDevice));
+            d.t.field.t = ft;
+            d.t.field.b = fbv;
+        }
         d.cp = ctrl_params(s);
         d.cp.n_err = d.nfin;
         d.cp.part_e = nullptr;  // the batched epilogue computes no energy

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -324,6 +325,7 @@ void conv_plan_init(ConvPlan& cp, const int N[3]) {
     cp.Q = cp.M + 1;
     cp.nK = cp.P[0] == 1 ? 4 : 6;
     cp.yxy = cp.P[0] > 1 && yxy_fits(cp.P[0], cp.P[1]);
+    cp.plane3 = cp.P[0] > 1 && !cp.yxy && plane3_fits(cp.P[0], cp.P[1], N[0]);
     GK_ASSERT(cp.M <= 2048 && cp.P[1] <= 4096 && cp.P[0] <= 1024, "grid too large for this build (padded %d %d %d)",
               cp.P[0], cp.P[1], cp.P[2]);
     const long hx = cp.P[0] / 2 + 1, hy = cp.P[1] / 2 + 1;
@@ -336,9 +338,80 @@ void conv_plan_init(ConvPlan& cp, const int N[3]) {
     cp.B2 = nullptr;  // layout [3][Q][Nx][Py]
     if (b2) GK_CHECK(hipMalloc(&cp.B2, b2 * sizeof(float2)));
     cp.bytes = kc * sizeof(float) + (b1 + b2) * sizeof(float2);
+    cp.tp = TuneParams{};
+    conv_tune_lookup(N, &cp.tp);
 }
 
-static int epi_rows_rt(int M);
+int conv_epi_blocks_max(const int N[3]) {
+    int P2 = pad_len(N[2]);
+    if (P2 < 2) P2 = 2;
+    const int M = P2 / 2;
+    return div_up(N[1], std::min(conv_epi_rows(M, 0), conv_epi_rows(M, 1))) * N[0];
+}
+
+// ---------------------------------------------------------------- tuning table
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 3>, TuneParams> g_tune;
+}  // namespace
+
+bool conv_tune_lookup(const int N[3], TuneParams* tp) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find({N[0], N[1], N[2]});
+    if (it == g_tune.end()) return false;
+    *tp = it->second;
+    return true;
+}
+
+void conv_tune_store(const int N[3], const TuneParams& tp) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[{N[0], N[1], N[2]}] = tp;
+}
+
+void conv_tune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+int conv_tune_save(const char* path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    fprintf(f, "# launch parameters per data size (internal Nx Ny Nz): zr er yb xb y2b pp\n");
+    for (auto& kv : g_tune) {
+        const TuneParams& t = kv.second;
+        fprintf(f, "%d %d %d %d %d %d %d %d %d\n", kv.first[0], kv.first[1], kv.first[2], t.zr, t.er, t.yb, t.xb,
+                t.y2b, t.pp);
+    }
+    const int n = (int)g_tune.size();
+    return fclose(f) == 0 ? n : -1;
+}
+
+int conv_tune_load(const char* path) {
+    FILE* f = fopen(path, "r");
+    if (!f) return -1;
+    char line[256];
+    int n = 0;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int N[3];
+        TuneParams t;
+        const int got = sscanf(line, "%d %d %d %d %d %d %d %d %d", &N[0], &N[1], &N[2], &t.zr, &t.er, &t.yb, &t.xb,
+                               &t.y2b, &t.pp);
+        if (got == 8) t.pp = 0;
+        else if (got != 9) continue;
+        if (N[0] < 1 || N[1] < 1 || N[2] < 2 || t.yb < 1 || t.xb < 1 || t.y2b < 1 || t.pp < 0 || t.pp > 1) continue;
+        int P2 = pad_len(N[2]);
+        if (P2 < 2) P2 = 2;
+        if (!conv_zr_valid(P2 / 2, t.zr)) t.zr = 0;
+        if (!conv_er_valid(P2 / 2, t.er)) t.er = 0;
+        if (t.pp && !plane3_fits(pad_len(N[0]), pad_len(N[1]), N[0])) t.pp = 0;
+        conv_tune_store(N, t);
+        n++;
+    }
+    fclose(f);
+    return n;
+}
 
 size_t conv_plan_footprint(const int N[3], int* epi_blocks) {
     int P[3];
@@ -349,7 +422,7 @@ size_t conv_plan_footprint(const int N[3], int* epi_blocks) {
     const long kc = P[0] == 1 ? (long)nK * Q * hy : (long)nK * Q * hy * hx;
     const long b1 = 3L * N[0] * Q * N[1];
     const long b2 = (P[0] == 1 || yxy_fits(P[0], P[1])) ? 0 : 3L * Q * P[1] * N[0];
-    if (epi_blocks) *epi_blocks = div_up(N[1], epi_rows_rt(M)) * N[0];
+    if (epi_blocks) *epi_blocks = conv_epi_blocks_max(N);
     return kc * sizeof(float) + (b1 + b2) * sizeof(float2);
 }
 
@@ -570,8 +643,38 @@ template <int M> constexpr int z_rows() {
     return tr;
 }
 
+// rows per Z forward block of shape variant v: 0 the default, 1 half of it, 2
+// double (while the block stays within 1024 threads and 80 KB of LDS).  Short
+// rows (M < 32) have one shape; a variant that does not exist gives the default.
+template <int M> constexpr int z_rows_v(int v) {
+    constexpr int d = z_rows<M>();
+    if (M < 32) return d;
+    if (v == 1) return d > 1 ? d / 2 : d;
+    if (v == 2) {
+        const int t = 2 * d;
+        const int fft = t * RegFFT<M>::LDS, tile = (M + 1) * (t + 1);
+        return (t * RegFFT<M>::G <= 1024 && (fft > tile ? fft : tile) * 8 <= 80 * 1024) ? t : d;
+    }
+    return d;
+}
+
+template <int M, int TR>
+static void zfwd_launch(const ConvPlan& cp, const StageIn& si, int NKp, const float2* twM, const float2* twP,
+                        hipStream_t s) {
+    const int Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
+    const dim3 g(div_up(Ny, TR), Nx, 3);
+    constexpr int nt = TR * RegFFT<M>::G;
+    switch (NKp) {
+    case 0: k_zfwd<M, TR, 0><<<g, nt, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz); break;
+    case 1: k_zfwd<M, TR, 1><<<g, nt, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz); break;
+    case 2: k_zfwd<M, TR, 2><<<g, nt, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz); break;
+    case 3: k_zfwd<M, TR, 3><<<g, nt, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz); break;
+    default: k_zfwd<M, TR, 5><<<g, nt, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz); break;
+    }
+}
+
 static void launch_zfwd(const ConvPlan& cp, const StageIn& in, hipStream_t s) {
-    const int M = cp.M, Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
+    const int M = cp.M;
     const float2* twM = twiddle_f32(M);
     const float2* twP = twiddle_f32(cp.P[2]);
     GK_ASSERT(in.nk >= 0 && in.nk <= 5, "stage build with %d earlier stages", in.nk);
@@ -584,26 +687,29 @@ static void launch_zfwd(const ConvPlan& cp, const StageIn& in, hipStream_t s) {
     }
     si.nk = NKp;
     switch (M) {
-#define GK_ZF(n, K) k_zfwd<n, TR, K><<<g, TR * RegFFT<n>::G, 0, s>>>(si, cp.B1, twM, twP, Nx, Ny, Nz)
-#define GK_CASE(n)                                                                              \
-    case n: {                                                                                   \
-        constexpr int TR = z_rows<n>();                                                         \
-        dim3 g(div_up(Ny, TR), Nx, 3);                                                          \
-        switch (NKp) {                                                                          \
-        case 0: GK_ZF(n, 0); break;                                                             \
-        case 1: GK_ZF(n, 1); break;                                                             \
-        case 2: GK_ZF(n, 2); break;                                                             \
-        case 3: GK_ZF(n, 3); break;                                                             \
-        default: GK_ZF(n, 5); break;                                                            \
-        }                                                                                       \
-        break;                                                                                  \
-    }
+#define GK_CASE(n)                                                                               \
+    case n:                                                                                      \
+        if (cp.tp.zr == 1) zfwd_launch<n, z_rows_v<n>(1)>(cp, si, NKp, twM, twP, s);             \
+        else if (cp.tp.zr == 2) zfwd_launch<n, z_rows_v<n>(2)>(cp, si, NKp, twM, twP, s);        \
+        else zfwd_launch<n, z_rows_v<n>(0)>(cp, si, NKp, twM, twP, s);                           \
+        break;
         GK_LEN_M(GK_CASE)
 #undef GK_CASE
-#undef GK_ZF
     default: abort();
     }
     GK_LAUNCH_CHECK();
+}
+
+bool conv_zr_valid(int M, int v) {
+    if (v == 0) return true;
+    if (v < 0 || v > 2) return false;
+    switch (M) {
+#define GK_CASE(n) \
+    case n: return z_rows_v<n>(v) != z_rows_v<n>(0);
+        GK_LEN_M(GK_CASE)
+#undef GK_CASE
+    default: return false;
+    }
 }
 
 static void launch_zinv(const ConvPlan& cp, float* h, const EngineState* st, hipStream_t s) {
@@ -638,20 +744,47 @@ template <int M> constexpr int epi_rows() {
     return tr;
 }
 
-static int epi_rows_rt(int M) {
+// rows per fused Z inverse block of shape variant v: 0 the default, 1 half of it
+template <int M> constexpr int epi_rows_v(int v) {
+    constexpr int d = epi_rows<M>();
+    return (v == 1 && M >= 32 && d > 1) ? d / 2 : d;
+}
+
+int conv_epi_rows(int M, int v) {
     switch (M) {
 #define GK_CASE(n) \
-    case n: return epi_rows<n>();
+    case n: return epi_rows_v<n>(v);
         GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
 }
 
-int conv_epi_blocks(const ConvPlan& cp) { return div_up(cp.N[1], epi_rows_rt(cp.M)) * cp.N[0]; }
+bool conv_er_valid(int M, int v) { return v == 0 || (v == 1 && conv_epi_rows(M, 1) != conv_epi_rows(M, 0)); }
+
+int conv_epi_blocks(const ConvPlan& cp) { return div_up(cp.N[1], conv_epi_rows(cp.M, cp.tp.er)) * cp.N[0]; }
+
+template <int M, int TR>
+static void zinv_epi_launch(const ConvPlan& cp, const EpiArgs& e, const EngineState* st, const float2* twM,
+                            const float2* twP, hipStream_t s) {
+    const int Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2], nu = e.nu;
+    const dim3 g(div_up(Ny, TR), Nx), b(3 * TR * RegFFT<M>::G);
+    if (e.fin == 1) {
+        if (nu <= 1) k_zinv_epi<M, TR, 1, 1><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+        else k_zinv_epi<M, TR, 3, 1><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+    } else if (e.fin == 2) {
+        if (nu <= 3) k_zinv_epi<M, TR, 3, 2><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+        else k_zinv_epi<M, TR, 6, 2><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+    } else {
+        if (nu == 0) k_zinv_epi<M, TR, 0, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+        else if (nu == 1) k_zinv_epi<M, TR, 1, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+        else if (nu <= 3) k_zinv_epi<M, TR, 3, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+        else k_zinv_epi<M, TR, 6, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);
+    }
+}
 
 static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineState* st, hipStream_t s) {
-    const int M = cp.M, Nx = cp.N[0], Ny = cp.N[1], Nz = cp.N[2];
+    const int M = cp.M;
     GK_ASSERT(st != nullptr && ea.nu >= 0 && ea.nu <= 6 && (ea.fin != 1 || ea.nu <= 3) &&
                   (ea.fin != 2 || (ea.mhigh && ea.part_err)),
               "fused Z inverse: bad epilogue");
@@ -668,40 +801,16 @@ static void launch_zinv_epi(const ConvPlan& cp, const EpiArgs& ea, const EngineS
     const float2* twP = twiddle_f32(cp.P[2]);
     switch (M) {
 #define GK_CASE(n)                                                                               \
-    case n: {                                                                                    \
-        constexpr int TR = epi_rows<n>();                                                        \
-        const dim3 g(div_up(Ny, TR), Nx), b(3 * TR * RegFFT<n>::G);                              \
-        if (e.fin == 1) {                                                                        \
-            if (nu <= 1) k_zinv_epi<n, TR, 1, 1><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else k_zinv_epi<n, TR, 3, 1><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);         \
-        } else if (e.fin == 2) {                                                                 \
-            if (nu <= 3) k_zinv_epi<n, TR, 3, 2><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else k_zinv_epi<n, TR, 6, 2><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);         \
-        } else {                                                                                 \
-            if (nu == 0) k_zinv_epi<n, TR, 0, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else if (nu == 1) k_zinv_epi<n, TR, 1, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else if (nu <= 3) k_zinv_epi<n, TR, 3, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz); \
-            else k_zinv_epi<n, TR, 6, 0><<<g, b, 0, s>>>(cp.B1, e, twM, twP, st, Nx, Ny, Nz);            \
-        }                                                                                        \
-        break;                                                                                   \
-    }
+    case n:                                                                                      \
+        if (cp.tp.er == 1) zinv_epi_launch<n, epi_rows_v<n>(1)>(cp, e, st, twM, twP, s);         \
+        else zinv_epi_launch<n, epi_rows_v<n>(0)>(cp, e, st, twM, twP, s);                       \
+        break;
         GK_LEN_M(GK_CASE)
 #undef GK_CASE
     default: abort();
     }
     GK_LAUNCH_CHECK();
 }
-
-// Blocks per CU of the pipelined Y passes (each walks several lines).
-#ifndef GK_Y_BLOCKS_PER_CU
-#define GK_Y_BLOCKS_PER_CU 8
-#endif
-constexpr int kYBlocksPerCU = GK_Y_BLOCKS_PER_CU;
-// ... and of the pipelined X . K . X pass
-#ifndef GK_X_BLOCKS_PER_CU
-#define GK_X_BLOCKS_PER_CU 4
-#endif
-constexpr int kXBlocksPerCU = GK_X_BLOCKS_PER_CU;
 
 static int cu_count() {
     static int n = 0;
@@ -731,7 +840,7 @@ static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipS
     case n: {                                                                                   \
         constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
         int g = 0;                                                                              \
-        const int nit = pipeline_split(div_up(nlines, LPB), (long)kYBlocksPerCU * cu_count(), &g); \
+        const int nit = pipeline_split(div_up(nlines, LPB), (long)cp.tp.yb * cu_count(), &g); \
         if (fwd) k_ypass<n, true><<<g, kNT, 0, s>>>(cp.B1, cp.B2, tw, st, Nx, Ny, Q, nit);      \
         else k_ypass<n, false><<<g, kNT, 0, s>>>(cp.B2, cp.B1, tw, st, Nx, Ny, Q, nit);          \
         break;                                                                                  \
@@ -752,7 +861,7 @@ static void launch_xmul(const ConvPlan& cp, const EngineState* st, hipStream_t s
         constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
         const int lpr = LPB < PY ? LPB : PY;                                                    \
         int g = 0;                                                                              \
-        const int nit = pipeline_split((long)Q * div_up(PY, lpr), (long)kXBlocksPerCU * cu_count(), &g); \
+        const int nit = pipeline_split((long)Q * div_up(PY, lpr), (long)cp.tp.xb * cu_count(), &g); \
         k_xmul<n><<<g, kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, Nx, PY, Q, nit);                      \
         break;                                                                                  \
     }
@@ -771,7 +880,7 @@ static void launch_y2d(const ConvPlan& cp, const EngineState* st, hipStream_t s)
     case n: {                                                                                   \
         constexpr int LPB = kNT / RegFFT<n>::G;                                                 \
         int g = 0;                                                                              \
-        const int nit = pipeline_split(div_up(Q, LPB), (long)kYBlocksPerCU * cu_count(), &g);  \
+        const int nit = pipeline_split(div_up(Q, LPB), (long)cp.tp.y2b * cu_count(), &g);  \
         k_y2r<n><<<g, kNT, 0, s>>>(cp.B1, cp.Kc, tw, st, Ny, Q, nit);                           \
         break;                                                                                  \
     }
@@ -779,6 +888,19 @@ static void launch_y2d(const ConvPlan& cp, const EngineState* st, hipStream_t s)
 #undef GK_CASE
     default: abort();
     }
+    GK_LAUNCH_CHECK();
+}
+
+static void launch_plane3(const ConvPlan& cp, const EngineState* st, hipStream_t s) {
+    const int PX = cp.P[0], Nx = cp.N[0], Ny = cp.N[1], Q = cp.Q;
+    GK_ASSERT(cp.plane3 && cp.P[1] == kPlanePY, "the plane pass does not fit this plan");
+    GK_ASSERT(Q >= 2 && 2 * Ny <= kPlanePY, "plane pass: Q = %d, Ny = %d", Q, Ny);
+    const float2* twL = twiddle_f32(kPlanePY / 2);
+    const float2* twP = twiddle_f32(kPlanePY);
+    const float2* twX = twiddle_f32(PX);
+    // Q - 1 blocks: block 0 takes the two real planes kz = 0 and kz = Q - 1 together
+    if (PX == 16) k_plane3<16><<<Q - 1, kPlaneThreads, 0, s>>>(cp.B1, cp.Kc, twL, twP, twX, st, Nx, Ny, Q);
+    else k_plane3<32><<<Q - 1, kPlaneThreads, 0, s>>>(cp.B1, cp.Kc, twL, twP, twX, st, Nx, Ny, Q);
     GK_LAUNCH_CHECK();
 }
 
@@ -843,15 +965,17 @@ Prof::~Prof() {
 
 void conv_run_pass(const ConvPlan& cp, int cls, const StageIn& in, float* h, const EpiArgs* epi, hipStream_t s) {
     const bool d3 = cp.P[0] > 1;
+    const bool pp = cp.plane3 && cp.tp.pp;
     switch (cls) {
     case 0: launch_zfwd(cp, in, s); break;
-    case 1: GK_ASSERT(d3 && !cp.yxy, "no separate Y forward pass"); launch_y3d(cp, in.st, true, s); break;
+    case 1: GK_ASSERT(d3 && !cp.yxy && !pp, "no separate Y forward pass"); launch_y3d(cp, in.st, true, s); break;
     case 2:
         if (cp.yxy) launch_yxy(cp, in.st, s);
+        else if (pp) launch_plane3(cp, in.st, s);
         else if (d3) launch_xmul(cp, in.st, s);
         else launch_y2d(cp, in.st, s);
         break;
-    case 3: GK_ASSERT(d3 && !cp.yxy, "no separate Y inverse pass"); launch_y3d(cp, in.st, false, s); break;
+    case 3: GK_ASSERT(d3 && !cp.yxy && !pp, "no separate Y inverse pass"); launch_y3d(cp, in.st, false, s); break;
     case 4: if (epi) launch_zinv_epi(cp, *epi, in.st, s); else launch_zinv(cp, h, in.st, s); break;
     default: GK_ASSERT(false, "pass class %d", cls);
     }
@@ -870,6 +994,10 @@ void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, 
     } else if (cp.yxy) {
         p.begin(2, s);
         launch_yxy(cp, in.st, s);
+        p.end(s);
+    } else if (cp.plane3 && cp.tp.pp) {
+        p.begin(2, s);
+        launch_plane3(cp, in.st, s);
         p.end(s);
     } else {
         p.begin(1, s);

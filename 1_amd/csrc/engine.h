@@ -146,6 +146,18 @@ struct KernelSpec {
     int type;
 };
 
+// Launch parameters of a plan's passes (tile shapes, grid sizes).  The
+// defaults are hand-picked; the autotuner (mm_sim_tune) times the candidates
+// on the device and a table of tuned values (per data size) feeds new plans.
+struct TuneParams {
+    int zr = 0;   // Z forward rows per block: variant 0 = z_rows<M>(), 1 = half, 2 = double
+    int er = 0;   // fused Z inverse rows per block: variant 0 = epi_rows<M>(), 1 = half
+    int yb = 8;   // blocks per CU of the pipelined 3-D Y passes
+    int xb = 4;   // blocks per CU of the pipelined X.K.X pass
+    int y2b = 8;  // blocks per CU of the 2-D Y.K.Y pass
+    int pp = 0;   // 3-D plans that fit it: 1 = Y.X K X.Y in one plane pass (k_plane3), 0 = separate passes
+};
+
 struct ConvPlan {
     int N[3];   // data size (internal X,Y,Z)
     int P[3];   // padded FFT size
@@ -156,8 +168,24 @@ struct ConvPlan {
     float2* B1; // [3][Nx][Q][Ny]
     float2* B2; // 3-D: [3][Q][Py][Nx] (not used by the thin plane pass)
     int yxy;    // thin 3-D: Y . X K X . Y inverse in one plane pass (no B2)
+    int plane3; // 3-D plan the register-resident plane pass fits (used when tp.pp is set)
     size_t bytes;
+    TuneParams tp;  // from the tuning table (conv_tune_lookup) when it holds N, else defaults
 };
+
+// Tuning table: data size N -> launch parameters (process-wide; conv_plan_init
+// reads it).  Text file: one line "Nx Ny Nz zr er yb xb y2b pp" per size (a
+// line without pp reads as pp = 0).
+bool conv_tune_lookup(const int N[3], TuneParams* tp);
+void conv_tune_store(const int N[3], const TuneParams& tp);
+int conv_tune_save(const char* path);  // entries written, or -1
+int conv_tune_load(const char* path);  // entries read, or -1
+void conv_tune_clear();
+// rows per block of the fused Z inverse for Z half-length M and variant v, and
+// whether variant v of the Z forward / fused Z inverse rows is a distinct shape
+int conv_epi_rows(int M, int v);
+bool conv_zr_valid(int M, int v);
+bool conv_er_valid(int M, int v);
 
 // Choose padded sizes, allocate buffers.  Kc is set by conv_acquire_kernel.
 void conv_plan_init(ConvPlan& cp, const int N[3]);
@@ -203,7 +231,8 @@ struct EpiArgs;
 void conv_apply(const ConvPlan& cp, const StageIn& in, float* h, hipStream_t s, Prof* prof = nullptr,
                 const EpiArgs* epi = nullptr);
 // number of blocks (= per-block partial maxima) of the fused Z inverse + torque pass
-int conv_epi_blocks(const ConvPlan& cp);
+int conv_epi_blocks(const ConvPlan& cp);  // blocks of the fused Z inverse at cp.tp.er
+int conv_epi_blocks_max(const int N[3]);   // ... the most over every row variant
 // one pass of class cls (0 Z fwd, 1 Y fwd, 2 X.K.X / Y.K.Y, 3 Y inv, 4 Z inv), for timing
 void conv_run_pass(const ConvPlan& cp, int cls, const StageIn& in, float* h, const EpiArgs* epi, hipStream_t s);
 // st->trial_active = 1 (stage kernels do their work until the next trial start)

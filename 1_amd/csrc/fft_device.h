@@ -297,10 +297,26 @@ template <int L> __device__ constexpr bool pad_half(int e) {
     return RegFFT<L>::E >= 2 && RegFFT<L>::E % 2 == 0 && e >= RegFFT<L>::E / 2;
 }
 
-template <typename T, int L, int DIR, int NS = 1>
+// The lanes of one line (G = L / E <= 64) are always in one wave, and the
+// exchange buffer belongs to that line, so a wave-level barrier orders its LDS
+// traffic (the pattern of rocPRIM's wave_barrier: release fence, wave barrier,
+// acquire fence).  Kernels whose buffers are private to a line during the
+// transform (WS = true) use it instead of the block barrier, so waves do not
+// wait for each other between radix passes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// EE: elements per lane (default RegFFT<L>::E; a smaller power-of-two EE spreads
+// a line over more lanes, L / EE, with the same exchange buffer); WS: wave-level
+// instead of block-level barriers around the exchanges (see wave_sync)
+template <typename T, int L, int DIR, int NS = 1, int EE = reg_fft_E(L), bool WS = false>
 __device__ __forceinline__ void reg_fft(C2<T>* v, int g, C2<T>* lds, const C2<T>* tw) {
     if constexpr (NS < L) {
-        constexpr int E = RegFFT<L>::E, G = RegFFT<L>::G;
+        constexpr int E = EE, G = L / EE;
+        static_assert(L % EE == 0, "elements per lane must divide the length");
         constexpr int R = reg_radix(L / NS, E);
         static_assert(R != 0 && E % R == 0, "radix must divide the elements per lane");
         constexpr int NB = E / R;  // butterflies per lane
@@ -328,11 +344,13 @@ __device__ __forceinline__ void reg_fft(C2<T>* v, int g, C2<T>* lds, const C2<T>
             }
         }
         if constexpr (!last) {
-            __syncthreads();
+            if constexpr (WS) wave_sync();
+            else __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = lds[lds_pad(g + G * e)];
-            __syncthreads();
-            reg_fft<T, L, DIR, NS * R>(v, g, lds, tw);
+            if constexpr (WS) wave_sync();
+            else __syncthreads();
+            reg_fft<T, L, DIR, NS * R, EE, WS>(v, g, lds, tw);
         }
     }
 }

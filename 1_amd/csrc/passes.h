@@ -684,6 +684,272 @@ __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const flo
     }
 }
 
+// 3-D Y forward . X.K.X . Y inverse of one kz plane in one pass, for padded
+// Y = 512 and padded X = 16 or 32 (Nx <= PX/2, Ny <= 256): per field
+// evaluation B1 is read and written once and the Y-padded B2 is not used
+// (48 A1 + kernel bytes instead of 48 A1 + 96 A2 + kernel for the three
+// separate passes).
+//
+// The length-512 Y transform of a line of 256 inputs splits by the parity p of
+// ky: Y(2k + p) = FFT_256(y[n] w^(p n))[k], w = exp(-2 pi i / 512), and the
+// inverse, keeping n < 256, is h[n] = sum_p w^(-p n) IFFT_256(H_p)[n].  The
+// two parities are independent through the X.K.X stage (a column ky only
+// meets the kernel at its own ky), so a block does them one after the other
+// and adds the two inverse halves.  Wave x (x < Nx) holds the 3 component
+// lines of x (a 256-line is one wave, 4 elements per lane, n or k = g + 64 e);
+// each round's spectra go to an LDS staging tile [k][c][x], whose columns the
+// block takes through X forward, the kernel multiply and X inverse 128 at a
+// time (8 lanes per column, 4 elements per lane), and come back.
+//
+// Block 0 packs the two real planes kz = 0 and kz = M (= Q - 1; their Z
+// spectra are real) as P = A + i B.  After the Y transform, A = (P(k) +
+// conj P(-k)) / 2 and B = (P(k) - conj P(-k)) / 2i (-k within the same
+// parity round: a lane shuffle), and only the columns k <= its Hermitian
+// partner are needed: K_A of A (kernel plane 0) and K_A of B (kernel plane M).
+// Coming back, H_P = H_A + i H_B, with the partner's conjugate for the other
+// half, and the real and imaginary parts go to planes 0 and M.  So Q planes
+// take M = Q - 1 blocks: 256 for the 256x256x16 cuboid, one per CU.
+constexpr int kPlaneThreads = 1024;
+constexpr int kPlanePY = 512;
+__host__ __device__ constexpr bool plane3_fits(int PX, int PY, int Nx) {
+    return PY == kPlanePY && (PX == 16 || PX == 32) && Nx >= 1 && 2 * Nx <= PX;
+}
+template <int PX> struct Plane3 {
+    static constexpr int L = kPlanePY / 2;             // half-length Y transform (256)
+    static constexpr int YE = 4;                       // elements per lane of a 256-line (64 lanes)
+    static constexpr int XE = 4, XG = PX / XE;         // X layout: elements / lanes per column
+    static constexpr int SUB = kPlaneThreads / XG;     // columns in the X layout at once
+    static constexpr int HX = PX / 2;                  // x slots of a column (Nx <= HX)
+    static constexpr int SQ = 3 * HX + 1;              // staging stride per column, padded
+    static constexpr int SLOTS = L + 2;                // columns of a round (block 0, even: 2 x 129)
+    static constexpr int STAGE = SLOTS * SQ;
+    static constexpr int XEXCH = SUB * (PX + PX / 8);
+    static constexpr int YEXCH = (kPlaneThreads / 64) * (L + L / 8);
+    static constexpr int EXCH = XEXCH > YEXCH ? XEXCH : YEXCH;
+    static constexpr int LDS = STAGE + EXCH + L + kPlanePY + PX;  // + twiddle tables
+};
+
+template <int PX>
+__global__ __launch_bounds__(kPlaneThreads) void k_plane3(float2* __restrict__ B1, const float* __restrict__ Kc,
+                                                          const float2* __restrict__ twLg,
+                                                          const float2* __restrict__ twPg,
+                                                          const float2* __restrict__ twXg, const EngineState* st,
+                                                          int Nx, int Ny, int Q) {
+    GK_FFT_FP
+    using P3 = Plane3<PX>;
+    constexpr int L = P3::L, YE = P3::YE, XE = P3::XE, XG = P3::XG, SQ = P3::SQ, HX = P3::HX, SUB = P3::SUB;
+    constexpr int PY = kPlanePY;
+    __shared__ float2 lds[P3::LDS];
+    if (st && !st->trial_active) return;
+    float2* stage = lds;                      // [SLOTS][SQ]
+    float2* exch = lds + P3::STAGE;           // Y (per wave) or X (per column) exchange buffers
+    float2* twL = exch + P3::EXCH;            // exp(-2 pi i j / 256)
+    float2* twP = twL + L;                    // exp(-2 pi i j / 512)
+    float2* twX = twP + PY;                   // exp(-2 pi i j / PX)
+    for (int i = threadIdx.x; i < L; i += kPlaneThreads) twL[i] = twLg[i];
+    for (int i = threadIdx.x; i < PY; i += kPlaneThreads) twP[i] = twPg[i];
+    if (threadIdx.x < PX) twX[threadIdx.x] = twXg[threadIdx.x];
+    const int M = Q - 1;
+    const bool packed = blockIdx.x == 0;
+    const int kz = blockIdx.x;  // plane of an unpacked block
+    const int x = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), g = threadIdx.x % 64;  // wave x, lane g
+    const bool xok = x < Nx;
+    const long cs = (long)Nx * Q * Ny;
+    const long row = ((long)(xok ? x : 0) * Q + kz) * Ny;
+    const long rowM = ((long)(xok ? x : 0) * Q + M) * Ny;
+    __syncthreads();  // twiddle tables
+    const int q = threadIdx.x / XG, gx = threadIdx.x % XG;  // X layout: column q of a sub-round, lane gx
+    const int hx = PX / 2 + 1, hy = PY / 2 + 1;
+    float2 half0[3][YE];  // the even round's inverse half of the lines
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        // columns of block 0: A for k < KA, B for KA <= slot < 2 KA (KA = 129 even, 128 odd)
+        const int KA = p ? L / 2 : L / 2 + 1;
+        const int nslots = packed ? 2 * KA : L;
+        // ---- the plane's lines (read again for the odd round rather than held in
+        // registers through the even one), then Y forward of this parity:
+        // FFT_256(y[n] w^(p n)); loads from clamped indices and a select, so no
+        // divergent branch sits around them
+        float2 u[3][YE];
+        if (packed) {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < YE; e++) {
+                    const int n = g + 64 * e, nn = n < Ny ? n : Ny - 1;
+                    const float2 a = B1[c * cs + row + nn], b = B1[c * cs + rowM + nn];
+                    u[c][e] = (xok && n < Ny) ? make_float2(a.x, b.x) : make_float2(0.f, 0.f);
+                }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < YE; e++) {
+                    const int n = g + 64 * e, nn = n < Ny ? n : Ny - 1;
+                    const float2 a = B1[c * cs + row + nn];
+                    u[c][e] = (xok && n < Ny) ? a : make_float2(0.f, 0.f);
+                }
+        }
+        if (p && xok) {
+            // the even round's half of h parks in B1 (its inputs are read): registers stay free for the odd round
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < YE; e++) {
+                    const int n = g + 64 * e;
+                    if (n < Ny) B1[c * cs + row + n] = half0[c][e];
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (p)
+#pragma unroll
+                for (int e = 0; e < YE; e++) u[c][e] = cmul<float>(u[c][e], twP[g + 64 * e]);
+            reg_fft<float, L, -1, 1, YE, true>(u[c], g, exch + x * (L + L / 8), twL);
+        }
+        // ---- into the staging tile
+        if (xok) {
+            if (!packed) {
+#pragma unroll
+                for (int e = 0; e < YE; e++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) stage[(g + 64 * e) * SQ + c * HX + x] = u[c][e];
+            } else {
+                // partner -k of k = g + 64 e: lane (64 - g) % 64 (even) or 63 - g (odd), element 3 - e;
+                // for g = 0 of the even round, this lane's element (4 - e) % 4
+                const int src = p ? 63 - g : (64 - g) & 63;
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int e = 0; e < YE; e++) {
+                        const float2 ta = shfl2<64>(u[c][YE - 1 - e], src);
+                        const float2 pm = (p == 0 && g == 0) ? u[c][(YE - e) % YE] : ta;
+                        const int k = g + 64 * e;
+                        if (k < KA) {
+                            const float2 a = u[c][e];
+                            stage[k * SQ + c * HX + x] = make_float2((a.x + pm.x) * 0.5f, (a.y - pm.y) * 0.5f);
+                            stage[(KA + k) * SQ + c * HX + x] =
+                                make_float2((a.y + pm.y) * 0.5f, (pm.x - a.x) * 0.5f);
+                        }
+                    }
+            }
+        }
+        __syncthreads();  // the Y exchange buffers become the X ones
+        // ---- X . K . X of every column, SUB at a time
+        for (int j = 0; j * SUB < nslots; j++) {
+            const int slot = j * SUB + q;
+            const bool ok = slot < nslots;
+            float2* col = stage + (ok ? slot : 0) * SQ;
+            // kernel plane and ky of the column
+            const bool bcol = packed && slot >= KA;
+            const int kzk = packed ? (bcol ? M : 0) : kz;
+            const int ky = 2 * (bcol ? slot - KA : slot) + p;
+            float2 xv[3][XE];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < XE; e++) {
+                    const int xx = gx + XG * e;
+                    xv[c][e] = (ok && 2 * e < XE && xx < Nx) ? col[c * HX + xx] : make_float2(0.f, 0.f);
+                }
+#pragma unroll
+            for (int c = 0; c < 3; c++) reg_fft<float, PX, -1, 1, XE, true>(xv[c], gx, exch + q * (PX + PX / 8), twX);
+            if (ok) {
+                // H = K M with the compressed kernel's parity signs restored (as xmul_kernel)
+                const bool fy = ky > PY / 2;
+                const float* kb = Kc + ((long)kzk * hy + (fy ? PY - ky : ky)) * hx * 6;
+#pragma unroll
+                for (int e = 0; e < XE; e++) {
+                    const int kx = gx + XG * e;
+                    const bool fx = kx > PX / 2;
+                    const float2* kk = reinterpret_cast<const float2*>(kb + (fx ? PX - kx : kx) * 6);
+                    const float2 k01 = kk[0], k23 = kk[1], k45 = kk[2];
+                    const float kxx = k01.x, kyy = k01.y, kzz = k23.x;
+                    const float kyz = fy ? -k23.y : k23.y, kxz = fx ? -k45.x : k45.x;
+                    const float kxy = (fx != fy) ? -k45.y : k45.y;
+                    const float2 a = xv[0][e], b = xv[1][e], cc = xv[2][e];
+                    xv[0][e] = make_float2(a.x * kxx + b.x * kxy + cc.x * kxz, a.y * kxx + b.y * kxy + cc.y * kxz);
+                    xv[1][e] = make_float2(a.x * kxy + b.x * kyy + cc.x * kyz, a.y * kxy + b.y * kyy + cc.y * kyz);
+                    xv[2][e] = make_float2(a.x * kxz + b.x * kyz + cc.x * kzz, a.y * kxz + b.y * kyz + cc.y * kzz);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) reg_fft<float, PX, +1, 1, XE, true>(xv[c], gx, exch + q * (PX + PX / 8), twX);
+            if (ok)
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int e = 0; e < XE; e++) {
+                        const int xx = gx + XG * e;
+                        if (2 * e < XE && xx < Nx) col[c * HX + xx] = xv[c][e];
+                    }
+        }
+        __syncthreads();  // the staging tile is complete; the X exchange buffers become the Y ones
+        // ---- back into the lines and Y inverse of this parity
+#pragma unroll
+        for (int e = 0; e < YE; e++) {
+            const int k = g + 64 * e;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float2 h = make_float2(0.f, 0.f);
+                if (xok) {
+                    if (!packed) {
+                        h = stage[k * SQ + c * HX + x];
+                    } else {
+                        // H_P = H_A + i H_B; beyond KA from the Hermitian partner, conjugated
+                        const bool low = k < KA;
+                        const int kk = low ? k : (p ? L - 1 - k : L - k);
+                        float2 ha = stage[kk * SQ + c * HX + x], hb = stage[(KA + kk) * SQ + c * HX + x];
+                        if (!low) {
+                            ha.y = -ha.y;
+                            hb.y = -hb.y;
+                        }
+                        h = make_float2(ha.x - hb.y, ha.y + hb.x);
+                    }
+                }
+                u[c][e] = h;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            reg_fft<float, L, +1, 1, YE, true>(u[c], g, exch + x * (L + L / 8), twL);
+            if (p) {
+#pragma unroll
+                for (int e = 0; e < YE; e++) {
+                    float2 w = twP[g + 64 * e];
+                    w.y = -w.y;
+                    u[c][e] = cmul<float>(u[c][e], w);
+                }
+            }
+        }
+        if (p == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < YE; e++) half0[c][e] = u[c][e];
+        } else if (xok) {
+            // h = even half (parked in B1 by this lane) + odd half
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < YE; e++) {
+                    const int n = g + 64 * e;
+                    if (n < Ny) {
+                        const float2 h0 = B1[c * cs + row + n];
+                        const float2 h = make_float2(h0.x + u[c][e].x, h0.y + u[c][e].y);
+                        if (packed) {
+                            B1[c * cs + row + n] = make_float2(h.x, 0.f);
+                            B1[c * cs + rowM + n] = make_float2(h.y, 0.f);
+                        } else {
+                            B1[c * cs + row + n] = h;
+                        }
+                    }
+                }
+        }
+        __syncthreads();  // the staging tile and exchange buffers are free for the next round
+    }
+}
+
 // Thin 3-D grids (padded X of 4 to 16): Y forward, X . K . X and Y inverse of
 // one kz plane in one pass.  The plane of Y spectra [3][Nx][PY] stays in LDS,
 // so per field evaluation the pass reads and writes B1 once, where the three

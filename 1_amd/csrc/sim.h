@@ -71,7 +71,6 @@ struct mmSim {
     bool wantEnergy = false;
     int fieldCap = 0;                // points in fT / fB
     double fieldTmaxHost = -INFINITY;  // ABI mode: latest time the field was evaluated at
-    std::vector<void*> retired;      // replaced field tables, freed with the simulation
     double* fT = nullptr;
     float* fB = nullptr;
     float* alphaMap = nullptr;       // device copy of mmParams.alphaMap (N floats)

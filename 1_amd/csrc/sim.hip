@@ -175,26 +175,29 @@ const double kGamma0 = 2.211e5;
 }  // namespace
 
 // device copy of the applied field table (static value in the table itself)
+// The pointwise field table lives in one buffer per simulation, replaced in
+// place (batches keep copies of their own), grown by doubling when a longer
+// table arrives.
 static void upload_field(mmSim* s) {
     const int n = s->p.nField;
-    if (s->fT) {
-        // a batch built earlier may still read the old table: retire it until
-        // the simulation is freed
-        s->retired.push_back(s->fT);
-        s->retired.push_back(s->fB);
-        s->fT = nullptr;
-        s->fB = nullptr;
-        s->fieldCap = 0;
-    }
     s->field = FieldTable{};
     s->field.n = n;
     if (n == 1)
         for (int c = 0; c < 3; c++) s->field.b0[c] = s->fieldB[c];
     if (n >= 2) {
-        GK_CHECK(hipMalloc(&s->fT, n * sizeof(double)));
-        GK_CHECK(hipMalloc(&s->fB, 3 * n * sizeof(float)));
-        s->fieldCap = n;
-        s->bytes += n * (sizeof(double) + 3 * sizeof(float));
+        if (s->s) GK_CHECK(hipStreamSynchronize(s->s));  // queued trials of this simulation read the old table
+        if (n > s->fieldCap) {
+            if (s->fT) {
+                GK_CHECK(hipFree(s->fT));
+                GK_CHECK(hipFree(s->fB));
+                s->bytes -= s->fieldCap * (sizeof(double) + 3 * sizeof(float));
+            }
+            const int cap = std::max(n, 2 * s->fieldCap);
+            GK_CHECK(hipMalloc(&s->fT, cap * sizeof(double)));
+            GK_CHECK(hipMalloc(&s->fB, 3 * cap * sizeof(float)));
+            s->fieldCap = cap;
+            s->bytes += cap * (sizeof(double) + 3 * sizeof(float));
+        }
         GK_CHECK(hipMemcpy(s->fT, s->fieldT.data(), n * sizeof(double), hipMemcpyHostToDevice));
         GK_CHECK(hipMemcpy(s->fB, s->fieldB.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice));
         s->field.t = s->fT;
@@ -627,7 +630,8 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
         fprintf(stderr, "Warning: FFT Kernel max imaginary part ratio = %g\n", s->kernelImagRatio);
     s->conv = true;
     s->nk_part = conv_epi_blocks(s->cp);
-    s->npart = s->nk_part > kRedBlocks ? s->nk_part : kRedBlocks;
+    // room for the partials of every row shape the tuner may choose
+    s->npart = std::max(kRedBlocks, conv_epi_blocks_max(s->cp.N));
     s->part = (float*)dmalloc(s, 3 * (size_t)s->npart * sizeof(float));
     GK_CHECK(hipMemset(s->part, 0, 3 * (size_t)s->npart * sizeof(float)));
     s->epart = (double*)dmalloc(s, (size_t)s->npart * sizeof(double));
@@ -651,7 +655,6 @@ void mm_sim_free(mmSim* s) {
     for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->log_err, (void*)s->dsum, (void*)s->fT,
                     (void*)s->part, (void*)s->epart})
         if (p) GK_CHECK(hipFree(p));
-    for (void* p : s->retired) GK_CHECK(hipFree(p));
     conv_plan_free(s->cp);
     if (s->plan) delete_gpuFFT3dPlan(s->plan);
     for (int c = 0; c < 3; c++)
@@ -936,7 +939,7 @@ void mm_sim_set_field_function(mmSim* s, int kind, const double* b, const double
     s->p.nField = 0;
     s->fieldT.clear();
     s->fieldB.clear();
-    upload_field(s);  // retires a table a batch may still read
+    upload_field(s);
     s->field.fn = kind;
     for (int i = 0; i < 4; i++) s->field.fp[i] = params[i];
     for (int c = 0; c < 3; c++) s->field.fb[c] = (float)b[c];
@@ -984,6 +987,97 @@ double mm_sim_time_pass(mmSim* s, int cls, int n) {
     GK_CHECK(hipEventDestroy(e1));
     return total;
 }
+
+int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[6], double ms_tuned[6]) {
+    GK_ASSERT(s->mode == MM_MODE_FUSED && s->conv, "tune: an initialised fused-mode simulation");
+    if (reps < 1) reps = 1;
+    ConvPlan& cp = s->cp;
+    const bool d3 = cp.P[0] > 1, sep = d3 && !cp.yxy;
+    const TuneParams dflt{};
+    // ms per launch of classes cls_a (+ cls_b): the least of 3 runs of `reps`
+    // back-to-back launches, after 2 untimed launches
+    auto timed = [&](int cls_a, int cls_b) {
+        double t = 0;
+        for (int cls : {cls_a, cls_b}) {
+            if (cls < 0) continue;
+            mm_sim_time_pass(s, cls, 2);
+            double best = INFINITY;
+            for (int r = 0; r < 3; r++) best = std::min(best, mm_sim_time_pass(s, cls, reps) / reps);
+            t += best;
+        }
+        return t;
+    };
+    // one tunable: try its candidates, keep the fastest (ties keep the earlier)
+    auto tune_one = [&](int slot, int* field, int dval, const std::vector<int>& cands, int cls_a, int cls_b) {
+        double t_d = -1, t_best = INFINITY;
+        int v_best = dval;
+        for (int v : cands) {
+            *field = v;
+            if (field == &cp.tp.er) s->nk_part = conv_epi_blocks(cp);
+            const double t = timed(cls_a, cls_b);
+            if (v == dval) t_d = t;
+            if (t < t_best) {
+                t_best = t;
+                v_best = v;
+            }
+        }
+        *field = v_best;
+        if (field == &cp.tp.er) s->nk_part = conv_epi_blocks(cp);
+        ms_default[slot] = t_d;
+        ms_tuned[slot] = t_best;
+    };
+    for (int i = 0; i < 6; i++) ms_default[i] = ms_tuned[i] = -1;
+    std::vector<int> zr{0}, er{0};
+    for (int v = 1; v <= 2; v++)
+        if (conv_zr_valid(cp.M, v)) zr.push_back(v);
+    if (conv_er_valid(cp.M, 1)) er.push_back(1);
+    const std::vector<int> grid{2, 3, 4, 6, 8, 12, 16};
+    tune_one(0, &cp.tp.zr, dflt.zr, zr, 0, -1);
+    tune_one(1, &cp.tp.er, dflt.er, er, 4, -1);
+    if (sep) {
+        // separate Y and X.K.X passes first (the plane pass off), then the choice between them
+        const int pp0 = cp.tp.pp;
+        cp.tp.pp = 0;
+        tune_one(2, &cp.tp.yb, dflt.yb, grid, 1, 3);
+        tune_one(3, &cp.tp.xb, dflt.xb, grid, 2, -1);
+        cp.tp.pp = pp0;
+        if (cp.plane3) {
+            double t[2];
+            for (int v = 0; v < 2; v++) {
+                cp.tp.pp = v;
+                t[v] = v ? timed(2, -1) : timed(1, 3) + timed(2, -1);
+            }
+            cp.tp.pp = t[1] < t[0] ? 1 : 0;
+            ms_default[5] = t[dflt.pp];
+            ms_tuned[5] = t[cp.tp.pp];
+        }
+    }
+    if (!d3) tune_one(4, &cp.tp.y2b, dflt.y2b, grid, 2, -1);
+    GK_CHECK(hipStreamSynchronize(s->s));
+    conv_tune_store(cp.N, cp.tp);
+    if (params) {
+        params[0] = cp.tp.zr;
+        params[1] = cp.tp.er;
+        params[2] = cp.tp.yb;
+        params[3] = cp.tp.xb;
+        params[4] = cp.tp.y2b;
+        params[5] = cp.tp.pp;
+    }
+    return 0;
+}
+
+void mm_sim_get_tune(mmSim* s, int params[6]) {
+    params[0] = s->cp.tp.zr;
+    params[1] = s->cp.tp.er;
+    params[2] = s->cp.tp.yb;
+    params[3] = s->cp.tp.xb;
+    params[4] = s->cp.tp.y2b;
+    params[5] = s->cp.tp.pp;
+}
+
+int mm_tune_save(const char* path) { return conv_tune_save(path); }
+int mm_tune_load(const char* path) { return conv_tune_load(path); }
+void mm_tune_clear(void) { conv_tune_clear(); }
 
 double mm_sim_time_steps(mmSim* s, int n) {
     GK_ASSERT(s->mode == MM_MODE_FUSED && !s->tb.adaptive, "timing needs the fused mode and a fixed-step solver");

@@ -9,6 +9,8 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgpukern.so")
+# launch parameters chosen by the autotuner on MI355X (1_amd/tune.py), read at load
+TUNE_TABLE = os.path.join(HERE, "tune_gfx950.txt")
 
 P = C.c_void_p
 F = C.c_float
@@ -64,6 +66,8 @@ GPUKERN = {
     "gpu_gaussian_noise": (None, [P, F, F, I]),
 }
 
+CP = C.c_char_p
+
 ENGINE = {
     "mm_sim_new": (P, [P, P, I]),
     "mm_sim_free": (None, [P]),
@@ -86,6 +90,11 @@ ENGINE = {
     "mm_sim_time_steps": (C.c_double, [P, I]),
     "mm_sim_profile_steps": (C.c_double, [P, I, P, P]),
     "mm_sim_time_pass": (C.c_double, [P, I, I]),
+    "mm_sim_tune": (I, [P, I, P, P, P]),
+    "mm_sim_get_tune": (None, [P, P]),
+    "mm_tune_save": (I, [CP]),
+    "mm_tune_load": (I, [CP]),
+    "mm_tune_clear": (None, []),
     "mm_sim_footprint": (C.c_ulonglong, [P]),
     "mm_sim_begin_run": (None, [P, C.c_double]),
     "mm_sim_relax": (C.c_long, [P, C.c_long]),
@@ -145,7 +154,6 @@ class Params(C.Structure):
 _lib = None
 
 
-CP = C.c_char_p
 PP = C.POINTER(C.c_char_p)
 
 IO = {
@@ -178,6 +186,8 @@ def load(path=LIB_PATH):
             fn.restype = res
             fn.argtypes = args
     _lib = lib
+    if os.path.exists(TUNE_TABLE):
+        lib.mm_tune_load(TUNE_TABLE.encode())
     return lib
 
 

@@ -778,9 +778,14 @@ class Sim:
     def hysteresis(self, bz0, by0, bx0, bz1, by1, bx1, steps, save=True, outdir=None):
         """Hysteresis() (simhysteresis.go:16-27): field scan in `steps`
         intervals, relaxing and saving m (omf) and a table row at each point."""
+        f32 = np.float32
+
+        def point(b0, b1, i):
+            # float32 in the reference's order: b0 + (b1 - b0) * float32(i) / float32(steps)
+            return float(f32(b0) + (f32(b1) - f32(b0)) * f32(i) / f32(steps))
+
         for i in range(steps + 1):
-            f = i / steps
-            self.applystatic("field", bz0 + (bz1 - bz0) * f, by0 + (by1 - by0) * f, bx0 + (bx1 - bx0) * f)
+            self.applystatic("field", point(bz0, bz1, i), point(by0, by1, i), point(bx0, bx1, i))
             self.relax()
             if save:
                 self.save("m", "omf", outdir)
@@ -885,6 +890,27 @@ class Sim:
     def device_bytes(self):
         self._init()
         return int(self._lib.mm_sim_device_bytes(self._ptr))
+
+    TUNABLES = ("zr", "er", "yb", "xb", "y2b", "pp")
+
+    def tune(self, reps=20):
+        """Autotune this simulation's launch shapes on the device
+        (mm_sim_tune): returns {tunable: (chosen value, ms per launch at the
+        default, ms at the chosen value)} for the tunables its plan has; the
+        choice is also stored in the process's tuning table."""
+        self._init()
+        par = np.zeros(6, np.int32)
+        md = np.zeros(6)
+        mt = np.zeros(6)
+        self._lib.mm_sim_tune(self._ptr, int(reps), par.ctypes.data, md.ctypes.data, mt.ctypes.data)
+        return {k: (int(par[i]), float(md[i]), float(mt[i])) for i, k in enumerate(self.TUNABLES) if md[i] >= 0}
+
+    def tune_params(self):
+        """The launch shapes this simulation runs with (mm_sim_get_tune)."""
+        self._init()
+        par = np.zeros(6, np.int32)
+        self._lib.mm_sim_get_tune(self._ptr, par.ctypes.data)
+        return dict(zip(self.TUNABLES, (int(v) for v in par)))
 
     def free(self):
         if self._ptr:

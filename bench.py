@@ -17,7 +17,16 @@ stream.  `traffic` is the HBM bytes per launch of that kernel from the
 committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload (scaled
 by the copy-kernel calibration recorded there), or null if none matches.
 `cpu_baseline` is the FP64 CPU oracle timed on this host on a bounded sample
-of the same workload.
+of the same workload; `parity` is the GPU engine against that oracle run
+(max |dm|, relative energy error) from the same m0 for the same steps.
+
+The same line carries two more objects, each against the same simulations run
+one at a time on this GPU: `sweep` (512 independent simulations of the sweep
+workload advanced in batches; serial rate from an evenly spaced sample of 32
+members) and `serve` (a Poisson stream of 200 requests served by continuous
+batching; the serial FIFO from an evenly spaced sample of 40 requests).
+`--no-sweep` / `--no-serve` leave them out; `--workload sweep|serve` runs one
+of them alone with the full serial side.
 """
 import argparse
 import ctypes as C
@@ -51,12 +60,16 @@ def committed_traffic(kernel_class, workload):
 CLASSES = ["z_fwd_r2c", "y_fwd", "x_fwd_kmul_x_inv", "y_inv", "z_inv_c2r_torque", "torque", "rk_final", "control"]
 
 
-def model_bytes(N, P, Nx, Ny, nK):
+def model_bytes(N, P, Nx, Ny, nK, plane=False):
     """Algorithmic bytes of one field evaluation, per kernel class, and of one
-    RK4 step (B_step = 4 B_conv + 12 N 27).  The pointwise RK work (12 N 27 per
-    step: stage builds, torques, final combination) runs in the epilogue of the
-    Z inverse pass, so that class carries a quarter of it per evaluation and
-    the classes of the 4 evaluations add up to B_step."""
+    RK4 step (B_step = 4 B_conv + 12 N 27, the fixed model of the reference's
+    pass structure).  The pointwise RK work (12 N 27 per step: stage builds,
+    torques, final combination) runs in the epilogue of the Z inverse pass, so
+    that class carries a quarter of it per evaluation and the classes of the 4
+    evaluations add up to B_step.  With the plane pass (`plane`), Y forward,
+    X.K.X and Y inverse are one pass that reads and writes B1 once and the
+    kernel (48 A1 + 4 nK A3, the kernel counted uncompressed as in the model);
+    `b_passes` is then the bytes of one step through these passes."""
     Q = P[2] // 2 + 1
     A1 = Nx * Ny * Q
     A2 = Nx * P[1] * Q
@@ -70,7 +83,12 @@ def model_bytes(N, P, Nx, Ny, nK):
     }
     b_conv = 3 * (8 * N + 32 * A1 + 32 * A2) + 4 * nK * A3
     b_step = 4 * b_conv + 12 * N * 27
-    return per, b_conv, b_step
+    b_passes = b_step
+    if plane:
+        del per["y_fwd"], per["y_inv"], per["x_fwd_kmul_x_inv"]
+        per["y_xkx_y_plane"] = 3 * 16 * A1 + 4 * nK * A3
+        b_passes = 4 * (3 * (8 * N + 32 * A1) + 4 * nK * A3) + 12 * N * 27
+    return per, b_conv, b_step, b_passes
 
 
 def cpu_baseline(cfg, seconds):
@@ -416,7 +434,9 @@ def main():
                          "serve: continuous batching of a request stream")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-serial", type=int, default=0, help="requests run alone for the serial side (0: all)")
+    ap.add_argument("--serve-serial", type=int, default=None,
+                    help="requests run alone for the serial side (0: all; default 0 for --workload serve, "
+                         "40 in the combined default run)")
     ap.add_argument("--serve-max-seconds", type=float, default=900.0)
     ap.add_argument("--sweep-sims", type=int, default=512)
     ap.add_argument("--sweep-ns", type=float, default=0.5)
@@ -427,6 +447,8 @@ def main():
     ap.add_argument("--lane-factor", type=float, default=3.0,
                     help="members with more than this times their group's median remaining trials move to a lane")
     ap.add_argument("--serial-sample", type=int, default=32, help="sweep members timed alone for the serial rate")
+    ap.add_argument("--no-sweep", action="store_true", help="cuboid workload: leave out the sweep object")
+    ap.add_argument("--no-serve", action="store_true", help="cuboid workload: leave out the serve object")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch a streaming kernel of known traffic (for HBM counter scaling)")
     args = ap.parse_args()
@@ -441,8 +463,12 @@ def main():
         print(json.dumps(bench_sweep(args, mm)), flush=True)
         return
     if args.workload == "serve":
+        if args.serve_serial is None:
+            args.serve_serial = 0
         print(json.dumps(bench_serve(args, mm)), flush=True)
         return
+    if args.serve_serial is None:
+        args.serve_serial = 40
     cfg = mm.workloads.cuboid_rk4()
     N = int(np.prod(cfg["size"]))
     sim = mm.workloads.make_sim(cfg)
@@ -458,10 +484,17 @@ def main():
     prof_n = np.zeros(8, np.int32)
     if args.profile_steps > 0:
         lib.mm_sim_profile_steps(sim._ptr, args.profile_steps, prof_ms.ctypes.data, prof_n.ctypes.data)
-    avg = {CLASSES[i]: (prof_ms[i] / prof_n[i] if prof_n[i] else 0.0) for i in range(8)}
-    per, b_conv, b_step = model_bytes(N, P, cfg["size"][0], cfg["size"][1], 6 if P[0] > 1 else 4)
-    dominant = max(per, key=lambda k: prof_ms[CLASSES.index(k)])
-    dom_ms = lib.mm_sim_time_pass(sim._ptr, CLASSES.index(dominant), args.pass_launches) / args.pass_launches
+    names = list(CLASSES)
+    if sim.tune_params()["pp"]:
+        names[2] = "y_xkx_y_plane"
+    avg = {names[i]: (prof_ms[i] / prof_n[i] if prof_n[i] else 0.0) for i in range(8) if prof_n[i]}
+    tune = sim.tune_params()
+    plane = bool(tune["pp"])
+    per, b_conv, b_step, b_passes = model_bytes(N, P, cfg["size"][0], cfg["size"][1], 6 if P[0] > 1 else 4, plane)
+    cls_of = {k: CLASSES.index(k) for k in per if k in CLASSES}
+    cls_of["y_xkx_y_plane"] = 2  # the plane pass is profiled as class 2
+    dominant = max(per, key=lambda k: prof_ms[cls_of[k]])
+    dom_ms = lib.mm_sim_time_pass(sim._ptr, cls_of[dominant], args.pass_launches) / args.pass_launches
     ach = per[dominant] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     traffic = committed_traffic(dominant, cfg["name"])
     step_gbs = b_step / (ms_per_step / 1e3) / 1e9
@@ -493,6 +526,9 @@ def main():
         },
         "hbm_pct_peak_model": round(100 * step_gbs / HBM_PEAK_GBS, 2),
         "step_model_bytes": b_step,
+        "step_pass_bytes": b_passes,
+        "hbm_pct_peak_passes": round(100 * b_passes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 2),
+        "launch_shapes": tune,
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
@@ -514,6 +550,14 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"], o, n = cpu_baseline(cfg, args.cpu_seconds)
         out["parity"] = parity_vs_oracle(mm, cfg, o, n)
+        del o
+    print("bench.py cuboid: %.2f Mcell-steps/s" % value, file=sys.stderr, flush=True)
+    # the batched sweep and the served request stream, against the same
+    # simulations run one at a time on this GPU
+    if not args.no_sweep:
+        out["sweep"] = bench_sweep(args, mm)
+    if not args.no_serve:
+        out["serve"] = bench_serve(args, mm)
     print(json.dumps(out), flush=True)
 
 

@@ -145,6 +145,30 @@ double mm_sim_profile_steps(mmSim* s, int n, double* ms, int* counts);
  * unchanged; the next step recomputes everything they touch. */
 double mm_sim_time_pass(mmSim* s, int cls, int n);
 
+/* Autotuner of the launch shapes.  The reference fixes them by one rule for
+ * every kernel (make1dconf, gpu_conf.cu:68-97, with the size filter of
+ * cufftsize.go:14-52); here each pass of a fused-mode simulation has its own
+ * tunables, timed on the device: params[6] = {zr: Z forward rows per block
+ * (variant 0 default, 1 half, 2 double), er: fused Z inverse rows per block
+ * (0 default, 1 half), yb: blocks per CU of the 3-D Y passes, xb: blocks per CU
+ * of the X.K.X pass, y2b: blocks per CU of the 2-D Y.K.Y pass, pp: for 3-D
+ * plans with padded Y 512 and padded X 16 or 32, 1 = Y forward, X.K.X and Y
+ * inverse as one register-resident plane pass, 0 = three passes}.  Each tunable's
+ * candidates run `reps` back-to-back launches (least of 3 runs); the fastest is
+ * kept in s and stored in the process's tuning table under s's data size.
+ * ms_default / ms_tuned: ms per launch of the affected pass(es) at the default
+ * and the chosen value (-1 where the plan has no such pass).  m is unchanged.
+ * The shapes change no result bit, except that the fused Z inverse's row count
+ * sets the summation order of the FP64 per-block energy sums. */
+int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[6], double ms_tuned[6]);
+void mm_sim_get_tune(mmSim* s, int params[6]);
+/* Tuning table: new simulations of a listed data size take its parameters.
+ * Text, one line "Nx Ny Nz zr er yb xb y2b pp" per size (internal order).  save
+ * and load return the number of entries, or -1 if the file cannot be used. */
+int mm_tune_save(const char* path);
+int mm_tune_load(const char* path);
+void mm_tune_clear(void);
+
 /* device bytes mm_sim_new(p, ..., MM_MODE_FUSED) allocates (admission by HBM
  * footprint); the kernel set-up needs transient FP64 buffers beyond this */
 unsigned long long mm_sim_footprint(const mmParams* p);
@@ -155,7 +179,9 @@ void mm_sim_begin_run(mmSim* s, double seconds);
 /* ApplyStatic / ApplyPointwise on a live simulation (simfield.go:87-153): n
  * points (t[n] seconds, b[3n] tesla, internal order; n = 1: static, n = 0:
  * none).  Takes effect at the next trial; a batch built before keeps the
- * table it copied (the old table stays allocated until mm_sim_free). */
+ * copy of the table it was built with.  The table is replaced in place (the
+ * call waits for the simulation's queued trials), in a buffer that grows by
+ * doubling. */
 void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b);
 /* Replace the applied field by a function of time (MM_FIELD_*), amplitude b[3]
  * in tesla (internal X, Y, Z), params[4]: PULSE (duration s, rise time s),

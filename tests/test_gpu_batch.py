@@ -161,3 +161,45 @@ def test_failed_member_is_reported_and_batch_mates_unaffected(mm):
     b.free()
     for s in sims:
         s.free()
+
+
+def test_batch_keeps_its_field_table_when_the_simulation_replaces_it(mm):
+    """A batch copies each member's pointwise field table when it is built
+    (mm_sim_set_field then replaces the simulation's own table in place): a
+    member whose table is replaced after the batch was built runs the ramp the
+    batch was built with, bit for bit as alone; a batch built afterwards runs
+    the new table, again as alone.  The second table is longer than the
+    first, so the simulation's buffer grows as well."""
+    base = mm.workloads.sweep_member(11, 3, duration=0.03e-9, **SMALL)  # the ramps part at 10 ps
+    ramp1 = [(0.0, (0.0, 0.0, 0.0)), (0.01e-9, (20e-3, 5e-3, 0.0))]   # (t, internal (X, Y, Z) tesla)
+    ramp2 = ramp1 + [(0.02e-9, (-30e-3, 0.0, 10e-3)), (0.05e-9, (0.0, 40e-3, 0.0))]
+
+    def with_ramp(ramp):
+        return mm.workloads.make_sim(dict(base, field=("pointwise", ramp)))
+
+    def run_alone(ramp, seconds):
+        s = with_ramp(ramp)
+        s.run(seconds)
+        out = s.m, s.trial_log()
+        s.free()
+        return out
+
+    m1, (dt1, _) = run_alone(ramp1, base["duration"])
+    s = with_ramp(ramp1)
+    s._init()
+    b = mm.Batch([s])
+    for t, (bX, bY, bZ) in ramp2[2:]:
+        s.applypointwise("field", t, bZ, bY, bX)   # replaces the simulation's table in place
+    b.run([base["duration"]])
+    b.free()
+    assert np.array_equal(s.trial_log()[0], dt1) and np.array_equal(s.m, m1)
+    s.free()
+
+    m2, (dt2, _) = run_alone(ramp2, base["duration"])
+    s = with_ramp(ramp2)
+    b = mm.Batch([s])
+    b.run([base["duration"]])
+    b.free()
+    assert np.array_equal(s.trial_log()[0], dt2) and np.array_equal(s.m, m2)
+    assert not np.array_equal(m1, m2)
+    s.free()

@@ -151,6 +151,8 @@ def pass_counts(mm, s, steps=1):
     """Launch counts per pass class over `steps` profiled steps of `s`:
     [Z forward, Y forward, X.K.X / plane / 2-D, Y inverse, Z inverse, ...]."""
     lib = mm.load()
+    s._init()  # the device simulation is created lazily
+    assert s._ptr
     ms = np.zeros(8)
     n = np.zeros(8, np.int32)
     lib.mm_sim_profile_steps(s._ptr, steps, ms.ctypes.data, n.ctypes.data)
@@ -167,12 +169,21 @@ def test_full_3d_path_runs_separate_passes(mm, size, cell):
     s.free()
 
 
-def test_cuboid_256x256x16_vs_oracle(mm, orc):
+def test_cuboid_256x256x16_vs_oracle(mm, lib, orc, tmp_path):
     """The benchmark's workload at full size (256x256x16 cells of 4 nm, padded
-    32 x 512 x 512; k_zfwd / k_ypass<512> / k_xmul<32> / k_zinv_epi): H_eff and
-    the torque of the seeded random m0 (5e-6 relative to max |H|), then 20
-    fixed-dt RK4 steps (1e-14 s) against the FP64 oracle (max |dm| <= 1e-5)."""
+    32 x 512 x 512) through the separate passes (k_zfwd / k_ypass<512> /
+    k_xmul<32> / k_zinv_epi): H_eff and the torque of the seeded random m0
+    (5e-6 relative to max |H|), then 20 fixed-dt RK4 steps (1e-14 s) against
+    the FP64 oracle (max |dm| <= 1e-5)."""
     cfg = mm.workloads.cuboid_rk4()
+    force_plane_pass(mm, lib, tmp_path, cfg["size"], on=0)
+    try:
+        _cuboid_separate_passes(mm, orc, cfg)
+    finally:
+        restore_tuning(mm, lib)
+
+
+def _cuboid_separate_passes(mm, orc, cfg):
     s = mm.workloads.make_sim(cfg)
     o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
     m0 = cfg["m0"]
@@ -193,6 +204,93 @@ def test_cuboid_256x256x16_vs_oracle(mm, orc):
     n = pass_counts(mm, s)
     assert n[1] == n[3] == 4, n
     s.free()
+
+
+# grids the register-resident plane pass fits (padded Y 512, padded X 16 or 32):
+# Y forward, X.K.X and Y inverse in one pass, block 0 packing the real planes
+# kz = 0 and kz = Q - 1 through their Hermitian symmetry
+PLANE_CASES = [
+    ((16, 256, 32), (4e-9, 4e-9, 4e-9)),       # padded 32 x 512 x 64 (X 32: two X sub-rounds)
+    ((13, 200, 20), (3e-9, 2.5e-9, 4e-9)),     # padded 32 x 512 x 48: Nx < PX/2, Ny < 256, Z 3 * 2^a
+    ((8, 256, 64), (4e-9, 4e-9, 4e-9)),        # padded 16 x 512 x 128 (X 16)
+    ((7, 193, 6), (3e-9, 3e-9, 3e-9)),         # padded 16 x 512 x 12: few planes
+]
+
+
+def force_plane_pass(mm, lib, tmp_path, size, on=1):
+    """New simulations of `size` take the plane pass (tuning table entry)."""
+    t = tmp_path / "plane.txt"
+    t.write_text("%d %d %d 0 0 8 4 8 %d\n" % (tuple(size) + (on,)))
+    lib.mm_tune_clear()
+    assert lib.mm_tune_load(str(t).encode()) == 1
+
+
+def restore_tuning(mm, lib):
+    import os
+
+    lib.mm_tune_clear()
+    if os.path.exists(mm.gpukern.TUNE_TABLE):
+        lib.mm_tune_load(mm.gpukern.TUNE_TABLE.encode())
+
+
+@pytest.mark.parametrize("size,cell", PLANE_CASES)
+def test_plane_pass_vs_oracle(mm, lib, orc, tmp_path, size, cell):
+    """H_eff and the torque through the plane pass (5e-6 relative to max |H|),
+    then 10 RK4 steps (1e-5), against the FP64 oracle; the pass counts show
+    that the separate Y passes did not run."""
+    print("plane pass: grid %s" % (size,), flush=True)
+    force_plane_pass(mm, lib, tmp_path, size)
+    try:
+        m0 = random_m(size, 7)
+        s = engine(mm, size, cell, solver="rk4", dt=2e-14, m0=m0, k1=3e5, axis=(0, 0, 1))
+        o = oracle(orc, size, cell, m0, solver="rk4", dt=2e-14, k1=3e5, axis=(0, 0, 1))
+        h = s.heff(m0)
+        assert s.tune_params()["pp"] == 1
+        href = o.heff(m0).reshape(h.shape)
+        k = s.torque(m0)
+        kref = o.torque(m0).reshape(k.shape)
+        eh = float(np.abs(h - href).max() / np.abs(href).max())
+        ek = float(np.abs(k - kref).max() / np.abs(href).max())
+        s.steps(10)
+        for _ in range(10):
+            o.step()
+        dm = float(np.abs(s.m - o.m.reshape(s.m.shape)).max())
+        print("plane pass %s: H_eff rel %.3g, torque rel %.3g, 10 RK4 steps max|dm| %.3g" % (size, eh, ek, dm))
+        assert eh < 5e-6 and ek < 5e-6
+        assert dm < 1e-5
+        n = pass_counts(mm, s)
+        assert n[1] == n[3] == 0 and n[2] == n[0] == n[4] == 4, n
+        s.free()
+    finally:
+        restore_tuning(mm, lib)
+
+
+def test_cuboid_256x256x16_plane_pass_vs_oracle(mm, lib, orc, tmp_path):
+    """The benchmark's cuboid through the plane pass (256 blocks for its 257
+    planes): H_eff and torque (5e-6), 20 RK4 steps (1e-5) against the FP64 oracle."""
+    cfg = mm.workloads.cuboid_rk4()
+    force_plane_pass(mm, lib, tmp_path, cfg["size"])
+    try:
+        s = mm.workloads.make_sim(cfg)
+        o = orc.sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+        m0 = cfg["m0"]
+        h = s.heff(m0)
+        assert s.tune_params()["pp"] == 1
+        href = o.heff(m0).reshape(h.shape)
+        k = s.torque(m0)
+        kref = o.torque(m0).reshape(k.shape)
+        eh = float(np.abs(h - href).max() / np.abs(href).max())
+        ek = float(np.abs(k - kref).max() / np.abs(href).max())
+        s.steps(20)
+        for _ in range(20):
+            o.step()
+        dm = float(np.abs(s.m - o.m.reshape(s.m.shape)).max())
+        print("cuboid 256x256x16, plane pass: H_eff rel %.3g, torque rel %.3g, 20 RK4 steps max|dm| %.3g" % (eh, ek, dm))
+        assert eh < 5e-6 and ek < 5e-6
+        assert dm < 1e-5
+        s.free()
+    finally:
+        restore_tuning(mm, lib)
 
 
 def replay_log(o, dts, acc, start=0, stop=None):
