@@ -300,9 +300,10 @@ template <int L> __device__ constexpr bool pad_half(int e) {
 // The lanes of one line (G = L / E <= 64) are always in one wave, and the
 // exchange buffer belongs to that line, so a wave-level barrier orders its LDS
 // traffic (the pattern of rocPRIM's wave_barrier: release fence, wave barrier,
-// acquire fence).  Kernels whose buffers are private to a line during the
-// transform (WS = true) use it instead of the block barrier, so waves do not
-// wait for each other between radix passes.
+// acquire fence), and waves do not wait for each other between radix passes.
+// Every caller's exchange buffer is private to its line during the transform;
+// where the buffers overlay other data, the callers separate the uses with
+// block barriers of their own.  WS = false keeps block barriers.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -312,7 +313,7 @@ __device__ __forceinline__ void wave_sync() {
 // EE: elements per lane (default RegFFT<L>::E; a smaller power-of-two EE spreads
 // a line over more lanes, L / EE, with the same exchange buffer); WS: wave-level
 // instead of block-level barriers around the exchanges (see wave_sync)
-template <typename T, int L, int DIR, int NS = 1, int EE = reg_fft_E(L), bool WS = false>
+template <typename T, int L, int DIR, int NS = 1, int EE = reg_fft_E(L), bool WS = true>
 __device__ __forceinline__ void reg_fft(C2<T>* v, int g, C2<T>* lds, const C2<T>* tw) {
     if constexpr (NS < L) {
         constexpr int E = EE, G = L / EE;

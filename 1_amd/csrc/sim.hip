@@ -988,7 +988,7 @@ double mm_sim_time_pass(mmSim* s, int cls, int n) {
     return total;
 }
 
-int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[6], double ms_tuned[6]) {
+int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double ms_tuned[7]) {
     GK_ASSERT(s->mode == MM_MODE_FUSED && s->conv, "tune: an initialised fused-mode simulation");
     if (reps < 1) reps = 1;
     ConvPlan& cp = s->cp;
@@ -1026,7 +1026,7 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[6], double 
         ms_default[slot] = t_d;
         ms_tuned[slot] = t_best;
     };
-    for (int i = 0; i < 6; i++) ms_default[i] = ms_tuned[i] = -1;
+    for (int i = 0; i < 7; i++) ms_default[i] = ms_tuned[i] = -1;
     std::vector<int> zr{0}, er{0};
     for (int v = 1; v <= 2; v++)
         if (conv_zr_valid(cp.M, v)) zr.push_back(v);
@@ -1050,6 +1050,8 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[6], double 
             cp.tp.pp = t[1] < t[0] ? 1 : 0;
             ms_default[5] = t[dflt.pp];
             ms_tuned[5] = t[cp.tp.pp];
+            ms_default[6] = t[0];  // the three passes
+            ms_tuned[6] = t[1];    // the plane pass
         }
     }
     if (!d3) tune_one(4, &cp.tp.y2b, dflt.y2b, grid, 2, -1);

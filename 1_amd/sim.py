@@ -896,14 +896,19 @@ class Sim:
     def tune(self, reps=20):
         """Autotune this simulation's launch shapes on the device
         (mm_sim_tune): returns {tunable: (chosen value, ms per launch at the
-        default, ms at the chosen value)} for the tunables its plan has; the
-        choice is also stored in the process's tuning table."""
+        default, ms at the chosen value)} for the tunables its plan has, and
+        for plans the plane pass fits "plane_vs_three": (ms of the three
+        separate passes, ms of the plane pass); the choice is also stored in
+        the process's tuning table."""
         self._init()
         par = np.zeros(6, np.int32)
-        md = np.zeros(6)
-        mt = np.zeros(6)
+        md = np.zeros(7)
+        mt = np.zeros(7)
         self._lib.mm_sim_tune(self._ptr, int(reps), par.ctypes.data, md.ctypes.data, mt.ctypes.data)
-        return {k: (int(par[i]), float(md[i]), float(mt[i])) for i, k in enumerate(self.TUNABLES) if md[i] >= 0}
+        out = {k: (int(par[i]), float(md[i]), float(mt[i])) for i, k in enumerate(self.TUNABLES) if md[i] >= 0}
+        if md[6] >= 0:
+            out["plane_vs_three"] = (float(md[6]), float(mt[6]))  # ms: three passes, plane pass
+        return out
 
     def tune_params(self):
         """The launch shapes this simulation runs with (mm_sim_get_tune)."""

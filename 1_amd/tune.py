@@ -59,18 +59,24 @@ def main():
     lines = ["| workload (internal X,Y,Z) | tunable | default | chosen | ms/launch default | ms/launch tuned | gain |",
              "|---|---|---|---|---|---|---|"]
     default = {"zr": 0, "er": 0, "yb": 8, "xb": 4, "y2b": 8, "pp": 0}
+    extra = ["", "Plane pass against the three passes it replaces (ms per evaluation, tuned shapes):", "",
+             "| workload (internal X,Y,Z) | Y fwd + X.K.X + Y inv | plane pass |", "|---|---|---|"]
     for name, cfg in shapes(mm):
         s = mm.workloads.make_sim(cfg)
         res = s.tune(args.reps)
         s.free()
-        for k, (v, md, mt) in res.items():
+        for k, val in res.items():
+            if k == "plane_vs_three":
+                extra.append("| %s %s | %.5f | %.5f |" % (name, tuple(cfg["size"]), val[0], val[1]))
+                continue
+            v, md, mt = val
             lines.append("| %s %s | %s | %d | %d | %.5f | %.5f | %.1f%% |" % (
                 name, tuple(cfg["size"]), k, default[k], v, md, mt, 100 * (md - mt) / md if md > 0 else 0))
         print("tune.py: %s done" % name, file=sys.stderr, flush=True)
     n = lib.mm_tune_save(args.out.encode())
     if n < 0:
         raise SystemExit("cannot write %s" % args.out)
-    text = "\n".join(lines)
+    text = "\n".join(lines + (extra if len(extra) > 5 else []))
     print(text)
     print("\n%d entries written to %s" % (n, args.out))
     print("\n" + "\n".join("- `%s`: %s" % kv for kv in DESCR.items()))

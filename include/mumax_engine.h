@@ -156,11 +156,13 @@ double mm_sim_time_pass(mmSim* s, int cls, int n);
  * inverse as one register-resident plane pass, 0 = three passes}.  Each tunable's
  * candidates run `reps` back-to-back launches (least of 3 runs); the fastest is
  * kept in s and stored in the process's tuning table under s's data size.
- * ms_default / ms_tuned: ms per launch of the affected pass(es) at the default
- * and the chosen value (-1 where the plan has no such pass).  m is unchanged.
+ * ms_default / ms_tuned [0..5]: ms per launch of the affected pass(es) at the
+ * default and the chosen value (-1 where the plan has no such pass); [6]: the
+ * three separate passes and the plane pass (-1 where it does not fit).  m is
+ * unchanged.
  * The shapes change no result bit, except that the fused Z inverse's row count
  * sets the summation order of the FP64 per-block energy sums. */
-int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[6], double ms_tuned[6]);
+int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double ms_tuned[7]);
 void mm_sim_get_tune(mmSim* s, int params[6]);
 /* Tuning table: new simulations of a listed data size take its parameters.
  * Text, one line "Nx Ny Nz zr er yb xb y2b pp" per size (internal order).  save
