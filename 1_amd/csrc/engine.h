@@ -43,6 +43,8 @@ struct EngineState {
     double relax_prev;  // energy after the previous step of the relaxation
     double field_tmax;  // latest stage time (internal) the applied field was evaluated at
     int nonfinite;      // max |torque| or the error estimate was NaN or infinite: the run stopped
+    float dt_noise;     // dt at the start of the current step: the thermal field's (rk.go:289-292)
+    long long noise_step;  // step the thermal field was drawn for (its counter)
 };
 
 // |x| for a per-block maximum that must not lose a NaN: fmaxf returns the other
@@ -243,6 +245,8 @@ void launch_set_relax(EngineState* st, int on, hipStream_t s);
 // ---------------------------------------------------------------- RK kernels
 struct TorqueArgs {
     float alpha, dtg;       // damping, 1/(1+alpha^2)
+    float thermalK;         // thermal field sigma * sqrt(dt) (internal units); 0: T = 0
+    unsigned long long noiseSeed;
     const float* alphaMap;  // per-cell damping multiplier (null: none); dtg keeps the global alpha (torque.go:49-51)
     int anisType;
     float kAnis;            // 2K1/(mu0 Ms^2) (init.go:103-108)
@@ -259,6 +263,8 @@ struct TorqueArgs {
 };
 // k = torque(m_i, H + local fields (+ exchange)); in place over hk.
 void launch_torque(const float* mi, float* hk, EngineState* st, const TorqueArgs& a, int N, hipStream_t s);
+// h += the thermal field of the current step (no-op at T = 0)
+void launch_add_thermal(float* h, const EngineState* st, const TorqueArgs& a, int N, hipStream_t s);
 
 // Applied field in internal units at the stage node: t = time0 + float64(c_i dt)
 // (rk.go:316), B(t) in tesla / B_unit (field.go:50-61).
@@ -268,6 +274,39 @@ __device__ __forceinline__ void stage_field(const TorqueArgs& a, const EngineSta
     float b[3];
     applied_field(a.field, t * a.unitTime, ts * a.unitTime, b);
     for (int i = 0; i < 3; i++) hext[i] = b[i] / a.unitField;
+}
+
+// Thermal field (temperature.go:29-36): per step and cell, three standard
+// normals from a counter-based stream -- SplitMix64's output function
+// (Steele, Lea & Flood 2014) over (seed, step, cell), 24-bit uniforms, the
+// Box-Muller transform -- times sigma = K / sqrt(dt_step).  The same draw in
+// every stage and trial of a step, in every launch shape and in batches.
+__host__ __device__ inline unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline unsigned long long thermal_key(unsigned long long seed, long long step, long cell, int j) {
+    const unsigned long long base = mix64(seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(step + 1));
+    return mix64(base + 0x9E3779B97F4A7C15ull * (unsigned long long)(4 * cell + j + 1));
+}
+// uniform in (0, 1) (half-offset) and in [0, 1) from the top 24 bits
+__host__ __device__ inline float thermal_u_open(unsigned long long r) { return ((float)(r >> 40) + 0.5f) * 0x1p-24f; }
+__host__ __device__ inline float thermal_u(unsigned long long r) { return (float)(r >> 40) * 0x1p-24f; }
+__device__ __forceinline__ void add_thermal(const TorqueArgs& a, const EngineState* st, long cell, float h[3]) {
+    if (a.thermalK == 0.f) return;
+    const float sigma = a.thermalK * rsqrtf(st->dt_noise);
+    const float u1 = thermal_u_open(thermal_key(a.noiseSeed, st->noise_step, cell, 0));
+    const float u2 = thermal_u(thermal_key(a.noiseSeed, st->noise_step, cell, 1));
+    const float u3 = thermal_u_open(thermal_key(a.noiseSeed, st->noise_step, cell, 2));
+    const float u4 = thermal_u(thermal_key(a.noiseSeed, st->noise_step, cell, 3));
+    const float r1 = sqrtf(-2.f * logf(u1)), r2 = sqrtf(-2.f * logf(u3));
+    float s1, c1, s2, c2;
+    sincospif(2.f * u2, &s1, &c1);
+    sincospif(2.f * u4, &s2, &c2);
+    h[0] += sigma * (r1 * c1);
+    h[1] += sigma * (r1 * s1);
+    h[2] += sigma * (r2 * c2);
 }
 
 // LL torque of one cell: h = H + B_ext + uniaxial anisotropy of m
@@ -350,6 +389,7 @@ struct CtrlParams {
     const double* part_e;   // non-null (wantEnergy): n_k per-block energy sums of this trial
     double cell_volume;     // internal units
     float cmax;             // largest stage node of the tableau (advances field_tmax)
+    int thermal;            // T > 0: no dt control after the first torque (rk.go:348)
 };
 void launch_ctrl_begin(EngineState* st, hipStream_t s);
 void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s);

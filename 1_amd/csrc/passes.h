@@ -305,8 +305,9 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
             }
             esum += phi;
         }
-        float k[3];
-        llg_torque_cell(ea.t, cell_alpha(ea.t, ci), hext, mi[0], mi[1], mi[2], h[0], h[1], h[2], k);
+        float k[3], ht[3] = {h[0], h[1], h[2]};
+        add_thermal(ea.t, st, ci, ht);  // not in the energy density (field.go:64-80)
+        llg_torque_cell(ea.t, cell_alpha(ea.t, ci), hext, mi[0], mi[1], mi[2], ht[0], ht[1], ht[2], k);
         lmax = fmaxf(lmax, fmaxf(abs_keep_nan(k[0]), fmaxf(abs_keep_nan(k[1]), abs_keep_nan(k[2]))));
         if constexpr (FIN == 2) {
 #pragma unroll

@@ -29,8 +29,9 @@ __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, flo
     const float hext[3] = {hext_s[0], hext_s[1], hext_s[2]};
     float lmax = 0.f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        float k[3];
-        llg_torque_cell(a, cell_alpha(a, i), hext, m[i], m[N + i], m[2 * N + i], hk[i], hk[N + i], hk[2 * N + i], k);
+        float k[3], h[3] = {hk[i], hk[N + i], hk[2 * N + i]};
+        add_thermal(a, st, i, h);
+        llg_torque_cell(a, cell_alpha(a, i), hext, m[i], m[N + i], m[2 * N + i], h[0], h[1], h[2], k);
         hk[i] = k[0];
         hk[N + i] = k[1];
         hk[2 * N + i] = k[2];
@@ -39,6 +40,17 @@ __global__ __launch_bounds__(256) void k_torque(const float* __restrict__ m, flo
     if (a.part) {
         const float bm = block_max_nonneg(lmax);
         if (threadIdx.x == 0) a.part[blockIdx.x] = bm;
+    }
+}
+
+// h += the thermal field of the current step (calcHeff's addThermalField, field.go:74-80)
+__global__ void k_add_thermal(float* __restrict__ h, const EngineState* st, TorqueArgs a, int N) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        float t[3] = {0.f, 0.f, 0.f};
+        add_thermal(a, st, i, t);
+        h[i] += t[0];
+        h[N + i] += t[1];
+        h[2 * N + i] += t[2];
     }
 }
 
@@ -152,6 +164,12 @@ void launch_set_target(EngineState* st, long long nsteps, double stop, hipStream
     GK_LAUNCH_CHECK();
 }
 
+
+void launch_add_thermal(float* h, const EngineState* st, const TorqueArgs& a, int N, hipStream_t s) {
+    if (a.thermalK == 0.f) return;
+    k_add_thermal<<<gridN(N), 256, 0, s>>>(h, st, a, N);
+    GK_LAUNCH_CHECK();
+}
 
 void launch_torque(const float* mi, float* hk, EngineState* st, const TorqueArgs& a, int N, hipStream_t s) {
     // a partial buffer has kRedBlocks entries: launch exactly that many blocks when reducing

@@ -104,6 +104,9 @@ __device__ __forceinline__ void ctrl_begin_body(EngineState* st) {
         st->trials = 0;
         st->new_step = 0;
         st->prev_energy = st->energy == 0 ? 9999999999.0 : st->energy;  // rk.go:297-298
+        // the thermal field of this step: drawn from dt at its start (rk.go:289-292)
+        st->dt_noise = st->dt;
+        st->noise_step = st->steps;
     }
     st->dt_used = st->dt;
 }
@@ -121,9 +124,10 @@ __device__ __forceinline__ void ctrl_after_k0_body(EngineState* st, int fsal_cop
     if (!st->trial_active) return;
     const float maxTorque = wave_max_partials(fsal_copy ? p.part_k6 : p.part_k0, p.n_k);
     if (threadIdx.x != 0) return;
+    if (!isfinite(maxTorque)) st->nonfinite = 1;
+    if (p.thermal) return;  // rk.go:348: the dt control after k0 only runs at T = 0
     if (!p.adaptive) st->dt = p.inputDt;
     st->max_torque = maxTorque;
-    if (!isfinite(maxTorque)) st->nonfinite = 1;
     float dt = st->dt;
     const float dm = dt * maxTorque;
     if (dm < p.minDm) dt = p.minDm / maxTorque;
@@ -150,7 +154,7 @@ __device__ __forceinline__ void ctrl_end_body(EngineState* st, const CtrlParams&
     if (p.fold_k0) {
         st->max_torque = k0_red;
         if (!isfinite(k0_red)) st->nonfinite = 1;
-        if (!p.adaptive) st->dt = p.inputDt;
+        if (!p.adaptive && !p.thermal) st->dt = p.inputDt;
     }
     const float dtu = st->dt_used;
     st->time = st->time0 + (double)dtu;

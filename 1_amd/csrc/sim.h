@@ -48,6 +48,7 @@ struct mmSim {
     double unitLength = 0, unitTime = 0, unitField = 0, unitEnergy = 0, unitEdens = 0;
     double cell[3] = {};  // internal units
     float kAnis = 0;
+    float thermalK = 0;  // thermal field sigma * sqrt(dt), internal units (0: T = 0)
     float u[3] = {};
     Tableau tb;
     std::vector<double> fieldT;

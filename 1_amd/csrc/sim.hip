@@ -258,6 +258,7 @@ CtrlParams gk::ctrl_params(const mmSim* s) {
     c.log_accept = s->log_acc;
     c.log_err = s->log_err;
     c.log_cap = s->log_cap;
+    c.thermal = s->p.temperature > 0;
     return c;
 }
 
@@ -274,6 +275,8 @@ TorqueArgs gk::torque_args(const mmSim* s, float c, float* part) {
     a.unitTime = s->unitTime;
     a.c = c;
     a.part = part;
+    a.thermalK = s->thermalK;
+    a.noiseSeed = s->p.seed;
     return a;
 }
 
@@ -558,6 +561,20 @@ static void abi_init(mmSim* s, const double cellInt[3]) {
 // ---------------------------------------------------------------- C ABI
 extern "C" {
 
+// thermal field: sigma(dt) = sqrt(2 alpha kB T / (gamma0 mu0 Ms V dt_SI)) / Ms (temperature.go:29-36),
+// dt_SI = dt UnitTime, so sigma = thermalK / sqrt(dt) in internal units
+static void set_thermal(mmSim* s, double T) {
+    GK_ASSERT(T >= 0, "temperature %g K", T);
+    s->p.temperature = T;
+    s->thermalK = 0.f;
+    if (T > 0) {
+        const double kB = 1.380650424e-23, gamma0 = 2.211e5, mu0 = 4e-7 * M_PI;  // material.go:27, init.go:82-83
+        const double V = s->p.cellsize[0] * s->p.cellsize[1] * s->p.cellsize[2];
+        s->thermalK = (float)(std::sqrt(2 * s->p.alpha * kB * T / (gamma0 * mu0 * s->p.msat * V * s->unitTime)) /
+                              s->p.msat);
+    }
+}
+
 mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     mmSim* s = new mmSim;
     s->p = *p;
@@ -582,6 +599,8 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->unitEdens = s->unitEnergy / (s->unitLength * s->unitLength * s->unitLength);
     for (int i = 0; i < 3; i++) s->cell[i] = p->cellsize[i] / s->unitLength;
     s->kAnis = (float)(p->k1 / s->unitEdens);
+    GK_ASSERT(p->temperature == 0 || mode == MM_MODE_FUSED, "a finite temperature needs the fused mode");
+    set_thermal(s, p->temperature);
     for (int i = 0; i < 3; i++) s->u[i] = (float)p->anisAxis[i];
     s->tb = make_tableau(p->solver);
     GK_ASSERT(!s->tb.adaptive || p->maxError > 0, "maxError must be > 0 for an adaptive solver");
@@ -611,6 +630,8 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     h.dt = (float)(p->dt / s->unitTime);  // init.go:280-284
     if (h.dt == 0.f) h.dt = 1e-4f;
     h.new_step = 1;
+    h.dt_noise = h.dt;
+    h.noise_step = 0;
     h.stop = 0;
     h.done = 1;
     h.field_tmax = -INFINITY;
@@ -802,6 +823,9 @@ static void eval_heff(mmSim* s, const float* m_host, float* h_dev) {
     float kk = s->kAnis;
     float ax[3] = {s->u[0], s->u[1], s->u[2]};
     gpu_add_local_fields(s->mi, h_dev, s->N, hext, nullptr, s->p.anisType, &kk, ax);
+    // the thermal field of the current step (field.go:74-80)
+    launch_add_thermal(h_dev, s->st, torque_args(s, 0.f, nullptr), s->N, s->s);
+    GK_CHECK(hipStreamSynchronize(s->s));
 }
 
 void mm_sim_heff(mmSim* s, const float* m, float* h) {
@@ -1066,6 +1090,12 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double 
         params[5] = cp.tp.pp;
     }
     return 0;
+}
+
+void mm_sim_set_temperature(mmSim* s, double temperature) {
+    GK_ASSERT(temperature == 0 || s->mode == MM_MODE_FUSED, "a finite temperature needs the fused mode");
+    GK_CHECK(hipStreamSynchronize(s->s));
+    set_thermal(s, temperature);
 }
 
 void mm_sim_get_tune(mmSim* s, int params[6]) {

@@ -149,6 +149,7 @@ def go_float(v, bits=32):
 COMMANDS = {
     # material (simmaterial.go, simanis.go)
     "msat": ("msat", "f"), "aexch": ("aexch", "f"), "alpha": ("alpha", "f"), "k1": ("k1", "f"),
+    "temperature": ("temperature", "f"),
     "anisuniaxial": ("anisuniaxial", "fff"),
     # geometry (simgeom.go)
     "gridsize": ("gridsize", "iii"), "cellsize": ("cellsize", "fff"), "partsize": ("partsize", "fff"),
@@ -188,7 +189,7 @@ COMMANDS = {
 # Commands of the reference whose physics or geometry this engine does not
 # implement; they stop the script with an InputErr naming the command.
 UNSUPPORTED = (
-    "k2", "aniscubic", "spinpolarization", "xi", "temperature", "currentdensity", "currentmask", "fieldmask",
+    "k2", "aniscubic", "spinpolarization", "xi", "currentdensity", "currentmask", "fieldmask",
     "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortex_in_array", "setm", "applyrotatingburst", "edgecorrection",
     "savebenchmark", "exchangetype",

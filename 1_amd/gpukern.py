@@ -103,6 +103,7 @@ ENGINE = {
     "mm_kernel_cache_size": (I, []),
     "mm_sim_set_field": (None, [P, I, P, P]),
     "mm_sim_set_field_function": (None, [P, I, P, P]),
+    "mm_sim_set_temperature": (None, [P, C.c_double]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
     "mm_batch_set_graph": (None, [P, I]),
@@ -148,6 +149,8 @@ class Params(C.Structure):
         ("wantEnergy", C.c_int),
         ("demagKernel", C.c_int),
         ("alphaMap", C.POINTER(C.c_float)),
+        ("temperature", C.c_double),
+        ("seed", C.c_ulonglong),
     ]
 
 

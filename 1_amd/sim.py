@@ -149,6 +149,8 @@ class Sim:
         self._want_energy = False
         self._kernel = 0
         self._alpha_map = None
+        self._temperature = 0.0
+        self._thermal_seed = 0
         self._m0 = None
         # output (sim.go:179-209): schedule, table columns, output id
         self._outdir = "."
@@ -337,6 +339,30 @@ class Sim:
         """SetAlphaRange (simgeom.go:253-264): multiplier over user x1..x2,
         y1..y2, z1..z2 (inclusive)."""
         self._alpha_mask()[z1:z2 + 1, y1:y2 + 1, x1:x2 + 1] = alpha
+
+    def temperature(self, T):
+        """Temperature(T) (temperature.go:47-56): a thermal field of standard
+        deviation sqrt(2 alpha kB T / (gamma0 mu0 Ms V dt)) / Ms per component,
+        drawn at the start of each step.  Before the simulation exists it also
+        selects the RK12 solver, as the reference does; afterwards the solver
+        must already be RK12, and the new temperature applies from the next
+        trial."""
+        T = float(np.float32(T))
+        if T < 0:
+            raise InputErr("temperature must be >= 0")
+        if not self._ptr:
+            self._solver = "rk12"
+        elif self._solver != "rk12":
+            raise InputErr("Finite temperature is only compatible with the RK12 solver type")
+        self._temperature = T
+        if self._ptr:
+            self._lib.mm_sim_set_temperature(self._ptr, T)
+
+    def thermalseed(self, seed):
+        """Seed of the thermal field's counter-based stream (the reference seeds
+        cuRAND with 0 at its first use, gpu_temperature.cu:31-39)."""
+        self._invalidate()
+        self._thermal_seed = int(seed) & ((1 << 64) - 1)
 
     def kerneltype(self, name):
         """KernelType (simfield.go): the demag kernel.  "facekernel6" (also the
@@ -960,6 +986,8 @@ class Sim:
         p.demagAccuracy = self._demag_accuracy
         p.wantEnergy = int(self._want_energy)
         p.demagKernel = self._kernel
+        p.temperature = self._temperature
+        p.seed = self._thermal_seed
         if self._alpha_map is not None:
             amap = np.ascontiguousarray(self._alpha_map, np.float32)
             self._keep_alpha = amap

@@ -211,6 +211,13 @@ def make_sim(cfg, mode="fused"):
         s.maxerror(cfg["maxerror"])
     if cfg.get("dt"):
         s.dt(cfg["dt"])
+    if cfg.get("max_dt"):
+        s.maxdt(cfg["max_dt"])
+    if cfg.get("min_dt"):
+        s.mindt(cfg["min_dt"])
+    if cfg.get("temperature"):
+        s.temperature(cfg["temperature"])
+        s.thermalseed(cfg.get("seed", 0))
     f = cfg.get("field")
     if f is not None:
         if f[0] == "static":
@@ -230,6 +237,11 @@ def oracle_kwargs(cfg):
               dt=cfg.get("dt") or 0.0)
     if cfg.get("anis_axis") is not None:
         kw.update(k1=cfg["k1"], anis_axis=cfg["anis_axis"])
+    if cfg.get("temperature"):
+        kw.update(temperature=cfg["temperature"], seed=cfg.get("seed", 0))
+    for k in ("max_dt", "min_dt"):
+        if cfg.get(k):
+            kw[k] = cfg[k]
     f = cfg.get("field")
     if f is not None:
         if f[0] == "static":

@@ -65,6 +65,14 @@ typedef struct {
                                SetAlphaRange, simgeom.go:226-264); null: uniform alpha.
                                The torque's 1/(1+alpha^2) keeps the global alpha
                                (torque.go:49-51). Copied at mm_sim_new. */
+    double temperature;     /* K (Temperature, temperature.go:29-36, sim.go:78); 0: none.
+                               Fused mode: a thermal field of standard deviation
+                               sqrt(2 alpha kB T / (gamma0 mu0 Ms V dt)) / Ms per component,
+                               drawn once per step from dt at the step's start and kept
+                               through its trials (rk.go:289-292) */
+    unsigned long long seed;  /* thermal noise stream: counter-based (SplitMix64 of seed,
+                               step and cell, Box-Muller), so it does not depend on launch
+                               shapes or batching */
 } mmParams;
 
 typedef struct mmSim mmSim;
@@ -185,6 +193,10 @@ void mm_sim_begin_run(mmSim* s, double seconds);
  * call waits for the simulation's queued trials), in a buffer that grows by
  * doubling. */
 void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b);
+/* Temperature(T) on a live simulation (temperature.go:47-56): the thermal field's
+ * standard deviation from the next trial on (fused mode); a batch built before
+ * keeps the temperature it was built with. */
+void mm_sim_set_temperature(mmSim* s, double temperature);
 /* Replace the applied field by a function of time (MM_FIELD_*), amplitude b[3]
  * in tesla (internal X, Y, Z), params[4]: PULSE (duration s, rise time s),
  * RF (frequency Hz), RF_RAMP (frequency, ramp time), SAWTOOTH (frequency),

@@ -53,6 +53,8 @@ class Params(C.Structure):
         ("fieldT", C.POINTER(C.c_double)),
         ("fieldB", C.POINTER(C.c_double)),
         ("wantEnergy", C.c_int),
+        ("temperature", C.c_double),
+        ("seed", C.c_ulonglong),
     ]
 
 
@@ -110,6 +112,7 @@ class Oracle:
         L.or_sim_trial_log.argtypes = [P, P, P, C.c_long]
         L.or_sim_trial_log.restype = C.c_long
         L.or_sim_energy.argtypes = [P]
+        L.or_thermal_normals.argtypes = [C.c_ulonglong, C.c_longlong, C.c_long, P]
         L.or_sim_energy.restype = C.c_double
 
     def _creal(self):
@@ -215,6 +218,15 @@ class Oracle:
     def sim(self, m0, **kw):
         return Sim(self, m0, **kw)
 
+    def thermal_normals(self, seed, step, cells):
+        """The thermal field's standard normals (3 per cell) of `cells` at `step`."""
+        out = np.empty((len(cells), 3))
+        buf = np.empty(3)
+        for j, i in enumerate(cells):
+            self.lib.or_thermal_normals(int(seed), int(step), int(i), buf.ctypes.data)
+            out[j] = buf
+        return out
+
 
 class Conv:
     def __init__(self, orc, ptr, N, P):
@@ -252,7 +264,7 @@ class Sim:
     def __init__(self, orc, m0, size, cellsize, msat, aexch, alpha, solver="rkdp", maxerror=1e-4, dt=0.0,
                  k1=0.0, anis_axis=None, field=None, periodic=(0, 0, 0), exch_type=6, exch_in_conv=True,
                  want_demag=True, min_dm=0.0, max_dm=0.0, min_dt=0.0, max_dt=0.0, want_energy=False,
-                 demag_accuracy=8):
+                 demag_accuracy=8, temperature=0.0, seed=0):
         self.orc = orc
         p = Params()
         p.size[:] = list(size)
@@ -280,6 +292,8 @@ class Sim:
             p.fieldT = self._fieldT.ctypes.data_as(C.POINTER(C.c_double))
             p.fieldB = self._fieldB.ctypes.data_as(C.POINTER(C.c_double))
         p.wantEnergy = int(want_energy)
+        p.temperature = float(temperature)
+        p.seed = int(seed) & ((1 << 64) - 1)
         self.params = p
         self.N = int(np.prod(size))
         m0 = np.ascontiguousarray(m0, np.float32).reshape(-1)

@@ -661,6 +661,27 @@ void or_add_exch6_stencil(const or_real* m, or_real* h, const int size[3], const
 #define MAX_STAGES 7
 #define MAX_STEP_TRIALS 10 /* rk.go:277 */
 
+static unsigned long long or_mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static unsigned long long thermal_key(unsigned long long seed, long long step, long cell, int j) {
+    unsigned long long base = or_mix64(seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(step + 1));
+    return or_mix64(base + 0x9E3779B97F4A7C15ull * (unsigned long long)(4 * cell + j + 1));
+}
+void or_thermal_normals(unsigned long long seed, long long step, long cell, double out[3]) {
+    const double u1 = ((double)(thermal_key(seed, step, cell, 0) >> 40) + 0.5) * 0x1p-24;
+    const double u2 = (double)(thermal_key(seed, step, cell, 1) >> 40) * 0x1p-24;
+    const double u3 = ((double)(thermal_key(seed, step, cell, 2) >> 40) + 0.5) * 0x1p-24;
+    const double u4 = (double)(thermal_key(seed, step, cell, 3) >> 40) * 0x1p-24;
+    const double pi = 3.14159265358979323846;
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    out[0] = r1 * cos(2.0 * pi * u2);
+    out[1] = r1 * sin(2.0 * pi * u2);
+    out[2] = r2 * cos(2.0 * pi * u4);
+}
+
 struct or_sim {
     or_params p;
     double* fieldT;
@@ -684,7 +705,24 @@ struct or_sim {
     or_real* amap; /* per-cell damping multiplier (SetAlpha, simgeom.go:246-264), NULL: none */
     int fn;        /* applied-field function of time (1 pulse, 2 RF, 3 RF ramp, 4 sawtooth, 5 rotating), 0: table */
     double fp[4], fb[3];
+    or_real* noise; /* thermal field of the current step, 3N (NULL at T = 0) */
 };
+
+/* updateTempNoise (temperature.go:29-36): the thermal field for this step from
+ * dt (internal) at its start, stddev = sqrt(2 alpha kB T / (gamma0 mu0 Ms V dt_SI)) / Ms */
+static void update_temp_noise(or_sim* s, double dt) {
+    const double kB = 1.380650424e-23, gamma0 = 2.211e5, mu0 = 4e-7 * 3.14159265358979323846;
+    const double V = s->p.cellsize[0] * s->p.cellsize[1] * s->p.cellsize[2];
+    const double sd = sqrt(2 * s->p.alpha * kB * s->p.temperature / (gamma0 * mu0 * s->p.msat * V * dt * s->units[1])) /
+                      s->p.msat;
+    const int N = s->N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; i++) {
+        double n[3];
+        or_thermal_normals(s->p.seed, s->steps, i, n);
+        for (int c = 0; c < 3; c++) s->noise[c * (size_t)N + i] = (or_real)(sd * n[c]);
+    }
+}
 
 static void log_trial(or_sim* s, double dt, int acc) {
     if (s->log_n == s->log_cap) {
@@ -864,6 +902,10 @@ or_sim* or_sim_new(const or_params* p, const float* m0) {
     s->dt = p->dt / s->units[1];
     if (s->dt == 0) s->dt = 1e-4; /* DEFAULT_DT_INTERNAL, sim.go:30 */
     set_tableau(s);
+    if (p->temperature > 0) {
+        s->noise = (or_real*)xcalloc(n3, sizeof(or_real));
+        update_temp_noise(s, s->dt);
+    }
     return s;
 }
 
@@ -875,7 +917,13 @@ void or_sim_free(or_sim* s) {
     free(s->fieldT); free(s->fieldB);
     free(s->log_dt); free(s->log_acc);
     free(s->amap);
+    free(s->noise);
     free(s);
+}
+
+static void madd(or_real* a, double c, const or_real* b, size_t n) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; i++) a[i] += (or_real)(c * b[i]);
 }
 
 static void update_hext(or_sim* s, or_real hint[3]) { /* field.go:50-61 */
@@ -900,6 +948,7 @@ static void calc_heff(or_sim* s, const or_real* m, or_real* h, int energy) {
     } else {
         or_local_fields(m, h, N, hint, s->p.anisType, s->kInt, s->axis);
     }
+    if (s->noise) madd(h, 1.0, s->noise, 3 * (size_t)N); /* addThermalField (temperature.go:20-23) */
 }
 
 static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53,98-102 */
@@ -907,10 +956,6 @@ static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53
     or_deltaM_map(m, h, (or_real)a, s->amap, (or_real)(1.0 / (1.0 + a * a)), s->N);
 }
 
-static void madd(or_real* a, double c, const or_real* b, size_t n) {
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < (long)n; i++) a[i] += (or_real)(c * b[i]);
-}
 
 /* one RK step, rk.go:279-445; replay != NULL overrides dt and acceptance per trial */
 static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_trial) {
@@ -921,6 +966,9 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
     memcpy(s->mbackup, m, n3 * sizeof(or_real));
     int goodstep = 0, trials = 0;
     double prevEnergy = s->energy == 0 ? 9999999999.0 : s->energy;
+    /* the thermal field is constant during the step (rk.go:289-292); a replayed
+     * step draws it from the dt its first trial used */
+    if (s->noise) update_temp_noise(s, (dt_trial && n_trial > 0) ? dt_trial[0] : s->dt);
     while (!goodstep && trials < MAX_STEP_TRIALS) {
         int t = trials;
         trials++;
@@ -938,15 +986,17 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
                 torque(s, m1, s->k[i]);
                 s->fsal_initiated = 1;
             }
-            if (i == 0) { /* rk.go:348-373 (T=0 always here) */
-                if (!s->has_b2) s->dt = s->p.dt / s->units[1];
-                double maxTorque = or_reduce(OR_REDUCE_MAXABS, s->k[0], (int)n3);
-                s->maxTorque = maxTorque;
-                double dm = s->dt * maxTorque;
-                if (dm < s->p.minDm) s->dt = s->p.minDm / maxTorque;
-                if (s->dt * s->units[1] < s->p.minDt) s->dt = s->p.minDt / s->units[1];
-                if (s->p.maxDm != 0 && dm > s->p.maxDm) s->dt = s->p.maxDm / maxTorque;
-                if (s->p.maxDt != 0 && s->dt * s->units[1] > s->p.maxDt) s->dt = s->p.maxDt / s->units[1];
+            if (i == 0) {
+                if (s->p.temperature == 0) { /* rk.go:348-373: only at T = 0 */
+                    if (!s->has_b2) s->dt = s->p.dt / s->units[1];
+                    double maxTorque = or_reduce(OR_REDUCE_MAXABS, s->k[0], (int)n3);
+                    s->maxTorque = maxTorque;
+                    double dm = s->dt * maxTorque;
+                    if (dm < s->p.minDm) s->dt = s->p.minDm / maxTorque;
+                    if (s->dt * s->units[1] < s->p.minDt) s->dt = s->p.minDt / s->units[1];
+                    if (s->p.maxDm != 0 && dm > s->p.maxDm) s->dt = s->p.maxDm / maxTorque;
+                    if (s->p.maxDt != 0 && s->dt * s->units[1] > s->p.maxDt) s->dt = s->p.maxDt / s->units[1];
+                }
                 if (dt_trial && t < n_trial) s->dt = dt_trial[t];
                 if (isnan(s->dt)) { fprintf(stderr, "oracle: dt = NaN\n"); return -1; }
             }

@@ -112,7 +112,16 @@ typedef struct {
     const double* fieldT;   /* nField times (s) */
     const double* fieldB;   /* nField*3 tesla, internal order */
     int wantEnergy;         /* compute energy at the FSAL stage (rk.go:323-337) */
+    double temperature;     /* K; thermal field drawn once per step (temperature.go:29-36, rk.go:289-292) */
+    unsigned long long seed; /* thermal stream seed */
 } or_params;
+
+/* The thermal field's standard normals of cell `cell` at step `step` (3 values):
+ * SplitMix64's output function over (seed, step, cell), the top 24 bits as uniforms
+ * ((r + 1/2) 2^-24 and r 2^-24), Box-Muller in double.  The engine draws the same
+ * uniforms; the reference's cuRAND stream is not reproduced (its seed and
+ * generator are cuRAND's, gpu_temperature.cu:31-39). */
+void or_thermal_normals(unsigned long long seed, long long step, long cell, double out[3]);
 
 typedef struct or_sim or_sim;
 or_sim* or_sim_new(const or_params* p, const float* m0 /* 3*N, internal planar, any norm */);

@@ -115,8 +115,10 @@ def test_shell_errors_and_exit_status(fe, tmp_path):
     with pytest.raises(fe.ShellExit) as e:
         sh.call("steps", ["1.5"])
     assert e.value.status == fe.EXIT_PARSE
-    with pytest.raises(fe.InputErr, match="temperature"):
-        sh.call("temperature", ["300"])
+    with pytest.raises(fe.InputErr, match="aniscubic"):
+        sh.call("aniscubic", ["1e3", "0"])       # physics this engine leaves out
+    sh.call("temperature", ["300"])              # Temperature before set-up selects RK12 (temperature.go:47-53)
+    assert sh.sim._temperature == 300.0 and sh.sim._solver == "rk12"
     with pytest.raises(fe.InputErr, match="output quantity"):
         sh.sim.autosave("h", "binary", 1e-12)
     sh.close()

@@ -555,3 +555,70 @@ def test_float32_coefficients_are_equivalent(orc):
     orc.lib.or_set_tableau_f32(0)
     d = np.abs(ms[0] - ms[1]).max()
     assert 0 < d < 1e-6
+
+
+# ------------------------------------------------------------------ thermal field (temperature.go)
+def test_thermal_normals_are_standard_independent_and_reproducible(orc):
+    """The thermal stream's normals: mean 0 and variance 1 within 5 standard
+    errors, a normal distribution by Kolmogorov-Smirnov, no correlation
+    between components, cells or steps, and the same draw for the same (seed,
+    step, cell)."""
+    from scipy import stats
+
+    n = 20000
+    a = orc.thermal_normals(7, 3, range(n))          # step 3, cells 0..n
+    b = orc.thermal_normals(7, 4, range(n))          # the next step
+    c = orc.thermal_normals(8, 3, range(n))          # another seed
+    x = a.reshape(-1)
+    se = 1 / np.sqrt(x.size)
+    assert abs(x.mean()) < 5 * se
+    assert abs(x.var() - 1) < 5 * np.sqrt(2 / x.size)
+    assert stats.kstest(x, "norm").pvalue > 1e-3
+    for u, v in [(a[:, 0], a[:, 1]), (a[:, 0], a[:, 2]), (a[:, 1], a[:, 2]), (a[:-1, 0], a[1:, 0]),
+                 (a[:, 0], b[:, 0]), (a[:, 2], c[:, 2])]:
+        assert abs(np.corrcoef(u, v)[0, 1]) < 5 / np.sqrt(n)
+    assert np.array_equal(orc.thermal_normals(7, 3, [0, 123, 19999]), a[[0, 123, 19999]])
+
+
+def _thermal_film(orc, T, dt=2e-13, seed=1):
+    size, cell = (1, 8, 16), (3e-9, 4e-9, 4e-9)
+    m0 = np.zeros((3,) + size, np.float32)
+    m0[2] = 1
+    return orc.sim(m0, size=size, cellsize=cell, msat=8e5, aexch=1.3e-11, alpha=0.1, solver="rk12",
+                   maxerror=1e-3, dt=dt, temperature=T, seed=seed), m0
+
+
+def test_thermal_field_in_heff_scales_with_temperature_and_dt(orc):
+    """calcHeff adds the thermal field (field.go:74-80): H(T) - H(0) is the
+    stream's normals times one standard deviation, which grows as sqrt(T) and
+    as 1/sqrt(dt) (updateTempNoise, temperature.go:29-36)."""
+    s0, m0 = _thermal_film(orc, 0.0)
+    h0 = s0.heff(m0)
+    d = {}
+    for T, dt in [(300.0, 2e-13), (1200.0, 2e-13), (300.0, 0.5e-13)]:
+        s, _ = _thermal_film(orc, T, dt)
+        d[(T, dt)] = (s.heff(m0) - h0).reshape(3, -1)
+    n = orc.thermal_normals(1, 0, range(d[(300.0, 2e-13)].shape[1])).T   # step 0, seed 1
+    sd = d[(300.0, 2e-13)] / n
+    assert np.allclose(sd, sd.flat[0], rtol=1e-9)                         # one sigma for every cell and component
+    assert sd.flat[0] > 0
+    assert np.allclose(d[(1200.0, 2e-13)], 2 * d[(300.0, 2e-13)], rtol=1e-9)   # sqrt(4 T)
+    assert np.allclose(d[(300.0, 0.5e-13)], 2 * d[(300.0, 2e-13)], rtol=1e-9)  # 1 / sqrt(dt / 4)
+
+
+def test_thermal_field_is_redrawn_per_step_and_kept_through_a_step(orc):
+    """The field is drawn at the start of each step (rk.go:289-292): after one
+    step it is the next step's draw; a step replayed with a rejected first
+    trial keeps one draw, from the dt of its first trial."""
+    s, m0 = _thermal_film(orc, 300.0)
+    s.step()
+    m = s.m.reshape(3, -1).astype(np.float32)
+    s0, _ = _thermal_film(orc, 0.0)
+    s0.m = s.m
+    ratio = (s.heff(m) - s0.heff(m)).reshape(3, -1) / orc.thermal_normals(1, 0, range(m.shape[1])).T
+    assert np.allclose(ratio, ratio.flat[0], rtol=1e-9)    # heff still shows step 0's draw until the next step starts
+    s2, _ = _thermal_film(orc, 300.0)
+    s3, _ = _thermal_film(orc, 300.0)
+    s2.step_replay([2e-13 / s2.state()["unit_time"]] * 2, [0, 1])
+    s3.step_replay([2e-13 / s3.state()["unit_time"]], [1])
+    assert np.array_equal(s2.m, s3.m)                       # the rejected trial left no trace, same draw
