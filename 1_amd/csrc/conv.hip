@@ -88,24 +88,76 @@ __device__ __forceinline__ bool unwrap(int w, int P, int& x) {
     return x >= x1;
 }
 
+__device__ __forceinline__ int wrap_dev(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// The displacements (in cells) along one axis whose face integrals add into
+// wrapped index w of padded length P: first, first + step, ... (count of them).
+// Without periodic images (period == 0) the one displacement unwrap(w); a
+// length of 1 keeps displacement 0 along X only, as FaceKernel6's loop range
+// does (demag.go:35-46: along Y and Z the range of a length 1 is empty).
+// With `images` periodic images of an axis of `period` cells, the reference
+// builds the kernel over the period itself (padSize, kernel.go:79-90) and adds
+// every displacement of [-((period-1)/2)(images+1), (period/2-1)(images+1)]
+// into its wrapped index (demag.go:48-79); on the padded length P >= 2 period
+// the linear convolution reproduces that circular one when index w holds the
+// sum for the residue of unwrap(w) modulo the period.
+struct AxisImages {
+    int first, step, count;
+};
+__device__ __forceinline__ AxisImages axis_images(int w, int P, int period, int images, bool axisX) {
+    AxisImages a{0, 1, 0};
+    if (period == 0) {
+        if (P == 1) {
+            a.count = axisX ? 1 : 0;
+            return a;
+        }
+        a.count = unwrap(w, P, a.first) ? 1 : 0;
+        return a;
+    }
+    // the residue: w itself on the reference's own grid (P == period), else that
+    // of the displacement of w (the index P/2 of an even P >= 2 period is never read)
+    int rsd = w;
+    if (P != period) {
+        if (!unwrap(w, P, rsd)) return a;
+        rsd = wrap_dev(rsd, period);
+    }
+    const int lo = -((period - 1) / 2) * (images + 1), hi = (period / 2 - 1) * (images + 1);
+    a.step = period;
+    a.first = lo + wrap_dev(rsd - lo, period);
+    a.count = a.first > hi ? 0 : (hi - a.first) / period + 1;
+    return a;
+}
+
+struct KernelImages {
+    int period[3], images[3];
+};
+
 __global__ void k_facekernel6(double* __restrict__ K, int P0, int P1, int P2, double c0, double c1, double c2,
-                              int acc) {
+                              int acc, KernelImages im) {
     const long NP = (long)P0 * P1 * P2;
     const double cell[3] = {c0, c1, c2};
     for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < NP; idx += (long)gridDim.x * blockDim.x) {
         const int zw = (int)(idx % P2);
         const long r = idx / P2;
         const int yw = (int)(r % P1), xw = (int)(r / P1);
-        int x, y, z;
+        const AxisImages ax = axis_images(xw, P0, im.period[0], im.images[0], true);
+        const AxisImages ay = axis_images(yw, P1, im.period[1], im.images[1], false);
+        const AxisImages az = axis_images(zw, P2, im.period[2], im.images[2], false);
         double kk[6] = {0, 0, 0, 0, 0, 0};
-        if (unwrap(xw, P0, x) && unwrap(yw, P1, y) && unwrap(zw, P2, z)) {
-            const double R[3] = {x * c0, y * c1, z * c2};
-            for (int s = 0; s < 3; s++) {
-                double B[3];
-                face_integral_f64(B, R, cell, s, acc);
-                for (int d = s; d < 3; d++) kk[c_kern_idx[s][d]] = B[d];
-            }
-        }
+        for (int i = 0; i < ax.count; i++)
+            for (int j = 0; j < ay.count; j++)
+                for (int l = 0; l < az.count; l++) {
+                    const double R[3] = {(ax.first + i * ax.step) * c0, (ay.first + j * ay.step) * c1,
+                                         (az.first + l * az.step) * c2};
+                    for (int s = 0; s < 3; s++) {
+                        double B[3];
+                        face_integral_f64(B, R, cell, s, acc);
+                        for (int d = s; d < 3; d++) kk[c_kern_idx[s][d]] += B[d];
+                    }
+                }
         for (int k = 0; k < 6; k++) K[k * NP + idx] = kk[k];
     }
 }
@@ -252,16 +304,38 @@ __global__ void k_newell_expand(const double* __restrict__ O, int H0, int H1, in
     }
 }
 
+// Exch6NgbrKernel's points as indices of the reference's kernel grid R (the
+// padded length, or the period along a periodic axis), added to the diagonal
+// components at every index w of the padded grid P whose residue maps to them
+// (the identity along axes without periodic images).
 struct ExchPoints {
     long idx[7];
     double val[7];
     int n;
+    int R[3], period[3];
 };
-__global__ void k_add_exch_points(double* K, long NP, ExchPoints p) {
-    const int t = threadIdx.x;
-    if (t < 3 * p.n) {
-        const int comp = t / p.n, i = t % p.n;  // comps 0,1,2 = xx, yy, zz
-        K[comp * NP + p.idx[i]] += p.val[i];
+__global__ void k_add_exch_points(double* __restrict__ K, int P0, int P1, int P2, ExchPoints p) {
+    const long NP = (long)P0 * P1 * P2;
+    for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < NP; w += (long)gridDim.x * blockDim.x) {
+        const int wz = (int)(w % P2);
+        const long r = w / P2;
+        const int wy = (int)(r % P1), wx = (int)(r / P1);
+        const int Pw[3] = {P0, P1, P2}, iw[3] = {wx, wy, wz};
+        int rho[3];
+        bool ok = true;
+        for (int a = 0; a < 3; a++) {
+            rho[a] = iw[a];
+            if (p.period[a] && Pw[a] != p.period[a]) {
+                int d = 0;
+                ok = ok && unwrap(iw[a], Pw[a], d);
+                rho[a] = wrap_dev(d, p.period[a]);
+            }
+        }
+        if (!ok) continue;
+        const long id = ((long)rho[0] * p.R[1] + rho[1]) * p.R[2] + rho[2];
+        for (int i = 0; i < p.n; i++)
+            if (p.idx[i] == id)
+                for (int comp = 0; comp < 3; comp++) K[comp * NP + w] += p.val[i];  // xx, yy, zz
     }
 }
 
@@ -434,8 +508,9 @@ size_t conv_plan_footprint(const int N[3], int* epi_blocks) {
 namespace {
 struct KernelKey {
     int dev, P0, P1, P2, type, accuracy, exch;
+    std::array<int, 6> images;  // period and image count per axis
     uint64_t c0, c1, c2;
-    auto tie() const { return std::tie(dev, P0, P1, P2, type, accuracy, exch, c0, c1, c2); }
+    auto tie() const { return std::tie(dev, P0, P1, P2, type, accuracy, exch, images, c0, c1, c2); }
     bool operator<(const KernelKey& o) const { return tie() < o.tie(); }
 };
 struct KernelEntry {
@@ -460,6 +535,10 @@ double conv_acquire_kernel(ConvPlan& cp, const KernelSpec& ks) {
     key.type = ks.type;
     key.accuracy = ks.type == MM_KERNEL_FACE6 ? ks.accuracy : 0;
     key.exch = ks.exch ? 1 : 0;
+    for (int a = 0; a < 3; a++) {
+        key.images[a] = ks.period[a];
+        key.images[3 + a] = ks.images[a];
+    }
     key.c0 = bits(ks.cell[0]), key.c1 = bits(ks.cell[1]), key.c2 = bits(ks.cell[2]);
     std::lock_guard<std::mutex> lock(g_kmu);
     auto it = g_kernels.find(key);
@@ -518,7 +597,12 @@ static double* realspace_kernel_f64(const int P[3], const KernelSpec& ks, hipStr
     const long NP = (long)P0 * P1 * P2;
     double* K = nullptr;
     GK_CHECK(hipMalloc(&K, 6 * NP * sizeof(double)));
+    for (int a = 0; a < 3; a++)
+        GK_ASSERT(ks.period[a] == 0 || (ks.images[a] > 0 && (ks.period[a] == P[a] || P[a] >= 2 * ks.period[a])),
+                  "periodic axis %d: period %d, %d images, padded length %d", a, ks.period[a], ks.images[a], P[a]);
     if (ks.type == MM_KERNEL_NEWELL) {
+        GK_ASSERT(!ks.period[0] && !ks.period[1] && !ks.period[2],
+                  "periodic boundaries use the FaceKernel6 demag kernel (demag.go:26-84)");
         // displacements 0 .. P/2 per axis cover unwrap's range [-(P-1)/2, P/2-1] in magnitude
         const int H0 = P0 / 2 + 1, H1 = P1 / 2 + 1, H2 = P2 / 2 + 1;
         const long NH = (long)H0 * H1 * H2;
@@ -532,15 +616,23 @@ static double* realspace_kernel_f64(const int P[3], const KernelSpec& ks, hipStr
         GK_CHECK(hipFree(O));
     } else {
         GK_ASSERT(ks.type == MM_KERNEL_FACE6, "unknown demag kernel type %d", ks.type);
-        k_facekernel6<<<grid1d(NP), kBlock, 0, s>>>(K, P0, P1, P2, cell[0], cell[1], cell[2], ks.accuracy);
+        KernelImages im{};
+        for (int a = 0; a < 3; a++) {
+            im.period[a] = ks.period[a];
+            im.images[a] = ks.images[a];
+        }
+        k_facekernel6<<<grid1d(NP), kBlock, 0, s>>>(K, P0, P1, P2, cell[0], cell[1], cell[2], ks.accuracy, im);
         GK_LAUNCH_CHECK();
     }
     if (ks.exch) {
-        // Exch6NgbrKernel written in the reference's order: with a padded size of 1,
-        // wrap(+-1,1) = 0 overwrites the centre (exchange.go:58-65)
+        // Exch6NgbrKernel written in the reference's order on the reference's grid R:
+        // with a length of 1, wrap(+-1,1) = 0 overwrites the centre; with a period
+        // of 2, +1 and -1 are one index (exchange.go:58-65)
+        int R[3];
+        for (int a = 0; a < 3; a++) R[a] = ks.period[a] ? ks.period[a] : P[a];
         std::vector<std::pair<long, double>> pts;
         auto put = [&](int a, int b, int c, double v) {
-            long id = ((long)wrapi(a, P0) * P1 + wrapi(b, P1)) * P2 + wrapi(c, P2);
+            long id = ((long)wrapi(a, R[0]) * R[1] + wrapi(b, R[1])) * R[2] + wrapi(c, R[2]);
             for (auto& pr : pts)
                 if (pr.first == id) {
                     pr.second = v;
@@ -562,7 +654,11 @@ static double* realspace_kernel_f64(const int P[3], const KernelSpec& ks, hipStr
             ep.idx[i] = pts[i].first;
             ep.val[i] = pts[i].second;
         }
-        k_add_exch_points<<<1, 64, 0, s>>>(K, NP, ep);
+        for (int a = 0; a < 3; a++) {
+            ep.R[a] = R[a];
+            ep.period[a] = ks.period[a];
+        }
+        k_add_exch_points<<<grid1d(NP), kBlock, 0, s>>>(K, P0, P1, P2, ep);
         GK_LAUNCH_CHECK();
     }
     return K;

@@ -146,6 +146,10 @@ struct KernelSpec {
     int accuracy;
     int exch;
     int type;
+    // periodic boundaries (Periodic, simgeom.go:84-88): per axis the period in
+    // cells (the data size) and the number of images, 0 and 0 for none
+    int period[3];
+    int images[3];
 };
 
 // Launch parameters of a plan's passes (tile shapes, grid sizes).  The

@@ -513,6 +513,10 @@ static KernelSpec kernel_spec(const mmSim* s, const double cell[3]) {
     ks.accuracy = s->p.demagAccuracy > 0 ? s->p.demagAccuracy : 8;
     ks.exch = s->p.exchType == 6 && s->p.exchInConv;
     ks.type = s->p.demagKernel;
+    for (int i = 0; i < 3; i++) {
+        ks.period[i] = s->p.periodic[i] ? s->p.size[i] : 0;
+        ks.images[i] = s->p.periodic[i];
+    }
     return ks;
 }
 
@@ -582,7 +586,16 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->s = stream();
     GK_ASSERT(p->size[0] > 0 && p->size[1] > 0 && p->size[2] > 1, "bad grid size %d %d %d", p->size[0], p->size[1],
               p->size[2]);
-    GK_ASSERT(!p->periodic[0] && !p->periodic[1] && !p->periodic[2], "periodic boundaries are not supported");
+    for (int i = 0; i < 3; i++) {
+        GK_ASSERT(p->periodic[i] >= 0, "periodic images must be >= 0, got %d", p->periodic[i]);
+        // the fused engine stores a quarter of the kernel spectrum and restores the rest from
+        // the kernel's parity along each axis; an odd period has an asymmetric image range
+        // (demag.go:42-53), whose kernel has no such parity
+        GK_ASSERT(!p->periodic[i] || mode != MM_MODE_FUSED || p->size[i] % 2 == 0 || p->size[i] == 1,
+                  "periodic axis %d of odd size %d: use MM_MODE_ABI", i, p->size[i]);
+        GK_ASSERT(!p->periodic[i] || p->demagKernel == MM_KERNEL_FACE6,
+                  "periodic boundaries use the FaceKernel6 demag kernel (demag.go:26-84)");
+    }
     GK_ASSERT(p->msat > 0 && p->aexch > 0, "msat and aexch must be set (init.go:87-97)");
     GK_ASSERT(p->wantDemag, "the engine needs demag on");
     GK_ASSERT(p->demagKernel == MM_KERNEL_FACE6 || p->demagKernel == MM_KERNEL_NEWELL, "unknown demag kernel %d",

@@ -152,7 +152,7 @@ COMMANDS = {
     "temperature": ("temperature", "f"),
     "anisuniaxial": ("anisuniaxial", "fff"),
     # geometry (simgeom.go)
-    "gridsize": ("gridsize", "iii"), "cellsize": ("cellsize", "fff"), "partsize": ("partsize", "fff"),
+    "gridsize": ("gridsize", "iii"), "periodic": ("periodic", "iii"), "cellsize": ("cellsize", "fff"), "partsize": ("partsize", "fff"),
     "demagaccuracy": ("demagaccuracy", "i"), "exchinconv": ("exchinconv", "b"), "exchtype": ("exchtype", "i"),
     "demag": ("_demag", "b"), "energy": ("_energy", "b"), "kerneltype": ("kerneltype", "s"),
     "setalpha": ("setalpha", "iiif"), "setalpharange": ("setalpharange", "iiiiiif"),
@@ -190,7 +190,7 @@ COMMANDS = {
 # implement; they stop the script with an InputErr naming the command.
 UNSUPPORTED = (
     "k2", "aniscubic", "spinpolarization", "xi", "currentdensity", "currentmask", "fieldmask",
-    "maxcellsize", "periodic", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
+    "maxcellsize", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortex_in_array", "setm", "applyrotatingburst", "edgecorrection",
     "savebenchmark", "exchangetype",
 )

@@ -132,6 +132,7 @@ class Sim:
         self.mode = mode
         self._size = None
         self._cell = None
+        self._periodic = (0, 0, 0)
         self._msat = 0.0
         self._aexch = 0.0
         self._alpha = 0.0
@@ -197,6 +198,15 @@ class Sim:
         if z == 1:
             raise InputErr("For a 2D geometry, use (X, Y, 1) cells, not (1, X, Y)")
         self._size = (int(x), int(y), int(z))
+
+    def periodic(self, z, y, x):
+        """Periodic(z, y, x) (simgeom.go:84-88): the number of periodic images
+        along each axis (0: free boundary), in the order of gridsize."""
+        self._invalidate()
+        v = (int(x), int(y), int(z))
+        if min(v) < 0:
+            raise InputErr("Periodic images should be >= 0")
+        self._periodic = v
 
     def cellsize(self, z, y, x):
         self._invalidate()
@@ -974,6 +984,11 @@ class Sim:
         if self._cell is None:
             raise InputErr("The cell size should be set first. E.g. cellsize 1E-9 1E-9 1E-9")
         p.size[:] = list(self._need_size())
+        p.periodic[:] = list(self._periodic)
+        if self.mode == "fused":
+            for n, k in zip(p.size, self._periodic):
+                if k and n % 2 and n > 1:
+                    raise InputErr("a periodic axis of odd size (%d) needs mode='abi'" % n)
         p.cellsize[:] = list(self._cell)
         p.msat, p.aexch, p.alpha = self._msat, self._aexch, self._alpha
         if self._axis is not None:

@@ -199,6 +199,9 @@ def make_sim(cfg, mode="fused"):
     s.gridsize(Z, Y, X)
     cx, cy, cz = cfg["cellsize"]
     s.cellsize(cz, cy, cx)
+    if cfg.get("periodic"):
+        pX, pY, pZ = cfg["periodic"]
+        s.periodic(pZ, pY, pX)
     s.msat(cfg["msat"])
     s.aexch(cfg["aexch"])
     s.alpha(cfg["alpha"])
@@ -239,6 +242,8 @@ def oracle_kwargs(cfg):
         kw.update(k1=cfg["k1"], anis_axis=cfg["anis_axis"])
     if cfg.get("temperature"):
         kw.update(temperature=cfg["temperature"], seed=cfg.get("seed", 0))
+    if cfg.get("periodic"):
+        kw["periodic"] = tuple(cfg["periodic"])
     for k in ("max_dt", "min_dt"):
         if cfg.get(k):
             kw[k] = cfg[k]

@@ -43,7 +43,10 @@ enum { MM_RK2 = 2, MM_RK12 = 12, MM_RK3 = 3, MM_RK23 = 23, MM_RK4 = 4, MM_RKCK =
  * already in internal axis order. */
 typedef struct {
     int size[3];            /* cells, internal X,Y,Z (gridsize z y x) */
-    int periodic[3];        /* must be 0 in this build */
+    int periodic[3];        /* periodic images per axis (Periodic, simgeom.go:84-88); 0: free.
+                               A periodic axis keeps its size in the reference's kernel grid
+                               (padSize, kernel.go:79-90) and the kernel adds the images
+                               (demag.go:48-79). Fused mode: even sizes (or 1) only */
     double cellsize[3];     /* metres, internal order (cellsize z y x) */
     double msat, aexch, alpha;
     int anisType;           /* MM_ANIS_* */

@@ -122,12 +122,15 @@ void or_facekernel6(const int P[3], const or_real cell[3], int accuracy, const i
     y1 *= periodic[Y] + 1; y2 *= periodic[Y] + 1;
     z1 *= periodic[Z] + 1; z2 *= periodic[Z] + 1;
     for (int s = 0; s < 3; s++) {
-        /* periodic images may add to the same wrapped cell: keep the loop serial over x */
+        /* periodic images add into the same wrapped cell: x stays serial and each
+         * thread owns one wrapped y, walking its images in ascending order, so every
+         * entry is summed in the serial loop's order (x, then y, then z ascending) */
         for (int x = x1; x <= x2; x++) {
             int xw = wrapi(x, P[X]);
 #pragma omp parallel for schedule(static)
-            for (int y = y1; y <= y2; y++) {
-                int yw = wrapi(y, P[Y]);
+            for (int yw = 0; yw < P[Y]; yw++) {
+                if (y1 > y2) continue;
+                for (int y = y1 + wrapi(yw - y1, P[Y]); y <= y2; y += P[Y])
                 for (int z = z1; z <= z2; z++) {
                     int zw = wrapi(z, P[Z]);
                     or_real R[3] = {(or_real)x * cell[X], (or_real)y * cell[Y], (or_real)z * cell[Z]};
@@ -193,8 +196,9 @@ void or_facekernel6_f32(const int P[3], const float cell[3], int accuracy, const
         for (int x = x1; x <= x2; x++) {
             int xw = wrapi(x, P[X]);
 #pragma omp parallel for schedule(static)
-            for (int y = y1; y <= y2; y++) {
-                int yw = wrapi(y, P[Y]);
+            for (int yw = 0; yw < P[Y]; yw++) {
+                if (y1 > y2) continue;
+                for (int y = y1 + wrapi(yw - y1, P[Y]); y <= y2; y += P[Y])
                 for (int z = z1; z <= z2; z++) {
                     int zw = wrapi(z, P[Z]);
                     float R[3] = {(float)x * cell[X], (float)y * cell[Y], (float)z * cell[Z]};

@@ -119,6 +119,10 @@ def test_shell_errors_and_exit_status(fe, tmp_path):
         sh.call("aniscubic", ["1e3", "0"])       # physics this engine leaves out
     sh.call("temperature", ["300"])              # Temperature before set-up selects RK12 (temperature.go:47-53)
     assert sh.sim._temperature == 300.0 and sh.sim._solver == "rk12"
+    sh.call("periodic", ["3", "2", "0"])         # Periodic(z, y, x) in gridsize's order (simgeom.go:84-88)
+    assert sh.sim._periodic == (0, 2, 3)
+    with pytest.raises(fe.InputErr, match="images"):
+        sh.call("periodic", ["-1", "0", "0"])
     with pytest.raises(fe.InputErr, match="output quantity"):
         sh.sim.autosave("h", "binary", 1e-12)
     sh.close()

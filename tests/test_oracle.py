@@ -57,6 +57,91 @@ def test_facekernel_thin_film_limit(orc):
     assert K[0].sum() + K[1].sum() + K[2].sum() == pytest.approx(-1.0, abs=5e-3)
 
 
+def _face_solid(h, v1, v2, w1, w2):
+    """Integral of h / r^3 over the rectangle [v1,v2] x [w1,w2] at height h (the
+    solid angle it subtends): the corner sum of atan(v w / (h r))."""
+    F = lambda v, w: np.arctan(v * w / (h * np.sqrt(h * h + v * v + w * w)))
+    return F(v2, w2) - F(v1, w2) - F(v2, w1) + F(v1, w1)
+
+
+def _box_demag(lo, hi, cell):
+    """-H_u / M_u at the centre of cell 0 of the box of cells lo..hi (per axis)
+    uniformly magnetised along u, from the charges of its two faces normal to u."""
+    out = []
+    for u in range(3):
+        v, w = (u + 1) % 3, (u + 2) % 3
+        e = [((lo[a] - 0.5) * cell[a], (hi[a] + 0.5) * cell[a]) for a in range(3)]
+        out.append(-(_face_solid(e[u][1], *e[v], *e[w]) + _face_solid(-e[u][0], *e[v], *e[w])) / (4 * np.pi))
+    return np.array(out)
+
+
+PERIODIC_CASES = [
+    # size, periodic images, cell, image range per axis (cells): demag.go:35-53 scaled by images + 1
+    ((4, 6, 8), (2, 1, 3), (1.0, 1.3, 0.8), ((-3, 3), (-4, 4), (-12, 12))),
+    ((5, 3, 7), (1, 2, 1), (1.0, 1.0, 1.0), ((-4, 2), (-3, 0), (-6, 4))),   # odd periods: asymmetric
+    ((1, 8, 8), (0, 4, 4), (0.5, 1.0, 1.0), ((0, 0), (-15, 15), (-15, 15))),  # film, periodic in plane
+    ((2, 6, 4), (0, 2, 0), (1.0, 1.0, 1.0), ((-1, 1), (-6, 6), (-3, 3))),    # free X and Z
+]
+
+
+@pytest.mark.parametrize("size,per,cell,box", PERIODIC_CASES)
+def test_facekernel_periodic_images_sum_to_the_box_field(orc, size, per, cell, box):
+    """A periodic axis keeps its size (padSize, kernel.go:79-90) and every
+    displacement of the image range adds into its wrapped cell: the kernel's
+    sum is the field at the centre cell of the box of all image cells, uniformly
+    magnetised -- the closed form of its face charges.  The face-charge
+    quadrature converges to it (x4 per doubled accuracy, <= 2.5e-5 at 8)."""
+    P = [n if (k or n == 1) else 2 * n for n, k in zip(size, per)]
+    want = _box_demag([b[0] for b in box], [b[1] for b in box], cell)
+    err = []
+    for acc in (8, 16):
+        got = orc.facekernel6(P, cell, acc, per).reshape(6, -1).sum(axis=1)
+        err.append(np.abs(got[:3] - want).max())
+        if box[0][0] == -box[0][1] and box[1][0] == -box[1][1] and box[2][0] == -box[2][1]:
+            assert np.abs(got[3:]).max() < 1e-12   # symmetric box: no shear field at its centre
+    assert err[0] < 2.5e-5 and err[1] < err[0] / 3, err
+
+
+def test_periodic_uniform_magnetisation_gives_the_box_field(orc):
+    """Fully periodic grid (sizes 4, 6, 8; 2, 1, 3 images): the convolution over
+    the unpadded period turns a uniform m into the uniform field of the image
+    box (exchange adds nothing to a uniform state), along each axis."""
+    size, per, cell, box = PERIODIC_CASES[0]
+    want = _box_demag([b[0] for b in box], [b[1] for b in box], cell)
+    N = int(np.prod(size))
+    for u in range(3):
+        m = np.zeros((3, N))
+        m[u] = 1.0
+        s = orc.sim(m.astype(np.float32), size=size, cellsize=tuple(c * 1e-9 for c in cell), msat=8e5,
+                    aexch=1.3e-11, alpha=0.1, periodic=per, demag_accuracy=16)
+        h = s.heff(m).reshape(3, N)
+        assert np.abs(h[u] - h[u].mean()).max() < 1e-9
+        assert h[u].mean() == pytest.approx(want[u], abs=1e-5)
+        assert np.abs(np.delete(h, u, axis=0)).max() < 1e-9
+
+
+@pytest.mark.parametrize("size,per", [((4, 6, 8), (2, 1, 3)), ((1, 6, 10), (0, 3, 2)), ((3, 5, 6), (1, 0, 1))])
+def test_periodic_field_commutes_with_shifts_along_periodic_axes(orc, size, per):
+    """Along a periodic axis the field of a random m shifted by one cell is the
+    field shifted by one cell (a circular convolution, demag + exchange); along a
+    free axis it is not."""
+    rng = np.random.default_rng(3)
+    m = rng.standard_normal((3,) + size)
+    m /= np.linalg.norm(m, axis=0)
+    s = orc.sim(m.astype(np.float32), size=size, cellsize=(2e-9, 3e-9, 2.5e-9), msat=8e5, aexch=1.3e-11, alpha=0.1,
+                periodic=per)
+    h = s.heff(m).reshape((3,) + size)
+    for a in range(3):
+        if size[a] == 1:
+            continue
+        hs = s.heff(np.roll(m, 1, axis=1 + a)).reshape((3,) + size)
+        d = np.abs(hs - np.roll(h, 1, axis=1 + a)).max()
+        if per[a]:
+            assert d < 1e-9 * np.abs(h).max(), (a, d)
+        else:
+            assert d > 1e-3 * np.abs(h).max(), (a, d)
+
+
 def test_facekernel_far_field_is_point_dipole(orc):
     P = (2, 2, 64)
     cell = (1.0, 1.0, 1.0)
