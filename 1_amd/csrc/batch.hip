@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -315,6 +316,20 @@ struct LaunchSet {
     int blocks = 0;  // total blocks
     int* idx = nullptr;
     int* start = nullptr;
+    size_t idx_off = 0, start_off = 0;  // offsets in the batch's table block (set-up)
+};
+
+// Host image of a batch's device tables, uploaded with one copy on the batch's
+// stream: a synchronous hipMemcpy per table would wait for other batches'
+// queued work.
+struct TableImage {
+    std::vector<char> bytes;
+    size_t put(const void* src, size_t n) {
+        const size_t off = (bytes.size() + 255) & ~(size_t)255;
+        bytes.resize(off + n);
+        memcpy(bytes.data() + off, src, n);
+        return off;
+    }
 };
 
 int epi_rows_host(int M);
@@ -330,6 +345,7 @@ struct mmBatch {
     int* ddone = nullptr;
     float* drem = nullptr;      // remaining trials per simulation (kb_done)
     std::vector<void*> owned;  // device allocations of the tables
+    TableImage image;          // set-up only: the tables before upload
     // per pass: one LaunchSet per length class
     std::vector<LaunchSet> zf, yf, xm, yi, y2, yx, zi, fin, all256;
     int useGraph = 1;
@@ -384,8 +400,7 @@ int epi_rows_host(int M) {
 int lanes_host(int L) { return reg_fft_G(L); }  // RegFFT<L>::G
 
 void* balloc(mmBatch* b, size_t bytes) {
-    void* p = nullptr;
-    GK_CHECK(hipMalloc(&p, bytes));
+    void* p = cached_malloc(bytes);
     b->owned.push_back(p);
     return p;
 }
@@ -403,10 +418,8 @@ std::vector<LaunchSet> make_sets(mmBatch* b, const std::vector<int>& members, Ke
         std::vector<int> st(ls.cnt + 1, 0);
         for (int i = 0; i < ls.cnt; i++) st[i + 1] = st[i] + blocks(b->sims[kv.second[i]]);
         ls.blocks = st[ls.cnt];
-        ls.idx = (int*)balloc(b, ls.cnt * sizeof(int));
-        ls.start = (int*)balloc(b, (ls.cnt + 1) * sizeof(int));
-        GK_CHECK(hipMemcpy(ls.idx, kv.second.data(), ls.cnt * sizeof(int), hipMemcpyHostToDevice));
-        GK_CHECK(hipMemcpy(ls.start, st.data(), (ls.cnt + 1) * sizeof(int), hipMemcpyHostToDevice));
+        ls.idx_off = b->image.put(kv.second.data(), ls.cnt * sizeof(int));
+        ls.start_off = b->image.put(st.data(), (ls.cnt + 1) * sizeof(int));
         if (ls.blocks > 0) out.push_back(ls);
     }
     return out;
@@ -619,7 +632,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         GK_ASSERT(s->mode == MM_MODE_FUSED && s->conv, "mm_batch_new: simulation %d is not a fused-mode engine", j);
         GK_ASSERT(s->p.solver == sims[0]->p.solver, "mm_batch_new: simulations must share the solver");
     }
-    GK_CHECK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+    b->s = cached_stream();
     // simulation table
     std::vector<BSim> h(n);
     for (int j = 0; j < n; j++) {
@@ -651,8 +664,8 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
             const int nf = s->field.n;
             double* ft = (double*)balloc(b, nf * sizeof(double));
             float* fbv = (float*)balloc(b, 3 * nf * sizeof(float));
-            GK_CHECK(hipMemcpy(ft, s->field.t, nf * sizeof(double), hipMemcpyDeviceToDevice));
-            GK_CHECK(hipMemcpy(fbv, s->field.b, 3 * nf * sizeof(float), hipMemcpyDeviceToDevice));
+            GK_CHECK(hipMemcpyAsync(ft, s->field.t, nf * sizeof(double), hipMemcpyDeviceToDevice, b->s));
+            GK_CHECK(hipMemcpyAsync(fbv, s->field.b, 3 * nf * sizeof(float), hipMemcpyDeviceToDevice, b->s));
             d.t.field.t = ft;
             d.t.field.b = fbv;
         }
@@ -660,8 +673,7 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
         d.cp.n_err = d.nfin;
         d.cp.part_e = nullptr;  // the batched epilogue computes no energy
     }
-    b->dsims = (BSim*)balloc(b, n * sizeof(BSim));
-    GK_CHECK(hipMemcpy(b->dsims, h.data(), n * sizeof(BSim), hipMemcpyHostToDevice));
+    const size_t sims_off = b->image.put(h.data(), n * sizeof(BSim));
     b->ddone = (int*)balloc(b, n * sizeof(int));
     b->drem = (float*)balloc(b, n * sizeof(float));
     // launch sets per pass and length class
@@ -689,6 +701,23 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
     b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });
     b->all256 = make_sets(b, all, [](const mmSim*) { return 0; },
                           [](const mmSim* s) { return std::max(1, std::min(1024, div_up(3L * s->N, 256 * 4))); });
+    // upload the tables: one block, one copy on the batch's stream from pinned memory
+    {
+        const size_t nb = b->image.bytes.size();
+        char* dev = (char*)balloc(b, nb);
+        char* host = (char*)cached_host_malloc(nb);
+        memcpy(host, b->image.bytes.data(), nb);
+        GK_CHECK(hipMemcpyAsync(dev, host, nb, hipMemcpyHostToDevice, b->s));
+        GK_CHECK(hipStreamSynchronize(b->s));
+        cached_host_free(host);
+        b->dsims = (BSim*)(dev + sims_off);
+        for (auto* sets : {&b->zf, &b->zi, &b->yf, &b->yi, &b->xm, &b->y2, &b->yx, &b->fin, &b->all256})
+            for (LaunchSet& ls : *sets) {
+                ls.idx = (int*)(dev + ls.idx_off);
+                ls.start = (int*)(dev + ls.start_off);
+            }
+        b->image = TableImage{};
+    }
     // twiddle tables are created on first use: make them now, outside any graph capture
     for (int j = 0; j < n; j++) {
         const ConvPlan& cp = sims[j]->cp;
@@ -705,13 +734,13 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
 void mm_batch_free(mmBatch* b) {
     if (!b) return;
     if (b->s) (void)hipStreamSynchronize(b->s);
-    if (b->hdone) (void)hipHostFree(b->hdone);
-    if (b->hrem) (void)hipHostFree(b->hrem);
+    cached_host_free(b->hdone);
+    cached_host_free(b->hrem);
     if (b->ev) (void)hipEventDestroy(b->ev);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (void* p : b->owned) (void)hipFree(p);
-    if (b->s) (void)hipStreamDestroy(b->s);
+    for (void* p : b->owned) cached_free(p);
+    cached_stream_release(b->s);
     delete b;
 }
 
@@ -752,8 +781,8 @@ void mm_batch_enqueue(mmBatch* b, int n) {
     kb_done<<<div_up(ns, 64), 64, 0, b->s>>>(b->dsims, ns, b->ddone, b->drem);
     GK_LAUNCH_CHECK();
     if (!b->hdone) {
-        GK_CHECK(hipHostMalloc(&b->hdone, ns * sizeof(int), hipHostMallocDefault));
-        GK_CHECK(hipHostMalloc(&b->hrem, ns * sizeof(float), hipHostMallocDefault));
+        b->hdone = (int*)cached_host_malloc(ns * sizeof(int));
+        b->hrem = (float*)cached_host_malloc(ns * sizeof(float));
         GK_CHECK(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
     }
     GK_CHECK(hipMemcpyAsync(b->hdone, b->ddone, ns * sizeof(int), hipMemcpyDeviceToHost, b->s));

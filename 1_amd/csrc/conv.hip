@@ -408,9 +408,9 @@ void conv_plan_init(ConvPlan& cp, const int N[3]) {
     const long b2 = (cp.P[0] == 1 || cp.yxy) ? 0 : 3L * cp.Q * cp.P[1] * N[0];
     cp.Kc = nullptr;  // conv_acquire_kernel
     cp.kc_elems = kc;
-    GK_CHECK(hipMalloc(&cp.B1, b1 * sizeof(float2)));
+    cp.B1 = (float2*)cached_malloc(b1 * sizeof(float2));
     cp.B2 = nullptr;  // layout [3][Q][Nx][Py]
-    if (b2) GK_CHECK(hipMalloc(&cp.B2, b2 * sizeof(float2)));
+    if (b2) cp.B2 = (float2*)cached_malloc(b2 * sizeof(float2));
     cp.bytes = kc * sizeof(float) + (b1 + b2) * sizeof(float2);
     cp.tp = TuneParams{};
     conv_tune_lookup(N, &cp.tp);
@@ -547,7 +547,7 @@ double conv_acquire_kernel(ConvPlan& cp, const KernelSpec& ks) {
         cp.Kc = it->second.Kc;
         return it->second.imag;
     }
-    GK_CHECK(hipMalloc(&cp.Kc, cp.kc_elems * sizeof(float)));
+    cp.Kc = (float*)cached_malloc(cp.kc_elems * sizeof(float));
     const double imag = conv_build_kernel(cp, ks, nullptr, nullptr);
     g_kernels[key] = KernelEntry{cp.Kc, 1, imag};
     return imag;
@@ -558,12 +558,12 @@ static void release_kernel(float* Kc) {
     for (auto it = g_kernels.begin(); it != g_kernels.end(); ++it)
         if (it->second.Kc == Kc) {
             if (--it->second.refs == 0) {
-                GK_CHECK(hipFree(Kc));
+                cached_free(Kc);
                 g_kernels.erase(it);
             }
             return;
         }
-    GK_CHECK(hipFree(Kc));
+    cached_free(Kc);
 }
 
 int conv_kernel_cache_size() {
@@ -577,8 +577,8 @@ int conv_kernel_cache_size() {
 
 void conv_plan_free(ConvPlan& cp) {
     if (cp.Kc) release_kernel(cp.Kc);
-    if (cp.B1) GK_CHECK(hipFree(cp.B1));
-    if (cp.B2) GK_CHECK(hipFree(cp.B2));
+    cached_free(cp.B1);
+    cached_free(cp.B2);
     cp.Kc = nullptr;
     cp.B1 = cp.B2 = nullptr;
 }
@@ -596,7 +596,7 @@ static double* realspace_kernel_f64(const int P[3], const KernelSpec& ks, hipStr
     const double* cell = ks.cell;
     const long NP = (long)P0 * P1 * P2;
     double* K = nullptr;
-    GK_CHECK(hipMalloc(&K, 6 * NP * sizeof(double)));
+    K = (double*)cached_malloc(6 * NP * sizeof(double));
     for (int a = 0; a < 3; a++)
         GK_ASSERT(ks.period[a] == 0 || (ks.images[a] > 0 && (ks.period[a] == P[a] || P[a] >= 2 * ks.period[a])),
                   "periodic axis %d: period %d, %d images, padded length %d", a, ks.period[a], ks.images[a], P[a]);
@@ -607,13 +607,13 @@ static double* realspace_kernel_f64(const int P[3], const KernelSpec& ks, hipStr
         const int H0 = P0 / 2 + 1, H1 = P1 / 2 + 1, H2 = P2 / 2 + 1;
         const long NH = (long)H0 * H1 * H2;
         double* O = nullptr;
-        GK_CHECK(hipMalloc(&O, 6 * NH * sizeof(double)));
+        O = (double*)cached_malloc(6 * NH * sizeof(double));
         k_newell_octant<<<grid1d(NH, 64), 64, 0, s>>>(O, H0, H1, H2, cell[0], cell[1], cell[2]);
         GK_LAUNCH_CHECK();
         k_newell_expand<<<grid1d(NP), kBlock, 0, s>>>(O, H0, H1, H2, K, P0, P1, P2);
         GK_LAUNCH_CHECK();
         GK_CHECK(hipStreamSynchronize(s));
-        GK_CHECK(hipFree(O));
+        cached_free(O);
     } else {
         GK_ASSERT(ks.type == MM_KERNEL_FACE6, "unknown demag kernel type %d", ks.type);
         KernelImages im{};
@@ -675,7 +675,7 @@ void conv_realspace_kernel_f64(const int P[3], const KernelSpec& ks, double* out
     double* K = realspace_kernel_f64(P, ks, s);
     GK_CHECK(hipMemcpyAsync(out_host, K, 6 * NP * sizeof(double), hipMemcpyDeviceToHost, s));
     sync();
-    GK_CHECK(hipFree(K));
+    cached_free(K);
 }
 
 void conv_realspace_kernel(const int P[3], const KernelSpec& ks, float* out_host) {
@@ -683,13 +683,13 @@ void conv_realspace_kernel(const int P[3], const KernelSpec& ks, float* out_host
     const long NP = (long)P[0] * P[1] * P[2];
     double* K = realspace_kernel_f64(P, ks, s);
     float* f = nullptr;
-    GK_CHECK(hipMalloc(&f, 6 * NP * sizeof(float)));
+    f = (float*)cached_malloc(6 * NP * sizeof(float));
     k_to_float<<<grid1d(6 * NP), kBlock, 0, s>>>(K, f, 6 * NP);
     GK_LAUNCH_CHECK();
     GK_CHECK(hipMemcpyAsync(out_host, f, 6 * NP * sizeof(float), hipMemcpyDeviceToHost, s));
     sync();
-    GK_CHECK(hipFree(K));
-    GK_CHECK(hipFree(f));
+    cached_free(K);
+    cached_free(f);
 }
 
 double conv_build_kernel(ConvPlan& cp, const KernelSpec& ks, float* full_host, float* full_dev) {
@@ -699,8 +699,8 @@ double conv_build_kernel(ConvPlan& cp, const KernelSpec& ks, float* full_host, f
     double* K = realspace_kernel_f64(cp.P, ks, s);
     double2* W = nullptr;
     unsigned long long* mx = nullptr;
-    GK_CHECK(hipMalloc(&W, NP * sizeof(double2)));
-    GK_CHECK(hipMalloc(&mx, 2 * sizeof(unsigned long long)));
+    W = (double2*)cached_malloc(NP * sizeof(double2));
+    mx = (unsigned long long*)cached_malloc(2 * sizeof(unsigned long long));
     GK_CHECK(hipMemsetAsync(mx, 0, 2 * sizeof(unsigned long long), s));
     const int hx = P0 / 2 + 1, hy = P1 / 2 + 1;
     const long fulls = (long)P0 * P1 * cp.Q;
@@ -718,9 +718,9 @@ double conv_build_kernel(ConvPlan& cp, const KernelSpec& ks, float* full_host, f
     if (full_host && full_dev)
         GK_CHECK(hipMemcpyAsync(full_host, full_dev, cp.nK * fulls * sizeof(float), hipMemcpyDeviceToHost, s));
     sync();
-    GK_CHECK(hipFree(K));
-    GK_CHECK(hipFree(W));
-    GK_CHECK(hipFree(mx));
+    cached_free(K);
+    cached_free(W);
+    cached_free(mx);
     double re, im;
     memcpy(&re, &h[0], 8);
     memcpy(&im, &h[1], 8);

@@ -41,6 +41,161 @@ void sync() {
     GK_CHECK(hipGetLastError());
     GK_CHECK(hipStreamSynchronize(stream()));
 }
+// ---------------------------------------------------------------- block cache
+namespace {
+struct BlockCache {
+    std::mutex mu;
+    std::map<void*, std::pair<int, size_t>> live;           // device block -> (device, class)
+    std::map<std::pair<int, size_t>, std::vector<void*>> free_dev;
+    std::map<int, size_t> held, cap;                         // cached bytes and their limit per device
+    std::map<void*, size_t> live_host;
+    std::map<size_t, std::vector<void*>> free_host;
+    std::map<int, std::vector<hipStream_t>> streams;
+};
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache;  // never destroyed: blocks may be freed during exit
+    return *c;
+}
+size_t size_class(size_t bytes) {
+    if (bytes <= (1u << 20)) {
+        size_t c = 512;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    return (bytes + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+}
+int current_device() {
+    int d = 0;
+    GK_CHECK(hipGetDevice(&d));
+    return d;
+}
+size_t trim_locked(BlockCache& c, int dev) {
+    size_t n = 0;
+    for (auto it = c.free_dev.begin(); it != c.free_dev.end();) {
+        if (it->first.first != dev) {
+            ++it;
+            continue;
+        }
+        for (void* p : it->second) {
+            GK_CHECK(hipFree(p));
+            n += it->first.second;
+        }
+        it = c.free_dev.erase(it);
+    }
+    c.held[dev] = 0;
+    return n;
+}
+}  // namespace
+
+void* cached_malloc(size_t bytes) {
+    BlockCache& c = block_cache();
+    const int dev = current_device();
+    const size_t cls = size_class(bytes > 0 ? bytes : 1);
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.free_dev.find({dev, cls});
+    void* p = nullptr;
+    if (it != c.free_dev.end() && !it->second.empty()) {
+        p = it->second.back();
+        it->second.pop_back();
+        c.held[dev] -= cls;
+    } else {
+        hipError_t e = hipMalloc(&p, cls);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            trim_locked(c, dev);
+            e = hipMalloc(&p, cls);
+        }
+        GK_CHECK(e);
+    }
+    c.live[p] = {dev, cls};
+    return p;
+}
+
+void cached_free(void* p) {
+    if (!p) return;
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.live.find(p);
+    GK_ASSERT(it != c.live.end(), "cached_free of a block the cache did not give out");
+    const int dev = it->second.first;
+    const size_t cls = it->second.second;
+    c.live.erase(it);
+    if (!c.cap.count(dev)) {
+        size_t fr = 0, tot = 0;
+        GK_CHECK(hipMemGetInfo(&fr, &tot));
+        c.cap[dev] = tot / 16;
+    }
+    if (c.held[dev] + cls > c.cap[dev]) {
+        GK_CHECK(hipFree(p));
+        return;
+    }
+    c.free_dev[{dev, cls}].push_back(p);
+    c.held[dev] += cls;
+}
+
+void* cached_host_malloc(size_t bytes) {
+    BlockCache& c = block_cache();
+    const size_t cls = size_class(bytes > 0 ? bytes : 1);
+    std::lock_guard<std::mutex> lk(c.mu);
+    void* p = nullptr;
+    auto& fl = c.free_host[cls];
+    if (!fl.empty()) {
+        p = fl.back();
+        fl.pop_back();
+    } else {
+        GK_CHECK(hipHostMalloc(&p, cls, hipHostMallocDefault));
+    }
+    c.live_host[p] = cls;
+    return p;
+}
+
+void cached_host_free(void* p) {
+    if (!p) return;
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.live_host.find(p);
+    GK_ASSERT(it != c.live_host.end(), "cached_host_free of a block the cache did not give out");
+    c.free_host[it->second].push_back(p);
+    c.live_host.erase(it);
+}
+
+hipStream_t cached_stream() {
+    BlockCache& c = block_cache();
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto& v = c.streams[dev];
+    if (!v.empty()) {
+        hipStream_t s = v.back();
+        v.pop_back();
+        return s;
+    }
+    hipStream_t s = nullptr;
+    GK_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void cached_stream_release(hipStream_t s) {
+    if (!s) return;
+    BlockCache& c = block_cache();
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.streams[dev].push_back(s);
+}
+
+size_t cached_trim() {
+    BlockCache& c = block_cache();
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(c.mu);
+    return trim_locked(c, dev);
+}
+
+size_t cached_bytes() {
+    BlockCache& c = block_cache();
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(c.mu);
+    return c.held[dev];
+}
+
 static thread_local bool g_tables_frozen = false;
 void tables_frozen(bool on) { g_tables_frozen = on; }
 bool tables_frozen() { return g_tables_frozen; }

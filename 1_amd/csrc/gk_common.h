@@ -62,4 +62,24 @@ const double2* twiddle_f64(int L);
 void tables_frozen(bool on);
 bool tables_frozen();
 
+// Device memory, pinned host memory and streams of the engine (simulations,
+// convolution plans, kernel spectra and their set-up, batches): freed blocks
+// go to a per-device cache and are reused by later requests of the same size
+// class.  hipFree and hipHostFree wait for every stream of the device, so a
+// simulation or batch freed while others run on their own streams (a sweep
+// rebuilding a batch, the serving loop retiring a request) would stall all of
+// them until their queued work drained.  The caller frees only blocks whose
+// work has finished (the engine synchronises the owner's stream first).
+// Classes: powers of two from 512 B to 1 MiB, then whole MiB.  A failed
+// hipMalloc releases the device's cache and retries; the cache holds at most
+// 1/16 of the device's memory (beyond that a free goes to hipFree).
+void* cached_malloc(size_t bytes);
+void cached_free(void* p);
+void* cached_host_malloc(size_t bytes);
+void cached_host_free(void* p);
+hipStream_t cached_stream();             // a non-blocking stream of the current device
+void cached_stream_release(hipStream_t s);  // after its work has finished
+size_t cached_trim();                    // hipFree the current device's cached blocks; bytes released
+size_t cached_bytes();                   // bytes the current device's cache holds
+
 }  // namespace gk

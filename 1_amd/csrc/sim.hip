@@ -188,26 +188,26 @@ static void upload_field(mmSim* s) {
         if (s->s) GK_CHECK(hipStreamSynchronize(s->s));  // queued trials of this simulation read the old table
         if (n > s->fieldCap) {
             if (s->fT) {
-                GK_CHECK(hipFree(s->fT));
-                GK_CHECK(hipFree(s->fB));
+                cached_free(s->fT);
+                cached_free(s->fB);
                 s->bytes -= s->fieldCap * (sizeof(double) + 3 * sizeof(float));
             }
             const int cap = std::max(n, 2 * s->fieldCap);
-            GK_CHECK(hipMalloc(&s->fT, cap * sizeof(double)));
-            GK_CHECK(hipMalloc(&s->fB, 3 * cap * sizeof(float)));
+            s->fT = (double*)cached_malloc(cap * sizeof(double));
+            s->fB = (float*)cached_malloc(3 * cap * sizeof(float));
             s->fieldCap = cap;
             s->bytes += cap * (sizeof(double) + 3 * sizeof(float));
         }
-        GK_CHECK(hipMemcpy(s->fT, s->fieldT.data(), n * sizeof(double), hipMemcpyHostToDevice));
-        GK_CHECK(hipMemcpy(s->fB, s->fieldB.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice));
+        GK_CHECK(hipMemcpyAsync(s->fT, s->fieldT.data(), n * sizeof(double), hipMemcpyHostToDevice, s->s));
+        GK_CHECK(hipMemcpyAsync(s->fB, s->fieldB.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice, s->s));
+        GK_CHECK(hipStreamSynchronize(s->s));
         s->field.t = s->fT;
         s->field.b = s->fB;
     }
 }
 
 void* gk::dmalloc(mmSim* s, size_t bytes) {
-    void* p = nullptr;
-    GK_CHECK(hipMalloc(&p, bytes));
+    void* p = cached_malloc(bytes);
     s->bytes += bytes;
     return p;
 }
@@ -627,10 +627,12 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     s->p.fieldB = nullptr;
     // device buffers
     s->m = (float*)dmalloc(s, s->N3 * sizeof(float));
-    GK_CHECK(hipMemcpy(s->m, m0, s->N3 * sizeof(float), hipMemcpyHostToDevice));
+    // set-up copies go to this simulation's stream: synchronous copies would wait for
+    // the work other simulations and batches have queued on their own streams
+    GK_CHECK(hipMemcpyAsync(s->m, m0, s->N3 * sizeof(float), hipMemcpyHostToDevice, s->s));
     for (int i = 0; i < s->tb.S; i++) {
         s->k[i] = (float*)dmalloc(s, s->N3 * sizeof(float));
-        GK_CHECK(hipMemset(s->k[i], 0, s->N3 * sizeof(float)));
+        GK_CHECK(hipMemsetAsync(s->k[i], 0, s->N3 * sizeof(float), s->s));
     }
     s->mi = (float*)dmalloc(s, s->N3 * sizeof(float));
     s->mtmp = (float*)dmalloc(s, s->N3 * sizeof(float));
@@ -648,12 +650,13 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     h.stop = 0;
     h.done = 1;
     h.field_tmax = -INFINITY;
-    GK_CHECK(hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
+    GK_CHECK(hipMemcpyAsync(s->st, &h, sizeof(h), hipMemcpyHostToDevice, s->s));
+    GK_CHECK(hipStreamSynchronize(s->s));  // h lives on this frame
     s->hdt = h.dt;
     upload_field(s);
     if (p->alphaMap) {
         s->alphaMap = (float*)dmalloc(s, (size_t)s->N * sizeof(float));
-        GK_CHECK(hipMemcpy(s->alphaMap, p->alphaMap, (size_t)s->N * sizeof(float), hipMemcpyHostToDevice));
+        GK_CHECK(hipMemcpyAsync(s->alphaMap, p->alphaMap, (size_t)s->N * sizeof(float), hipMemcpyHostToDevice, s->s));
     }
     s->p.alphaMap = nullptr;  // the caller's array is not kept
     // convolution (init.go:234-274)
@@ -667,9 +670,9 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     // room for the partials of every row shape the tuner may choose
     s->npart = std::max(kRedBlocks, conv_epi_blocks_max(s->cp.N));
     s->part = (float*)dmalloc(s, 3 * (size_t)s->npart * sizeof(float));
-    GK_CHECK(hipMemset(s->part, 0, 3 * (size_t)s->npart * sizeof(float)));
+    GK_CHECK(hipMemsetAsync(s->part, 0, 3 * (size_t)s->npart * sizeof(float), s->s));
     s->epart = (double*)dmalloc(s, (size_t)s->npart * sizeof(double));
-    GK_CHECK(hipMemset(s->epart, 0, (size_t)s->npart * sizeof(double)));
+    GK_CHECK(hipMemsetAsync(s->epart, 0, (size_t)s->npart * sizeof(double), s->s));
     s->wantEnergy = p->wantEnergy != 0;
     if (mode == MM_MODE_ABI) abi_init(s, s->cell);
     // set-up work ran on the library stream and this simulation's stream (the
@@ -683,12 +686,11 @@ void mm_sim_free(mmSim* s) {
     if (!s) return;
     GK_CHECK(hipStreamSynchronize(s->s));
     for (int i = 0; i < 7; i++)
-        if (s->k[i]) GK_CHECK(hipFree(s->k[i]));
-    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap})
-        if (p) GK_CHECK(hipFree(p));
+        cached_free(s->k[i]);
+    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap}) cached_free(p);
     for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->log_err, (void*)s->dsum, (void*)s->fT,
                     (void*)s->part, (void*)s->epart})
-        if (p) GK_CHECK(hipFree(p));
+        cached_free(p);
     conv_plan_free(s->cp);
     if (s->plan) delete_gpuFFT3dPlan(s->plan);
     for (int c = 0; c < 3; c++)
@@ -891,12 +893,12 @@ long mm_sim_kernel_spectrum(mmSim* s, float* out) {
     const long total = s->cp.nK * fulls;
     if (!out) return total;
     float* dev = nullptr;
-    GK_CHECK(hipMalloc(&dev, total * sizeof(float)));
+    dev = (float*)cached_malloc(total * sizeof(float));
     ConvPlan tmp = s->cp;  // a private spectrum: the engine's may be shared
-    GK_CHECK(hipMalloc(&tmp.Kc, tmp.kc_elems * sizeof(float)));
+    tmp.Kc = (float*)cached_malloc(tmp.kc_elems * sizeof(float));
     conv_build_kernel(tmp, kernel_spec(s, s->cell), out, dev);
-    GK_CHECK(hipFree(tmp.Kc));
-    GK_CHECK(hipFree(dev));
+    cached_free(tmp.Kc);
+    cached_free(dev);
     return total;
 }
 
@@ -910,6 +912,9 @@ void mm_demag_kernel(const int P[3], const double cellsize[3], int kernel, int a
     ks.type = kernel;
     conv_realspace_kernel_f64(P, ks, out);
 }
+
+unsigned long long mm_memory_cached(void) { return cached_bytes(); }
+unsigned long long mm_memory_trim(void) { return cached_trim(); }
 
 int mm_kernel_cache_size(void) {
     return conv_kernel_cache_size();

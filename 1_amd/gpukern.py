@@ -101,6 +101,8 @@ ENGINE = {
     "mm_sim_run_until": (C.c_long, [P, C.c_double]),
     "mm_demag_kernel": (None, [P, P, I, I, P]),
     "mm_kernel_cache_size": (I, []),
+    "mm_memory_cached": (C.c_ulonglong, []),
+    "mm_memory_trim": (C.c_ulonglong, []),
     "mm_sim_set_field": (None, [P, I, P, P]),
     "mm_sim_set_field_function": (None, [P, I, P, P]),
     "mm_sim_set_temperature": (None, [P, C.c_double]),

@@ -110,6 +110,18 @@ def cpu_baseline(cfg, seconds):
         if el >= seconds or (el / n) * (n + 1) > 2.0 * seconds:
             break
     N = int(np.prod(cfg["size"]))
+    # the FP32 transcription of the same path (the oracle compiled in float), a
+    # shorter sample
+    o32 = Oracle("f32").sim(cfg["m0"], **kw)
+    n32 = 0
+    t2 = time.perf_counter()
+    while True:
+        o32.step()
+        n32 += 1
+        el32 = time.perf_counter() - t2
+        if el32 >= seconds / 3 or (el32 / n32) * (n32 + 1) > seconds:
+            break
+    del o32
     cores = used_cores()
     model = ""
     try:
@@ -127,6 +139,8 @@ def cpu_baseline(cfg, seconds):
         "kind": "port",
         "sample": "%d RK4 step(s) of the same 256x256x16 workload in %.1f s (FP64 oracle, OpenMP, kernel setup "
                   "%.1f s excluded); cpu: %s" % (n, el, setup, model),
+        "fp32_transcription": {"value": N * n32 / el32 / 1e6, "unit": "Mcell-steps/s",
+                               "sample": "%d RK4 step(s) in %.1f s (the oracle compiled in float32)" % (n32, el32)},
     }, s, n
 
 
@@ -488,6 +502,12 @@ def main():
     if sim.tune_params()["pp"]:
         names[2] = "y_xkx_y_plane"
     avg = {names[i]: (prof_ms[i] / prof_n[i] if prof_n[i] else 0.0) for i in range(8) if prof_n[i]}
+    # per pass, back-to-back launches between two events (no per-launch events): these add up to the step
+    b2b = {}
+    for i in range(5):
+        if (i in (1, 3)) and names[2] == "y_xkx_y_plane":
+            continue  # the plane pass does the Y passes' work
+        b2b[names[i]] = lib.mm_sim_time_pass(sim._ptr, i, args.pass_launches) / args.pass_launches
     tune = sim.tune_params()
     plane = bool(tune["pp"])
     per, b_conv, b_step, b_passes = model_bytes(N, P, cfg["size"][0], cfg["size"][1], 6 if P[0] > 1 else 4, plane)
@@ -541,7 +561,9 @@ def main():
             "avg_launch_ms": round(dom_ms, 5),
             "launches_timed": args.pass_launches,
         },
-        "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+        "kernel_avg_ms": {k: round(v, 5) for k, v in b2b.items()},
+        "step_ms_from_passes": round(4 * sum(b2b.values()), 5),
+        "kernel_avg_ms_event_bracketed": {k: round(v, 5) for k, v in avg.items()},
         "norm_max_dev": norm_err,
     }
     sim.free()

@@ -141,6 +141,13 @@ void mm_demag_kernel(const int P[3], const double cellsize[3], int kernel, int a
 unsigned long long mm_sim_device_bytes(mmSim* s);
 /* distinct kernel spectra held on the current device by live simulations */
 int mm_kernel_cache_size(void);
+/* Freed engine memory (simulations, plans, spectra, batches) is kept per
+ * device and reused by later simulations and batches, so that freeing does not
+ * wait for the device's other streams (hipFree does).  Bytes the current
+ * device's cache holds, and a release of all of them to the device (this
+ * waits for the device, as hipFree does). */
+unsigned long long mm_memory_cached(void);
+unsigned long long mm_memory_trim(void);
 
 /* Fixed-step fused mode only.  Enqueue n steps bracketed by HIP events on the
  * engine's stream; returns the elapsed ms.  The _profile variant also brackets
