@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -46,6 +47,11 @@ struct BSim {
     int nfin;     // blocks of the RK final (error partials)
     TorqueArgs t; // c and part are set per stage
     CtrlParams cp;
+    // twiddle tables of the simulation's lengths (the length-dispatch kernels)
+    const float2* twM;  // Z half length M
+    const float2* twP;  // Z padded length 2M
+    const float2* twY;  // padded Y
+    const float2* twX;  // padded X
 };
 
 // one RK stage, the same for every simulation of the batch
@@ -91,16 +97,14 @@ __device__ __forceinline__ bool bskip(const BSim& d, const BStage& sg) {
 }
 
 // ------------------------------------------------------------------ passes
+// The bodies of the batched passes: block lb of simulation d.  The per-length
+// kernels (kb_*) and the length-dispatch kernels (kb_*_any, one launch for
+// many length classes) both run them, with the same block shapes.
 template <int M, int TR, int NK>
-__global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, const int* idx, const int* start,
-                                                               int n, BStage sg, const float2* __restrict__ twM,
-                                                               const float2* __restrict__ twP) {
-    __shared__ float2 lds[zfwd_lds<M, TR>()];
-    const BBlock bb = bblock(sims, idx, start, n);
-    const BSim& d = *bb.d;
-    if (bskip(d, sg)) return;
+__device__ __forceinline__ void bz_fwd(const BSim& d, int lb, const BStage& sg, const float2* __restrict__ twM,
+                                       const float2* __restrict__ twP, float2* lds) {
     const int nyb = (d.Ny + TR - 1) / TR;
-    const int yb = bb.lb % nyb, r = bb.lb / nyb, x = r % d.Nx, c = r / d.Nx;
+    const int yb = lb % nyb, r = lb / nyb, x = r % d.Nx, c = r / d.Nx;
     StageIn si{};
     si.m = d.m;
     si.st = d.st;
@@ -111,6 +115,17 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, c
         si.a[j] = sg.a[j];
     }
     zfwd_tile<M, TR, NK>(si, d.B1, twM, twP, d.Nx, d.Ny, d.Nz, c, x, yb * TR, lds);
+}
+
+template <int M, int TR, int NK>
+__global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, const int* idx, const int* start,
+                                                               int n, BStage sg, const float2* __restrict__ twM,
+                                                               const float2* __restrict__ twP) {
+    __shared__ float2 lds[zfwd_lds<M, TR>()];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    bz_fwd<M, TR, NK>(d, bb.lb, sg, twM, twP, lds);
 }
 
 template <int PY, bool FWD>
@@ -164,37 +179,43 @@ __global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx,
 }
 
 template <int PY, int PX>
-__global__ __launch_bounds__(kYxyThreads) void kb_yxy(const BSim* sims, const int* idx, const int* start, int n,
-                                                       BStage sg, const float2* __restrict__ twY,
-                                                       const float2* __restrict__ twX) {
+__device__ __forceinline__ void b_yxy(const BSim& d, int lb, const float2* __restrict__ twY,
+                                      const float2* __restrict__ twX, float2* lds) {
     using Y = Yxy<PY, PX>;
-    __shared__ float2 lds[yxy_lds<PY, PX>()];
-    const BBlock bb = bblock(sims, idx, start, n);
-    const BSim& d = *bb.d;
-    if (bskip(d, sg)) return;
     float2 wx[PX];
 #pragma unroll
     for (int j = 0; j < PX; j++) wx[j] = twX[j];
     float2 v[Y::RMAX][Y::E];
-    yxy_load<PY, PX>(v, d.B1, d.Nx, d.Ny, d.Q, bb.lb);
-    yxy_plane<PY, PX>(v, d.B1, d.Kc, twY, wx, d.Nx, d.Ny, d.Q, bb.lb, lds);
+    yxy_load<PY, PX>(v, d.B1, d.Nx, d.Ny, d.Q, lb);
+    yxy_plane<PY, PX>(v, d.B1, d.Kc, twY, wx, d.Nx, d.Ny, d.Q, lb, lds);
 }
 
-template <int PY>
-__global__ __launch_bounds__(kNT) void kb_y2r(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
-                                              const float2* __restrict__ tw) {
-    using F = RegFFT<PY>;
-    constexpr int E = F::E, G = F::G, LPB = kNT / G;
-    constexpr bool TWL = PY <= 1024;
-    __shared__ float2 lds[LPB * F::LDS];
-    __shared__ float2 twl[TWL ? PY : 1];
+template <int PY, int PX>
+__global__ __launch_bounds__(kYxyThreads) void kb_yxy(const BSim* sims, const int* idx, const int* start, int n,
+                                                       BStage sg, const float2* __restrict__ twY,
+                                                       const float2* __restrict__ twX) {
+    __shared__ float2 lds[yxy_lds<PY, PX>()];
     const BBlock bb = bblock(sims, idx, start, n);
     const BSim& d = *bb.d;
     if (bskip(d, sg)) return;
+    b_yxy<PY, PX>(d, bb.lb, twY, twX, lds);
+}
+
+// 2-D: lds holds the exchange buffers (LPB RegFFT<PY>::LDS) and then the
+// twiddle copy (PY, for PY <= 1024)
+template <int PY> constexpr int y2r_lds() {
+    return (kNT / RegFFT<PY>::G) * RegFFT<PY>::LDS + (PY <= 1024 ? PY : 1);
+}
+template <int PY>
+__device__ __forceinline__ void b_y2r(const BSim& d, int lb, const float2* __restrict__ tw, float2* lds) {
+    using F = RegFFT<PY>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    constexpr bool TWL = PY <= 1024;
+    float2* twl = lds + LPB * F::LDS;
     if constexpr (TWL)
         for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
     const int li = threadIdx.x / G, g = threadIdx.x % G;
-    const long kz = (long)bb.lb * LPB + li;
+    const long kz = (long)lb * LPB + li;
     __syncthreads();  // twiddle table
     if constexpr (E > 16) {
         y2r_column_split<PY>(d.B1, d.Kc, kz, kz < d.Q, d.Ny, d.Q, g, lds + li * F::LDS, TWL ? twl : tw);
@@ -206,19 +227,21 @@ __global__ __launch_bounds__(kNT) void kb_y2r(const BSim* sims, const int* idx, 
     }
 }
 
-template <int M, int TR, int NU, int FIN>
-__global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim* sims, const int* idx,
-                                                                       const int* start, int n, BStage sg,
-                                                                       const float2* __restrict__ twM,
-                                                                       const float2* __restrict__ twP) {
-    __shared__ float2 lds[zinv_epi_lds<M, TR>()];
-    __shared__ float hext_s[3];
-    __shared__ float red[3 * TR * RegFFT<M>::G];
+template <int PY>
+__global__ __launch_bounds__(kNT) void kb_y2r(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+                                              const float2* __restrict__ tw) {
+    __shared__ float2 lds[y2r_lds<PY>()];
     const BBlock bb = bblock(sims, idx, start, n);
     const BSim& d = *bb.d;
     if (bskip(d, sg)) return;
+    b_y2r<PY>(d, bb.lb, tw, lds);
+}
+
+template <int M, int TR, int NU, int FIN>
+__device__ __forceinline__ void bz_inv(const BSim& d, int lb, const BStage& sg, const float2* __restrict__ twM,
+                                       const float2* __restrict__ twP, float2* lds, float* hext_s, float* red) {
     const int nyb = (d.Ny + TR - 1) / TR;
-    const int yb = bb.lb % nyb, x = bb.lb / nyb;
+    const int yb = lb % nyb, x = lb / nyb;
     EpiArgs ea{};
     ea.m = sg.use_mi ? d.mi : d.m;
     ea.out = FIN == 1 ? d.m : d.k[sg.i];
@@ -239,7 +262,21 @@ __global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim*
     ea.t.c = sg.c;
     ea.t.cmax = sg.cmax;
     ea.t.part = sg.partslot < 0 ? nullptr : d.part + (long)sg.partslot * d.npart;
-    zinv_epi_tile<M, TR, NU, FIN>(d.B1, ea, twM, twP, d.st, d.Nx, d.Ny, d.Nz, x, yb * TR, bb.lb, lds, hext_s, red);
+    zinv_epi_tile<M, TR, NU, FIN>(d.B1, ea, twM, twP, d.st, d.Nx, d.Ny, d.Nz, x, yb * TR, lb, lds, hext_s, red);
+}
+
+template <int M, int TR, int NU, int FIN>
+__global__ __launch_bounds__(3 * TR * RegFFT<M>::G) void kb_zinv_epi(const BSim* sims, const int* idx,
+                                                                       const int* start, int n, BStage sg,
+                                                                       const float2* __restrict__ twM,
+                                                                       const float2* __restrict__ twP) {
+    __shared__ float2 lds[zinv_epi_lds<M, TR>()];
+    __shared__ float hext_s[3];
+    __shared__ float red[3 * TR * RegFFT<M>::G];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    bz_inv<M, TR, NU, FIN>(d, bb.lb, sg, twM, twP, lds, hext_s, red);
 }
 
 // ------------------------------------------------------------------ RK pointwise and control
@@ -317,6 +354,9 @@ struct LaunchSet {
     int* idx = nullptr;
     int* start = nullptr;
     size_t idx_off = 0, start_off = 0;  // offsets in the batch's table block (set-up)
+    std::vector<int> hidx, hst;         // host copies of idx and start (set-up)
+    int any = 0;                        // 1: small length classes merged, launched by a dispatch kernel
+    int nt = 0, lds = 0;                // dispatch launch: threads per block and dynamic LDS bytes
 };
 
 // Host image of a batch's device tables, uploaded with one copy on the batch's
@@ -399,6 +439,167 @@ int epi_rows_host(int M) {
 }
 int lanes_host(int L) { return reg_fft_G(L); }  // RegFFT<L>::G
 
+// ---------------------------------------------------------------- length-dispatch kernels
+// One launch for the small length classes of a pass: each block reads its
+// simulation's lengths and runs the body of that length, with the block shape
+// of the per-length kernel (threads beyond it leave at once) and its LDS carved
+// from dynamic shared memory sized for the launch's largest class.  A batch of
+// many small simulations of different sizes then issues a few launches per
+// stage instead of one per length class and pass.
+constexpr int kAnyLds = 80 * 1024;
+#define GK_ANY_M(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#define GK_ANY_Y(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+bool any_m(int M) {
+    switch (M) {
+#define GK_CASE(n) case n:
+        GK_ANY_M(GK_CASE)
+#undef GK_CASE
+        return true;
+    }
+    return false;
+}
+bool any_y(int PY) {
+    switch (PY) {
+#define GK_CASE(n) case n:
+        GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+        return true;
+    }
+    return false;
+}
+bool any_yx(int PY, int PX) { return any_y(PY) && (PX == 4 || PX == 8) && yxy_fits(PX, PY); }
+
+template <int M> constexpr int zinv_any_bytes() {
+    constexpr int TR = epi_rows_c<M>(), NR = TR * RegFFT<M>::G;
+    return zinv_epi_lds<M, TR>() * 8 + 16 + 3 * NR * 4;
+}
+
+template <int NK>
+__global__ __launch_bounds__(1024) void kb_zfwd_any(const BSim* sims, const int* idx, const int* start, int n,
+                                                    BStage sg) {
+    extern __shared__ float2 dyn[];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    switch (d.Q - 1) {
+#define GK_CASE(m)                                                              case m: {                                                                       constexpr int TR = z_rows_c<m>();                                           if (threadIdx.x >= TR * RegFFT<m>::G) return;                               bz_fwd<m, TR, NK>(d, bb.lb, sg, d.twM, d.twP, dyn);                         break;                                                                  }
+        GK_ANY_M(GK_CASE)
+#undef GK_CASE
+    }
+}
+
+template <int NU, int FIN>
+__global__ __launch_bounds__(768) void kb_zinv_any(const BSim* sims, const int* idx, const int* start, int n,
+                                                   BStage sg) {
+    extern __shared__ float2 dyn[];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    switch (d.Q - 1) {
+#define GK_CASE(m)                                                                              case m: {                                                                                       constexpr int TR = epi_rows_c<m>(), NT = 3 * TR * RegFFT<m>::G;                             if (threadIdx.x >= NT) return;                                                              float* hext_s = reinterpret_cast<float*>(dyn + zinv_epi_lds<m, TR>());                      bz_inv<m, TR, NU, FIN>(d, bb.lb, sg, d.twM, d.twP, dyn, hext_s, hext_s + 4);                break;                                                                                  }
+        GK_ANY_M(GK_CASE)
+#undef GK_CASE
+    }
+}
+
+__global__ __launch_bounds__(kYxyThreads) void kb_yxy_any(const BSim* sims, const int* idx, const int* start, int n,
+                                                          BStage sg) {
+    extern __shared__ float2 dyn[];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    if (d.PX == 4) {
+        switch (d.PY) {
+#define GK_CASE(py)     case py: if constexpr (yxy_fits(4, py)) b_yxy<py, 4>(d, bb.lb, d.twY, d.twX, dyn); break;
+            GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+        }
+    } else {
+        switch (d.PY) {
+#define GK_CASE(py)     case py: if constexpr (yxy_fits(8, py)) b_yxy<py, 8>(d, bb.lb, d.twY, d.twX, dyn); break;
+            GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+        }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void kb_y2r_any(const BSim* sims, const int* idx, const int* start, int n,
+                                                  BStage sg) {
+    extern __shared__ float2 dyn[];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    switch (d.PY) {
+#define GK_CASE(py)     case py: b_y2r<py>(d, bb.lb, d.twY, dyn); break;
+        GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+    }
+}
+
+// per length: threads and dynamic LDS bytes of a block in the dispatch kernels
+struct AnyShape {
+    int nt, lds;
+};
+AnyShape any_shape(int pass, int L) {
+    switch (pass) {
+    case 0:  // Z forward, L = M
+        switch (L) {
+#define GK_CASE(n) case n: return {z_rows_c<n>() * RegFFT<n>::G, zfwd_lds<n, z_rows_c<n>()>() * 8};
+            GK_ANY_M(GK_CASE)
+#undef GK_CASE
+        }
+        break;
+    case 1:  // Z inverse + epilogue, L = M
+        switch (L) {
+#define GK_CASE(n) case n: return {3 * epi_rows_c<n>() * RegFFT<n>::G, zinv_any_bytes<n>()};
+            GK_ANY_M(GK_CASE)
+#undef GK_CASE
+        }
+        break;
+    case 2: {  // thin plane, L = PY * 64 + PX
+        const int PY = L / 64, PX = L % 64;
+        switch (PY) {
+#define GK_CASE(n)     case n: return {kYxyThreads, PX == 4 ? yxy_lds_elems(4, n) * 8 : yxy_lds_elems(8, n) * 8};
+            GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+        }
+        break;
+    }
+    case 3:  // 2-D, L = PY
+        switch (L) {
+#define GK_CASE(n) case n: return {kNT, y2r_lds<n>() * 8};
+            GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+        }
+        break;
+    }
+    abort();
+}
+
+void any_kernel_attributes() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        auto set = [](const void* f) {
+            GK_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kAnyLds));
+        };
+        set((const void*)kb_zfwd_any<0>);
+        set((const void*)kb_zfwd_any<1>);
+        set((const void*)kb_zfwd_any<2>);
+        set((const void*)kb_zfwd_any<3>);
+        set((const void*)kb_zfwd_any<5>);
+        set((const void*)kb_zinv_any<0, 0>);
+        set((const void*)kb_zinv_any<1, 0>);
+        set((const void*)kb_zinv_any<3, 0>);
+        set((const void*)kb_zinv_any<6, 0>);
+        set((const void*)kb_zinv_any<1, 1>);
+        set((const void*)kb_zinv_any<3, 1>);
+        set((const void*)kb_zinv_any<3, 2>);
+        set((const void*)kb_zinv_any<6, 2>);
+        set((const void*)kb_yxy_any);
+        set((const void*)kb_y2r_any);
+    });
+}
+
 void* balloc(mmBatch* b, size_t bytes) {
     void* p = cached_malloc(bytes);
     b->owned.push_back(p);
@@ -418,17 +619,70 @@ std::vector<LaunchSet> make_sets(mmBatch* b, const std::vector<int>& members, Ke
         std::vector<int> st(ls.cnt + 1, 0);
         for (int i = 0; i < ls.cnt; i++) st[i + 1] = st[i] + blocks(b->sims[kv.second[i]]);
         ls.blocks = st[ls.cnt];
-        ls.idx_off = b->image.put(kv.second.data(), ls.cnt * sizeof(int));
-        ls.start_off = b->image.put(st.data(), (ls.cnt + 1) * sizeof(int));
+        ls.hidx = kv.second;
+        ls.hst = st;
         if (ls.blocks > 0) out.push_back(ls);
     }
     return out;
+}
+
+// Length classes of a pass (0 Z forward, 1 Z inverse, 2 thin plane, 3 2-D) with
+// fewer blocks than this are merged into one dispatch launch when two or more
+// of them are small and have a body in the dispatch kernel.
+constexpr int kMergeBlocks = 1024;
+std::vector<LaunchSet> merge_small(std::vector<LaunchSet> sets, int pass) {
+    std::vector<LaunchSet> out, small;
+    for (auto& ls : sets) {
+        const bool has = pass <= 1 ? any_m(ls.L) : pass == 2 ? any_yx(ls.L / 64, ls.L % 64) : any_y(ls.L);
+        (has && ls.blocks < kMergeBlocks ? small : out).push_back(std::move(ls));
+    }
+    if (small.size() < 2) {
+        for (auto& ls : small) out.push_back(std::move(ls));
+        return out;
+    }
+    LaunchSet m;
+    m.any = 1;
+    m.L = -1;
+    m.hst.push_back(0);
+    for (auto& ls : small) {
+        const AnyShape a = any_shape(pass, ls.L);
+        m.nt = std::max(m.nt, a.nt);
+        m.lds = std::max(m.lds, a.lds);
+        for (int i = 0; i < ls.cnt; i++) {
+            m.hidx.push_back(ls.hidx[i]);
+            m.hst.push_back(m.hst.back() + ls.hst[i + 1] - ls.hst[i]);
+        }
+        m.cnt += ls.cnt;
+    }
+    m.blocks = m.hst.back();
+    GK_ASSERT(m.lds <= kAnyLds && m.nt <= 1024, "dispatch launch: %d threads, %d bytes of LDS", m.nt, m.lds);
+    out.push_back(std::move(m));
+    return out;
+}
+
+// the launch sets' simulation lists and block starts into the table image
+void stage_sets(mmBatch* b, std::vector<LaunchSet>& sets) {
+    for (auto& ls : sets) {
+        ls.idx_off = b->image.put(ls.hidx.data(), ls.cnt * sizeof(int));
+        ls.start_off = b->image.put(ls.hst.data(), (ls.cnt + 1) * sizeof(int));
+    }
 }
 
 int fin_blocks(const mmSim* s) { return std::max(1, std::min(kRedBlocks, div_up(s->N, 256))); }
 
 // ---------------------------------------------------------------- launchers (one length class each)
 void bl_zfwd(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    if (ls.any) {
+        switch (zfwd_nk(sg.nk)) {
+        case 0: kb_zfwd_any<0><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg); break;
+        case 1: kb_zfwd_any<1><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg); break;
+        case 2: kb_zfwd_any<2><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg); break;
+        case 3: kb_zfwd_any<3><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg); break;
+        default: kb_zfwd_any<5><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg); break;
+        }
+        GK_LAUNCH_CHECK();
+        return;
+    }
     const float2* twM = twiddle_f32(ls.L);
     const float2* twP = twiddle_f32(2 * ls.L);
     switch (ls.L) {
@@ -479,6 +733,11 @@ void bl_xmul(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
 }
 // thin 3-D plane pass; the class key is PY * 64 + PX
 void bl_yxy(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    if (ls.any) {
+        kb_yxy_any<<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg);
+        GK_LAUNCH_CHECK();
+        return;
+    }
     const int PY = ls.L / 64, PX = ls.L % 64;
     const float2* twY = twiddle_f32(PY);
     const float2* twX = twiddle_f32(PX);
@@ -505,6 +764,11 @@ void bl_yxy(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     GK_LAUNCH_CHECK();
 }
 void bl_y2d(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    if (ls.any) {
+        kb_y2r_any<<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg);
+        GK_LAUNCH_CHECK();
+        return;
+    }
     const float2* tw = twiddle_f32(ls.L);
     switch (ls.L) {
 #define GK_CASE(n)                                                                                           \
@@ -517,6 +781,24 @@ void bl_y2d(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     GK_LAUNCH_CHECK();
 }
 void bl_zinv_epi(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    if (ls.any) {
+#define GK_ZA(nu, fin) kb_zinv_any<nu, fin><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg)
+        if (sg.fin == 1) {
+            if (sg.nu <= 1) GK_ZA(1, 1);
+            else GK_ZA(3, 1);
+        } else if (sg.fin == 2) {
+            if (sg.nu <= 3) GK_ZA(3, 2);
+            else GK_ZA(6, 2);
+        } else {
+            if (sg.nu == 0) GK_ZA(0, 0);
+            else if (sg.nu == 1) GK_ZA(1, 0);
+            else if (sg.nu <= 3) GK_ZA(3, 0);
+            else GK_ZA(6, 0);
+        }
+#undef GK_ZA
+        GK_LAUNCH_CHECK();
+        return;
+    }
     const float2* twM = twiddle_f32(ls.L);
     const float2* twP = twiddle_f32(2 * ls.L);
     switch (ls.L) {
@@ -669,6 +951,10 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
             d.t.field.t = ft;
             d.t.field.b = fbv;
         }
+        d.twM = twiddle_f32(s->cp.M);
+        d.twP = twiddle_f32(s->cp.P[2]);
+        d.twY = s->cp.P[1] > 1 ? twiddle_f32(s->cp.P[1]) : nullptr;
+        d.twX = s->cp.P[0] > 1 ? twiddle_f32(s->cp.P[0]) : nullptr;
         d.cp = ctrl_params(s);
         d.cp.n_err = d.nfin;
         d.cp.part_e = nullptr;  // the batched epilogue computes no energy
@@ -701,6 +987,14 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
     b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });
     b->all256 = make_sets(b, all, [](const mmSim*) { return 0; },
                           [](const mmSim* s) { return std::max(1, std::min(1024, div_up(3L * s->N, 256 * 4))); });
+    // small length classes of the Z, thin-plane and 2-D passes: one dispatch launch each
+    any_kernel_attributes();
+    b->zf = merge_small(std::move(b->zf), 0);
+    b->zi = merge_small(std::move(b->zi), 1);
+    b->yx = merge_small(std::move(b->yx), 2);
+    b->y2 = merge_small(std::move(b->y2), 3);
+    for (auto* sets : {&b->zf, &b->zi, &b->yf, &b->yi, &b->xm, &b->y2, &b->yx, &b->fin, &b->all256})
+        stage_sets(b, *sets);
     // upload the tables: one block, one copy on the batch's stream from pinned memory
     {
         const size_t nb = b->image.bytes.size();
