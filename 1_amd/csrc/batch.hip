@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <cmath>
 #include <cstring>
@@ -629,12 +630,13 @@ std::vector<LaunchSet> make_sets(mmBatch* b, const std::vector<int>& members, Ke
 // Length classes of a pass (0 Z forward, 1 Z inverse, 2 thin plane, 3 2-D) with
 // fewer blocks than this are merged into one dispatch launch when two or more
 // of them are small and have a body in the dispatch kernel.
-constexpr int kMergeBlocks = 1024;
+std::atomic<int> g_merge_blocks{1024};
 std::vector<LaunchSet> merge_small(std::vector<LaunchSet> sets, int pass) {
     std::vector<LaunchSet> out, small;
+    const int limit = g_merge_blocks.load();
     for (auto& ls : sets) {
         const bool has = pass <= 1 ? any_m(ls.L) : pass == 2 ? any_yx(ls.L / 64, ls.L % 64) : any_y(ls.L);
-        (has && ls.blocks < kMergeBlocks ? small : out).push_back(std::move(ls));
+        (has && ls.blocks < limit ? small : out).push_back(std::move(ls));
     }
     if (small.size() < 2) {
         for (auto& ls : small) out.push_back(std::move(ls));
@@ -1039,6 +1041,11 @@ void mm_batch_free(mmBatch* b) {
 }
 
 void mm_batch_set_graph(mmBatch* b, int on) { b->useGraph = on; }
+
+int mm_batch_set_merge_blocks(int blocks) {
+    GK_ASSERT(blocks >= 0, "mm_batch_set_merge_blocks: %d", blocks);
+    return g_merge_blocks.exchange(blocks);
+}
 
 int mm_batch_size(mmBatch* b) { return (int)b->sims.size(); }
 

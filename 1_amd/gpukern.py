@@ -109,6 +109,7 @@ ENGINE = {
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
     "mm_batch_set_graph": (None, [P, I]),
+    "mm_batch_set_merge_blocks": (I, [I]),
     "mm_batch_size": (I, [P]),
     "mm_batch_start": (None, [P, P, P]),
     "mm_batch_trials": (I, [P, I]),

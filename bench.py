@@ -374,6 +374,15 @@ def bench_serve(args, mm):
     sv = importlib.import_module("1_amd.serve")
     reqs = sv.request_stream(args.serve_requests, rate=args.serve_rate, seed=5)
     n = len(reqs)
+    finished = [0]
+
+    def progress(r):
+        finished[0] += 1
+        if finished[0] % 25 == 0:
+            print("bench.py serve: %d of %d requests served" % (finished[0], n), file=sys.stderr, flush=True)
+
+    for r in reqs:
+        r.on_finish = progress
     srv = sv.Server(poll=args.poll, graph=not args.no_graph, lane_factor=None if args.no_lanes else args.lane_factor)
     stats = srv.serve(reqs, max_seconds=args.serve_max_seconds)
     print("bench.py serve: %d requests served in batches in %.2f s" % (len(reqs), stats["elapsed"]), file=sys.stderr,

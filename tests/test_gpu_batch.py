@@ -54,6 +54,22 @@ def test_batch_graph_replay_equals_eager_launches(mm):
     check_same(mm, cfgs, [c["duration"] for c in cfgs], graph=False)
 
 
+@pytest.mark.parametrize("merge", [0, 1 << 30])
+def test_batch_merged_and_per_length_launches_equal_alone(mm, lib, merge):
+    """Members of 2-D, thin and 3-D plans and many length classes, with every
+    small class of the Z, thin-plane and 2-D passes merged into one dispatch
+    launch (1 << 30) or each launched alone (0): the same, bit for bit, as
+    alone."""
+    old = lib.mm_batch_set_merge_blocks(merge)
+    try:
+        cfgs = [mm.workloads.sweep_member(12, j, duration=0.008e-9, sizes=(32, 40, 48, 64, 80, 96),
+                                          thick=(1, 2, 4, 16)) for j in range(12)]
+        assert len({c["size"] for c in cfgs}) > 6
+        check_same(mm, cfgs, [c["duration"] for c in cfgs])
+    finally:
+        lib.mm_batch_set_merge_blocks(old)
+
+
 def test_batch_uneven_stop_times(mm):
     """Members stop at different times; finished members stay untouched while
     the others go on."""
