@@ -129,22 +129,21 @@ __global__ __launch_bounds__(TR * RegFFT<M>::G) void kb_zfwd(const BSim* sims, c
     bz_fwd<M, TR, NK>(d, bb.lb, sg, twM, twP, lds);
 }
 
+// 3-D Y passes: lds holds the exchange buffers and then the twiddle copy
+template <int PY> constexpr int ypass_lds() {
+    return (kNT / RegFFT<PY>::G) * RegFFT<PY>::LDS + (PY <= 1024 ? PY : 1);
+}
 template <int PY, bool FWD>
-__global__ __launch_bounds__(kNT) void kb_ypass(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
-                                                const float2* __restrict__ tw) {
+__device__ __forceinline__ void b_ypass(const BSim& d, int lb, const float2* __restrict__ tw, float2* lds) {
     using F = RegFFT<PY>;
     constexpr int E = F::E, G = F::G, LPB = kNT / G;
     constexpr bool TWL = PY <= 1024;
-    __shared__ float2 lds[LPB * F::LDS];
-    __shared__ float2 twl[TWL ? PY : 1];
-    const BBlock bb = bblock(sims, idx, start, n);
-    const BSim& d = *bb.d;
-    if (bskip(d, sg)) return;
+    float2* twl = lds + LPB * F::LDS;
     if constexpr (TWL)
         for (int i = threadIdx.x; i < PY; i += kNT) twl[i] = tw[i];
     const int li = threadIdx.x / G, g = threadIdx.x % G;
     const long nlines = 3L * d.Nx * d.Q;
-    const long line = (long)bb.lb * LPB + li;
+    const long line = (long)lb * LPB + li;
     float2 v[E];
     ypass_load<PY, FWD>(v, FWD ? d.B1 : d.B2, line, line < nlines, d.Nx, d.Ny, d.Q, g);
     __syncthreads();  // twiddle table
@@ -152,22 +151,29 @@ __global__ __launch_bounds__(kNT) void kb_ypass(const BSim* sims, const int* idx
     if (line < nlines) ypass_store<PY, FWD>(v, FWD ? d.B2 : d.B1, line, d.Nx, d.Ny, d.Q, g);
 }
 
-template <int PX>
-__global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
-                                               const float2* __restrict__ tw) {
-    using F = RegFFT<PX>;
-    constexpr int E = F::E, G = F::G, LPB = kNT / G;
-    __shared__ float2 lds[LPB * F::LDS];
-    __shared__ float2 twl[PX];
+template <int PY, bool FWD>
+__global__ __launch_bounds__(kNT) void kb_ypass(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+                                                const float2* __restrict__ tw) {
+    __shared__ float2 lds[ypass_lds<PY>()];
     const BBlock bb = bblock(sims, idx, start, n);
     const BSim& d = *bb.d;
     if (bskip(d, sg)) return;
+    b_ypass<PY, FWD>(d, bb.lb, tw, lds);
+}
+
+// 3-D X . K . X: lds holds the exchange buffers and then the twiddle copy
+template <int PX> constexpr int xmul_lds() { return (kNT / RegFFT<PX>::G) * RegFFT<PX>::LDS + PX; }
+template <int PX>
+__device__ __forceinline__ void b_xmul(const BSim& d, int lb, const float2* __restrict__ tw, float2* lds) {
+    using F = RegFFT<PX>;
+    constexpr int E = F::E, G = F::G, LPB = kNT / G;
+    float2* twl = lds + LPB * F::LDS;
     for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
     const int li = threadIdx.x / G, g = threadIdx.x % G;
     const int lpr = LPB < d.PY ? LPB : d.PY;
     const int rows = (d.PY + lpr - 1) / lpr;
     int kz, ky;
-    xmul_column(bb.lb, (long)d.Q * rows, rows, lpr, li, d.PY, kz, ky);
+    xmul_column(lb, (long)d.Q * rows, rows, lpr, li, d.PY, kz, ky);
     float2 v[3][E];
     xmul_load<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
     __syncthreads();  // twiddle table
@@ -177,6 +183,16 @@ __global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx,
 #pragma unroll
     for (int c = 0; c < 3; c++) reg_fft<float, PX, +1>(v[c], g, lds + li * F::LDS, twl);
     if (ky >= 0) xmul_store<PX>(v, d.B2, kz, ky, d.Nx, d.PY, d.Q, g);
+}
+
+template <int PX>
+__global__ __launch_bounds__(kNT) void kb_xmul(const BSim* sims, const int* idx, const int* start, int n, BStage sg,
+                                               const float2* __restrict__ tw) {
+    __shared__ float2 lds[xmul_lds<PX>()];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    b_xmul<PX>(d, bb.lb, tw, lds);
 }
 
 template <int PY, int PX>
@@ -537,6 +553,38 @@ __global__ __launch_bounds__(kNT) void kb_y2r_any(const BSim* sims, const int* i
     }
 }
 
+#define GK_ANY_X(X) X(4) X(8) X(16) X(32)
+bool any_x(int PX) { return PX == 4 || PX == 8 || PX == 16 || PX == 32; }
+
+template <bool FWD>
+__global__ __launch_bounds__(kNT) void kb_ypass_any(const BSim* sims, const int* idx, const int* start, int n,
+                                                    BStage sg) {
+    extern __shared__ float2 dyn[];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    switch (d.PY) {
+#define GK_CASE(py) \
+    case py: b_ypass<py, FWD>(d, bb.lb, d.twY, dyn); break;
+        GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+    }
+}
+
+__global__ __launch_bounds__(kNT) void kb_xmul_any(const BSim* sims, const int* idx, const int* start, int n,
+                                                   BStage sg) {
+    extern __shared__ float2 dyn[];
+    const BBlock bb = bblock(sims, idx, start, n);
+    const BSim& d = *bb.d;
+    if (bskip(d, sg)) return;
+    switch (d.PX) {
+#define GK_CASE(px) \
+    case px: b_xmul<px>(d, bb.lb, d.twX, dyn); break;
+        GK_ANY_X(GK_CASE)
+#undef GK_CASE
+    }
+}
+
 // per length: threads and dynamic LDS bytes of a block in the dispatch kernels
 struct AnyShape {
     int nt, lds;
@@ -573,6 +621,20 @@ AnyShape any_shape(int pass, int L) {
 #undef GK_CASE
         }
         break;
+    case 4:  // 3-D Y passes, L = PY
+        switch (L) {
+#define GK_CASE(n) case n: return {kNT, ypass_lds<n>() * 8};
+            GK_ANY_Y(GK_CASE)
+#undef GK_CASE
+        }
+        break;
+    case 5:  // 3-D X . K . X, L = PX
+        switch (L) {
+#define GK_CASE(n) case n: return {kNT, xmul_lds<n>() * 8};
+            GK_ANY_X(GK_CASE)
+#undef GK_CASE
+        }
+        break;
     }
     abort();
 }
@@ -598,6 +660,9 @@ void any_kernel_attributes() {
         set((const void*)kb_zinv_any<6, 2>);
         set((const void*)kb_yxy_any);
         set((const void*)kb_y2r_any);
+        set((const void*)kb_ypass_any<true>);
+        set((const void*)kb_ypass_any<false>);
+        set((const void*)kb_xmul_any);
     });
 }
 
@@ -627,7 +692,8 @@ std::vector<LaunchSet> make_sets(mmBatch* b, const std::vector<int>& members, Ke
     return out;
 }
 
-// Length classes of a pass (0 Z forward, 1 Z inverse, 2 thin plane, 3 2-D) with
+// Length classes of a pass (0 Z forward, 1 Z inverse, 2 thin plane, 3 2-D, 4 3-D
+// Y forward or inverse, 5 3-D X . K . X) with
 // fewer blocks than this are merged into one dispatch launch when two or more
 // of them are small and have a body in the dispatch kernel.
 std::atomic<int> g_merge_blocks{1024};
@@ -635,7 +701,10 @@ std::vector<LaunchSet> merge_small(std::vector<LaunchSet> sets, int pass) {
     std::vector<LaunchSet> out, small;
     const int limit = g_merge_blocks.load();
     for (auto& ls : sets) {
-        const bool has = pass <= 1 ? any_m(ls.L) : pass == 2 ? any_yx(ls.L / 64, ls.L % 64) : any_y(ls.L);
+        const bool has = pass <= 1   ? any_m(ls.L)
+                         : pass == 2 ? any_yx(ls.L / 64, ls.L % 64)
+                         : pass == 5 ? any_x(ls.L)
+                                     : any_y(ls.L);
         (has && ls.blocks < limit ? small : out).push_back(std::move(ls));
     }
     if (small.size() < 2) {
@@ -709,6 +778,12 @@ void bl_zfwd(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
     GK_LAUNCH_CHECK();
 }
 void bl_ypass(mmBatch* b, const LaunchSet& ls, const BStage& sg, bool fwd) {
+    if (ls.any) {
+        if (fwd) kb_ypass_any<true><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg);
+        else kb_ypass_any<false><<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg);
+        GK_LAUNCH_CHECK();
+        return;
+    }
     const float2* tw = twiddle_f32(ls.L);
     switch (ls.L) {
 #define GK_CASE(n)                                                                                       \
@@ -723,6 +798,11 @@ void bl_ypass(mmBatch* b, const LaunchSet& ls, const BStage& sg, bool fwd) {
     GK_LAUNCH_CHECK();
 }
 void bl_xmul(mmBatch* b, const LaunchSet& ls, const BStage& sg) {
+    if (ls.any) {
+        kb_xmul_any<<<ls.blocks, ls.nt, ls.lds, b->s>>>(b->dsims, ls.idx, ls.start, ls.cnt, sg);
+        GK_LAUNCH_CHECK();
+        return;
+    }
     const float2* tw = twiddle_f32(ls.L);
     switch (ls.L) {
 #define GK_CASE(n)                                                                            \
@@ -989,12 +1069,15 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n) {
     b->fin = make_sets(b, all, [](const mmSim*) { return 0; }, [](const mmSim* s) { return fin_blocks(s); });
     b->all256 = make_sets(b, all, [](const mmSim*) { return 0; },
                           [](const mmSim* s) { return std::max(1, std::min(1024, div_up(3L * s->N, 256 * 4))); });
-    // small length classes of the Z, thin-plane and 2-D passes: one dispatch launch each
+    // small length classes of each pass: one dispatch launch per pass
     any_kernel_attributes();
     b->zf = merge_small(std::move(b->zf), 0);
     b->zi = merge_small(std::move(b->zi), 1);
     b->yx = merge_small(std::move(b->yx), 2);
     b->y2 = merge_small(std::move(b->y2), 3);
+    b->yf = merge_small(std::move(b->yf), 4);
+    b->yi = merge_small(std::move(b->yi), 4);
+    b->xm = merge_small(std::move(b->xm), 5);
     for (auto* sets : {&b->zf, &b->zi, &b->yf, &b->yi, &b->xm, &b->y2, &b->yx, &b->fin, &b->all256})
         stage_sets(b, *sets);
     // upload the tables: one block, one copy on the batch's stream from pinned memory

@@ -233,8 +233,9 @@ mmBatch* mm_batch_new(mmSim* const* sims, int n);
 void mm_batch_free(mmBatch* b);
 /* 1 (default): replay a captured graph per trial; 0: enqueue the launches */
 void mm_batch_set_graph(mmBatch* b, int on);
-/* Batches created from now on launch the length classes of the Z, thin-plane
- * and 2-D passes that have fewer than `blocks` blocks together, in one launch
+/* Batches created from now on launch the length classes of each pass (Z,
+ * thin-plane, 2-D, 3-D Y and X.K.X) that have fewer than `blocks` blocks
+ * together, in one launch
  * of a kernel that runs each simulation's per-length body (default 1024; 0:
  * one launch per length class).  Results do not depend on it.  Returns the
  * previous value. */

@@ -57,7 +57,7 @@ def test_batch_graph_replay_equals_eager_launches(mm):
 @pytest.mark.parametrize("merge", [0, 1 << 30])
 def test_batch_merged_and_per_length_launches_equal_alone(mm, lib, merge):
     """Members of 2-D, thin and 3-D plans and many length classes, with every
-    small class of the Z, thin-plane and 2-D passes merged into one dispatch
+    small class of every pass merged into one dispatch
     launch (1 << 30) or each launched alone (0): the same, bit for bit, as
     alone."""
     old = lib.mm_batch_set_merge_blocks(merge)
