@@ -451,11 +451,11 @@ int conv_tune_save(const char* path) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     FILE* f = fopen(path, "w");
     if (!f) return -1;
-    fprintf(f, "# launch parameters per data size (internal Nx Ny Nz): zr er yb xb y2b pp\n");
+    fprintf(f, "# launch parameters per data size (internal Nx Ny Nz): zr er yb xb y2b pp xv\n");
     for (auto& kv : g_tune) {
         const TuneParams& t = kv.second;
-        fprintf(f, "%d %d %d %d %d %d %d %d %d\n", kv.first[0], kv.first[1], kv.first[2], t.zr, t.er, t.yb, t.xb,
-                t.y2b, t.pp);
+        fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", kv.first[0], kv.first[1], kv.first[2], t.zr, t.er, t.yb, t.xb,
+                t.y2b, t.pp, t.xv);
     }
     const int n = (int)g_tune.size();
     return fclose(f) == 0 ? n : -1;
@@ -470,11 +470,13 @@ int conv_tune_load(const char* path) {
         if (line[0] == '#') continue;
         int N[3];
         TuneParams t;
-        const int got = sscanf(line, "%d %d %d %d %d %d %d %d %d", &N[0], &N[1], &N[2], &t.zr, &t.er, &t.yb, &t.xb,
-                               &t.y2b, &t.pp);
-        if (got == 8) t.pp = 0;
-        else if (got != 9) continue;
+        const int got = sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &N[0], &N[1], &N[2], &t.zr, &t.er, &t.yb,
+                               &t.xb, &t.y2b, &t.pp, &t.xv);
+        if (got < 8) continue;
+        if (got < 9) t.pp = 0;  // older tables: 8 or 9 columns
+        if (got < 10) t.xv = 0;
         if (N[0] < 1 || N[1] < 1 || N[2] < 2 || t.yb < 1 || t.xb < 1 || t.y2b < 1 || t.pp < 0 || t.pp > 1) continue;
+        if (!conv_xv_valid(pad_len(N[0]), t.xv)) t.xv = 0;
         int P2 = pad_len(N[2]);
         if (P2 < 2) P2 = 2;
         if (!conv_zr_valid(P2 / 2, t.zr)) t.zr = 0;
@@ -948,9 +950,36 @@ static void launch_y3d(const ConvPlan& cp, const EngineState* st, bool fwd, hipS
     GK_LAUNCH_CHECK();
 }
 
+// X . K . X launch variants (TuneParams::xv) for padded X 16, 32, 64: 1 = no
+// prefetch of the next item, 2 = half the elements per lane over twice the
+// lanes, 3 = both; 0 and other lengths: k_xmul
+template <int PX, int EE, bool PIPE>
+static void xmul_v_launch(const ConvPlan& cp, const EngineState* st, const float2* tw, hipStream_t s) {
+    constexpr int LPB = kNT / (PX / EE);
+    const int PY = cp.P[1];
+    const int lpr = LPB < PY ? LPB : PY;
+    int g = 0;
+    const int nit = pipeline_split((long)cp.Q * div_up(PY, lpr), (long)cp.tp.xb * cu_count(), &g);
+    k_xmul_v<PX, EE, PIPE><<<g, kNT, 0, s>>>(cp.B2, cp.Kc, tw, st, cp.N[0], PY, cp.Q, nit);
+}
+bool conv_xv_valid(int PX, int v) { return v == 0 || ((PX == 16 || PX == 32 || PX == 64) && v >= 1 && v <= 3); }
+
 static void launch_xmul(const ConvPlan& cp, const EngineState* st, hipStream_t s) {
     const int PX = cp.P[0], PY = cp.P[1], Nx = cp.N[0], Q = cp.Q;
     const float2* tw = twiddle_f32(PX);
+    if (cp.tp.xv && conv_xv_valid(PX, cp.tp.xv)) {
+        switch (PX * 4 + cp.tp.xv) {
+#define GK_XV(n)                                                                            \
+    case n * 4 + 1: xmul_v_launch<n, RegFFT<n>::E, false>(cp, st, tw, s); break;            \
+    case n * 4 + 2: xmul_v_launch<n, RegFFT<n>::E / 2, true>(cp, st, tw, s); break;        \
+    case n * 4 + 3: xmul_v_launch<n, RegFFT<n>::E / 2, false>(cp, st, tw, s); break;
+            GK_XV(16) GK_XV(32) GK_XV(64)
+#undef GK_XV
+        default: abort();
+        }
+        GK_LAUNCH_CHECK();
+        return;
+    }
     switch (PX) {
 #define GK_CASE(n)                                                                              \
     case n: {                                                                                   \

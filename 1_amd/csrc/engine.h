@@ -162,6 +162,7 @@ struct TuneParams {
     int xb = 4;   // blocks per CU of the pipelined X.K.X pass
     int y2b = 8;  // blocks per CU of the 2-D Y.K.Y pass
     int pp = 0;   // 3-D plans that fit it: 1 = Y.X K X.Y in one plane pass (k_plane3), 0 = separate passes
+    int xv = 0;   // X.K.X variant (padded X 16-64): 0 default, 1 no prefetch, 2 half the elements per lane, 3 both
 };
 
 struct ConvPlan {
@@ -192,6 +193,7 @@ void conv_tune_clear();
 int conv_epi_rows(int M, int v);
 bool conv_zr_valid(int M, int v);
 bool conv_er_valid(int M, int v);
+bool conv_xv_valid(int PX, int v);
 
 // Choose padded sizes, allocate buffers.  Kc is set by conv_acquire_kernel.
 void conv_plan_init(ConvPlan& cp, const int N[3]);

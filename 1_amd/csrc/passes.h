@@ -640,6 +640,116 @@ __device__ __forceinline__ void xmul_column(long t, long items, int rows, int lp
     ky = (t < items && li < lpr && r * lpr + li < PY) ? r * lpr + li : -1;
 }
 
+// Launch variants of the X . K . X pass (TuneParams::xv): EE elements per lane
+// (the default RegFFT<PX>::E, or half of it over twice the lanes: fewer live
+// registers, one more LDS exchange per transform) and PIPE, the next item's
+// columns loaded before the current one is transformed (twice the data
+// registers) or after.  Lane g of a line holds x = g + (PX / EE) e.
+template <int PX, int EE>
+__device__ __forceinline__ void xmul_load_e(float2 (*v)[EE], const float2* __restrict__ B2, int kz, int ky, int Nx,
+                                            int PY, int Q, int g) {
+    constexpr int G = PX / EE;
+    const long cstride = (long)Q * Nx * PY;
+    const float2* base = B2 + (long)kz * Nx * PY + (ky < 0 ? 0 : ky);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int e = 0; e < EE; e++) {
+            const int x = g + G * e;
+            const bool pad = EE >= 2 && EE % 2 == 0 && e >= EE / 2;  // x >= PX / 2 >= Nx
+            v[c][e] = (!pad && ky >= 0 && x < Nx) ? base[c * cstride + (long)x * PY] : make_float2(0.f, 0.f);
+        }
+}
+template <int PX, int EE>
+__device__ __forceinline__ void xmul_store_e(float2 (*v)[EE], float2* __restrict__ B2, int kz, int ky, int Nx, int PY,
+                                             int Q, int g) {
+    constexpr int G = PX / EE;
+    const long cstride = (long)Q * Nx * PY;
+    float2* base = B2 + (long)kz * Nx * PY + ky;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int e = 0; e < EE; e++) {
+            const int x = g + G * e;
+            const bool pad = EE >= 2 && EE % 2 == 0 && e >= EE / 2;
+            if (!pad && x < Nx) base[c * cstride + (long)x * PY] = v[c][e];
+        }
+}
+template <int PX, int EE>
+__device__ __forceinline__ void xmul_kernel_e(float2 (*v)[EE], const float* __restrict__ Kc, int kz, int ky, int PY,
+                                              int g) {
+    GK_FFT_FP
+    constexpr int G = PX / EE;
+    const int hx = PX / 2 + 1, hy = PY / 2 + 1;
+    const bool fy = ky > PY / 2;
+    const int kyp = fy ? PY - ky : ky;
+    const float* kb = Kc + ((long)kz * hy + kyp) * hx * 6;
+#pragma unroll
+    for (int e = 0; e < EE; e++) {
+        const int kx = g + G * e;
+        const bool fx = kx > PX / 2;
+        const int kxp = fx ? PX - kx : kx;
+        const float2* kk = reinterpret_cast<const float2*>(kb + kxp * 6);
+        const float2 k01 = kk[0], k23 = kk[1], k45 = kk[2];
+        const float kxx = k01.x, kyy = k01.y, kzz = k23.x;
+        float kyz = k23.y, kxz = k45.x, kxy = k45.y;
+        if (fy) kyz = -kyz;
+        if (fx) kxz = -kxz;
+        if (fx != fy) kxy = -kxy;
+        const float2 a = v[0][e], b = v[1][e], c = v[2][e];
+        v[0][e] = make_float2(a.x * kxx + b.x * kxy + c.x * kxz, a.y * kxx + b.y * kxy + c.y * kxz);
+        v[1][e] = make_float2(a.x * kxy + b.x * kyy + c.x * kyz, a.y * kxy + b.y * kyy + c.y * kyz);
+        v[2][e] = make_float2(a.x * kxz + b.x * kyz + c.x * kzz, a.y * kxz + b.y * kyz + c.y * kzz);
+    }
+}
+template <int PX, int EE, bool PIPE>
+__global__ __launch_bounds__(kNT) void k_xmul_v(float2* __restrict__ B2, const float* __restrict__ Kc,
+                                                const float2* __restrict__ tw, const EngineState* st, int Nx, int PY,
+                                                int Q, int nit) {
+    constexpr int E = EE, G = PX / EE, LPB = kNT / G, LDS = RegFFT<PX>::LDS;
+    __shared__ float2 lds[LPB * LDS];
+    __shared__ float2 twl[PX];
+    if (st && !st->trial_active) return;
+    for (int i = threadIdx.x; i < PX; i += kNT) twl[i] = tw[i];
+    const int li = threadIdx.x / G, g = threadIdx.x % G;
+    const int lpr = LPB < PY ? LPB : PY;
+    const int rows = (PY + lpr - 1) / lpr;
+    const long items = (long)Q * rows;
+    const long t0 = (long)blockIdx.x * nit;
+    float2 nxt[3][E];
+    if constexpr (PIPE) {
+        int kz, ky;
+        xmul_column(t0, items, rows, lpr, li, PY, kz, ky);
+        xmul_load_e<PX, E>(nxt, B2, kz, ky, Nx, PY, Q, g);
+    }
+    __syncthreads();  // twiddle table
+    for (int it = 0; it < nit; it++) {
+        const long t = t0 + it;
+        int kz, ky;
+        xmul_column(t, items, rows, lpr, li, PY, kz, ky);
+        float2 v[3][E];
+        if constexpr (PIPE) {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int e = 0; e < E; e++) v[c][e] = nxt[c][e];
+            if (it + 1 < nit) {
+                int kz1, ky1;
+                xmul_column(t + 1, items, rows, lpr, li, PY, kz1, ky1);
+                xmul_load_e<PX, E>(nxt, B2, kz1, ky1, Nx, PY, Q, g);
+            }
+        } else {
+            xmul_load_e<PX, E>(v, B2, kz, ky, Nx, PY, Q, g);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) reg_fft<float, PX, -1, 1, E>(v[c], g, lds + li * LDS, twl);
+        if (ky >= 0) xmul_kernel_e<PX, E>(v, Kc, kz, ky, PY, g);
+#pragma unroll
+        for (int c = 0; c < 3; c++) reg_fft<float, PX, +1, 1, E>(v[c], g, lds + li * LDS, twl);
+        if (ky >= 0) xmul_store_e<PX, E>(v, B2, kz, ky, Nx, PY, Q, g);
+    }
+}
+
 template <int PX>
 __global__ __launch_bounds__(kNT) void k_xmul(float2* __restrict__ B2, const float* __restrict__ Kc,
                                               const float2* __restrict__ tw, const EngineState* st, int Nx, int PY,

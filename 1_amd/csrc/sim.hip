@@ -1030,7 +1030,7 @@ double mm_sim_time_pass(mmSim* s, int cls, int n) {
     return total;
 }
 
-int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double ms_tuned[7]) {
+int mm_sim_tune(mmSim* s, int reps, int params[7], double ms_default[8], double ms_tuned[8]) {
     GK_ASSERT(s->mode == MM_MODE_FUSED && s->conv, "tune: an initialised fused-mode simulation");
     if (reps < 1) reps = 1;
     ConvPlan& cp = s->cp;
@@ -1068,7 +1068,7 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double 
         ms_default[slot] = t_d;
         ms_tuned[slot] = t_best;
     };
-    for (int i = 0; i < 7; i++) ms_default[i] = ms_tuned[i] = -1;
+    for (int i = 0; i < 8; i++) ms_default[i] = ms_tuned[i] = -1;
     std::vector<int> zr{0}, er{0};
     for (int v = 1; v <= 2; v++)
         if (conv_zr_valid(cp.M, v)) zr.push_back(v);
@@ -1082,6 +1082,10 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double 
         cp.tp.pp = 0;
         tune_one(2, &cp.tp.yb, dflt.yb, grid, 1, 3);
         tune_one(3, &cp.tp.xb, dflt.xb, grid, 2, -1);
+        std::vector<int> xv{0};
+        for (int v = 1; v <= 3; v++)
+            if (conv_xv_valid(cp.P[0], v)) xv.push_back(v);
+        if (xv.size() > 1) tune_one(6, &cp.tp.xv, dflt.xv, xv, 2, -1);
         cp.tp.pp = pp0;
         if (cp.plane3) {
             double t[2];
@@ -1092,8 +1096,8 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double 
             cp.tp.pp = t[1] < t[0] ? 1 : 0;
             ms_default[5] = t[dflt.pp];
             ms_tuned[5] = t[cp.tp.pp];
-            ms_default[6] = t[0];  // the three passes
-            ms_tuned[6] = t[1];    // the plane pass
+            ms_default[7] = t[0];  // the three passes
+            ms_tuned[7] = t[1];    // the plane pass
         }
     }
     if (!d3) tune_one(4, &cp.tp.y2b, dflt.y2b, grid, 2, -1);
@@ -1106,6 +1110,7 @@ int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double 
         params[3] = cp.tp.xb;
         params[4] = cp.tp.y2b;
         params[5] = cp.tp.pp;
+        params[6] = cp.tp.xv;
     }
     return 0;
 }
@@ -1116,13 +1121,14 @@ void mm_sim_set_temperature(mmSim* s, double temperature) {
     set_thermal(s, temperature);
 }
 
-void mm_sim_get_tune(mmSim* s, int params[6]) {
+void mm_sim_get_tune(mmSim* s, int params[7]) {
     params[0] = s->cp.tp.zr;
     params[1] = s->cp.tp.er;
     params[2] = s->cp.tp.yb;
     params[3] = s->cp.tp.xb;
     params[4] = s->cp.tp.y2b;
     params[5] = s->cp.tp.pp;
+    params[6] = s->cp.tp.xv;
 }
 
 int mm_tune_save(const char* path) { return conv_tune_save(path); }

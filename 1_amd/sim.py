@@ -927,7 +927,7 @@ class Sim:
         self._init()
         return int(self._lib.mm_sim_device_bytes(self._ptr))
 
-    TUNABLES = ("zr", "er", "yb", "xb", "y2b", "pp")
+    TUNABLES = ("zr", "er", "yb", "xb", "y2b", "pp", "xv")
 
     def tune(self, reps=20):
         """Autotune this simulation's launch shapes on the device
@@ -937,19 +937,19 @@ class Sim:
         separate passes, ms of the plane pass); the choice is also stored in
         the process's tuning table."""
         self._init()
-        par = np.zeros(6, np.int32)
-        md = np.zeros(7)
-        mt = np.zeros(7)
+        par = np.zeros(7, np.int32)
+        md = np.zeros(8)
+        mt = np.zeros(8)
         self._lib.mm_sim_tune(self._ptr, int(reps), par.ctypes.data, md.ctypes.data, mt.ctypes.data)
         out = {k: (int(par[i]), float(md[i]), float(mt[i])) for i, k in enumerate(self.TUNABLES) if md[i] >= 0}
-        if md[6] >= 0:
-            out["plane_vs_three"] = (float(md[6]), float(mt[6]))  # ms: three passes, plane pass
+        if md[7] >= 0:
+            out["plane_vs_three"] = (float(md[7]), float(mt[7]))  # ms: three passes, plane pass
         return out
 
     def tune_params(self):
         """The launch shapes this simulation runs with (mm_sim_get_tune)."""
         self._init()
-        par = np.zeros(6, np.int32)
+        par = np.zeros(7, np.int32)
         self._lib.mm_sim_get_tune(self._ptr, par.ctypes.data)
         return dict(zip(self.TUNABLES, (int(v) for v in par)))
 

@@ -32,6 +32,8 @@ DESCR = {
     "y2b": "2-D Y.K.Y pass, blocks per CU",
     "pp": "3-D plans with padded Y 512, padded X 16/32: 1 = one register-resident plane pass for Y, X.K.X, Y "
           "inverse; 0 = three passes (ms: the passes it replaces)",
+    "xv": "X.K.X variant, padded X 16-64: 0 default, 1 no prefetch of the next item, 2 half the elements per lane "
+          "over twice the lanes, 3 both",
 }
 
 
@@ -58,7 +60,7 @@ def main():
     lib.mm_tune_clear()  # every shape starts from the defaults
     lines = ["| workload (internal X,Y,Z) | tunable | default | chosen | ms/launch default | ms/launch tuned | gain |",
              "|---|---|---|---|---|---|---|"]
-    default = {"zr": 0, "er": 0, "yb": 8, "xb": 4, "y2b": 8, "pp": 0}
+    default = {"zr": 0, "er": 0, "yb": 8, "xb": 4, "y2b": 8, "pp": 0, "xv": 0}
     extra = ["", "Plane pass against the three passes it replaces (ms per evaluation, tuned shapes):", "",
              "| workload (internal X,Y,Z) | Y fwd + X.K.X + Y inv | plane pass |", "|---|---|---|"]
     for name, cfg in shapes(mm):

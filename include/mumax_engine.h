@@ -171,19 +171,25 @@ double mm_sim_time_pass(mmSim* s, int cls, int n);
  * (0 default, 1 half), yb: blocks per CU of the 3-D Y passes, xb: blocks per CU
  * of the X.K.X pass, y2b: blocks per CU of the 2-D Y.K.Y pass, pp: for 3-D
  * plans with padded Y 512 and padded X 16 or 32, 1 = Y forward, X.K.X and Y
- * inverse as one register-resident plane pass, 0 = three passes}.  Each tunable's
+ * inverse as one register-resident plane pass, 0 = three passes, xv: X.K.X
+ * variant for padded X 16-64, 0 default, 1 without the next item's prefetch,
+ * 2 half the elements per lane over twice the lanes, 3 both}.  Each tunable's
  * candidates run `reps` back-to-back launches (least of 3 runs); the fastest is
  * kept in s and stored in the process's tuning table under s's data size.
- * ms_default / ms_tuned [0..5]: ms per launch of the affected pass(es) at the
- * default and the chosen value (-1 where the plan has no such pass); [6]: the
+ * ms_default / ms_tuned [0..6]: ms per launch of the affected pass(es) at the
+ * default and the chosen value (-1 where the plan has no such pass); [7]: the
  * three separate passes and the plane pass (-1 where it does not fit).  m is
  * unchanged.
- * The shapes change no result bit, except that the fused Z inverse's row count
- * sets the summation order of the FP64 per-block energy sums. */
-int mm_sim_tune(mmSim* s, int reps, int params[6], double ms_default[7], double ms_tuned[7]);
-void mm_sim_get_tune(mmSim* s, int params[6]);
+ * Row counts, blocks per CU and the prefetch change no result bit, except that
+ * the fused Z inverse's row count sets the summation order of the FP64
+ * per-block energy sums; the plane pass and xv 2, 3 compute the same
+ * transforms in another order (FP32 rounding apart), and the batched engine
+ * keeps the default shapes. */
+int mm_sim_tune(mmSim* s, int reps, int params[7], double ms_default[8], double ms_tuned[8]);
+void mm_sim_get_tune(mmSim* s, int params[7]);
 /* Tuning table: new simulations of a listed data size take its parameters.
- * Text, one line "Nx Ny Nz zr er yb xb y2b pp" per size (internal order).  save
+ * Text, one line "Nx Ny Nz zr er yb xb y2b pp xv" per size (internal order;
+ * lines of 8 or 9 columns read pp and xv as 0).  save
  * and load return the number of entries, or -1 if the file cannot be used. */
 int mm_tune_save(const char* path);
 int mm_tune_load(const char* path);

@@ -56,16 +56,17 @@ def test_every_launch_shape_gives_the_same_steps(mm, lib, tmp_path, size, cell):
     lib.mm_tune_clear()
     try:
         ref, shapes = run(mm, size, cell)
-        assert shapes == {"zr": 0, "er": 0, "yb": 8, "xb": 4, "y2b": 8, "pp": 0}
-        for zr, er, grid, pp in [(1, 1, 2, 0), (2, 0, 16, 1)]:
+        assert shapes == {"zr": 0, "er": 0, "yb": 8, "xb": 4, "y2b": 8, "pp": 0, "xv": 0}
+        for zr, er, grid, pp, xv in [(1, 1, 2, 0, 1), (2, 0, 16, 1, 0), (0, 0, 4, 0, 2), (0, 0, 8, 0, 3)]:
             table = tmp_path / "tune.txt"
-            table.write_text("# test\n%d %d %d %d %d %d %d %d %d\n" % (size + (zr, er, grid, grid, grid, pp)))
+            table.write_text("# test\n%d %d %d %d %d %d %d %d %d %d\n" % (size + (zr, er, grid, grid, grid, pp, xv)))
             lib.mm_tune_clear()
             assert lib.mm_tune_load(str(table).encode()) == 1
             m, shapes = run(mm, size, cell)
             assert shapes["yb"] == shapes["xb"] == shapes["y2b"] == grid
-            if shapes["pp"]:
-                # the plane pass computes the same transforms in another order: FP32 rounding apart
+            if shapes["pp"] or shapes["xv"] >= 2:
+                # the plane pass and the half-element X.K.X compute the same transforms in another
+                # order: FP32 rounding apart
                 assert np.abs(m - ref).max() < 1e-6, (zr, er, grid, shapes)
             else:
                 assert np.array_equal(m.view(np.uint32), ref.view(np.uint32)), (zr, er, grid, shapes)
@@ -80,14 +81,14 @@ def test_every_launch_shape_gives_the_same_steps(mm, lib, tmp_path, size, cell):
             v, t_default, t_tuned = val
             assert 0 < t_tuned <= t_default, (k, res)
         s.steps(4)
-        if s.tune_params()["pp"]:
+        if s.tune_params()["pp"] or s.tune_params()["xv"] >= 2:
             assert np.abs(s.m - ref).max() < 1e-6
         else:
             assert np.array_equal(s.m.view(np.uint32), ref.view(np.uint32))
         s.free()
         # the choice is in the table: a new simulation of this size takes it
         s = make(mm, size, cell)
-        assert s.tune_params() == {k: res.get(k, (d,))[0] for k, d in zip(s.TUNABLES, (0, 0, 8, 4, 8, 0))}
+        assert s.tune_params() == {k: res.get(k, (d,))[0] for k, d in zip(s.TUNABLES, (0, 0, 8, 4, 8, 0, 0))}
         s.free()
     finally:
         lib.mm_tune_clear()
