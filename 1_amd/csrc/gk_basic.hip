@@ -50,7 +50,7 @@ struct BlockCache {
     std::map<int, size_t> held, cap;                         // cached bytes and their limit per device
     std::map<void*, size_t> live_host;
     std::map<size_t, std::vector<void*>> free_host;
-    std::map<int, std::vector<hipStream_t>> streams;
+    std::map<int, std::vector<hipStream_t>> streams;  // idle, last freed on top
 };
 BlockCache& block_cache() {
     static BlockCache* c = new BlockCache;  // never destroyed: blocks may be freed during exit
@@ -186,6 +186,12 @@ size_t cached_trim() {
     BlockCache& c = block_cache();
     const int dev = current_device();
     std::lock_guard<std::mutex> lk(c.mu);
+    // idle streams too: the next batches create new ones.  A stream keeps the
+    // hardware queue it got at creation (4 per process here), so which of a new
+    // workload's concurrent batches share a queue depends on the streams it
+    // inherits; starting from new ones gives it the assignment of a fresh process.
+    for (hipStream_t st : c.streams[dev]) GK_CHECK(hipStreamDestroy(st));
+    c.streams[dev].clear();
     return trim_locked(c, dev);
 }
 

@@ -79,7 +79,8 @@ void* cached_host_malloc(size_t bytes);
 void cached_host_free(void* p);
 hipStream_t cached_stream();             // a non-blocking stream of the current device
 void cached_stream_release(hipStream_t s);  // after its work has finished
-size_t cached_trim();                    // hipFree the current device's cached blocks; bytes released
+size_t cached_trim();                    // hipFree the current device's cached blocks (and destroy its idle
+                                         // streams); bytes released
 size_t cached_bytes();                   // bytes the current device's cache holds
 
 }  // namespace gk

@@ -273,7 +273,9 @@ def run_groups(mm, sims, groups, poll, graph, clock, t1, done_at, lane_factor=3.
 
 
 def bench_sweep(args, mm):
-    """Batched sweep against members run one at a time on the same GPU.  All
+    """Batched sweep against members run one at a time on the same GPU.  Runs
+    the batched phase and returns a function that runs the serial sample and
+    returns the result object.  All
     members start together (t = 0); after a first stretch of trials with every
     member in one batch (which gives the full-batch trial time), they are
     advanced in size groups (cells <= 20480, <= 81920, above), each group a
@@ -318,55 +320,60 @@ def bench_sweep(args, mm):
     for s in sims:
         s.free()
     print("bench.py sweep: %d simulations batched in %.2f s" % (n, t_batch), file=sys.stderr, flush=True)
-    # serial: an evenly spaced sample of members, each alone
-    k = min(n, args.serial_sample)
-    pick = np.unique(np.linspace(0, n - 1, k).astype(int))
-    t_alone = []
-    for j in pick:
-        s = mm.workloads.make_sim(cfgs[j])
-        s._init()
-        s.sync()
-        ta = time.perf_counter()
-        s.run(cfgs[j]["duration"])
-        t_alone.append(time.perf_counter() - ta)
-        s.free()
-    t_alone = np.array(t_alone)
-    serial_rate = 1.0 / t_alone.mean()
-    rate = n / t_batch
-    return {
-        "metric": "sweep solves/s",
-        "value": round(rate, 3),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "higher_is_better": True,
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: seeded sweep (SplitMix64 seed 4), random m0 on the sphere",
-        "config": {"workload": "sweep_%d_sims_%gns_rkdp" % (n, args.sweep_ns), "sims": n,
-                   "cells_total": int(cells.sum()), "solver": "rkdp", "max_error": 1e-5},
-        "batch_seconds": round(t_batch, 4),
-        "batched_trials": trials,
-        "batch_rebuilds": rebuilds,
-        "groups_cells_upto": [20480, 81920, None],
-        "moved_to_lanes": moved,
-        "full_batch_trial": {"ms": round(t_full * 1e3, 4), "model_bytes": int(full_bytes),
-                             "hbm_frac": round(full_bytes / t_full / (HBM_PEAK_GBS * 1e9), 4)},
-        "accepted_steps_total": int(steps.sum()),
-        "mcell_steps_per_s": round(float((steps * cells).sum()) / t_batch / 1e6, 2),
-        "latency_s": {"p50": round(float(np.percentile(done_at, 50)), 4),
-                      "p99": round(float(np.percentile(done_at, 99)), 4)},
-        "serial": {"sample": int(len(pick)), "mean_s_per_sim": round(float(t_alone.mean()), 5),
-                   "solves_per_s": round(serial_rate, 3),
-                   "latency_s": {"p50": round(float(np.percentile(t_alone, 50)), 4),
-                                 "p99": round(float(np.percentile(t_alone, 99)), 4)}},
-        "speedup_vs_serial": round(rate / serial_rate, 2),
-        "setup_seconds": round(setup, 2),
-    }
+
+    def finish():
+        # serial: an evenly spaced sample of members, each alone
+        k = min(n, args.serial_sample)
+        pick = np.unique(np.linspace(0, n - 1, k).astype(int))
+        t_alone = []
+        for j in pick:
+            s = mm.workloads.make_sim(cfgs[j])
+            s._init()
+            s.sync()
+            ta = time.perf_counter()
+            s.run(cfgs[j]["duration"])
+            t_alone.append(time.perf_counter() - ta)
+            s.free()
+        t_alone = np.array(t_alone)
+        serial_rate = 1.0 / t_alone.mean()
+        rate = n / t_batch
+        return {
+            "metric": "sweep solves/s",
+            "value": round(rate, 3),
+            "unit": "solves/s",
+            "n_gpus": 1,
+            "higher_is_better": True,
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic: seeded sweep (SplitMix64 seed 4), random m0 on the sphere",
+            "config": {"workload": "sweep_%d_sims_%gns_rkdp" % (n, args.sweep_ns), "sims": n,
+                       "cells_total": int(cells.sum()), "solver": "rkdp", "max_error": 1e-5},
+            "batch_seconds": round(t_batch, 4),
+            "batched_trials": trials,
+            "batch_rebuilds": rebuilds,
+            "groups_cells_upto": [20480, 81920, None],
+            "moved_to_lanes": moved,
+            "full_batch_trial": {"ms": round(t_full * 1e3, 4), "model_bytes": int(full_bytes),
+                                 "hbm_frac": round(full_bytes / t_full / (HBM_PEAK_GBS * 1e9), 4)},
+            "accepted_steps_total": int(steps.sum()),
+            "mcell_steps_per_s": round(float((steps * cells).sum()) / t_batch / 1e6, 2),
+            "latency_s": {"p50": round(float(np.percentile(done_at, 50)), 4),
+                          "p99": round(float(np.percentile(done_at, 99)), 4)},
+            "serial": {"sample": int(len(pick)), "mean_s_per_sim": round(float(t_alone.mean()), 5),
+                       "solves_per_s": round(serial_rate, 3),
+                       "latency_s": {"p50": round(float(np.percentile(t_alone, 50)), 4),
+                                     "p99": round(float(np.percentile(t_alone, 99)), 4)}},
+            "speedup_vs_serial": round(rate / serial_rate, 2),
+            "setup_seconds": round(setup, 2),
+        }
+
+    return finish
 
 
 def bench_serve(args, mm):
     """Continuous batching of a Poisson request stream against one-at-a-time
-    service in arrival order.  The serial side runs `--serve-serial` requests
+    service in arrival order.  Runs the served stream and returns a function
+    that runs the serial side and returns the result object.  The serial side runs `--serve-serial` requests
     (all by default) one at a time; when that is a sample, every request's
     service time is taken from the sample member with the nearest cells x
     duration and the FIFO queue is computed from the arrivals (stated in
@@ -390,56 +397,60 @@ def bench_serve(args, mm):
     lat = sv.latency_stats(reqs)
     rate = n / stats["elapsed"]
     work = np.array([np.prod(r.cfg["size"]) * r.duration for r in reqs])
-    k = n if args.serve_serial <= 0 else min(n, args.serve_serial)
-    pick = np.unique(np.linspace(0, n - 1, k).astype(int))
-    service = {}
-    for i, j in enumerate(pick):
-        r = reqs[j]
-        s = mm.workloads.make_sim(r.cfg)
-        ta = time.perf_counter()
-        s._init()
-        s.run(r.duration)
-        s.sync()
-        service[j] = time.perf_counter() - ta
-        s.free()
-        if (i + 1) % 32 == 0:
-            print("bench.py serve: %d of %d requests run alone" % (i + 1, len(pick)), file=sys.stderr, flush=True)
-    svc = np.array([service[j] if j in service else service[min(service, key=lambda i: abs(np.log(work[i] / work[j])))]
-                    for j in range(n)])
-    t_free, fin = 0.0, np.zeros(n)
-    arr = np.array([r.arrival for r in reqs])
-    for j in np.argsort(arr, kind="stable"):
-        t_free = max(t_free, arr[j]) + svc[j]
-        fin[j] = t_free
-    slat = fin - arr
-    serial_rate = n / fin.max()
-    return {
-        "metric": "served solves/s",
-        "value": round(rate, 3),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "higher_is_better": True,
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: seeded Poisson request stream (SplitMix64 seed 5)",
-        "config": {"workload": "serve_%d_requests_poisson_%g_per_s" % (n, args.serve_rate), "requests": n,
-                   "cells_total": int(sum(np.prod(r.cfg["size"]) for r in reqs)), "solver": "rkdp",
-                   "hbm_budget_bytes": sv.HBM_BYTES},
-        "elapsed_s": round(stats["elapsed"], 3),
-        "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4), "mean": round(lat["mean"], 4)},
-        "batch_rebuilds": stats["rebuilds"],
-        "moved_to_lanes": stats["moved_to_lanes"],
-        "max_concurrent": stats["max_active"],
-        "max_device_bytes": stats["max_bytes"],
-        "serial": {"measured": int(len(pick)), "of": n,
-                   "service_s_total": round(float(svc.sum()), 3),
-                   "solves_per_s": round(serial_rate, 3),
-                   "latency_s": {"p50": round(float(np.percentile(slat, 50)), 4),
-                                 "p99": round(float(np.percentile(slat, 99)), 4)},
-                   "note": "one request at a time in arrival order (FIFO)" +
-                           ("" if len(pick) == n else "; unmeasured service times from the nearest measured request")},
-        "speedup_vs_serial": round(rate / serial_rate, 2),
-    }
+
+    def finish():
+        k = n if args.serve_serial <= 0 else min(n, args.serve_serial)
+        pick = np.unique(np.linspace(0, n - 1, k).astype(int))
+        service = {}
+        for i, j in enumerate(pick):
+            r = reqs[j]
+            s = mm.workloads.make_sim(r.cfg)
+            ta = time.perf_counter()
+            s._init()
+            s.run(r.duration)
+            s.sync()
+            service[j] = time.perf_counter() - ta
+            s.free()
+            if (i + 1) % 32 == 0:
+                print("bench.py serve: %d of %d requests run alone" % (i + 1, len(pick)), file=sys.stderr, flush=True)
+        svc = np.array([service[j] if j in service else service[min(service, key=lambda i: abs(np.log(work[i] / work[j])))]
+                        for j in range(n)])
+        t_free, fin = 0.0, np.zeros(n)
+        arr = np.array([r.arrival for r in reqs])
+        for j in np.argsort(arr, kind="stable"):
+            t_free = max(t_free, arr[j]) + svc[j]
+            fin[j] = t_free
+        slat = fin - arr
+        serial_rate = n / fin.max()
+        return {
+            "metric": "served solves/s",
+            "value": round(rate, 3),
+            "unit": "solves/s",
+            "n_gpus": 1,
+            "higher_is_better": True,
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic: seeded Poisson request stream (SplitMix64 seed 5)",
+            "config": {"workload": "serve_%d_requests_poisson_%g_per_s" % (n, args.serve_rate), "requests": n,
+                       "cells_total": int(sum(np.prod(r.cfg["size"]) for r in reqs)), "solver": "rkdp",
+                       "hbm_budget_bytes": sv.HBM_BYTES},
+            "elapsed_s": round(stats["elapsed"], 3),
+            "latency_s": {"p50": round(lat["p50"], 4), "p99": round(lat["p99"], 4), "mean": round(lat["mean"], 4)},
+            "batch_rebuilds": stats["rebuilds"],
+            "moved_to_lanes": stats["moved_to_lanes"],
+            "max_concurrent": stats["max_active"],
+            "max_device_bytes": stats["max_bytes"],
+            "serial": {"measured": int(len(pick)), "of": n,
+                       "service_s_total": round(float(svc.sum()), 3),
+                       "solves_per_s": round(serial_rate, 3),
+                       "latency_s": {"p50": round(float(np.percentile(slat, 50)), 4),
+                                     "p99": round(float(np.percentile(slat, 99)), 4)},
+                       "note": "one request at a time in arrival order (FIFO)" +
+                               ("" if len(pick) == n else "; unmeasured service times from the nearest measured request")},
+            "speedup_vs_serial": round(rate / serial_rate, 2),
+        }
+
+    return finish
 
 
 def main():
@@ -483,12 +494,12 @@ def main():
     mm = importlib.import_module("1_amd")
     lib = mm.load()
     if args.workload == "sweep":
-        print(json.dumps(bench_sweep(args, mm)), flush=True)
+        print(json.dumps(bench_sweep(args, mm)()), flush=True)
         return
     if args.workload == "serve":
         if args.serve_serial is None:
             args.serve_serial = 0
-        print(json.dumps(bench_serve(args, mm)), flush=True)
+        print(json.dumps(bench_serve(args, mm)()), flush=True)
         return
     if args.serve_serial is None:
         args.serve_serial = 40
@@ -583,12 +594,20 @@ def main():
         out["parity"] = parity_vs_oracle(mm, cfg, o, n)
         del o
     print("bench.py cuboid: %.2f Mcell-steps/s" % value, file=sys.stderr, flush=True)
-    # the batched sweep and the served request stream, against the same
-    # simulations run one at a time on this GPU
-    if not args.no_sweep:
-        out["sweep"] = bench_sweep(args, mm)
-    if not args.no_serve:
-        out["serve"] = bench_serve(args, mm)
+    # the batched sweep and the served request stream, then the same
+    # simulations run one at a time on this GPU.  Each batched workload starts
+    # as in a fresh process: the engine's cached blocks and idle streams of the
+    # earlier ones are released (mm_memory_trim).  A stream keeps the hardware
+    # queue it got at creation, so streams inherited from the sweep put the
+    # served stream's concurrent batches on other queues (~9 % slower).
+    lib.mm_memory_trim()
+    sweep_serial = bench_sweep(args, mm) if not args.no_sweep else None
+    lib.mm_memory_trim()
+    serve_serial = bench_serve(args, mm) if not args.no_serve else None
+    if sweep_serial:
+        out["sweep"] = sweep_serial()
+    if serve_serial:
+        out["serve"] = serve_serial()
     print(json.dumps(out), flush=True)
 
 

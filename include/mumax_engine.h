@@ -145,7 +145,9 @@ int mm_kernel_cache_size(void);
  * device and reused by later simulations and batches, so that freeing does not
  * wait for the device's other streams (hipFree does).  Bytes the current
  * device's cache holds, and a release of all of them to the device (this
- * waits for the device, as hipFree does). */
+ * waits for the device, as hipFree does) together with the idle batch streams,
+ * so that the next batches get new streams (and the hardware queues new
+ * streams get). */
 unsigned long long mm_memory_cached(void);
 unsigned long long mm_memory_trim(void);
 
