@@ -155,6 +155,43 @@ def used_cores():
     return min(omp, avail) if omp > 0 else avail
 
 
+def member_parity(mm, cfg, batched_m, seconds):
+    """One member of a batched workload against the FP64 oracle: its final m
+    in the batch against the same simulation run alone (bit for bit), and the
+    first `seconds` of it run alone with its trial log (dt, accepted) replayed
+    in the oracle: max |dm| and the relative energy error at the end."""
+    from oracle import Oracle
+
+    s = mm.workloads.make_sim(cfg)
+    s.run(cfg["duration"])
+    same = bool(np.array_equal(s.m.view(np.uint32), batched_m.view(np.uint32)))
+    s.free()
+    s = mm.workloads.make_sim(cfg)
+    s.run(seconds)
+    dts, acc = s.trial_log()
+    replay = []
+    for prec in ("f64", "f32"):  # the FP64 oracle, and its FP32 twin on the same trials
+        o = Oracle(prec).sim(cfg["m0"], **mm.workloads.oracle_kwargs(cfg))
+        i = 0
+        while i < len(dts):
+            j = i
+            while j < len(dts) - 1 and not acc[j]:
+                j += 1
+            o.step_replay(dts[i:j + 1], acc[i:j + 1])
+            i = j + 1
+        replay.append(o)
+    o, o32 = replay
+    e_gpu, e_o = s.energy(), o.energy()
+    out = {"member": cfg["name"], "size_internal_xyz": list(cfg["size"]), "batch_equals_alone_bitwise": same,
+           "vs": "FP64 CPU oracle replaying the member's first %.3g ns alone (%d trials, %d steps)" % (
+               seconds * 1e9, len(dts), int(s.state()["steps"])),
+           "max_abs_dm": float(np.abs(s.m.reshape(-1) - o.m).max()),
+           "rel_energy_err": float(abs(e_gpu - e_o) / abs(e_o)),
+           "fp32_twin_max_abs_dm": float(np.abs(o32.m - o.m).max())}
+    s.free()
+    return out
+
+
 def parity_vs_oracle(mm, cfg, o, n):
     """The GPU engine run from the same m0 for the same n RK4 steps as the
     oracle sim `o` of the CPU leg: max |dm| and the relative energy error of
@@ -317,6 +354,8 @@ def bench_sweep(args, mm):
     full_bytes = sum(dp_trial_bytes(c["size"]) for c in cfgs)
     steps = np.array([s.state()["steps"] for s in sims])
     cells = np.array([np.prod(c["size"]) for c in cfgs])
+    j_par = int(np.argmin(cells))
+    m_par = sims[j_par].m
     for s in sims:
         s.free()
     print("bench.py sweep: %d simulations batched in %.2f s" % (n, t_batch), file=sys.stderr, flush=True)
@@ -364,6 +403,7 @@ def bench_sweep(args, mm):
                        "latency_s": {"p50": round(float(np.percentile(t_alone, 50)), 4),
                                      "p99": round(float(np.percentile(t_alone, 99)), 4)}},
             "speedup_vs_serial": round(rate / serial_rate, 2),
+            "parity": None if args.no_cpu else member_parity(mm, cfgs[j_par], m_par, 0.02e-9),
             "setup_seconds": round(setup, 2),
         }
 
@@ -382,9 +422,13 @@ def bench_serve(args, mm):
     reqs = sv.request_stream(args.serve_requests, rate=args.serve_rate, seed=5)
     n = len(reqs)
     finished = [0]
+    j_par = int(np.argmin([np.prod(r.cfg["size"]) for r in reqs]))
+    m_par = [None]
 
     def progress(r):
         finished[0] += 1
+        if r.rid == reqs[j_par].rid:
+            m_par[0] = r.sim.m  # the smallest request's result, for the parity check
         if finished[0] % 25 == 0:
             print("bench.py serve: %d of %d requests served" % (finished[0], n), file=sys.stderr, flush=True)
 
@@ -448,6 +492,8 @@ def bench_serve(args, mm):
                        "note": "one request at a time in arrival order (FIFO)" +
                                ("" if len(pick) == n else "; unmeasured service times from the nearest measured request")},
             "speedup_vs_serial": round(rate / serial_rate, 2),
+            "parity": None if args.no_cpu or m_par[0] is None else member_parity(
+                mm, dict(reqs[j_par].cfg, duration=reqs[j_par].duration), m_par[0], 0.02e-9),
         }
 
     return finish
