@@ -315,13 +315,14 @@ __global__ __launch_bounds__(256) void kb_rk_final(const BSim* sims, const int* 
     for (int s = 0; s < 7; s++) kp.k[s] = d.k[s] ? d.k[s] : d.k[0];
     FinalArgs a = fa;
     a.part_err = d.part + 2L * d.npart;
+    a.normMap = d.t.normMap;
     rk_final_body(d.m, kp, d.mtmp, d.st, a, (int)d.N, bb.lb, d.nfin);
 }
 
 __global__ __launch_bounds__(256) void kb_commit(const BSim* sims, const int* idx, const int* start, int n) {
     const BBlock bb = bblock(sims, idx, start, n);
     const BSim& d = *bb.d;
-    commit_body(d.m, d.mtmp, d.st, (int)d.N, bb.lb, d.nfin);
+    commit_body(d.m, d.mtmp, d.st, d.t.normMap, (int)d.N, bb.lb, d.nfin);
 }
 
 // one thread per simulation
@@ -1141,7 +1142,7 @@ void mm_batch_start(mmBatch* b, const double* seconds, const long long* steps) {
         GK_CHECK(hipStreamSynchronize(s->s));
         EngineState h = read_state(s);
         const double stop = seconds ? h.time + seconds[j] / s->unitTime : INFINITY;
-        if (seconds) launch_normalize_active(s->m, nullptr, s->N, b->s);
+        if (seconds) launch_normalize_active(s->m, nullptr, s->normMap, s->N, b->s);
         launch_set_target(s->st, steps ? steps[j] : -1, stop, b->s);
     }
     GK_CHECK(hipStreamSynchronize(b->s));

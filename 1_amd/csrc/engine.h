@@ -254,6 +254,7 @@ struct TorqueArgs {
     float thermalK;         // thermal field sigma * sqrt(dt) (internal units); 0: T = 0
     unsigned long long noiseSeed;
     const float* alphaMap;  // per-cell damping multiplier (null: none); dtg keeps the global alpha (torque.go:49-51)
+    const float* normMap;   // per-cell |m| after each normalisation (null: 1; sim.go:230-239)
     int anisType;
     float kAnis;            // 2K1/(mu0 Ms^2) (init.go:103-108)
     float u[3];
@@ -321,6 +322,14 @@ __device__ __forceinline__ void add_thermal(const TorqueArgs& a, const EngineSta
 __device__ __forceinline__ float cell_alpha(const TorqueArgs& a, long i) {
     return a.alphaMap ? a.alpha * a.alphaMap[i] : a.alpha;
 }
+// scale that normalises (x, y, z): 1/|m|, times the cell's norm when a norm map
+// is given, 0 where the map is 0 (gpu_normalize.cu:20-30 uniform, :43-71 map)
+__device__ __forceinline__ float norm_scale(float x, float y, float z, const float* nmap, long i) {
+    const float s = rsqrtf(x * x + y * y + z * z);
+    if (!nmap) return s;
+    const float w = nmap[i];
+    return w == 0.f ? 0.f : s * w;
+}
 __device__ __forceinline__ void llg_torque_cell(const TorqueArgs& a, float alpha, const float hext[3], float Mx,
                                                 float My, float Mz, float Hx, float Hy, float Hz, float k[3]) {
     Hx += hext[0];
@@ -367,6 +376,7 @@ struct FinalArgs {
     float b[7], b2[7];
     int adaptive;
     float* part_err;        // per-block max |m_low - m_high| (adaptive)
+    const float* normMap;   // fixed step: the norm map of the final normalisation (null: 1)
 };
 
 // Pointwise kernels run a fixed grid of kRedBlocks blocks (grid-stride), so
@@ -400,8 +410,8 @@ struct CtrlParams {
 void launch_ctrl_begin(EngineState* st, hipStream_t s);
 void launch_ctrl_after_k0(EngineState* st, int fsal_copy, const CtrlParams& p, hipStream_t s);
 void launch_ctrl_end(EngineState* st, const CtrlParams& p, hipStream_t s);
-void launch_commit(float* m, const float* mtmp, EngineState* st, int N, hipStream_t s);
-void launch_normalize_active(float* m, const EngineState* st, int N, hipStream_t s);
+void launch_commit(float* m, const float* mtmp, EngineState* st, const float* nmap, int N, hipStream_t s);
+void launch_normalize_active(float* m, const EngineState* st, const float* nmap, int N, hipStream_t s);
 
 void launch_energy(const float* m, const float* h, const float hext[3], int anisType, float kAnis, const float u[3],
                    int N, double* out, hipStream_t s);

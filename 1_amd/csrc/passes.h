@@ -317,7 +317,7 @@ __device__ __forceinline__ void zinv_epi_tile(const float2* __restrict__ B1, con
             float f[3];
 #pragma unroll
             for (int q = 0; q < 3; q++) f[q] = fmaf(cbs, k[q], hi[q]);
-            const float sc = rsqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+            const float sc = norm_scale(f[0], f[1], f[2], ea.t.normMap, ci);
 #pragma unroll
             for (int q = 0; q < 3; q++) o[q] = f[q] * sc;
         } else {

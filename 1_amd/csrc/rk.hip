@@ -68,15 +68,16 @@ __global__ void k_copy_if_active(float* __restrict__ dst, const float* __restric
 }
 
 __global__ __launch_bounds__(256) void k_commit(float* __restrict__ m, const float* __restrict__ mtmp,
-                                                const EngineState* st, int N) {
-    commit_body(m, mtmp, st, N, blockIdx.x, gridDim.x);
+                                                const EngineState* st, const float* __restrict__ nmap, int N) {
+    commit_body(m, mtmp, st, nmap, N, blockIdx.x, gridDim.x);
 }
 
-__global__ void k_normalize_active(float* __restrict__ m, const EngineState* st, int N) {
+__global__ void k_normalize_active(float* __restrict__ m, const EngineState* st, const float* __restrict__ nmap,
+                                   int N) {
     if (st && !st->trial_active) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const float x = m[i], y = m[N + i], z = m[2 * N + i];
-        const float s = rsqrtf(x * x + y * y + z * z);
+        const float s = norm_scale(x, y, z, nmap, i);
         m[i] = x * s;
         m[N + i] = y * s;
         m[2 * N + i] = z * s;
@@ -202,12 +203,12 @@ void launch_ctrl_end(EngineState* st, const CtrlParams& p, hipStream_t s) {
     k_ctrl_end<<<1, 64, 0, s>>>(st, p);
     GK_LAUNCH_CHECK();
 }
-void launch_commit(float* m, const float* mtmp, EngineState* st, int N, hipStream_t s) {
-    k_commit<<<gridN(N), 256, 0, s>>>(m, mtmp, st, N);
+void launch_commit(float* m, const float* mtmp, EngineState* st, const float* nmap, int N, hipStream_t s) {
+    k_commit<<<gridN(N), 256, 0, s>>>(m, mtmp, st, nmap, N);
     GK_LAUNCH_CHECK();
 }
-void launch_normalize_active(float* m, const EngineState* st, int N, hipStream_t s) {
-    k_normalize_active<<<gridN(N), 256, 0, s>>>(m, st, N);
+void launch_normalize_active(float* m, const EngineState* st, const float* nmap, int N, hipStream_t s) {
+    k_normalize_active<<<gridN(N), 256, 0, s>>>(m, st, nmap, N);
     GK_LAUNCH_CHECK();
 }
 void launch_sum3(const float* v, int N, double* out, hipStream_t s) {

@@ -64,7 +64,7 @@ __device__ __forceinline__ void rk_final_body(float* __restrict__ m, const KPtrs
                 lmax = fmaxf(lmax, abs_keep_nan((float)er[c]));
             }
         } else {
-            const float s = rsqrtf(hi[0] * hi[0] + hi[1] * hi[1] + hi[2] * hi[2]);
+            const float s = norm_scale(hi[0], hi[1], hi[2], a.normMap, i);
 #pragma unroll
             for (int c = 0; c < 3; c++) m[c * N + i] = hi[c] * s;
         }
@@ -81,14 +81,16 @@ __device__ __forceinline__ void copy_body(float* __restrict__ dst, const float* 
 }
 
 // accepted: m = normalize(m_high); step ended after the last rejected trial:
-// m (restored, i.e. untouched) is normalised (rk.go:426-444)
+// m (restored, i.e. untouched) is normalised (rk.go:426-444); with a norm map
+// each cell to its own norm (sim.go:230-239)
 __device__ __forceinline__ void commit_body(float* __restrict__ m, const float* __restrict__ mtmp,
-                                            const EngineState* st, int N, int blk, int nblk) {
+                                            const EngineState* st, const float* __restrict__ nmap, int N, int blk,
+                                            int nblk) {
     if (!st->trial_active || !st->step_end) return;
     const float* src = st->accept ? mtmp : m;
     for (int i = blk * blockDim.x + threadIdx.x; i < N; i += nblk * blockDim.x) {
         const float x = src[i], y = src[N + i], z = src[2 * N + i];
-        const float s = rsqrtf(x * x + y * y + z * z);
+        const float s = norm_scale(x, y, z, nmap, i);
         m[i] = x * s;
         m[N + i] = y * s;
         m[2 * N + i] = z * s;

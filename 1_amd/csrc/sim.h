@@ -75,6 +75,7 @@ struct mmSim {
     double* fT = nullptr;
     float* fB = nullptr;
     float* alphaMap = nullptr;       // device copy of mmParams.alphaMap (N floats)
+    float* normMap = nullptr;        // device copy of mmParams.normMap (N floats)
     FieldTable field{};
     ConvPlan cp{};
     bool conv = false;

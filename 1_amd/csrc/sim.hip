@@ -267,6 +267,7 @@ TorqueArgs gk::torque_args(const mmSim* s, float c, float* part) {
     a.alpha = (float)s->p.alpha;
     a.dtg = (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha));
     a.alphaMap = s->alphaMap;
+    a.normMap = s->normMap;
     a.anisType = s->p.anisType;
     a.kAnis = s->kAnis;
     for (int i = 0; i < 3; i++) a.u[i] = s->u[i];
@@ -358,6 +359,7 @@ static void enqueue_trial(mmSim* s) {
     }
     fa.adaptive = t.adaptive;
     fa.part_err = s->part + 2 * s->npart;
+    fa.normMap = s->normMap;
     const float* kk[7];
     for (int i = 0; i < 7; i++) kk[i] = s->k[i] ? s->k[i] : s->k[0];
     if (t.adaptive && !t.fsal_final()) {
@@ -367,7 +369,7 @@ static void enqueue_trial(mmSim* s) {
     }
     pf.begin(7, s->s);
     launch_ctrl_end(s->st, cp, s->s);
-    if (t.adaptive) launch_commit(s->m, s->mtmp, s->st, s->N, s->s);
+    if (t.adaptive) launch_commit(s->m, s->mtmp, s->st, s->normMap, s->N, s->s);
     pf.end(s->s);
     if (t.fsal) s->fsalHost = true;
 }
@@ -428,6 +430,14 @@ static void abi_torque(mmSim* s, float* m, float* h) {
     // torque.go:33-53 with dt = 1
     const float alpha = (float)s->p.alpha;
     gpu_deltaM(m, h, alpha, s->alphaMap, 1.f / (1.f + alpha * alpha), s->N);
+}
+
+// Sim.Normalize (sim.go:230-239): to the norm map when there is one
+static void normalize_abi(mmSim* s, float* m) {
+    if (s->normMap)
+        gpu_normalize_map(m, s->normMap, s->N);
+    else
+        gpu_normalize_uniform(m, s->N);
 }
 
 static void abi_step(mmSim* s) {
@@ -502,7 +512,7 @@ static void abi_step(mmSim* s) {
         s->hlog_acc.push_back(!bad);
         s->hlog_err.push_back(t.adaptive ? s->herr : 0.f);
     }
-    gpu_normalize_uniform(m, s->N);
+    normalize_abi(s, m);  // rk.go:444
     s->hsteps++;
 }
 
@@ -659,6 +669,11 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
         GK_CHECK(hipMemcpyAsync(s->alphaMap, p->alphaMap, (size_t)s->N * sizeof(float), hipMemcpyHostToDevice, s->s));
     }
     s->p.alphaMap = nullptr;  // the caller's array is not kept
+    if (p->normMap) {
+        s->normMap = (float*)dmalloc(s, (size_t)s->N * sizeof(float));
+        GK_CHECK(hipMemcpyAsync(s->normMap, p->normMap, (size_t)s->N * sizeof(float), hipMemcpyHostToDevice, s->s));
+    }
+    s->p.normMap = nullptr;
     // convolution (init.go:234-274)
     conv_plan_init(s->cp, p->size);
     s->bytes += s->cp.bytes;
@@ -687,7 +702,7 @@ void mm_sim_free(mmSim* s) {
     GK_CHECK(hipStreamSynchronize(s->s));
     for (int i = 0; i < 7; i++)
         cached_free(s->k[i]);
-    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap}) cached_free(p);
+    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap, s->normMap}) cached_free(p);
     for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->log_err, (void*)s->dsum, (void*)s->fT,
                     (void*)s->part, (void*)s->epart})
         cached_free(p);
@@ -705,7 +720,7 @@ void mm_sim_free(mmSim* s) {
 long mm_sim_run(mmSim* s, double seconds) {
     if (s->mode == MM_MODE_ABI) {
         const double stop = s->htime + seconds / s->unitTime;
-        gpu_normalize_uniform(s->m, s->N);
+        normalize_abi(s, s->m);  // simrun.go:24
         long n = 0;
         while (s->htime < stop) {
             abi_step(s);
@@ -715,7 +730,7 @@ long mm_sim_run(mmSim* s, double seconds) {
     }
     EngineState h = read_state(s);
     const double stop = h.time + seconds / s->unitTime;
-    launch_normalize_active(s->m, nullptr, s->N, s->s);  // simrun.go:24
+    launch_normalize_active(s->m, nullptr, s->normMap, s->N, s->s);  // simrun.go:24
     return fused_run(s, -1, stop, true);
 }
 
@@ -955,6 +970,7 @@ unsigned long long mm_sim_footprint(const mmParams* p) {
     b += 3 * (size_t)npart * sizeof(float) + (size_t)npart * sizeof(double);  // part, epart
     if (p->nField >= 2) b += (size_t)p->nField * (sizeof(double) + 3 * sizeof(float));
     if (p->alphaMap) b += (size_t)(N3 / 3) * sizeof(float);
+    if (p->normMap) b += (size_t)(N3 / 3) * sizeof(float);
     return b;
 }
 
@@ -1001,7 +1017,7 @@ void mm_sim_begin_run(mmSim* s, double seconds) {
     GK_ASSERT(s->mode == MM_MODE_FUSED, "mm_sim_begin_run needs the fused mode");
     EngineState h = read_state(s);
     const double stop = h.time + seconds / s->unitTime;
-    launch_normalize_active(s->m, nullptr, s->N, s->s);  // simrun.go:24
+    launch_normalize_active(s->m, nullptr, s->normMap, s->N, s->s);  // simrun.go:24
     launch_set_target(s->st, -1, stop, s->s);
     GK_CHECK(hipStreamSynchronize(s->s));
 }

@@ -156,6 +156,8 @@ COMMANDS = {
     "demagaccuracy": ("demagaccuracy", "i"), "exchinconv": ("exchinconv", "b"), "exchtype": ("exchtype", "i"),
     "demag": ("_demag", "b"), "energy": ("_energy", "b"), "kerneltype": ("kerneltype", "s"),
     "setalpha": ("setalpha", "iiif"), "setalpharange": ("setalpharange", "iiiiiif"),
+    "setmsat": ("setmsat", "iiif"), "setmsatrange": ("setmsatrange", "iiiiiif"),
+    "setmsatellips": ("setmsatellips", "fffff"),
     # excitation (simfield.go)
     "applystatic": ("applystatic", "sfff"), "applypointwise": ("applypointwise", "sdfff"),
     "applypulse": ("applypulse", "sfffdd"), "applyrf": ("applyrf", "sfffd"), "applyrframp": ("applyrframp", "sfffdd"),
@@ -190,7 +192,7 @@ COMMANDS = {
 # implement; they stop the script with an InputErr naming the command.
 UNSUPPORTED = (
     "k2", "aniscubic", "spinpolarization", "xi", "currentdensity", "currentmask", "fieldmask",
-    "maxcellsize", "ellipsoid", "mask", "setmsat", "setmsatrange", "setmsatellips", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
+    "maxcellsize", "ellipsoid", "mask", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortex_in_array", "setm", "applyrotatingburst", "edgecorrection",
     "savebenchmark", "exchangetype",
 )

@@ -154,6 +154,7 @@ class Params(C.Structure):
         ("alphaMap", C.POINTER(C.c_float)),
         ("temperature", C.c_double),
         ("seed", C.c_ulonglong),
+        ("normMap", C.POINTER(C.c_float)),
     ]
 
 

@@ -150,6 +150,7 @@ class Sim:
         self._want_energy = False
         self._kernel = 0
         self._alpha_map = None
+        self._norm_map = None
         self._temperature = 0.0
         self._thermal_seed = 0
         self._m0 = None
@@ -349,6 +350,63 @@ class Sim:
         """SetAlphaRange (simgeom.go:253-264): multiplier over user x1..x2,
         y1..y2, z1..z2 (inclusive)."""
         self._alpha_mask()[z1:z2 + 1, y1:y2 + 1, x1:x2 + 1] = alpha
+
+    def _msat_mask(self):
+        # initMsatMask (simgeom.go:266-278): a norm of 1 in every cell
+        self._invalidate()
+        if self._norm_map is None:
+            self._norm_map = np.ones(tuple(self._need_size()), np.float32)
+        return self._norm_map
+
+    def _msat_box(self, lo, hi, msat):
+        # the reference's Set panics outside the grid: an input error here
+        n = self._msat_mask()
+        for a, b, size in zip(lo, hi, n.shape):
+            if a <= b and (a < 0 or b >= size):
+                raise InputErr("cells %s..%s are outside the grid %s" % (tuple(lo), tuple(hi), n.shape))
+        n[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = np.float32(msat)
+
+    def setmsat(self, z, y, x, msat):
+        """SetMsat (simgeom.go:279-284): the normalised saturation magnetisation
+        of one cell, i.e. the norm every normalisation gives m there (0: an
+        empty cell); indexed as SetAlpha."""
+        x, y, z = int(x), int(y), int(z)
+        self._msat_box((x, y, z), (x, y, z), msat)
+
+    def setmsatrange(self, z1, y1, x1, z2, y2, x2, msat):
+        """SetMsatRange (simgeom.go:287-299): the norm over cells x1..x2 (first
+        internal index), y1..y2, z1..z2 (third), inclusive."""
+        self._msat_box((int(x1), int(y1), int(z1)), (int(x2), int(y2), int(z2)), msat)
+
+    def setmsat_array(self, nmap):
+        """The whole norm map at once, internal layout (X, Y, Z)."""
+        n = self._msat_mask()
+        n[...] = np.asarray(nmap, np.float32).reshape(n.shape)
+
+    def setmsatellips(self, cx, cy, sx, sy, msat):
+        """SetMsatEllips (simgeom.go:301-340): the norm inside the ellipse of
+        centre (cx, cy) and axes (sx, sy) (metres; cx along the last internal
+        axis, cy along the middle one) through every layer, in the reference's
+        float32 arithmetic over its cell range."""
+        if self._cell is None:
+            raise InputErr("The cell size should be set first. E.g. cellsize 1E-9 1E-9 1E-9")
+        f = np.float32
+        n = self._msat_mask()
+        cx, cy, sx, sy = f(cx), f(cy), f(sx), f(sy)
+        rx, ry = sx / f(2.0), sy / f(2.0)
+        c1, c2 = f(self._cell[1]), f(self._cell[2])
+        start_x = int(np.floor(np.float64((cx - rx) / c2)))
+        start_y = int(np.floor(np.float64((cy - ry) / c1)))
+        stop_x = int(np.ceil(np.float64((cx + rx) / c2)))
+        stop_y = int(np.ceil(np.float64((cy + ry) / c1)))
+        for j in range(start_y, stop_y + 1):
+            y = f(j) * c1 - cy
+            for k in range(start_x, stop_x + 1):
+                x = f(k) * c2 - cx
+                if x * x / rx / rx + y * y / ry / ry < f(1):
+                    if not (0 <= j < n.shape[1] and 0 <= k < n.shape[2]):
+                        raise InputErr("the ellipse covers cell (%d, %d) outside the grid %s" % (j, k, n.shape))
+                    n[:, j, k] = f(msat)
 
     def temperature(self, T):
         """Temperature(T) (temperature.go:47-56): a thermal field of standard
@@ -1007,6 +1065,10 @@ class Sim:
             amap = np.ascontiguousarray(self._alpha_map, np.float32)
             self._keep_alpha = amap
             p.alphaMap = amap.ctypes.data_as(C.POINTER(C.c_float))
+        if self._norm_map is not None:
+            nmap = np.ascontiguousarray(self._norm_map, np.float32)
+            self._keep_norm = nmap
+            p.normMap = nmap.ctypes.data_as(C.POINTER(C.c_float))
         p.solver = SOLVERS[self._solver]
         p.maxError, p.dt = self._maxerror, self._dt
         p.minDm, p.maxDm, p.minDt, p.maxDt = self._mindm, self._maxdm, self._mindt, self._maxdt

@@ -229,6 +229,8 @@ def make_sim(cfg, mode="fused"):
         else:
             for t, (bX, bY, bZ) in f[1]:
                 s.applypointwise("field", t, bZ, bY, bX)
+    if cfg.get("norm_map") is not None:
+        s.setmsat_array(cfg["norm_map"])
     s.setm_array(cfg["m0"])
     return s
 
@@ -244,6 +246,8 @@ def oracle_kwargs(cfg):
         kw.update(temperature=cfg["temperature"], seed=cfg.get("seed", 0))
     if cfg.get("periodic"):
         kw["periodic"] = tuple(cfg["periodic"])
+    if cfg.get("norm_map") is not None:
+        kw["norm_map"] = cfg["norm_map"]
     for k in ("max_dt", "min_dt"):
         if cfg.get(k):
             kw[k] = cfg[k]

@@ -76,6 +76,14 @@ typedef struct {
     unsigned long long seed;  /* thermal noise stream: counter-based (SplitMix64 of seed,
                                step and cell, Box-Muller), so it does not depend on launch
                                shapes or batching */
+    const float* normMap;   /* N host floats: per-cell norm of m, the "normalized
+                               saturation magnetization" of SetMsat / SetMsatRange /
+                               SetMsatEllips (simgeom.go:266-340); every normalisation
+                               (the run's start, each step's end) scales a cell to its
+                               value, 0 leaving it empty (Sim.Normalize, sim.go:230-239;
+                               gpu_normalize.cu:43-71). Null: 1 everywhere. <m> still
+                               divides by the cell count (avgNorm, init.go:135).
+                               Copied at mm_sim_new. */
 } mmParams;
 
 typedef struct mmSim mmSim;

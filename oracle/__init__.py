@@ -107,6 +107,8 @@ class Oracle:
         L.or_sim_set_field.argtypes = [P, C.c_int, P, P]
         L.or_sim_set_want_energy.argtypes = [P, C.c_int]
         L.or_sim_set_alpha_map.argtypes = [P, P]
+        L.or_sim_set_norm_map.argtypes = [P, P]
+        L.or_normalize_map.argtypes = [P, P, C.c_int]
         L.or_sim_set_field_function.argtypes = [P, C.c_int, P, P]
         L.or_sim_field_at.argtypes = [P, C.c_double, P]
         L.or_sim_trial_log.argtypes = [P, P, P, C.c_long]
@@ -201,6 +203,13 @@ class Oracle:
         self.lib.or_normalize(m.ctypes.data, m.size // 3)
         return m
 
+    def normalize_map(self, m, nmap):
+        m = np.ascontiguousarray(m, self.real).copy()
+        nmap = np.ascontiguousarray(nmap, self.real).reshape(-1)
+        assert nmap.size == m.size // 3
+        self.lib.or_normalize_map(m.ctypes.data, nmap.ctypes.data, m.size // 3)
+        return m
+
     def reduce(self, op, x):
         x, xp = self._arr(x)
         return self.lib.or_reduce(op, xp, x.size)
@@ -264,7 +273,7 @@ class Sim:
     def __init__(self, orc, m0, size, cellsize, msat, aexch, alpha, solver="rkdp", maxerror=1e-4, dt=0.0,
                  k1=0.0, anis_axis=None, field=None, periodic=(0, 0, 0), exch_type=6, exch_in_conv=True,
                  want_demag=True, min_dm=0.0, max_dm=0.0, min_dt=0.0, max_dt=0.0, want_energy=False,
-                 demag_accuracy=8, temperature=0.0, seed=0):
+                 demag_accuracy=8, temperature=0.0, seed=0, norm_map=None):
         self.orc = orc
         p = Params()
         p.size[:] = list(size)
@@ -299,6 +308,8 @@ class Sim:
         m0 = np.ascontiguousarray(m0, np.float32).reshape(-1)
         assert m0.size == 3 * self.N
         self.ptr = orc.lib.or_sim_new(C.byref(p), m0.ctypes.data)
+        if norm_map is not None:
+            self.set_norm_map(norm_map)
 
     def __del__(self):
         if getattr(self, "ptr", None):
@@ -373,6 +384,11 @@ class Sim:
         """per-cell damping multiplier (internal [X][Y][Z] order), None: none"""
         self._amap = None if amap is None else np.ascontiguousarray(amap, np.float32).reshape(-1)
         self.orc.lib.or_sim_set_alpha_map(self.ptr, None if self._amap is None else self._amap.ctypes.data)
+
+    def set_norm_map(self, nmap):
+        """per-cell norm of m after each normalisation (internal [X][Y][Z] order), None: 1"""
+        self._nmap = None if nmap is None else np.ascontiguousarray(nmap, np.float32).reshape(-1)
+        self.orc.lib.or_sim_set_norm_map(self.ptr, None if self._nmap is None else self._nmap.ctypes.data)
 
     def set_field(self, t, b):
         """replace the applied field table (t: n times in s, b: n x 3 tesla)"""

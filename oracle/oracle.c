@@ -590,6 +590,20 @@ void or_normalize(or_real* m, int N) {
     }
 }
 
+/* gpu_normalize.cu:43-71 (_gpu_normalize_map): each cell to the norm nmap[i];
+ * a cell whose norm is 0 becomes 0 (the kernel's special case, so m = 0 there
+ * gives 0 rather than 0/0) */
+void or_normalize_map(or_real* m, const or_real* nmap, int N) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        double n2 = (double)m[i] * m[i] + (double)m[N + i] * m[N + i] + (double)m[2 * N + i] * m[2 * N + i];
+        double sc = nmap[i] == 0 ? 0.0 : (double)nmap[i] / sqrt(n2);
+        m[i] = (or_real)(m[i] * sc);
+        m[N + i] = (or_real)(m[N + i] * sc);
+        m[2 * N + i] = (or_real)(m[2 * N + i] * sc);
+    }
+}
+
 void or_scale_dot(or_real* r, const or_real* a, const or_real* b, or_real s, int N) {
     for (int i = 0; i < N; i++) r[i] = s * (a[i] * b[i] + a[N + i] * b[N + i] + a[2 * N + i] * b[2 * N + i]);
 }
@@ -707,10 +721,19 @@ struct or_sim {
     int* log_acc;
     long log_n, log_cap;
     or_real* amap; /* per-cell damping multiplier (SetAlpha, simgeom.go:246-264), NULL: none */
+    or_real* nmap; /* per-cell norm of m (SetMsat, simgeom.go:266-340), NULL: 1 */
     int fn;        /* applied-field function of time (1 pulse, 2 RF, 3 RF ramp, 4 sawtooth, 5 rotating), 0: table */
     double fp[4], fb[3];
     or_real* noise; /* thermal field of the current step, 3N (NULL at T = 0) */
 };
+
+/* Sim.Normalize (sim.go:230-239): to the norm map when there is one */
+static void sim_normalize(or_sim* s, or_real* m) {
+    if (s->nmap)
+        or_normalize_map(m, s->nmap, s->N);
+    else
+        or_normalize(m, s->N);
+}
 
 /* updateTempNoise (temperature.go:29-36): the thermal field for this step from
  * dt (internal) at its start, stddev = sqrt(2 alpha kB T / (gamma0 mu0 Ms V dt_SI)) / Ms */
@@ -921,6 +944,7 @@ void or_sim_free(or_sim* s) {
     free(s->fieldT); free(s->fieldB);
     free(s->log_dt); free(s->log_acc);
     free(s->amap);
+    free(s->nmap);
     free(s->noise);
     free(s);
 }
@@ -1040,7 +1064,7 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
             log_trial(s, dt_used, 1);
         }
     }
-    or_normalize(m, s->N); /* rk.go:444 */
+    sim_normalize(s, m); /* rk.go:444 */
     return trials;
 }
 
@@ -1058,7 +1082,7 @@ int or_sim_step_replay(or_sim* s, const double* dt_trial, const int* accept, int
 
 int or_sim_run(or_sim* s, double seconds) { /* simrun.go:19-57 */
     double stop = s->time + seconds / s->units[1];
-    or_normalize(s->m, s->N);
+    sim_normalize(s, s->m); /* simrun.go:24 */
     int n = 0;
     while (s->time < stop) {
         if (or_sim_step(s) < 0) return -1;
@@ -1141,6 +1165,15 @@ void or_sim_set_alpha_map(or_sim* s, const float* amap) {
     s->amap = (or_real*)malloc((size_t)s->N * sizeof(or_real));
     if (!s->amap) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
     for (int i = 0; i < s->N; i++) s->amap[i] = (or_real)amap[i];
+}
+
+void or_sim_set_norm_map(or_sim* s, const float* nmap) {
+    free(s->nmap);
+    s->nmap = NULL;
+    if (!nmap) return;
+    s->nmap = (or_real*)malloc((size_t)s->N * sizeof(or_real));
+    if (!s->nmap) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    for (int i = 0; i < s->N; i++) s->nmap[i] = (or_real)nmap[i];
 }
 
 double or_sim_energy(or_sim* s) {

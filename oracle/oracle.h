@@ -86,6 +86,7 @@ void or_local_fields_phi(const or_real* m, or_real* h, or_real* phi, int N, cons
 void or_deltaM(const or_real* m, or_real* h, or_real alpha, or_real dtgilb, int N);
 void or_deltaM_map(const or_real* m, or_real* h, or_real alpha_mul, const or_real* amap, or_real dtgilb, int N);
 void or_normalize(or_real* m, int N);
+void or_normalize_map(or_real* m, const or_real* nmap, int N);
 void or_scale_dot(or_real* r, const or_real* a, const or_real* b, or_real s, int N);
 double or_reduce(int op, const or_real* x, int n);
 /* 6-neighbour exchange stencil with free (zero-halo) or periodic boundaries (gpu_exch1.cu:21-301) */
@@ -155,6 +156,8 @@ void or_sim_set_field_function(or_sim* s, int kind, const double* b, const doubl
 void or_sim_field_at(or_sim* s, double t_si, double* out);
 /* per-cell damping multiplier, N floats (NULL: none); copied */
 void or_sim_set_alpha_map(or_sim* s, const float* amap);
+/* per-cell norm of m after each normalisation, N floats (NULL: 1); copied */
+void or_sim_set_norm_map(or_sim* s, const float* nmap);
 /* total energy (internal units * cell volume), energy.go:38-52 */
 double or_sim_energy(or_sim* s);
 
