@@ -76,6 +76,7 @@ struct mmSim {
     float* fB = nullptr;
     float* alphaMap = nullptr;       // device copy of mmParams.alphaMap (N floats)
     float* normMap = nullptr;        // device copy of mmParams.normMap (N floats)
+    float* jMask = nullptr;          // device copy of mmParams.currentMask (3N floats)
     FieldTable field{};
     ConvPlan cp{};
     bool conv = false;

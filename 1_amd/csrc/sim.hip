@@ -426,8 +426,31 @@ static void abi_calc_heff(mmSim* s, float* m, float* h) {
     gpu_add_local_fields(m, h, N, hext, nullptr, s->p.anisType, &kk, ax);
 }
 
+static bool has_current(const mmParams& p) { return p.current[0] != 0 || p.current[1] != 0 || p.current[2] != 0; }
+
+// SpintorqueDeltaM (torque.go:64-91) with dt = 1, in the reference's float32:
+// b = P muB / (e (1 + xi^2)), beta = b (1 + alpha xi), eps = b (xi - alpha),
+// u = j / (2 cell), muB, e and j in internal units (material.go:83-110)
+static void abi_spintorque(mmSim* s, float* m, float* h) {
+    const float Ms = (float)s->p.msat, L = (float)s->unitLength, T = (float)s->unitTime;
+    const float alpha = (float)s->p.alpha;
+    const float dtGilb = 1.f / (1.f + alpha * alpha);
+    const float muB = 9.2740091523E-24f / (Ms * (L * L * L));  // init.go:84 / UnitMoment
+    const float e = 1.60217646E-19f / ((Ms * L) * T);          // init.go:85 / UnitCharge
+    const float P = (float)s->p.spinPol, xi = (float)s->p.xi;
+    const float b = P * muB / (e * 1 * (1 + xi * xi));
+    const float beta = b * (1 + alpha * xi), eps = b * (xi - alpha);
+    float u[3];
+    for (int i = 0; i < 3; i++) u[i] = 0.5f * ((float)s->p.current[i] / (Ms / L)) / (float)s->cell[i];
+    gpu_spintorque_deltaM(m, h, alpha, beta, eps, u, s->jMask, dtGilb, s->p.size[0], s->p.size[1], s->p.size[2]);
+}
+
 static void abi_torque(mmSim* s, float* m, float* h) {
-    // torque.go:33-53 with dt = 1
+    // torque.go:33-53 with dt = 1; the spin-transfer form with a current (:40-49)
+    if (has_current(s->p)) {
+        abi_spintorque(s, m, h);
+        return;
+    }
     const float alpha = (float)s->p.alpha;
     gpu_deltaM(m, h, alpha, s->alphaMap, 1.f / (1.f + alpha * alpha), s->N);
 }
@@ -623,6 +646,11 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     for (int i = 0; i < 3; i++) s->cell[i] = p->cellsize[i] / s->unitLength;
     s->kAnis = (float)(p->k1 / s->unitEdens);
     GK_ASSERT(p->temperature == 0 || mode == MM_MODE_FUSED, "a finite temperature needs the fused mode");
+    if (has_current(*p)) {
+        GK_ASSERT(mode == MM_MODE_ABI, "spin-transfer torque runs in MM_MODE_ABI");
+        GK_ASSERT(p->currentMask, "Spin-transfer torque requires a currentMask to be set (torque.go:68-71)");
+        GK_ASSERT(!p->alphaMap, "spin-transfer torque with a space-dependent alpha (torque.go:66)");
+    }
     set_thermal(s, p->temperature);
     for (int i = 0; i < 3; i++) s->u[i] = (float)p->anisAxis[i];
     s->tb = make_tableau(p->solver);
@@ -674,6 +702,11 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
         GK_CHECK(hipMemcpyAsync(s->normMap, p->normMap, (size_t)s->N * sizeof(float), hipMemcpyHostToDevice, s->s));
     }
     s->p.normMap = nullptr;
+    if (p->currentMask) {
+        s->jMask = (float*)dmalloc(s, (size_t)s->N3 * sizeof(float));
+        GK_CHECK(hipMemcpyAsync(s->jMask, p->currentMask, (size_t)s->N3 * sizeof(float), hipMemcpyHostToDevice, s->s));
+    }
+    s->p.currentMask = nullptr;
     // convolution (init.go:234-274)
     conv_plan_init(s->cp, p->size);
     s->bytes += s->cp.bytes;
@@ -702,7 +735,7 @@ void mm_sim_free(mmSim* s) {
     GK_CHECK(hipStreamSynchronize(s->s));
     for (int i = 0; i < 7; i++)
         cached_free(s->k[i]);
-    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap, s->normMap}) cached_free(p);
+    for (float* p : {s->m, s->mi, s->mtmp, s->fB, s->alphaMap, s->normMap, s->jMask}) cached_free(p);
     for (void* p : {(void*)s->st, (void*)s->log_dt, (void*)s->log_acc, (void*)s->log_err, (void*)s->dsum, (void*)s->fT,
                     (void*)s->part, (void*)s->epart})
         cached_free(p);
@@ -865,8 +898,12 @@ void mm_sim_heff(mmSim* s, const float* m, float* h) {
 
 void mm_sim_torque(mmSim* s, const float* m, float* k) {
     eval_heff(s, m, s->mtmp);
-    const float alpha = (float)s->p.alpha;
-    gpu_deltaM(s->mi, s->mtmp, alpha, s->alphaMap, (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha)), s->N);
+    if (has_current(s->p)) {
+        abi_spintorque(s, s->mi, s->mtmp);  // torque.go:40-49
+    } else {
+        const float alpha = (float)s->p.alpha;
+        gpu_deltaM(s->mi, s->mtmp, alpha, s->alphaMap, (float)(1.0 / (1.0 + s->p.alpha * s->p.alpha)), s->N);
+    }
     GK_CHECK(hipMemcpy(k, s->mtmp, s->N3 * sizeof(float), hipMemcpyDeviceToHost));
 }
 
@@ -971,6 +1008,7 @@ unsigned long long mm_sim_footprint(const mmParams* p) {
     if (p->nField >= 2) b += (size_t)p->nField * (sizeof(double) + 3 * sizeof(float));
     if (p->alphaMap) b += (size_t)(N3 / 3) * sizeof(float);
     if (p->normMap) b += (size_t)(N3 / 3) * sizeof(float);
+    if (p->currentMask) b += (size_t)N3 * sizeof(float);
     return b;
 }
 
@@ -1148,6 +1186,16 @@ void mm_sim_set_temperature(mmSim* s, double temperature) {
     GK_ASSERT(temperature == 0 || s->mode == MM_MODE_FUSED, "a finite temperature needs the fused mode");
     GK_CHECK(hipStreamSynchronize(s->s));
     set_thermal(s, temperature);
+}
+
+void mm_sim_set_spintorque(mmSim* s, const double j[3], double spinPol, double xi) {
+    const bool on = j[0] != 0 || j[1] != 0 || j[2] != 0;
+    GK_ASSERT(!on || s->mode == MM_MODE_ABI, "spin-transfer torque runs in MM_MODE_ABI");
+    GK_ASSERT(!on || s->jMask, "Spin-transfer torque requires a currentMask to be set (torque.go:68-71)");
+    GK_ASSERT(!on || !s->alphaMap, "spin-transfer torque with a space-dependent alpha (torque.go:66)");
+    for (int i = 0; i < 3; i++) s->p.current[i] = j[i];
+    s->p.spinPol = spinPol;
+    s->p.xi = xi;
 }
 
 void mm_sim_get_tune(mmSim* s, int params[7]) {

@@ -158,6 +158,9 @@ COMMANDS = {
     "setalpha": ("setalpha", "iiif"), "setalpharange": ("setalpharange", "iiiiiif"),
     "setmsat": ("setmsat", "iiif"), "setmsatrange": ("setmsatrange", "iiiiiif"),
     "setmsatellips": ("setmsatellips", "fffff"),
+    # spin-transfer torque (simspintorque.go)
+    "currentdensity": ("currentdensity", "fff"), "spinpolarization": ("spinpolarization", "f"),
+    "xi": ("xi", "f"), "currentmask": ("currentmask", "s"),
     # excitation (simfield.go)
     "applystatic": ("applystatic", "sfff"), "applypointwise": ("applypointwise", "sdfff"),
     "applypulse": ("applypulse", "sfffdd"), "applyrf": ("applyrf", "sfffd"), "applyrframp": ("applyrframp", "sfffdd"),
@@ -191,7 +194,7 @@ COMMANDS = {
 # Commands of the reference whose physics or geometry this engine does not
 # implement; they stop the script with an InputErr naming the command.
 UNSUPPORTED = (
-    "k2", "aniscubic", "spinpolarization", "xi", "currentdensity", "currentmask", "fieldmask",
+    "k2", "aniscubic", "fieldmask",
     "maxcellsize", "ellipsoid", "mask", "dotarray", "dotarrayrectangle", "dotarrayellips", "antidotarrayrectangle",
     "antidotarrayellips", "squareholearray", "vortex_in_array", "setm", "applyrotatingburst", "edgecorrection",
     "savebenchmark", "exchangetype",

@@ -106,6 +106,7 @@ ENGINE = {
     "mm_sim_set_field": (None, [P, I, P, P]),
     "mm_sim_set_field_function": (None, [P, I, P, P]),
     "mm_sim_set_temperature": (None, [P, C.c_double]),
+    "mm_sim_set_spintorque": (None, [P, P, C.c_double, C.c_double]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
     "mm_batch_set_graph": (None, [P, I]),
@@ -155,6 +156,10 @@ class Params(C.Structure):
         ("temperature", C.c_double),
         ("seed", C.c_ulonglong),
         ("normMap", C.POINTER(C.c_float)),
+        ("current", C.c_double * 3),
+        ("spinPol", C.c_double),
+        ("xi", C.c_double),
+        ("currentMask", C.POINTER(C.c_float)),
     ]
 
 

@@ -153,6 +153,10 @@ class Sim:
         self._norm_map = None
         self._temperature = 0.0
         self._thermal_seed = 0
+        self._current = (0.0, 0.0, 0.0)   # A/m^2, internal order
+        self._spin_pol = 0.0
+        self._xi = 0.0
+        self._jmask = None
         self._m0 = None
         # output (sim.go:179-209): schedule, table columns, output id
         self._outdir = "."
@@ -599,6 +603,50 @@ class Sim:
         m = np.ascontiguousarray(m, np.float32).reshape((3,) + self._need_size())
         self._m0 = m.copy()
 
+    # ------------------------------------------------------------ spin-transfer torque (simspintorque.go)
+    def _push_spintorque(self):
+        if self._ptr:
+            j = np.asarray(self._current, np.float64)
+            if j.any() and self._jmask is None:
+                raise InputErr("Spin-transfer torque requires a currentMask to be set")
+            self._lib.mm_sim_set_spintorque(self._ptr, j.ctypes.data, self._spin_pol, self._xi)
+
+    def currentdensity(self, jz, jy, jx):
+        """CurrentDensity(jz, jy, jx) (simspintorque.go:30-36), A/m^2: a nonzero
+        current makes every torque the spin-transfer (Zhang-Li) form
+        (torque.go:40-91); the ABI mode runs it.  Takes effect at the next
+        torque on a running simulation, as in the reference."""
+        f = np.float32
+        self._current = (float(f(jx)), float(f(jy)), float(f(jz)))
+        self._push_spintorque()
+
+    def spinpolarization(self, p):
+        """SpinPolarization(p) (simspintorque.go:24-27)."""
+        self._spin_pol = float(np.float32(p))
+        self._push_spintorque()
+
+    def xi(self, xi):
+        """Xi(xi), the degree of non-adiabaticity (simspintorque.go:19-22)."""
+        self._xi = float(np.float32(xi))
+        self._push_spintorque()
+
+    def currentmask_array(self, mask):
+        """The current mask, internal layout (3, X, Y, Z): component d scales the
+        current along axis d per cell (gpu_spintorque.cu:39-47)."""
+        self._invalidate()
+        self._jmask = np.ascontiguousarray(mask, np.float32).reshape((3,) + tuple(self._need_size())).copy()
+
+    def currentmask(self, file):
+        """CurrentMask(file) (simspintorque.go:39-52): the mask from an OVF file
+        of the grid size (the reference resamples other sizes; here they are an
+        input error)."""
+        from . import ovf
+
+        mask, meta = ovf.read_ovf(file)
+        if meta["size"] != tuple(self._need_size()):
+            raise InputErr("%s holds a %s grid, the simulation is %s" % (file, meta["size"], self._size))
+        self.currentmask_array(mask)
+
     def loadm(self, file):
         """Initial magnetisation from an OVF 1.0 / 2.0 file of the grid size
         (simconfig.go:291-297; the header's multiplier is not applied)."""
@@ -751,7 +799,7 @@ class Sim:
             b = self.field_at(t)
             row += [b[2], b[1], b[0]]
         if self._tab["j"]:
-            row += [0.0, 0.0, 0.0]
+            row += [self._current[2], self._current[1], self._current[0]]  # simoutput.go:244
         if self._tab["e"]:
             row += [self.energy_si()]
         if self._tab["maxdm/dt"]:
@@ -1065,6 +1113,18 @@ class Sim:
             amap = np.ascontiguousarray(self._alpha_map, np.float32)
             self._keep_alpha = amap
             p.alphaMap = amap.ctypes.data_as(C.POINTER(C.c_float))
+        if any(self._current):
+            if self.mode != "abi":
+                raise InputErr("spin-transfer torque runs in the ABI mode (Sim(mode='abi'))")
+            if self._jmask is None:
+                raise InputErr("Spin-transfer torque requires a currentMask to be set")
+            if self._alpha_map is not None:
+                raise InputErr("spin-transfer torque with a space-dependent alpha is not implemented (torque.go:66)")
+        p.current[:] = list(self._current)
+        p.spinPol, p.xi = self._spin_pol, self._xi
+        if self._jmask is not None:
+            self._keep_jmask = self._jmask
+            p.currentMask = self._jmask.ctypes.data_as(C.POINTER(C.c_float))
         if self._norm_map is not None:
             nmap = np.ascontiguousarray(self._norm_map, np.float32)
             self._keep_norm = nmap

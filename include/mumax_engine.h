@@ -84,6 +84,14 @@ typedef struct {
                                gpu_normalize.cu:43-71). Null: 1 everywhere. <m> still
                                divides by the cell count (avgNorm, init.go:135).
                                Copied at mm_sim_new. */
+    double current[3];      /* current density, A/m^2, internal order (CurrentDensity,
+                               simspintorque.go:30-36). Nonzero: every torque is the
+                               spin-transfer (Zhang-Li) form (torque.go:40-91,
+                               gpu_spintorque.cu). MM_MODE_ABI only */
+    double spinPol, xi;     /* SpinPolarization, Xi (simspintorque.go:19-28) */
+    const float* currentMask; /* 3N host floats (CurrentMask): each component of the
+                               current scaled per cell; required with a current, as in
+                               the reference (torque.go:68-71). Copied at mm_sim_new. */
 } mmParams;
 
 typedef struct mmSim mmSim;
@@ -223,6 +231,9 @@ void mm_sim_set_field(mmSim* s, int n, const double* t, const double* b);
  * standard deviation from the next trial on (fused mode); a batch built before
  * keeps the temperature it was built with. */
 void mm_sim_set_temperature(mmSim* s, double temperature);
+/* CurrentDensity / SpinPolarization / Xi on a live simulation (ABI mode; the
+ * mask stays the one given at creation): j[3] in A/m^2, internal order. */
+void mm_sim_set_spintorque(mmSim* s, const double j[3], double spinPol, double xi);
 /* Replace the applied field by a function of time (MM_FIELD_*), amplitude b[3]
  * in tesla (internal X, Y, Z), params[4]: PULSE (duration s, rise time s),
  * RF (frequency Hz), RF_RAMP (frequency, ramp time), SAWTOOTH (frequency),

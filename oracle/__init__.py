@@ -108,6 +108,7 @@ class Oracle:
         L.or_sim_set_want_energy.argtypes = [P, C.c_int]
         L.or_sim_set_alpha_map.argtypes = [P, P]
         L.or_sim_set_norm_map.argtypes = [P, P]
+        L.or_sim_set_spintorque.argtypes = [P, P, C.c_double, C.c_double, P]
         L.or_normalize_map.argtypes = [P, P, C.c_int]
         L.or_sim_set_field_function.argtypes = [P, C.c_int, P, P]
         L.or_sim_field_at.argtypes = [P, C.c_double, P]
@@ -384,6 +385,14 @@ class Sim:
         """per-cell damping multiplier (internal [X][Y][Z] order), None: none"""
         self._amap = None if amap is None else np.ascontiguousarray(amap, np.float32).reshape(-1)
         self.orc.lib.or_sim_set_alpha_map(self.ptr, None if self._amap is None else self._amap.ctypes.data)
+
+    def set_spintorque(self, j, spin_pol, xi, jmask=None):
+        """spin-transfer torque: current density j (A/m^2, internal order), spin
+        polarisation, non-adiabaticity xi, mask (3, X, Y, Z) or None (1)"""
+        jj = np.ascontiguousarray(j, np.float64)
+        self._jmask = None if jmask is None else np.ascontiguousarray(jmask, np.float32).reshape(-1)
+        self.orc.lib.or_sim_set_spintorque(self.ptr, jj.ctypes.data, float(spin_pol), float(xi),
+                                           None if self._jmask is None else self._jmask.ctypes.data)
 
     def set_norm_map(self, nmap):
         """per-cell norm of m after each normalisation (internal [X][Y][Z] order), None: 1"""

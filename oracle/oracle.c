@@ -590,6 +590,41 @@ void or_normalize(or_real* m, int N) {
     }
 }
 
+/* gpu_spintorque.cu:24-180 (Zhang-Li torque): Hspin = sum_d u_d (m(+1) - m(-1))
+ * along each axis, the cell itself standing in for a missing neighbour at the
+ * box edges (:58-83), u_d scaled by the mask's component d of the cell (NULL: 1);
+ * then h <- dtg (m x (H + eps Hspin) - m x (m x (alpha H + beta Hspin))) */
+void or_spintorque_deltaM(const or_real* m, or_real* h, double alpha, double beta, double eps, const double u[3],
+                          const or_real* jmap, double dtg, const int n[3]) {
+    const long N = (long)n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
+    for (long I = 0; I < N; I++) {
+        const int k = (int)(I % n[2]);
+        const long r = I / n[2];
+        const int j = (int)(r % n[1]), i = (int)(r / n[1]);
+        double ud[3] = {u[0], u[1], u[2]};
+        if (jmap)
+            for (int d = 0; d < 3; d++) ud[d] *= jmap[d * N + I];
+        const long st[3] = {(long)n[1] * n[2], n[2], 1};
+        const int at[3] = {i, j, k};
+        double hs[3] = {0, 0, 0};
+        for (int d = 0; d < 3; d++) {
+            const long lo = at[d] > 0 ? I - st[d] : I, hi = at[d] + 1 < n[d] ? I + st[d] : I;
+            for (int c = 0; c < 3; c++) hs[c] += ud[d] * ((double)m[c * N + hi] - (double)m[c * N + lo]);
+        }
+        const double M[3] = {m[I], m[N + I], m[2 * N + I]}, H[3] = {h[I], h[N + I], h[2 * N + I]};
+        double p[3], q[3], a[3], c[3], e[3];
+        for (int x = 0; x < 3; x++) {
+            p[x] = H[x] + eps * hs[x];
+            q[x] = alpha * H[x] + beta * hs[x];
+        }
+        a[0] = M[1] * p[2] - M[2] * p[1]; a[1] = M[2] * p[0] - M[0] * p[2]; a[2] = M[0] * p[1] - M[1] * p[0];
+        c[0] = M[1] * q[2] - M[2] * q[1]; c[1] = M[2] * q[0] - M[0] * q[2]; c[2] = M[0] * q[1] - M[1] * q[0];
+        e[0] = M[1] * c[2] - M[2] * c[1]; e[1] = M[2] * c[0] - M[0] * c[2]; e[2] = M[0] * c[1] - M[1] * c[0];
+        for (int x = 0; x < 3; x++) h[x * N + I] = (or_real)(dtg * (a[x] - e[x]));
+    }
+}
+
 /* gpu_normalize.cu:43-71 (_gpu_normalize_map): each cell to the norm nmap[i];
  * a cell whose norm is 0 becomes 0 (the kernel's special case, so m = 0 there
  * gives 0 rather than 0/0) */
@@ -722,6 +757,9 @@ struct or_sim {
     long log_n, log_cap;
     or_real* amap; /* per-cell damping multiplier (SetAlpha, simgeom.go:246-264), NULL: none */
     or_real* nmap; /* per-cell norm of m (SetMsat, simgeom.go:266-340), NULL: 1 */
+    double j[3];   /* current density, A/m^2 (CurrentDensity, simspintorque.go:30-36) */
+    double spinPol, xi;
+    or_real* jmask; /* 3N current mask (CurrentMask), NULL: 1 */
     int fn;        /* applied-field function of time (1 pulse, 2 RF, 3 RF ramp, 4 sawtooth, 5 rotating), 0: table */
     double fp[4], fb[3];
     or_real* noise; /* thermal field of the current step, 3N (NULL at T = 0) */
@@ -945,6 +983,7 @@ void or_sim_free(or_sim* s) {
     free(s->log_dt); free(s->log_acc);
     free(s->amap);
     free(s->nmap);
+    free(s->jmask);
     free(s->noise);
     free(s);
 }
@@ -981,6 +1020,18 @@ static void calc_heff(or_sim* s, const or_real* m, or_real* h, int energy) {
 
 static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53,98-102 */
     double a = s->p.alpha;
+    if (s->j[0] != 0 || s->j[1] != 0 || s->j[2] != 0) { /* torque.go:40-49: the STT form */
+        /* SpintorqueDeltaM (torque.go:64-91) with dt = 1; units material.go:83-110 */
+        const double L = s->units[0], T = s->units[1], Ms = s->p.msat;
+        const double muB = 9.2740091523E-24 / (Ms * L * L * L); /* init.go:84, / UnitMoment */
+        const double e = 1.60217646E-19 / (Ms * L * T);         /* init.go:85, / UnitCharge */
+        const double xi = s->xi;
+        const double b = s->spinPol * muB / (e * (1 + xi * xi));
+        double u[3];
+        for (int i = 0; i < 3; i++) u[i] = 0.5 * (s->j[i] / (Ms / L)) / (double)s->cell[i];
+        or_spintorque_deltaM(m, h, a, b * (1 + a * xi), b * (xi - a), u, s->jmask, 1.0 / (1.0 + a * a), s->p.size);
+        return;
+    }
     or_deltaM_map(m, h, (or_real)a, s->amap, (or_real)(1.0 / (1.0 + a * a)), s->N);
 }
 
@@ -1165,6 +1216,18 @@ void or_sim_set_alpha_map(or_sim* s, const float* amap) {
     s->amap = (or_real*)malloc((size_t)s->N * sizeof(or_real));
     if (!s->amap) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
     for (int i = 0; i < s->N; i++) s->amap[i] = (or_real)amap[i];
+}
+
+void or_sim_set_spintorque(or_sim* s, const double j[3], double spinPol, double xi, const float* jmask) {
+    for (int i = 0; i < 3; i++) s->j[i] = j[i];
+    s->spinPol = spinPol;
+    s->xi = xi;
+    free(s->jmask);
+    s->jmask = NULL;
+    if (!jmask) return;
+    s->jmask = (or_real*)malloc(3 * (size_t)s->N * sizeof(or_real));
+    if (!s->jmask) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    for (int i = 0; i < 3 * s->N; i++) s->jmask[i] = (or_real)jmask[i];
 }
 
 void or_sim_set_norm_map(or_sim* s, const float* nmap) {

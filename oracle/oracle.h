@@ -156,6 +156,12 @@ void or_sim_set_field_function(or_sim* s, int kind, const double* b, const doubl
 void or_sim_field_at(or_sim* s, double t_si, double* out);
 /* per-cell damping multiplier, N floats (NULL: none); copied */
 void or_sim_set_alpha_map(or_sim* s, const float* amap);
+/* spin-transfer torque (CurrentDensity, SpinPolarization, Xi, CurrentMask;
+ * simspintorque.go): current density j (A/m^2, internal order), 3N mask floats
+ * (NULL: 1); copied.  A nonzero j selects the STT torque (torque.go:40-49) */
+void or_sim_set_spintorque(or_sim* s, const double j[3], double spinPol, double xi, const float* jmask);
+void or_spintorque_deltaM(const or_real* m, or_real* h, double alpha, double beta, double eps, const double u[3],
+                          const or_real* jmap, double dtg, const int n[3]);
 /* per-cell norm of m after each normalisation, N floats (NULL: 1); copied */
 void or_sim_set_norm_map(or_sim* s, const float* nmap);
 /* total energy (internal units * cell volume), energy.go:38-52 */
