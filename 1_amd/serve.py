@@ -141,10 +141,12 @@ class Server:
     (``mm_batch_remaining``) exceed `lane_factor` times the group's median
     move to the class's lane, a batch of their own whose trials are cheap, so
     the few simulations that need many more steps than the rest no longer set
-    the pace of everything else (lane_factor=None: no lanes)."""
+    the pace of everything else (lane_factor=None: no lanes).  A lane enqueues
+    `lane_poll` times `poll` trials at a time: its few members' trials are
+    cheap, and they set the end of the run."""
 
     def __init__(self, hbm_bytes=HBM_BYTES, poll=16, graph=True, keep_results=False, split_cells=(16384, 131072),
-                 lane_factor=3.0, lane_floor=1024, lane_after=4):
+                 lane_factor=3.0, lane_floor=1024, lane_after=4, lane_poll=4):
         self.hbm_bytes = hbm_bytes
         self.poll = poll
         self.graph = graph
@@ -153,6 +155,7 @@ class Server:
         self.lane_factor = lane_factor
         self.lane_floor = lane_floor
         self.lane_after = lane_after
+        self.lane_poll = max(1, int(lane_poll))
         self.results = {}
 
     def _group_of(self, r):
@@ -250,10 +253,11 @@ class Server:
                     g.dirty = False
                     g.polls = 0
                 if g.batch is not None:
-                    g.batch.enqueue(self.poll)
+                    n = self.poll * (self.lane_poll if g.lane else 1)
+                    g.batch.enqueue(n)
                     g.busy = True
                     g.polls += 1
-                    trials += self.poll
+                    trials += n
                     launched = True
             if not launched:
                 if any(g.busy for g in groups):

@@ -434,7 +434,8 @@ def bench_serve(args, mm):
 
     for r in reqs:
         r.on_finish = progress
-    srv = sv.Server(poll=args.poll, graph=not args.no_graph, lane_factor=None if args.no_lanes else args.lane_factor)
+    srv = sv.Server(poll=args.poll, graph=not args.no_graph, lane_factor=None if args.no_lanes else args.lane_factor,
+                    lane_poll=args.lane_poll)
     stats = srv.serve(reqs, max_seconds=args.serve_max_seconds)
     print("bench.py serve: %d requests served in batches in %.2f s" % (len(reqs), stats["elapsed"]), file=sys.stderr,
           flush=True)
@@ -526,6 +527,7 @@ def main():
                     help="keep long-running members in their size group (no lanes of their own)")
     ap.add_argument("--lane-factor", type=float, default=3.0,
                     help="members with more than this times their group's median remaining trials move to a lane")
+    ap.add_argument("--lane-poll", type=int, default=4, help="serving: a lane enqueues this many times --poll trials")
     ap.add_argument("--serial-sample", type=int, default=32, help="sweep members timed alone for the serial rate")
     ap.add_argument("--no-sweep", action="store_true", help="cuboid workload: leave out the sweep object")
     ap.add_argument("--no-serve", action="store_true", help="cuboid workload: leave out the serve object")
