@@ -72,6 +72,15 @@ struct mmSim {
     bool wantEnergy = false;
     int fieldCap = 0;                // points in fT / fB
     double fieldTmaxHost = -INFINITY;  // ABI mode: latest time the field was evaluated at
+    // ABI mode: an applied current density (Apply*("j", ...)), evaluated on the host
+    // at each torque's stage time with its own forward-only search
+    bool curOn = false;
+    int curN = 0, curFn = 0;
+    std::vector<double> curT;
+    std::vector<float> curV;  // 3 per point, A/m^2, internal order
+    double curFp[4] = {0, 0, 0, 0};
+    float curFb[3] = {0, 0, 0};
+    double curTmaxHost = -INFINITY;
     double* fT = nullptr;
     float* fB = nullptr;
     float* alphaMap = nullptr;       // device copy of mmParams.alphaMap (N floats)

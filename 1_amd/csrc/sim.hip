@@ -427,6 +427,24 @@ static void abi_calc_heff(mmSim* s, float* m, float* h) {
 }
 
 static bool has_current(const mmParams& p) { return p.current[0] != 0 || p.current[1] != 0 || p.current[2] != 0; }
+static bool has_current(const mmSim* s) { return s->curOn || has_current(s->p); }
+
+// s.input.j for this torque: the applied current at the stage time when there is
+// one (torque.go:42-44, simfield.go:87-260), else CurrentDensity's value
+static void current_at(mmSim* s, float j[3]) {
+    if (!s->curOn) {
+        for (int i = 0; i < 3; i++) j[i] = (float)s->p.current[i];
+        return;
+    }
+    s->curTmaxHost = std::max(s->curTmaxHost, s->htime);
+    const double t = s->htime * s->unitTime;
+    if (s->curFn) {
+        field_function(s->curFn, s->curFp, s->curFb, t, j);
+        return;
+    }
+    const float* v = s->curV.data();
+    applied_field_at(s->curN, s->curT.data(), v, v, t, s->curTmaxHost * s->unitTime, j);
+}
 
 // SpintorqueDeltaM (torque.go:64-91) with dt = 1, in the reference's float32:
 // b = P muB / (e (1 + xi^2)), beta = b (1 + alpha xi), eps = b (xi - alpha),
@@ -440,14 +458,15 @@ static void abi_spintorque(mmSim* s, float* m, float* h) {
     const float P = (float)s->p.spinPol, xi = (float)s->p.xi;
     const float b = P * muB / (e * 1 * (1 + xi * xi));
     const float beta = b * (1 + alpha * xi), eps = b * (xi - alpha);
-    float u[3];
-    for (int i = 0; i < 3; i++) u[i] = 0.5f * ((float)s->p.current[i] / (Ms / L)) / (float)s->cell[i];
+    float j[3], u[3];
+    current_at(s, j);
+    for (int i = 0; i < 3; i++) u[i] = 0.5f * (j[i] / (Ms / L)) / (float)s->cell[i];
     gpu_spintorque_deltaM(m, h, alpha, beta, eps, u, s->jMask, dtGilb, s->p.size[0], s->p.size[1], s->p.size[2]);
 }
 
 static void abi_torque(mmSim* s, float* m, float* h) {
     // torque.go:33-53 with dt = 1; the spin-transfer form with a current (:40-49)
-    if (has_current(s->p)) {
+    if (has_current(s)) {
         abi_spintorque(s, m, h);
         return;
     }
@@ -898,7 +917,7 @@ void mm_sim_heff(mmSim* s, const float* m, float* h) {
 
 void mm_sim_torque(mmSim* s, const float* m, float* k) {
     eval_heff(s, m, s->mtmp);
-    if (has_current(s->p)) {
+    if (has_current(s)) {
         abi_spintorque(s, s->mi, s->mtmp);  // torque.go:40-49
     } else {
         const float alpha = (float)s->p.alpha;
@@ -1196,6 +1215,26 @@ void mm_sim_set_spintorque(mmSim* s, const double j[3], double spinPol, double x
     for (int i = 0; i < 3; i++) s->p.current[i] = j[i];
     s->p.spinPol = spinPol;
     s->p.xi = xi;
+}
+
+void mm_sim_set_current(mmSim* s, int n, const double* t, const double* j, int kind, const double* params) {
+    const bool on = kind > 0 || n > 0;
+    GK_ASSERT(!on || s->mode == MM_MODE_ABI, "spin-transfer torque runs in MM_MODE_ABI");
+    GK_ASSERT(!on || s->jMask, "Spin-transfer torque requires a currentMask to be set (torque.go:68-71)");
+    GK_ASSERT(!on || !s->alphaMap, "spin-transfer torque with a space-dependent alpha (torque.go:66)");
+    GK_ASSERT(kind >= 0 && kind <= MM_FIELD_ROTATING && n >= 0 && (kind || n <= 1 || t), "mm_sim_set_current: %d %d",
+              kind, n);
+    s->curOn = on;
+    s->curFn = kind;
+    s->curN = kind ? 0 : n;
+    s->curT.assign(t && !kind && n > 1 ? t : nullptr, t && !kind && n > 1 ? t + n : nullptr);
+    s->curV.resize(kind ? 0 : 3 * (size_t)n);
+    for (size_t i = 0; i < s->curV.size(); i++) s->curV[i] = (float)j[i];
+    if (kind) {
+        for (int i = 0; i < 4; i++) s->curFp[i] = params[i];
+        for (int c = 0; c < 3; c++) s->curFb[c] = (float)j[c];
+    }
+    s->curTmaxHost = -INFINITY;
 }
 
 void mm_sim_get_tune(mmSim* s, int params[7]) {

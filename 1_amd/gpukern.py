@@ -107,6 +107,7 @@ ENGINE = {
     "mm_sim_set_field_function": (None, [P, I, P, P]),
     "mm_sim_set_temperature": (None, [P, C.c_double]),
     "mm_sim_set_spintorque": (None, [P, P, C.c_double, C.c_double]),
+    "mm_sim_set_current": (None, [P, I, P, P, I, P]),
     "mm_batch_new": (P, [P, I]),
     "mm_batch_free": (None, [P]),
     "mm_batch_set_graph": (None, [P, I]),

@@ -157,6 +157,9 @@ class Sim:
         self._spin_pol = 0.0
         self._xi = 0.0
         self._jmask = None
+        self._jstatic = None   # applied current density (Apply*("j", ...)): static, points, function
+        self._jfield = []
+        self._jfn = None
         self._m0 = None
         # output (sim.go:179-209): schedule, table columns, output id
         self._outdir = "."
@@ -228,9 +231,22 @@ class Sim:
     # ------------------------------------------------------------ fields (simfield.go)
     # Applying a field does not invalidate a running simulation
     # (simfield.go:87-110): the engine's field table is replaced in place.
+    @staticmethod
+    def _is_current(what):
+        # apply (simfield.go:61-70): the field, or the current density
+        w = str(what).lower()
+        if w in ("currentdensity", "j"):
+            return True
+        if w in ("field", "b"):
+            return False
+        raise InputErr("Illegal Argument: %s Options are: [field (or B), currentdensity (or j)]" % what)
+
     def applystatic(self, what, hz, hy, hx):
-        if what.lower() not in ("field", "b"):
-            raise InputErr("only 'field' can be applied")
+        if self._is_current(what):
+            f = np.float32
+            self._jstatic, self._jfield, self._jfn = (float(f(hx)), float(f(hy)), float(f(hz))), [], None
+            self._push_current()
+            return
         self._static = (hx, hy, hz)
         self._field = []
         self._fn = None
@@ -238,8 +254,11 @@ class Sim:
 
     # functions of time (simfield.go:155-260); amplitude in the reference's (hz, hy, hx) order
     def _apply_fn(self, what, kind, hz, hy, hx, params):
-        if what.lower() not in ("field", "b"):
-            raise InputErr("only 'field' can be applied")
+        if self._is_current(what):
+            self._jstatic, self._jfield = None, []
+            self._jfn = (kind, (float(hx), float(hy), float(hz)), tuple(float(p) for p in params) + (0.0,) * (4 - len(params)))
+            self._push_current()
+            return
         self._static = None
         self._field = []
         self._fn = (kind, (float(hx), float(hy), float(hz)), tuple(float(p) for p in params) + (0.0,) * (4 - len(params)))
@@ -266,8 +285,13 @@ class Sim:
         self._apply_fn(what, 5, hz, hy, hx, [freq, phasex, phasey, phasez])
 
     def applypointwise(self, what, t, hz, hy, hx):
-        if what.lower() not in ("field", "b"):
-            raise InputErr("only 'field' can be applied")
+        if self._is_current(what):
+            if self._jstatic is not None or self._jfn is not None:
+                self._jfield = []
+            self._jstatic, self._jfn = None, None
+            self._jfield.append((float(t), (float(hx), float(hy), float(hz))))
+            self._push_current()
+            return
         if self._static is not None:
             self._field = []
         self._static = None
@@ -611,6 +635,47 @@ class Sim:
                 raise InputErr("Spin-transfer torque requires a currentMask to be set")
             self._lib.mm_sim_set_spintorque(self._ptr, j.ctypes.data, self._spin_pol, self._xi)
 
+    def _applied_current(self):
+        return self._jfn is not None or self._jstatic is not None or len(self._jfield) >= 2
+
+    def _push_current(self):
+        # like the field: a pointwise current needs two points (simfield.go:123-125)
+        if not self._ptr or not (self._applied_current() or not self._jfield):
+            return
+        if self._applied_current() and self._jmask is None:
+            raise InputErr("Spin-transfer torque requires a currentMask to be set")
+        prm = np.zeros(4)
+        if self._jfn is not None:
+            kind, amp, p4 = self._jfn
+            j, t, n, prm = np.asarray(amp, np.float64), None, 0, np.asarray(p4, np.float64)
+        elif self._jstatic is not None:
+            kind, j, t, n = 0, np.asarray(self._jstatic, np.float64), np.zeros(1), 1
+        else:
+            kind, n = 0, len(self._jfield)
+            t = np.asarray([p[0] for p in self._jfield], np.float64)
+            j = np.asarray([p[1] for p in self._jfield], np.float64).reshape(-1)
+            if n == 0:
+                t = j = None
+        self._lib.mm_sim_set_current(self._ptr, n, None if t is None else t.ctypes.data,
+                                     None if j is None else j.ctypes.data, kind, prm.ctypes.data)
+
+    def current_at(self, t):
+        """The current density in A/m^2 (internal X, Y, Z) the torque uses at
+        time t: the applied one (simfield.go:87-260) or CurrentDensity's."""
+        if self._jfn is not None:
+            return field_function(self._jfn[0], self._jfn[2], self._jfn[1], t)
+        if self._jstatic is not None:
+            return np.array(self._jstatic, float)
+        if len(self._jfield) < 2:
+            return np.array(self._current, float)
+        ts = [f[0] for f in self._jfield]
+        i = int(np.searchsorted(ts, t, side="left"))
+        if i == 0 or i >= len(ts):
+            return np.zeros(3)
+        (t0, b0), (t1, b1) = self._jfield[i - 1], self._jfield[i]
+        w = (t - t0) / (t1 - t0)
+        return np.asarray(b0, float) + w * (np.asarray(b1, float) - np.asarray(b0, float))
+
     def currentdensity(self, jz, jy, jx):
         """CurrentDensity(jz, jy, jx) (simspintorque.go:30-36), A/m^2: a nonzero
         current makes every torque the spin-transfer (Zhang-Li) form
@@ -799,7 +864,8 @@ class Sim:
             b = self.field_at(t)
             row += [b[2], b[1], b[0]]
         if self._tab["j"]:
-            row += [self._current[2], self._current[1], self._current[0]]  # simoutput.go:244
+            j = self.current_at(t)
+            row += [j[2], j[1], j[0]]  # simoutput.go:244
         if self._tab["e"]:
             row += [self.energy_si()]
         if self._tab["maxdm/dt"]:
@@ -1113,7 +1179,7 @@ class Sim:
             amap = np.ascontiguousarray(self._alpha_map, np.float32)
             self._keep_alpha = amap
             p.alphaMap = amap.ctypes.data_as(C.POINTER(C.c_float))
-        if any(self._current):
+        if any(self._current) or self._applied_current():
             if self.mode != "abi":
                 raise InputErr("spin-transfer torque runs in the ABI mode (Sim(mode='abi'))")
             if self._jmask is None:
@@ -1154,6 +1220,8 @@ class Sim:
             raise RuntimeError("mm_sim_new failed")
         if self._fn is not None:
             self._push_field()
+        if self._applied_current():
+            self._push_current()
 
 
 class Batch:

@@ -234,6 +234,15 @@ void mm_sim_set_temperature(mmSim* s, double temperature);
 /* CurrentDensity / SpinPolarization / Xi on a live simulation (ABI mode; the
  * mask stays the one given at creation): j[3] in A/m^2, internal order. */
 void mm_sim_set_spintorque(mmSim* s, const double j[3], double spinPol, double xi);
+/* ApplyStatic / ApplyPointwise / ApplyRf ... ("j", ...) on an ABI-mode
+ * simulation (simfield.go:61-70): the current density as n points (t[n]
+ * seconds, j[3n] A/m^2 internal order; n = 1 static; n = 0 and kind = 0: off,
+ * back to CurrentDensity's value) or, kind = MM_FIELD_*, a function of time of
+ * amplitude j[3] with params[4] as mm_sim_set_field_function.  Evaluated at each
+ * torque's stage time with its own forward-only search, it replaces the current
+ * density there; any applied current selects the spin-transfer torque
+ * (torque.go:40-45).  Needs the current mask. */
+void mm_sim_set_current(mmSim* s, int n, const double* t, const double* j, int kind, const double* params);
 /* Replace the applied field by a function of time (MM_FIELD_*), amplitude b[3]
  * in tesla (internal X, Y, Z), params[4]: PULSE (duration s, rise time s),
  * RF (frequency Hz), RF_RAMP (frequency, ramp time), SAWTOOTH (frequency),

@@ -109,6 +109,7 @@ class Oracle:
         L.or_sim_set_alpha_map.argtypes = [P, P]
         L.or_sim_set_norm_map.argtypes = [P, P]
         L.or_sim_set_spintorque.argtypes = [P, P, C.c_double, C.c_double, P]
+        L.or_sim_set_applied_current.argtypes = [P, C.c_int, P, P, C.c_int, P]
         L.or_normalize_map.argtypes = [P, P, C.c_int]
         L.or_sim_set_field_function.argtypes = [P, C.c_int, P, P]
         L.or_sim_field_at.argtypes = [P, C.c_double, P]
@@ -393,6 +394,18 @@ class Sim:
         self._jmask = None if jmask is None else np.ascontiguousarray(jmask, np.float32).reshape(-1)
         self.orc.lib.or_sim_set_spintorque(self.ptr, jj.ctypes.data, float(spin_pol), float(xi),
                                            None if self._jmask is None else self._jmask.ctypes.data)
+
+    def set_applied_current(self, t=None, j=None, kind=0, params=(0.0, 0.0, 0.0, 0.0)):
+        """an applied current density: points t (s) and j (n x 3 A/m^2, internal
+        order; one point: static; none: off) or, kind > 0, a function of time
+        with amplitude j (3,)"""
+        self._jT = None if t is None else np.ascontiguousarray(t, np.float64)
+        self._jV = None if j is None else np.ascontiguousarray(j, np.float64).reshape(-1)
+        self._jP = np.ascontiguousarray(params, np.float64)
+        n = 0 if self._jT is None else len(self._jT)
+        self.orc.lib.or_sim_set_applied_current(self.ptr, n, None if self._jT is None else self._jT.ctypes.data,
+                                                None if self._jV is None else self._jV.ctypes.data, int(kind),
+                                                self._jP.ctypes.data)
 
     def set_norm_map(self, nmap):
         """per-cell norm of m after each normalisation (internal [X][Y][Z] order), None: 1"""

@@ -760,6 +760,9 @@ struct or_sim {
     double j[3];   /* current density, A/m^2 (CurrentDensity, simspintorque.go:30-36) */
     double spinPol, xi;
     or_real* jmask; /* 3N current mask (CurrentMask), NULL: 1 */
+    int japplied;   /* an applied current (Apply*("j", ...), simfield.go:61-70) replaces j at each torque */
+    int jn, jfn, jLastIdx;
+    double *jT, *jV, jfp[4], jfb[3];
     int fn;        /* applied-field function of time (1 pulse, 2 RF, 3 RF ramp, 4 sawtooth, 5 rotating), 0: table */
     double fp[4], fb[3];
     or_real* noise; /* thermal field of the current step, 3N (NULL at T = 0) */
@@ -893,12 +896,12 @@ static void set_tableau(or_sim* s) {
 /* pulsedField, rfField, rfRamp, sawtooth, rotatingField .GetAppliedField
  * (simfield.go:155-260): scale in double, rounded to float32, times the
  * float32 amplitude */
-static void field_function(const or_sim* s, double t, double B[3]) {
-    const double pi = 3.14159265358979323846, f = s->fp[0];
+static void field_function(int fn, const double fp[4], const double fb[3], double t, double B[3]) {
+    const double pi = 3.14159265358979323846, f = fp[0];
     double sc[3] = {0, 0, 0};
-    switch (s->fn) {
+    switch (fn) {
     case 1: {
-        const double d = s->fp[0], r = s->fp[1];
+        const double d = fp[0], r = fp[1];
         double v = 0;
         if (t > 0 && t < r) v = 0.5 - 0.5 * cos(t * pi / r);
         else if (t >= r && t < d + r) v = 1;
@@ -907,25 +910,32 @@ static void field_function(const or_sim* s, double t, double B[3]) {
         break;
     }
     case 2: sc[0] = sc[1] = sc[2] = sin(f * 2 * pi * t); break;
-    case 3: sc[0] = sc[1] = sc[2] = sin(f * 2 * pi * t) * (t < s->fp[1] ? t / s->fp[1] : 1.0); break;
+    case 3: sc[0] = sc[1] = sc[2] = sin(f * 2 * pi * t) * (t < fp[1] ? t / fp[1] : 1.0); break;
     case 4: sc[0] = sc[1] = sc[2] = asin(sin(f * 2 * pi * t)) / (pi / 2); break;
-    case 5: for (int c = 0; c < 3; c++) sc[c] = sin(f * 2 * pi * t + s->fp[1 + c]); break;
+    case 5: for (int c = 0; c < 3; c++) sc[c] = sin(f * 2 * pi * t + fp[1 + c]); break;
     }
-    for (int c = 0; c < 3; c++) B[c] = (double)((float)sc[c] * (float)s->fb[c]);
+    for (int c = 0; c < 3; c++) B[c] = (double)((float)sc[c] * (float)fb[c]);
+}
+
+/* an AppliedField (simfield.go:87-260): a function of time, a static value (n = 1)
+ * or n points interpolated linearly with the forward-only search index *last */
+static void applied_eval(int n, const double* T, const double* V, int* last, int fn, const double fp[4],
+                         const double fb[3], double t, double B[3]) {
+    B[0] = B[1] = B[2] = 0;
+    if (fn) { field_function(fn, fp, fb, t, B); return; }
+    if (n == 0) return;
+    if (n == 1) { for (int i = 0; i < 3; i++) B[i] = V[i]; return; }
+    for (; *last < n; (*last)++)
+        if (T[*last] >= t) break;
+    int i = *last;
+    if (i - 1 < 0 || i >= n) return;
+    double dt = T[i] - T[i - 1];
+    double f = (t - T[i - 1]) / dt;
+    for (int c = 0; c < 3; c++) B[c] = V[3 * (i - 1) + c] + f * (V[3 * i + c] - V[3 * (i - 1) + c]);
 }
 
 static void applied_field(or_sim* s, double t, double B[3]) {
-    B[0] = B[1] = B[2] = 0;
-    if (s->fn) { field_function(s, t, B); return; }
-    if (s->p.nField == 0) return;
-    if (s->p.nField == 1) { for (int i = 0; i < 3; i++) B[i] = s->fieldB[i]; return; }
-    for (; s->fieldLastIdx < s->p.nField; s->fieldLastIdx++)
-        if (s->fieldT[s->fieldLastIdx] >= t) break;
-    int i = s->fieldLastIdx;
-    if (i - 1 < 0 || i >= s->p.nField) return;
-    double dt = s->fieldT[i] - s->fieldT[i - 1];
-    double f = (t - s->fieldT[i - 1]) / dt;
-    for (int c = 0; c < 3; c++) B[c] = s->fieldB[3 * (i - 1) + c] + f * (s->fieldB[3 * i + c] - s->fieldB[3 * (i - 1) + c]);
+    applied_eval(s->p.nField, s->fieldT, s->fieldB, &s->fieldLastIdx, s->fn, s->fp, s->fb, t, B);
 }
 
 or_sim* or_sim_new(const or_params* p, const float* m0) {
@@ -984,6 +994,8 @@ void or_sim_free(or_sim* s) {
     free(s->amap);
     free(s->nmap);
     free(s->jmask);
+    free(s->jT);
+    free(s->jV);
     free(s->noise);
     free(s);
 }
@@ -1020,7 +1032,10 @@ static void calc_heff(or_sim* s, const or_real* m, or_real* h, int energy) {
 
 static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53,98-102 */
     double a = s->p.alpha;
-    if (s->j[0] != 0 || s->j[1] != 0 || s->j[2] != 0) { /* torque.go:40-49: the STT form */
+    double j[3] = {s->j[0], s->j[1], s->j[2]};
+    if (s->japplied) /* s.input.j from the applied current at the stage time (torque.go:42-44) */
+        applied_eval(s->jn, s->jT, s->jV, &s->jLastIdx, s->jfn, s->jfp, s->jfb, s->time * s->units[1], j);
+    if (s->japplied || j[0] != 0 || j[1] != 0 || j[2] != 0) { /* torque.go:40-49: the STT form */
         /* SpintorqueDeltaM (torque.go:64-91) with dt = 1; units material.go:83-110 */
         const double L = s->units[0], T = s->units[1], Ms = s->p.msat;
         const double muB = 9.2740091523E-24 / (Ms * L * L * L); /* init.go:84, / UnitMoment */
@@ -1028,7 +1043,7 @@ static void torque(or_sim* s, const or_real* m, or_real* h) { /* torque.go:33-53
         const double xi = s->xi;
         const double b = s->spinPol * muB / (e * (1 + xi * xi));
         double u[3];
-        for (int i = 0; i < 3; i++) u[i] = 0.5 * (s->j[i] / (Ms / L)) / (double)s->cell[i];
+        for (int i = 0; i < 3; i++) u[i] = 0.5 * (j[i] / (Ms / L)) / (double)s->cell[i];
         or_spintorque_deltaM(m, h, a, b * (1 + a * xi), b * (xi - a), u, s->jmask, 1.0 / (1.0 + a * a), s->p.size);
         return;
     }
@@ -1228,6 +1243,27 @@ void or_sim_set_spintorque(or_sim* s, const double j[3], double spinPol, double 
     s->jmask = (or_real*)malloc(3 * (size_t)s->N * sizeof(or_real));
     if (!s->jmask) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
     for (int i = 0; i < 3 * s->N; i++) s->jmask[i] = (or_real)jmask[i];
+}
+
+void or_sim_set_applied_current(or_sim* s, int n, const double* t, const double* j, int kind, const double* params) {
+    free(s->jT);
+    free(s->jV);
+    s->jT = s->jV = NULL;
+    s->jn = kind ? 0 : n;
+    s->jfn = kind;
+    s->jLastIdx = 0;
+    s->japplied = kind || n > 0;
+    if (kind) {
+        for (int i = 0; i < 4; i++) s->jfp[i] = params[i];
+        for (int c = 0; c < 3; c++) s->jfb[c] = j[c];
+        return;
+    }
+    if (n > 0) {
+        s->jT = (double*)xcalloc((size_t)n, sizeof(double));
+        s->jV = (double*)xcalloc(3 * (size_t)n, sizeof(double));
+        memcpy(s->jT, t, (size_t)n * sizeof(double));
+        memcpy(s->jV, j, 3 * (size_t)n * sizeof(double));
+    }
 }
 
 void or_sim_set_norm_map(or_sim* s, const float* nmap) {

@@ -160,6 +160,11 @@ void or_sim_set_alpha_map(or_sim* s, const float* amap);
  * simspintorque.go): current density j (A/m^2, internal order), 3N mask floats
  * (NULL: 1); copied.  A nonzero j selects the STT torque (torque.go:40-49) */
 void or_sim_set_spintorque(or_sim* s, const double j[3], double spinPol, double xi, const float* jmask);
+/* an applied current density (Apply*("j", ...), simfield.go:61-70): n points
+ * (t s, j A/m^2 internal order; n = 1 static, n = 0 none) or, kind > 0, a function
+ * of time with amplitude j[3] (kinds and params as or_sim_set_field_function);
+ * evaluated at each stage's time, it replaces the current density (torque.go:42-44) */
+void or_sim_set_applied_current(or_sim* s, int n, const double* t, const double* j, int kind, const double* params);
 void or_spintorque_deltaM(const or_real* m, or_real* h, double alpha, double beta, double eps, const double u[3],
                           const or_real* jmap, double dtg, const int n[3]);
 /* per-cell norm of m after each normalisation, N floats (NULL: 1); copied */

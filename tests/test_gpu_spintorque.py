@@ -119,3 +119,32 @@ def test_fused_mode_refuses_a_current(mm):
     s.uniform(1, 0, 0)
     with pytest.raises(mm.InputErr):
         s.steps(1)
+
+
+def test_applied_pointwise_current_run_replayed(mm, orc):
+    """ApplyPointwise("j", ...) (simfield.go:61-70): a current ramp evaluated at
+    each stage's time replaces CurrentDensity (torque.go:42-44); the
+    Dormand-Prince run replayed in the oracle with the same table."""
+    size = (1, 32, 64)
+    print("spin-transfer torque: RKDP run of %s under a pointwise current" % (size,), flush=True)
+    s, o, m0 = pair(mm, orc, size, 7, solver="rkdp", dt=0.0)
+    tt = [-1e-12, 3e-12, 6e-12]
+    jj = np.array([(0.0, 0.0, 0.0), (0.0, 3e12, 1e12), (0.0, -1e12, 2e12)])   # internal (jx, jy, jz)
+    for t, (jx, jy, jz) in zip(tt, jj):
+        s.applypointwise("j", t, jz, jy, jx)
+    o.set_applied_current(tt, jj)
+    o.m = orc.normalize(o.m)
+    s.run(5e-12)
+    dts, acc = s.trial_log()
+    i = 0
+    while i < len(dts):
+        j = i
+        while not acc[j]:
+            j += 1
+        o.step_replay(dts[i:j + 1], acc[i:j + 1])
+        i = j + 1
+    d = float(np.abs(s.m.reshape(-1) - o.m).max())
+    print("%d trials, max |dm| vs FP64 replay %.3g" % (len(dts), d))
+    assert d < 1e-5
+    assert np.allclose(s.current_at(4.5e-12), jj[1] + 0.5 * (jj[2] - jj[1]))
+    s.free()

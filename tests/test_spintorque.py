@@ -108,3 +108,39 @@ def test_spintorque_commands_and_their_checks(mm, tmp_path):
     f.currentmask_array(np.ones((3, 2, 4, 8), np.float32))
     with pytest.raises(mm.InputErr):         # the fused engine has no STT: mode='abi'
         f.params()
+
+
+# ------------------------------------------------------------------ an applied current (Apply*("j", ...))
+def test_applied_current_replaces_the_current_density(orc):
+    """simfield.go:61-70, torque.go:42-44: the applied current, evaluated at the
+    torque's time, is the current the torque uses; a static one equals
+    CurrentDensity of the same value, a pointwise one its interpolation."""
+    n, k = 32, 0.25
+    m0 = spiral(n, k)
+    J0, J1 = np.array([0.0, 0.0, 1e12]), np.array([0.0, 0.0, 3e12])
+    a, b = sim(orc, (1, 1, n), m0), sim(orc, (1, 1, n), m0)
+    a.set_spintorque(J0, 0.5, 0.0, None)
+    b.set_spintorque((0.0, 0.0, 0.0), 0.5, 0.0, None)
+    b.set_applied_current([0.0], J0)
+    assert np.array_equal(a.torque(m0), b.torque(m0))
+    t0, t1 = -2e-12, 6e-12                       # the torque is taken at t = 0
+    b.set_applied_current([t0, t1], np.stack([J0, J1]))
+    w = (0.0 - t0) / (t1 - t0)
+    a.set_spintorque(J0 + w * (J1 - J0), 0.5, 0.0, None)
+    assert np.allclose(a.torque(m0), b.torque(m0), rtol=1e-12, atol=0)
+    b.set_applied_current()                      # off: back to CurrentDensity (0 here): the LL torque
+    c = sim(orc, (1, 1, n), m0)
+    assert np.array_equal(b.torque(m0), c.torque(m0))
+
+
+def test_applied_current_commands(mm):
+    s = mm.Sim("abi")
+    s.gridsize(8, 4, 1)
+    s.cellsize(2e-9, 2e-9, 2e-9)
+    s.applypointwise("j", 0.0, 1e12, 0.0, 0.0)
+    s.applypointwise("currentdensity", 1e-9, 3e12, 0.0, 0.0)   # (jz, jy, jx): the first is internal Z
+    assert np.allclose(s.current_at(0.5e-9), [0.0, 0.0, 2e12])
+    s.applystatic("j", 0.0, 0.0, 5e11)
+    assert np.allclose(s.current_at(1.0), [5e11, 0.0, 0.0])
+    with pytest.raises(mm.InputErr):
+        s.applystatic("h", 0, 0, 0)
