@@ -639,9 +639,10 @@ class Sim:
         return self._jfn is not None or self._jstatic is not None or len(self._jfield) >= 2
 
     def _push_current(self):
-        # like the field: a pointwise current needs two points (simfield.go:123-125)
-        if not self._ptr or not (self._applied_current() or not self._jfield):
+        if not self._ptr:
             return
+        if len(self._jfield) == 1 and self._jstatic is None and self._jfn is None:
+            return  # like the field: a pointwise current needs two points (simfield.go:123-125)
         if self._applied_current() and self._jmask is None:
             raise InputErr("Spin-transfer torque requires a currentMask to be set")
         prm = np.zeros(4)
