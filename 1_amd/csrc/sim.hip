@@ -28,6 +28,8 @@ namespace {
 // rk.go:62-212, coefficients rounded to float32 as in the reference
 Tableau make_tableau(int solver) {
     Tableau t;
+    // the semi-analytic solvers keep RK2's buffers: k_0 holds h, k_1 the corrector's m3
+    if (solver == MM_SEMIANAL1 || solver == MM_SEMIANAL2) solver = MM_RK2;
     switch (solver) {
     case MM_RK2:
     case MM_RK12:
@@ -482,7 +484,48 @@ static void normalize_abi(mmSim* s, float* m) {
         gpu_normalize_uniform(m, s->N);
 }
 
+static bool semianal(const mmSim* s) { return s->p.solver == MM_SEMIANAL1 || s->p.solver == MM_SEMIANAL2; }
+
+// SemiAnal1.step / SemiAnal2.step (semianal.go:23-58, 87-140) with Step's counter
+// (simrun.go:76-77): m2 (am1) carries over between steps; the time is not advanced
+static void abi_semianal_step(mmSim* s) {
+    const int N = s->N, n3 = (int)s->N3;
+    const float dt = (float)(s->p.dt / s->unitTime), alpha = (float)s->p.alpha;
+    float *m1 = s->m, *m2 = s->am1, *m3 = s->k[1], *h = s->k[0];
+    if (s->hsteps == 0) {
+        abi_calc_heff(s, m1, h);
+        gpu_anal_fw_step(dt / 2.f, alpha, N, m1, m2, h);
+        abi_calc_heff(s, m2, h);
+        gpu_anal_fw_step(dt, alpha, N, m1, m1, h);
+    } else if (s->p.solver == MM_SEMIANAL1) {
+        abi_calc_heff(s, m1, h);
+        gpu_anal_fw_step(dt, alpha, N, m2, m2, h);
+        abi_calc_heff(s, m2, h);
+        gpu_anal_fw_step(dt, alpha, N, m1, m1, h);
+    } else {
+        abi_calc_heff(s, m1, h);
+        gpu_anal_fw_step(dt, alpha, N, m2, m3, h);
+        gpu_linear_combination(m3, m2, 0.5f, 0.5f, n3);
+        abi_calc_heff(s, m3, h);
+        gpu_anal_fw_step(dt, alpha, N, m2, m2, h);
+        abi_calc_heff(s, m2, h);
+        gpu_anal_fw_step(dt, alpha, N, m1, m3, h);
+        gpu_linear_combination(m3, m1, 0.5f, 0.5f, n3);
+        abi_calc_heff(s, m3, h);
+        gpu_anal_fw_step(dt, alpha, N, m1, m1, h);
+    }
+    if (s->hsteps % 100 == 0) {
+        normalize_abi(s, m1);
+        normalize_abi(s, m2);
+    }
+    s->hsteps++;
+}
+
 static void abi_step(mmSim* s) {
+    if (semianal(s)) {
+        abi_semianal_step(s);
+        return;
+    }
     // rk.go:279-445
     const Tableau& t = s->tb;
     const int order = t.S;
@@ -673,6 +716,8 @@ mmSim* mm_sim_new(const mmParams* p, const float* m0, int mode) {
     set_thermal(s, p->temperature);
     for (int i = 0; i < 3; i++) s->u[i] = (float)p->anisAxis[i];
     s->tb = make_tableau(p->solver);
+    GK_ASSERT(!(p->solver == MM_SEMIANAL1 || p->solver == MM_SEMIANAL2) || mode == MM_MODE_ABI,
+              "the semi-analytic solvers run in MM_MODE_ABI");
     GK_ASSERT(!s->tb.adaptive || p->maxError > 0, "maxError must be > 0 for an adaptive solver");
     GK_ASSERT(s->tb.adaptive || p->dt > 0, "a fixed-step solver needs dt > 0");
     if (p->nField > 0) {
@@ -770,6 +815,7 @@ void mm_sim_free(mmSim* s) {
 }
 
 long mm_sim_run(mmSim* s, double seconds) {
+    GK_ASSERT(!semianal(s), "the semi-analytic solvers do not advance the time (semianal.go): use Steps");
     if (s->mode == MM_MODE_ABI) {
         const double stop = s->htime + seconds / s->unitTime;
         normalize_abi(s, s->m);  // simrun.go:24
@@ -787,6 +833,7 @@ long mm_sim_run(mmSim* s, double seconds) {
 }
 
 long mm_sim_run_until(mmSim* s, double stop) {
+    GK_ASSERT(!semianal(s), "the semi-analytic solvers do not advance the time (semianal.go): use Steps");
     if (s->mode == MM_MODE_ABI) {
         long n = 0;
         while (s->htime < stop) {

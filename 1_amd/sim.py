@@ -17,7 +17,8 @@ import numpy as np
 from . import gpukern
 
 MODES = {"fused": 0, "abi": 1}
-SOLVERS = {"rk2": 2, "rk12": 12, "rk3": 3, "rk23": 23, "rk4": 4, "rkck": 45, "rkdp": 54, "rk45": 54}
+SOLVERS = {"rk2": 2, "rk12": 12, "rk3": 3, "rk23": 23, "rk4": 4, "rkck": 45, "rkdp": 54, "rk45": 54,
+           "semianal1": 101, "semianal2": 102}
 
 
 class InputErr(ValueError):
@@ -930,6 +931,7 @@ class Sim:
         """Run(seconds) (simrun.go:19-57): normalise m, then step until the
         time has advanced by `seconds`, saving scheduled output before each
         step.  Returns the number of steps taken."""
+        self._no_semianal("run")
         self._init()
         if not self._schedule:
             n = int(self._lib.mm_sim_run(self._ptr, float(seconds)))
@@ -969,6 +971,7 @@ class Sim:
         """Relax() (relax.go:14-25): steps with energy control until a step
         raises the total energy (or max_steps > 0 steps were taken); scheduled
         output is saved after each step.  Returns the number of steps taken."""
+        self._no_semianal("relax")
         self._init()
         self._m_fresh = False
         if not self._schedule:
@@ -1143,6 +1146,11 @@ class Sim:
             raise InputErr("The gridsize should be set first. E.g.: gridsize 32 32 4")
         return self._size
 
+    def _no_semianal(self, what):
+        # SemiAnal1/2 (semianal.go) never advance the time: only Steps ends with them
+        if self._solver.startswith("semianal"):
+            raise InputErr("%s with %s would not end (its steps do not advance the time): use steps" % (what, self._solver))
+
     def _invalidate(self):
         if self._ptr:
             # init.go:20-26: re-initialisation is not allowed
@@ -1180,6 +1188,8 @@ class Sim:
             amap = np.ascontiguousarray(self._alpha_map, np.float32)
             self._keep_alpha = amap
             p.alphaMap = amap.ctypes.data_as(C.POINTER(C.c_float))
+        if self._solver.startswith("semianal") and self.mode != "abi":
+            raise InputErr("the semi-analytic solvers run in the ABI mode (Sim(mode='abi'))")
         if any(self._current) or self._applied_current():
             if self.mode != "abi":
                 raise InputErr("spin-transfer torque runs in the ABI mode (Sim(mode='abi'))")

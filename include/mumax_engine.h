@@ -38,6 +38,10 @@ enum { MM_KERNEL_FACE6 = 0, MM_KERNEL_NEWELL = 1 };
 enum { MM_FIELD_PULSE = 1, MM_FIELD_RF = 2, MM_FIELD_RF_RAMP = 3, MM_FIELD_SAWTOOTH = 4, MM_FIELD_ROTATING = 5 };
 /* solver codes: rk.go tableaux (solver.go:16-53) */
 enum { MM_RK2 = 2, MM_RK12 = 12, MM_RK3 = 3, MM_RK23 = 23, MM_RK4 = 4, MM_RKCK = 45, MM_RKDP = 54 };
+/* The semi-analytic solvers (SemiAnal1 / SemiAnal2, semianal.go; solver.go:46-49):
+ * MM_MODE_ABI, fixed dt, mm_sim_steps only -- their steps never advance the time,
+ * so a Run would not end (the reference's do not either). */
+enum { MM_SEMIANAL1 = 101, MM_SEMIANAL2 = 102 };
 
 /* Simulation parameters: the reference's input commands (sim.go:50-170),
  * already in internal axis order. */

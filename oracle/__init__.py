@@ -19,7 +19,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 XX, YY, ZZ, YZ, XZ, XY = range(6)
 REDUCE_ADD, REDUCE_MAX, REDUCE_MAXABS, REDUCE_MIN, REDUCE_SUMABS = 1, 2, 3, 4, 5
 ANIS_NONE, ANIS_UNIAXIAL = 0, 1
-SOLVERS = {"rk2": 2, "rk12": 12, "rk3": 3, "rk23": 23, "rk4": 4, "rkck": 45, "rkdp": 54, "rk45": 54}
+SOLVERS = {"rk2": 2, "rk12": 12, "rk3": 3, "rk23": 23, "rk4": 4, "rkck": 45, "rkdp": 54, "rk45": 54,
+           "semianal1": 101, "semianal2": 102}
 
 
 def build():
@@ -110,6 +111,7 @@ class Oracle:
         L.or_sim_set_norm_map.argtypes = [P, P]
         L.or_sim_set_spintorque.argtypes = [P, P, C.c_double, C.c_double, P]
         L.or_sim_set_applied_current.argtypes = [P, C.c_int, P, P, C.c_int, P]
+        L.or_anal_step.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int]
         L.or_normalize_map.argtypes = [P, P, C.c_int]
         L.or_sim_set_field_function.argtypes = [P, C.c_int, P, P]
         L.or_sim_field_at.argtypes = [P, C.c_double, P]
@@ -203,6 +205,12 @@ class Oracle:
     def normalize(self, m):
         m = np.ascontiguousarray(m, self.real).copy()
         self.lib.or_normalize(m.ctypes.data, m.size // 3)
+        return m
+
+    def anal_step(self, m, h, dt, alpha):
+        m = np.ascontiguousarray(m, self.real).copy()
+        h = np.ascontiguousarray(h, self.real)
+        self.lib.or_anal_step(m.ctypes.data, m.ctypes.data, h.ctypes.data, dt, alpha, m.size // 3)
         return m
 
     def normalize_map(self, m, nmap):

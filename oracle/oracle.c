@@ -590,6 +590,38 @@ void or_normalize(or_real* m, int N) {
     }
 }
 
+/* gpu_anal.cu:23-84: the exact solution of the LL equation in a constant field over dt.
+ * m is rotated into a frame whose first axis is h (the fixed frame for h along z),
+ * the polar component advanced by the damping, the azimuth by the precession,
+ * and rotated back; a cell with m = 0 is left as it is. */
+void or_anal_step(const or_real* min, or_real* mout, const or_real* h, double dt, double alpha, int N) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        const double mx = min[i], my = min[N + i], mz = min[2 * N + i];
+        if (mx == 0 && my == 0 && mz == 0) continue;
+        const double hx = h[i], hy = h[N + i], hz = h[2 * N + i];
+        double r0, r1, r2, r3, r4, r5, r6, r8, hr; /* rows of the rotation; hr = 1/|h| */
+        if (hx == 0 && hy == 0) {
+            r0 = 0; r1 = 0; r2 = -1; r3 = 0; r4 = 1; r5 = 0; r6 = 1; r8 = 0;
+            hr = 1.0 / hz;
+        } else {
+            const double t = hx * hx + hy * hy, hxy = 1.0 / sqrt(t);
+            hr = 1.0 / sqrt(t + hz * hz);
+            r0 = hx * hr; r1 = -hy * hxy; r2 = -r0 * hz * hxy;
+            r3 = hy * hr; r4 = hx * hxy; r5 = r1 * hz * hr;
+            r6 = hz * hr; r8 = hr / hxy;
+        }
+        const double a = mx * r0 + my * r3 + mz * r6, b = mx * r1 + my * r4, c = mx * r2 + my * r5 + mz * r8;
+        const double qt = dt / (1 + alpha * alpha), ex = exp(alpha * qt / hr), sn = sin(qt / hr), cs = cos(qt / hr);
+        const double den = ex * (1 + a) + (1 - a) / ex;
+        const double an = (ex * (1 + a) - (1 - a) / ex) / den, bn = 2 * (b * cs - c * sn) / den,
+                     cn = 2 * (b * sn + c * cs) / den;
+        mout[i] = (or_real)(an * r0 + bn * r1 + cn * r2);
+        mout[N + i] = (or_real)(an * r3 + bn * r4 + cn * r5);
+        mout[2 * N + i] = (or_real)(an * r6 + cn * r8);
+    }
+}
+
 /* gpu_spintorque.cu:24-180 (Zhang-Li torque): Hspin = sum_d u_d (m(+1) - m(-1))
  * along each axis, the cell itself standing in for a missing neighbour at the
  * box edges (:58-83), u_d scaled by the mask's component d of the cell (NULL: 1);
@@ -760,6 +792,7 @@ struct or_sim {
     double j[3];   /* current density, A/m^2 (CurrentDensity, simspintorque.go:30-36) */
     double spinPol, xi;
     or_real* jmask; /* 3N current mask (CurrentMask), NULL: 1 */
+    or_real *sa_m2, *sa_m3, *sa_h; /* semi-analytic solvers' m2, m3 and field (allocated at their first step) */
     int japplied;   /* an applied current (Apply*("j", ...), simfield.go:61-70) replaces j at each torque */
     int jn, jfn, jLastIdx;
     double *jT, *jV, jfp[4], jfb[3];
@@ -815,7 +848,7 @@ static void set_tableau(or_sim* s) {
     memset(s->b2, 0, sizeof(s->b2));
     s->fsal = 0;
     s->has_b2 = 0;
-    switch (s->p.solver) {
+    switch (s->p.solver == OR_SEMIANAL1 || s->p.solver == OR_SEMIANAL2 ? OR_RK2 : s->p.solver) {
     case OR_RK2:
     case OR_RK12:
         s->stages = 2;
@@ -996,6 +1029,9 @@ void or_sim_free(or_sim* s) {
     free(s->jmask);
     free(s->jT);
     free(s->jV);
+    free(s->sa_m2);
+    free(s->sa_m3);
+    free(s->sa_h);
     free(s->noise);
     free(s);
 }
@@ -1134,7 +1170,52 @@ static int rk_step(or_sim* s, const double* dt_trial, const int* accept, int n_t
     return trials;
 }
 
+/* SemiAnal1.step / SemiAnal2.step (semianal.go:23-58, 87-140): m2 carries over
+ * between steps, the time is not advanced; m, m2 normalised every 100 steps */
+static void semianal_step(or_sim* s) {
+    const int N = s->N;
+    const size_t n3 = 3 * (size_t)N;
+    if (!s->sa_m2) {
+        s->sa_m2 = (or_real*)xcalloc(n3, sizeof(or_real));
+        s->sa_m3 = (or_real*)xcalloc(n3, sizeof(or_real));
+        s->sa_h = (or_real*)xcalloc(n3, sizeof(or_real));
+    }
+    or_real *m1 = s->m, *m2 = s->sa_m2, *m3 = s->sa_m3, *h = s->sa_h;
+    const double dt = s->p.dt / s->units[1], a = s->p.alpha;
+    if (s->steps == 0) {
+        calc_heff(s, m1, h, 0);
+        or_anal_step(m1, m2, h, dt / 2, a, N);
+        calc_heff(s, m2, h, 0);
+        or_anal_step(m1, m1, h, dt, a, N);
+    } else if (s->p.solver == OR_SEMIANAL1) {
+        calc_heff(s, m1, h, 0);
+        or_anal_step(m2, m2, h, dt, a, N);
+        calc_heff(s, m2, h, 0);
+        or_anal_step(m1, m1, h, dt, a, N);
+    } else {
+        calc_heff(s, m1, h, 0);
+        or_anal_step(m2, m3, h, dt, a, N);
+        for (size_t i = 0; i < n3; i++) m3[i] = (or_real)(0.5 * m3[i] + 0.5 * m2[i]); /* LinearCombination */
+        calc_heff(s, m3, h, 0);
+        or_anal_step(m2, m2, h, dt, a, N);
+        calc_heff(s, m2, h, 0);
+        or_anal_step(m1, m3, h, dt, a, N);
+        for (size_t i = 0; i < n3; i++) m3[i] = (or_real)(0.5 * m3[i] + 0.5 * m1[i]);
+        calc_heff(s, m3, h, 0);
+        or_anal_step(m1, m1, h, dt, a, N);
+    }
+    if (s->steps % 100 == 0) {
+        sim_normalize(s, m1);
+        sim_normalize(s, m2);
+    }
+}
+
 int or_sim_step(or_sim* s) {
+    if (s->p.solver == OR_SEMIANAL1 || s->p.solver == OR_SEMIANAL2) {
+        semianal_step(s);
+        s->steps++;
+        return 1;
+    }
     int r = rk_step(s, NULL, NULL, 0);
     s->steps++;
     return r;

@@ -38,6 +38,7 @@ enum { OR_REDUCE_ADD = 1, OR_REDUCE_MAX = 2, OR_REDUCE_MAXABS = 3, OR_REDUCE_MIN
 enum { OR_ANIS_NONE = 0, OR_ANIS_UNIAXIAL = 1 };
 /* solvers (solver.go:16-53; rk.go tableaux) */
 enum { OR_RK2 = 2, OR_RK12 = 12, OR_RK3 = 3, OR_RK23 = 23, OR_RK4 = 4, OR_RKCK = 45, OR_RKDP = 54 };
+enum { OR_SEMIANAL1 = 101, OR_SEMIANAL2 = 102 }; /* semianal.go: fixed dt, the time is not advanced */
 
 int or_real_size(void);
 
@@ -86,6 +87,8 @@ void or_local_fields_phi(const or_real* m, or_real* h, or_real* phi, int N, cons
 void or_deltaM(const or_real* m, or_real* h, or_real alpha, or_real dtgilb, int N);
 void or_deltaM_map(const or_real* m, or_real* h, or_real alpha_mul, const or_real* amap, or_real dtgilb, int N);
 void or_normalize(or_real* m, int N);
+/* gpu_anal.cu:23-84: m advanced over dt in a constant field h in closed form (in place allowed) */
+void or_anal_step(const or_real* min, or_real* mout, const or_real* h, double dt, double alpha, int N);
 void or_normalize_map(or_real* m, const or_real* nmap, int N);
 void or_scale_dot(or_real* r, const or_real* a, const or_real* b, or_real s, int N);
 double or_reduce(int op, const or_real* x, int n);
